@@ -1,0 +1,219 @@
+"""ctypes binding of libdimspers_stokes.so (the C ABI in include/dimspers_stokes.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libdimspers_stokes.so"
+
+STK_OK = 0
+STK_E_HIP, STK_E_SHAPE, STK_E_STATE, STK_E_NOTCONV, STK_E_NAN, STK_E_ARG, STK_E_NOMESH = -1, -2, -3, -4, -5, -6, -7
+
+ASM_FP64, ASM_FP32, ASM_CARTESIAN, ASM_NO_BC, ASM_ISOTROPIC, ASM_NO_MG = 1, 2, 4, 8, 16, 32
+MAT_K, MAT_G, MAT_GT, MAT_M, MAT_S = 0, 1, 2, 3, 4
+SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED = 0, 1, 2
+INNER_MG, INNER_JACOBI = 0, 1
+STOP_SCHUR, STOP_FULL = 0, 1
+NULL_PROJECT, NULL_DEFLATE = 0, 1
+
+EXPORTED = [
+    "stk_default_opts", "stk_create", "stk_destroy", "stk_last_error", "stk_info", "stk_set_fields",
+    "stk_assemble", "stk_export_csc", "stk_export_rhs", "stk_spmv", "stk_solve", "stk_get_solution",
+    "stk_updatevelocity",
+]
+
+
+class StkOpts(C.Structure):
+    _fields_ = [
+        ("method", C.c_int32), ("inner", C.c_int32), ("stop", C.c_int32), ("itmax", C.c_int32),
+        ("itmin", C.c_int32), ("inner_itmax", C.c_int32), ("refine_max", C.c_int32), ("cheb_degree", C.c_int32),
+        ("null_mode", C.c_int32), ("reserved", C.c_int32),
+        ("rtol", C.c_double), ("inner_rtol", C.c_double), ("mixed_inner_rtol", C.c_double),
+        ("mixed_schur_rtol", C.c_double), ("stall_ratio", C.c_double),
+    ]
+
+
+class StkReport(C.Structure):
+    _fields_ = [
+        ("converged", C.c_int32), ("outer_iterations", C.c_int32), ("inner_iterations", C.c_int64),
+        ("refinements", C.c_int32), ("fallbacks", C.c_int32), ("schur_residual", C.c_double),
+        ("full_residual", C.c_double), ("solve_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class StkError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+dp = C.POINTER(C.c_double)
+i64p = C.POINTER(C.c_int64)
+vp = C.c_void_p
+
+
+def load(path: Path | str | None = None):
+    """Load (once) and return the shared library with argument types set."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(str(p))
+    lib.stk_default_opts.argtypes = [C.POINTER(StkOpts)]
+    lib.stk_default_opts.restype = None
+    lib.stk_create.argtypes = [C.POINTER(vp), C.c_int64, C.c_int64, C.c_int64, i64p, dp, dp, dp, dp, i64p, C.c_int64]
+    lib.stk_destroy.argtypes = [vp]
+    lib.stk_destroy.restype = None
+    lib.stk_last_error.argtypes = [vp]
+    lib.stk_last_error.restype = C.c_char_p
+    lib.stk_info.argtypes = [vp, i64p]
+    lib.stk_set_fields.argtypes = [vp, C.c_double, dp, dp, C.c_int64, C.POINTER(dp)]
+    lib.stk_assemble.argtypes = [vp, C.c_int32]
+    lib.stk_export_csc.argtypes = [vp, C.c_int32, C.c_int32, i64p, i64p, i64p, i64p, i64p, dp]
+    lib.stk_export_rhs.argtypes = [vp, dp]
+    lib.stk_spmv.argtypes = [vp, C.c_int32, C.c_int32, dp, dp]
+    lib.stk_solve.argtypes = [vp, C.POINTER(StkOpts), C.POINTER(StkReport)]
+    lib.stk_get_solution.argtypes = [vp, dp, dp]
+    lib.stk_updatevelocity.argtypes = [vp, dp, dp, dp]
+    for name in EXPORTED:
+        if name not in ("stk_default_opts", "stk_destroy", "stk_last_error"):
+            getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def _d(a: np.ndarray):
+    return a.ctypes.data_as(dp)
+
+
+def _i(a: np.ndarray):
+    return a.ctypes.data_as(i64p)
+
+
+def default_opts(**kw) -> StkOpts:
+    o = StkOpts()
+    load().stk_default_opts(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(k)
+        setattr(o, k, v)
+    return o
+
+
+class Context:
+    """Owns one stk_ctx (device memory + stream on the current device)."""
+
+    def __init__(self, nel, nnod, nvert, e2n_1based, el_theta, el_r, node_theta, node_r, fixed_1based):
+        self.lib = load()
+        e2n = np.ascontiguousarray(np.asfortranarray(e2n_1based, dtype=np.int64).T)   # element-major = Julia 6 x nel
+        self._keep = [
+            e2n,
+            np.ascontiguousarray(np.asarray(el_theta, dtype=np.float64).T),
+            np.ascontiguousarray(np.asarray(el_r, dtype=np.float64).T),
+            np.ascontiguousarray(node_theta, dtype=np.float64),
+            np.ascontiguousarray(node_r, dtype=np.float64),
+            np.ascontiguousarray(fixed_1based, dtype=np.int64),
+        ]
+        h = vp()
+        rc = self.lib.stk_create(C.byref(h), nel, nnod, nvert, _i(self._keep[0]), _d(self._keep[1]),
+                                 _d(self._keep[2]), _d(self._keep[3]), _d(self._keep[4]), _i(self._keep[5]),
+                                 self._keep[5].size)
+        if rc != STK_OK:
+            raise StkError(rc, self.lib.stk_last_error(None).decode())
+        self.h = h
+        info = np.zeros(7, dtype=np.int64)
+        self.lib.stk_info(self.h, _i(info))
+        self.nel, self.nnod, self.nvert, self.nU, self.nP, self.ncolours, self.mg_levels = (int(x) for x in info)
+
+    @classmethod
+    def from_mesh(cls, mesh):
+        fixed, _ = mesh.free_slip()
+        return cls(mesh.nel, mesh.nnod, mesh.nvert, mesh.e2n + 1, mesh.el_theta, mesh.el_r, mesh.theta, mesh.r,
+                   fixed + 1)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.stk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, allow=()):
+        if rc != STK_OK and rc not in allow:
+            raise StkError(rc, self.lib.stk_last_error(self.h).decode())
+        return rc
+
+    def set_fields(self, g, rho, eta_ip, C6=None):
+        eta = np.asfortranarray(eta_ip, dtype=np.float64)
+        nip = eta.shape[1]
+        self._fields = [np.ascontiguousarray(rho, dtype=np.float64), eta]
+        arr = None
+        if C6 is not None:
+            comps = [np.asfortranarray(c, dtype=np.float64) for c in C6]
+            self._fields += comps
+            arr = (dp * 6)(*[_d(c) for c in comps])
+        self._check(self.lib.stk_set_fields(self.h, float(g), _d(self._fields[0]), _d(eta), nip, arr))
+
+    def assemble(self, flags=ASM_FP64):
+        self._check(self.lib.stk_assemble(self.h, int(flags)))
+
+    def export_csc(self, which, lower_only=False):
+        """Assembled operator as a 0-based scipy CSC matrix."""
+        import scipy.sparse as sp
+        nr, nc, nnz = C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(self.lib.stk_export_csc(self.h, which, int(lower_only), C.byref(nr), C.byref(nc), C.byref(nnz),
+                                            None, None, None))
+        colptr = np.zeros(nc.value + 1, dtype=np.int64)
+        rowval = np.zeros(nnz.value, dtype=np.int64)
+        val = np.zeros(nnz.value)
+        self._check(self.lib.stk_export_csc(self.h, which, int(lower_only), None, None, None, _i(colptr), _i(rowval),
+                                            _d(val)))
+        return sp.csc_matrix((val, rowval - 1, colptr - 1), shape=(nr.value, nc.value))
+
+    def export_rhs(self):
+        F = np.zeros(self.nU)
+        self._check(self.lib.stk_export_rhs(self.h, _d(F)))
+        return F
+
+    def spmv(self, which, x, precision=64):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        ny = {MAT_K: self.nU, MAT_G: self.nU, MAT_GT: self.nP, MAT_M: self.nP, MAT_S: self.nP}[which]
+        y = np.zeros(ny)
+        self._check(self.lib.stk_spmv(self.h, which, precision, _d(x), _d(y)))
+        return y
+
+    def solve(self, opts: StkOpts | None = None, allow_notconv=False):
+        rep = StkReport()
+        o = opts if opts is not None else default_opts()
+        rc = self.lib.stk_solve(self.h, C.byref(o), C.byref(rep))
+        self._check(rc, allow=(STK_E_NOTCONV,) if allow_notconv else ())
+        return rep
+
+    def solution(self):
+        U = np.zeros(self.nU)
+        P = np.zeros(self.nP)
+        self._check(self.lib.stk_get_solution(self.h, _d(U), _d(P)))
+        return U, P
+
+    def updatevelocity(self):
+        U = np.zeros(self.nU)
+        Uc = np.zeros(self.nU)
+        w = C.c_double()
+        self._check(self.lib.stk_updatevelocity(self.h, _d(U), _d(Uc), C.byref(w)))
+        return U, Uc, w.value

@@ -1,0 +1,415 @@
+// extern "C" entry points of libdimspers_stokes.so (declared in include/dimspers_stokes.h).
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "device_ops.h"
+#include "solver.h"
+
+namespace stk {
+template <typename S, typename D> void convert(Context &c, const S *src, D *dst, int64_t n);
+template <typename T> void lumped_inverse(Context &c, const Pattern &p, const T *M, T *pc);
+template <typename T> void csr_diag_inverse(Context &c, const Pattern &p, const T *A, T *dinv);
+void host_spmv(Context &c, int which, int prec, const double *xh, double *yh);
+double update_velocity(Context &c, double *Ucart_dev);
+
+Context::~Context() {
+    if (h_pinned) (void)hipHostFree(h_pinned);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+}  // namespace stk
+
+struct stk_ctx {
+    stk::Context c;
+};
+
+using namespace stk;
+
+static thread_local std::string g_err;
+
+template <typename F>
+static int guarded(stk_ctx *ctx, F f) {
+    try {
+        if (ctx) {
+            STK_HIP(hipSetDevice(ctx->c.device));
+            ctx->c.err.clear();
+        }
+        return f();
+    } catch (const Error &e) {
+        (ctx ? ctx->c.err : g_err) = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        (ctx ? ctx->c.err : g_err) = e.what();
+        return STK_E_HIP;
+    }
+}
+
+extern "C" {
+
+void stk_default_opts(stk_opts *o) {
+    o->method = STK_SOLVE_PLAIN;
+    o->inner = STK_INNER_MG;
+    o->stop = STK_STOP_SCHUR;
+    o->itmax = 200;
+    o->itmin = 10;
+    o->inner_itmax = 2000;
+    o->refine_max = 12;
+    o->cheb_degree = 3;
+    o->rtol = 1e-8;
+    o->inner_rtol = 1e-11;
+    o->mixed_inner_rtol = 1e-3;
+    o->mixed_schur_rtol = 1e-3;
+    o->stall_ratio = 0.5;
+}
+
+const char *stk_last_error(const stk_ctx *ctx) { return ctx ? ctx->c.err.c_str() : g_err.c_str(); }
+
+void stk_destroy(stk_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->c.device);
+    delete ctx;
+}
+
+int stk_create(stk_ctx **out, int64_t nel, int64_t nnod, int64_t nvert, const int64_t *e2n,
+               const double *el_theta, const double *el_r, const double *node_theta, const double *node_r,
+               const int64_t *fixed_dofs, int64_t nfixed) {
+    g_err.clear();
+    if (!out) { g_err = "ctx pointer is NULL"; return STK_E_ARG; }
+    *out = nullptr;
+    stk_ctx *ctx = new stk_ctx();
+    int rc = guarded(nullptr, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK(nel > 0 && nnod > 0 && nvert > 0 && nvert <= nnod, STK_E_SHAPE, "bad mesh sizes");
+        STK_CHECK(nnod < (int64_t(1) << 30), STK_E_SHAPE, "mesh too large for 32-bit node ids");
+        STK_CHECK(e2n && el_theta && el_r && node_theta && node_r, STK_E_ARG, "NULL mesh array");
+        STK_HIP(hipGetDevice(&c.device));
+        STK_HIP(hipStreamCreate(&c.stream));
+        c.nel = nel; c.nnod = nnod; c.nvert = nvert; c.nU = 2 * nnod; c.nP = nvert;
+        c.e2n.resize(6 * nel);
+        for (int64_t e = 0; e < nel; ++e)
+            for (int a = 0; a < 6; ++a) {
+                const int64_t v = e2n[e * 6 + a] - 1;
+                STK_CHECK(v >= 0 && v < nnod, STK_E_SHAPE, "e2n entry out of range");
+                if (a < 3) STK_CHECK(v < nvert, STK_E_SHAPE, "vertex id beyond nvert");
+                c.e2n[e * 6 + a] = (int32_t)v;
+            }
+        c.el_th.assign(el_theta, el_theta + 3 * nel);
+        c.el_r.assign(el_r, el_r + 3 * nel);
+        c.node_th.assign(node_theta, node_theta + nnod);
+        c.node_r.assign(node_r, node_r + nnod);
+        c.fixed.assign(2 * nnod, 0);
+        for (int64_t i = 0; i < nfixed; ++i) {
+            const int64_t d = fixed_dofs[i] - 1;
+            STK_CHECK(d >= 0 && d < 2 * nnod, STK_E_SHAPE, "fixed dof out of range");
+            c.fixed[d] = 1;
+        }
+        for (int64_t d = 0; d < 2 * nnod; ++d)
+            if (c.fixed[d]) c.fixed_list.push_back(d);
+        c.d_fixlist.upload(c.fixed_list, c.stream);
+        build_patterns(c);
+        build_colouring(c);
+        build_slot_maps(c);
+        detect_structured(c);
+        build_mg_structure(c);
+        if (!c.mg.empty()) {
+            STK_CHECK(c.mg[0].pat.ptr == c.MN.ptr && c.mg[0].pat.col == c.MN.col, STK_E_SHAPE,
+                      "vertex pattern mismatch between mesh and multigrid level 0");
+            for (MGLevel &L : c.mg) {
+                std::vector<int64_t> fl;
+                const int64_t nr = L.ncr + 1;
+                for (int64_t v = 0; v < L.nvert; ++v)
+                    if (v % nr == 0 || v % nr == L.ncr) fl.push_back(2 * v + 1);
+                L.nfix = (int64_t)fl.size();
+                L.fixlist.upload(fl, c.stream);
+            }
+        }
+        // reduction partials: the largest node-row grid of any SpMV with a fused dot
+        const int64_t maxrows = std::max<int64_t>(c.nnod, c.nvert);
+        c.red64.alloc(std::max<int64_t>(kReduceBlocks, maxrows / 32 + 64));
+        c.d_sc.alloc(SC_COUNT);
+        c.d_sc.zero(c.stream);
+        STK_HIP(hipHostMalloc(&c.h_pinned, SC_COUNT * sizeof(double), hipHostMallocDefault));
+        upload_quadrature(c.stream);
+        STK_HIP(hipStreamSynchronize(c.stream));
+        return STK_OK;
+    });
+    if (rc != STK_OK) {
+        delete ctx;
+        return rc;
+    }
+    *out = ctx;
+    return STK_OK;
+}
+
+int stk_info(const stk_ctx *ctx, int64_t *o) {
+    if (!ctx || !o) return STK_E_ARG;
+    const Context &c = ctx->c;
+    o[0] = c.nel; o[1] = c.nnod; o[2] = c.nvert; o[3] = c.nU; o[4] = c.nP;
+    o[5] = (int64_t)c.col_off.size() - 1;
+    o[6] = (int64_t)c.mg.size();
+    return STK_OK;
+}
+
+int stk_set_fields(stk_ctx *ctx, double g, const double *rho, const double *eta_ip, int64_t nip,
+                   const double *const C[6]) {
+    if (!ctx) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK(rho && eta_ip, STK_E_ARG, "NULL field array");
+        STK_CHECK(nip >= 6, STK_E_SHAPE, "eta.ip needs at least 6 integration-point columns");
+        c.g = g;
+        c.d_rho.upload(rho, c.nnod, c.stream);
+        // Julia nel x nip column-major: the first six columns are contiguous
+        c.d_eta.upload(eta_ip, 6 * c.nel, c.stream);
+        if (C) {
+            std::vector<double> buf(36 * c.nel);
+            for (int k = 0; k < 6; ++k) {
+                STK_CHECK(C[k], STK_E_ARG, "NULL tensor component");
+                std::copy(C[k], C[k] + 6 * c.nel, buf.begin() + (int64_t)k * 6 * c.nel);
+            }
+            c.d_C.upload(buf, c.stream);
+            c.have_C = true;
+            STK_HIP(hipStreamSynchronize(c.stream));
+        } else {
+            c.have_C = false;
+        }
+        STK_HIP(hipStreamSynchronize(c.stream));
+        c.have_fields = true;
+        return STK_OK;
+    });
+}
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+void assemble_prec(Context &c, int flags, bool mg, DBuf<T> &K, DBuf<T> &G, DBuf<T> &GT, DBuf<T> &M, DBuf<T> &F,
+                   bool f64) {
+    K.alloc(4 * c.KN.nnz()); K.zero(c.stream);
+    G.alloc(2 * c.GN.nnz()); G.zero(c.stream);
+    GT.alloc(2 * c.GTN.nnz()); GT.zero(c.stream);
+    M.alloc(c.MN.nnz()); M.zero(c.stream);
+    F.alloc(c.nU); F.zero(c.stream);
+    T *A1 = nullptr, *E1 = nullptr;
+    auto Abuf = [&](MGLevel &L) -> DBuf<T> & {
+        if constexpr (sizeof(T) == 8) return L.A64; else return L.A32;
+    };
+    auto Ebuf = [&](MGLevel &L) -> DBuf<T> & {
+        if constexpr (sizeof(T) == 8) return L.E64; else return L.E32;
+    };
+    if (mg) {
+        for (MGLevel &L : c.mg) {
+            Abuf(L).alloc(4 * L.pat.nnz());
+            Abuf(L).zero(c.stream);
+            Ebuf(L).alloc(36 * L.nel);
+        }
+        A1 = Abuf(c.mg[0]).p;
+        E1 = Ebuf(c.mg[0]).p;
+    }
+    assemble_impl<T>(c, flags, K.p, G.p, GT.p, M.p, F.p, A1, E1);
+    if (mg)
+        for (size_t l = 0; l + 1 < c.mg.size(); ++l)
+            galerkin_impl<T>(c, l, Ebuf(c.mg[l]).p, Ebuf(c.mg[l + 1]).p, Abuf(c.mg[l + 1]).p);
+    if (!(flags & STK_ASM_NO_BC)) fix_diag<T>(c, c.KN, (int64_t)c.fixed_list.size(), c.d_fixlist.p, K.p);
+    if (mg)
+        for (MGLevel &L : c.mg) fix_diag<T>(c, L.pat, L.nfix, L.fixlist.p, Abuf(L).p);
+    (void)f64;
+}
+
+}  // namespace
+
+extern "C" {
+
+int stk_assemble(stk_ctx *ctx, int32_t flags) {
+    if (!ctx) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK(c.have_fields, STK_E_STATE, "stk_set_fields must precede stk_assemble");
+        const bool fp32 = flags & STK_ASM_FP32;
+        const bool fp64 = (flags & STK_ASM_FP64) || !fp32;
+        const bool solve_form = !(flags & (STK_ASM_CARTESIAN | STK_ASM_NO_BC));
+        const bool mg = solve_form && c.structured && !(flags & STK_ASM_NO_MG) && !c.mg.empty();
+        c.assembled = false;
+        c.loaded = false;
+        c.mg_built = false;
+        c.have_mg64 = c.have_mg32 = false;
+        // stale products of an earlier assembly
+        c.K32.release(); c.G32.release(); c.GT32.release(); c.M32.release(); c.F32.release(); c.pc32.release();
+        c.S64.release(); c.S32.release(); c.Sdinv64.release(); c.Sdinv32.release();
+        if (fp64) {
+            assemble_prec<double>(c, flags, mg, c.K64, c.G64, c.GT64, c.M64, c.F64, true);
+            c.pc64.alloc(c.nP);
+            lumped_inverse<double>(c, c.MN, c.M64.p, c.pc64.p);
+            if (fp32) {
+                c.K32.alloc(c.K64.n); convert<double, float>(c, c.K64.p, c.K32.p, c.K64.n);
+                c.G32.alloc(c.G64.n); convert<double, float>(c, c.G64.p, c.G32.p, c.G64.n);
+                c.GT32.alloc(c.GT64.n); convert<double, float>(c, c.GT64.p, c.GT32.p, c.GT64.n);
+                c.M32.alloc(c.M64.n); convert<double, float>(c, c.M64.p, c.M32.p, c.M64.n);
+                c.F32.alloc(c.F64.n); convert<double, float>(c, c.F64.p, c.F32.p, c.F64.n);
+                c.pc32.alloc(c.nP); convert<double, float>(c, c.pc64.p, c.pc32.p, c.nP);
+                if (mg)
+                    for (MGLevel &L : c.mg) {
+                        L.A32.alloc(L.A64.n);
+                        convert<double, float>(c, L.A64.p, L.A32.p, L.A64.n);
+                    }
+            }
+        } else {
+            c.K64.release(); c.G64.release(); c.GT64.release(); c.M64.release(); c.pc64.release();
+            assemble_prec<float>(c, flags, mg, c.K32, c.G32, c.GT32, c.M32, c.F32, false);
+            c.pc32.alloc(c.nP);
+            lumped_inverse<float>(c, c.MN, c.M32.p, c.pc32.p);
+            c.F64.alloc(c.nU);
+            convert<float, double>(c, c.F32.p, c.F64.p, c.nU);
+        }
+        STK_HIP(hipStreamSynchronize(c.stream));
+        c.mg_built = mg;
+        if (solve_form) {
+            if (fp64) {
+                setup_solver_data<double>(c);
+                c.have_mg64 = mg;
+            }
+            if (fp32) {
+                setup_solver_data<float>(c);
+                c.have_mg32 = mg;
+            }
+        }
+        c.asm_flags = flags;
+        c.assembled = true;
+        STK_HIP(hipStreamSynchronize(c.stream));
+        return STK_OK;
+    });
+}
+
+int stk_export_csc(stk_ctx *ctx, int32_t which, int32_t lower_only, int64_t *nrows, int64_t *ncols, int64_t *nnz,
+                   int64_t *colptr, int64_t *rowval, double *nzval) {
+    if (!ctx) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK(c.assembled, STK_E_STATE, "nothing assembled");
+        const Pattern *p = nullptr;
+        int br = 1, bc = 1;
+        int64_t nr = 0, nc = 0;
+        const double *v64 = nullptr;
+        const float *v32 = nullptr;
+        switch (which) {
+            case STK_MAT_K: p = &c.KN; br = bc = 2; nr = nc = c.nU; v64 = c.K64.p; v32 = c.K32.p; break;
+            case STK_MAT_G: p = &c.GN; br = 2; nr = c.nU; nc = c.nP; v64 = c.G64.p; v32 = c.G32.p; break;
+            case STK_MAT_GT: p = &c.GTN; bc = 2; nr = c.nP; nc = c.nU; v64 = c.GT64.p; v32 = c.GT32.p; break;
+            case STK_MAT_M: p = &c.MN; nr = nc = c.nP; v64 = c.M64.p; v32 = c.M32.p; break;
+            case STK_MAT_S: p = &c.MN; nr = nc = c.nP; v64 = c.S64.p; v32 = c.S32.p; break;
+            default: throw Error(STK_E_ARG, "unknown operator");
+        }
+        STK_CHECK(v64 || v32, STK_E_STATE, "operator not available");
+        const int64_t nvals = p->nnz() * br * bc;
+        std::vector<double> vals(nvals);
+        if (v64) {
+            STK_HIP(hipMemcpyAsync(vals.data(), v64, nvals * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        } else {
+            std::vector<float> f(nvals);
+            STK_HIP(hipMemcpyAsync(f.data(), v32, nvals * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+            STK_HIP(hipStreamSynchronize(c.stream));
+            std::copy(f.begin(), f.end(), vals.begin());
+        }
+        STK_HIP(hipStreamSynchronize(c.stream));
+        // count entries per column
+        std::vector<int64_t> cnt(nc + 1, 0);
+        auto visit = [&](auto fn) {
+            for (int64_t i = 0; i < p->nrows; ++i)
+                for (int32_t k = p->ptr[i]; k < p->ptr[i + 1]; ++k)
+                    for (int r = 0; r < br; ++r)
+                        for (int q = 0; q < bc; ++q) {
+                            const int64_t row = i * br + r, col = (int64_t)p->col[k] * bc + q;
+                            if (lower_only && row < col) continue;
+                            fn(row, col, vals[(int64_t)k * br * bc + r * bc + q]);
+                        }
+        };
+        visit([&](int64_t, int64_t col, double) { cnt[col + 1]++; });
+        for (int64_t j = 0; j < nc; ++j) cnt[j + 1] += cnt[j];
+        if (nrows) *nrows = nr;
+        if (ncols) *ncols = nc;
+        if (nnz) *nnz = cnt[nc];
+        if (!colptr || !rowval || !nzval) return STK_OK;
+        std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+        // rows are visited in increasing order, so each column comes out sorted
+        visit([&](int64_t row, int64_t col, double v) {
+            const int64_t d = pos[col]++;
+            rowval[d] = row + 1;
+            nzval[d] = v;
+        });
+        for (int64_t j = 0; j <= nc; ++j) colptr[j] = cnt[j] + 1;
+        return STK_OK;
+    });
+}
+
+int stk_export_rhs(stk_ctx *ctx, double *F) {
+    if (!ctx || !F) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK(c.assembled && c.F64, STK_E_STATE, "nothing assembled");
+        c.F64.download(F, c.nU, c.stream);
+        STK_HIP(hipStreamSynchronize(c.stream));
+        return STK_OK;
+    });
+}
+
+int stk_spmv(stk_ctx *ctx, int32_t which, int32_t precision, const double *x, double *y) {
+    if (!ctx || !x || !y) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        STK_CHECK(ctx->c.assembled, STK_E_STATE, "nothing assembled");
+        host_spmv(ctx->c, which, precision, x, y);
+        return STK_OK;
+    });
+}
+
+int stk_solve(stk_ctx *ctx, const stk_opts *opts, stk_report *report) {
+    if (!ctx) return STK_E_ARG;
+    stk_report local{};
+    stk_report &rep = report ? *report : local;
+    std::memset(&rep, 0, sizeof(rep));
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK(c.assembled, STK_E_STATE, "stk_assemble must precede stk_solve");
+        STK_CHECK(!(c.asm_flags & (STK_ASM_CARTESIAN | STK_ASM_NO_BC)), STK_E_STATE,
+                  "the solve needs the rotated operators with boundary conditions");
+        stk_opts o;
+        stk_default_opts(&o);
+        if (opts) o = *opts;
+        STK_CHECK(o.method >= STK_SOLVE_PLAIN && o.method <= STK_SOLVE_LEARNED, STK_E_ARG, "bad method");
+        STK_CHECK(o.inner == STK_INNER_MG || o.inner == STK_INNER_JACOBI, STK_E_ARG, "bad inner solver");
+        STK_CHECK(o.itmax > 0 && o.inner_itmax > 0 && o.rtol > 0 && o.cheb_degree >= 1, STK_E_ARG, "bad options");
+        if (o.method == STK_SOLVE_PLAIN) STK_CHECK(c.K64, STK_E_STATE, "plain solve needs FP64 operators");
+        const int rc = run_solve(c, o, rep);
+        if (rc == STK_E_NOTCONV) c.err = "stopped at the iteration cap before the tolerance";
+        return rc;
+    });
+}
+
+int stk_get_solution(stk_ctx *ctx, double *U, double *P) {
+    if (!ctx) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK(c.U && c.P, STK_E_STATE, "no solution");
+        if (U) c.U.download(U, c.nU, c.stream);
+        if (P) c.P.download(P, c.nP, c.stream);
+        STK_HIP(hipStreamSynchronize(c.stream));
+        return STK_OK;
+    });
+}
+
+int stk_updatevelocity(stk_ctx *ctx, double *U, double *Ucart, double *omega) {
+    if (!ctx) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK(c.U && c.have_fields, STK_E_STATE, "no solution");
+        DBuf<double> uc(c.nU);
+        const double w = update_velocity(c, uc.p);
+        if (omega) *omega = w;
+        if (U) c.U.download(U, c.nU, c.stream);
+        if (Ucart) uc.download(Ucart, c.nU, c.stream);
+        STK_HIP(hipStreamSynchronize(c.stream));
+        return STK_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,582 @@
+// Element assembly of the anisotropic P2-P1 Stokes operators on gfx950.
+//
+// One thread per element, 64 elements per workgroup, elements of one colour per
+// launch (no two share a node, so the read-modify-write scatter is race free and
+// the summation order is fixed: assembly is bit-reproducible).  Per element the
+// kernel follows the reference block assembly (src/Stokes/StokesSolver.jl:247-404):
+//   _fill_R_J!            (:522-541)   straight-sided Jacobian in (theta, r),
+//   _derivative_weights!  (:641-686)   polar -> Cartesian "double Jacobian",
+//   _fill_Kblk! aniso     (src/src/Stokes/StokesSolver.jl:744-808, unclamped),
+//   _fill_Kblk! iso       (:688-742), _fill_Gblk! (:812-823), _fill_Mblk! (:825-836),
+//   _fill_Fbblk!          (:838-845) with _element_density (:1017-1022).
+// The 12x12 element matrix is accumulated as 21 node-pair 2x2 blocks in LDS,
+// rotated to the (u_theta, u_r) node frames (TT^T K TT of _prepare_matrices,
+// :192-201, applied per element since TT is block diagonal) and added into the
+// node-block CSR pattern at offsets precomputed per element.  Fixed (radial
+// boundary) dofs get zero rows/columns; a later pass puts 1 on their diagonal.
+//
+// The same kernel projects the rotated element matrix onto the P1 vertex space
+// (E1 = P^T K_e P with P the element's P1 -> P2 interpolation in rotated frames)
+// and adds it into the finest multigrid operator; coarser levels are built from
+// these element matrices by k_galerkin below (Galerkin products computed element
+// by element on nested meshes).
+#include <cmath>
+
+#include "device_ops.h"
+#include "stk_internal.h"
+
+namespace stk {
+
+struct QuadTables {
+    double xi[6], eta[6], w[6];
+};
+__constant__ QuadTables c_quad;
+
+namespace {
+
+constexpr int kAsmThreads = 64;
+
+// 6-point rule (src/Algebra/Quadrature.jl:80-107), weights include the 1/2.
+QuadTables make_quad() {
+    QuadTables q;
+    const double s = std::sqrt(38.0 - 44.0 * std::sqrt(2.0 / 5.0));
+    const double g1 = (8.0 - std::sqrt(10.0) + s) / 18.0;
+    const double g2 = (8.0 - std::sqrt(10.0) - s) / 18.0;
+    const double w1 = (620.0 + std::sqrt(213125.0 - 53320.0 * std::sqrt(10.0))) / 3720.0;
+    const double w2 = (620.0 - std::sqrt(213125.0 - 53320.0 * std::sqrt(10.0))) / 3720.0;
+    const double x[6][2] = {{1 - 2 * g1, g1}, {g1, 1 - 2 * g1}, {g1, g1},
+                            {1 - 2 * g2, g2}, {g2, 1 - 2 * g2}, {g2, g2}};
+    for (int i = 0; i < 6; ++i) {
+        q.xi[i] = x[i][0];
+        q.eta[i] = x[i][1];
+        q.w[i] = 0.5 * (i < 3 ? w1 : w2);
+    }
+    return q;
+}
+
+// P2 shape functions and derivatives (Quadrature.jl:264-307); node order v1 v2 v3 m12 m23 m31.
+template <typename T>
+__device__ __forceinline__ void p2_shape(T r, T s, T N[6], T dr[6], T ds[6]) {
+    const T t = T(1) - r - s;
+    N[0] = t * (T(2) * t - T(1)); N[1] = r * (T(2) * r - T(1)); N[2] = s * (T(2) * s - T(1));
+    N[3] = T(4) * r * t; N[4] = T(4) * r * s; N[5] = T(4) * s * t;
+    dr[0] = -(T(4) * t - T(1)); dr[1] = T(4) * r - T(1); dr[2] = T(0);
+    dr[3] = T(4) * (t - r); dr[4] = T(4) * s; dr[5] = -T(4) * s;
+    ds[0] = -(T(4) * t - T(1)); ds[1] = T(0); ds[2] = T(4) * s - T(1);
+    ds[3] = -T(4) * r; ds[4] = T(4) * r; ds[5] = T(4) * (t - s);
+}
+
+// index of lower block (a >= b) among the 21 node-pair blocks
+__device__ __forceinline__ int lblk(int a, int b) { return a * (a + 1) / 2 + b; }
+
+template <typename T>
+struct AsmArgs {
+    int64_t nel, off, cnt;
+    const int32_t *perm, *e2n;          // e2n: 6 x nel, permuted SoA
+    const double *th, *r;               // 3 x nel permuted SoA (vertex angles after fixangles)
+    const double *nodecs;               // 2 x nnod
+    const uint8_t *fixed;               // per dof
+    const double *rho, *eta, *C;        // rho: nnod; eta: 6 x nel (ip-major, element id); C: 6 x 6 x nel
+    double g;
+    const uint8_t *sK, *sG, *sGT, *sM;  // slot offsets, SoA by position
+    const int32_t *Kptr, *Gptr, *GTptr, *Mptr;
+    T *K, *G, *GT, *M, *F;
+    // finest multigrid level (vertex space)
+    T *A1, *E1;                         // A1: BSR 2x2 over the M pattern; E1: 36 x nel by element id
+    int cart, bc, iso;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kAsmThreads) void k_assemble(AsmArgs<T> a) {
+    __shared__ T sK[84][kAsmThreads];
+    __shared__ T sM[6][kAsmThreads];
+    T sG[36], sF[12];
+    const int tid = threadIdx.x;
+    const int64_t j = (int64_t)blockIdx.x * kAsmThreads + tid;
+    if (j >= a.cnt) return;   // no block-level synchronisation below: early exit is safe
+    const int64_t p = a.off + j;
+    const int64_t nel = a.nel;
+    const int64_t e = a.perm[p];
+
+    int32_t n[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) n[k] = a.e2n[k * nel + p];
+    T th[3], rr[3], rhov[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        th[k] = (T)a.th[k * nel + p];
+        rr[k] = (T)a.r[k * nel + p];
+        rhov[k] = (T)a.rho[n[k]];
+    }
+    // _fill_R_J!
+    const T R21 = rr[1] - rr[0], R31 = rr[2] - rr[0];
+    const T Th21 = th[1] - th[0], Th31 = th[2] - th[0];
+    const T detJ = Th21 * R31 - Th31 * R21;
+
+    for (int k = 0; k < 84; ++k) sK[k][tid] = T(0);
+#pragma unroll
+    for (int k = 0; k < 36; ++k) sG[k] = T(0);
+    for (int k = 0; k < 6; ++k) sM[k][tid] = T(0);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) sF[k] = T(0);
+
+    for (int q = 0; q < 6; ++q) {
+        const T xi = (T)c_quad.xi[q], et = (T)c_quad.eta[q];
+        T N[6], dr[6], ds[6];
+        p2_shape<T>(xi, et, N, dr, ds);
+        const T NP[3] = {T(1) - xi - et, xi, et};
+        const T thq = NP[0] * th[0] + NP[1] * th[1] + NP[2] * th[2];
+        const T rq = NP[0] * rr[0] + NP[1] * rr[1] + NP[2] * rr[2];
+        T sn, cs;
+        if constexpr (sizeof(T) == 8) sincos(thq, &sn, &cs);
+        else sincosf(thq, &sn, &cs);
+        const T inv = T(1) / rq / detJ;
+        const T jx0 = (R31 * cs - Th31 * sn * rq) * inv;
+        const T jx1 = (-R21 * cs + Th21 * sn * rq) * inv;
+        const T jz0 = (-R31 * sn - Th31 * cs * rq) * inv;
+        const T jz1 = (R21 * sn + Th21 * cs * rq) * inv;
+        const T om = rq * detJ * (T)c_quad.w[q];
+        T dx[6], dz[6];
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+            dx[b] = jx0 * dr[b] + jx1 * ds[b];
+            dz[b] = jz0 * dr[b] + jz1 * ds[b];
+        }
+        const T eta = (T)a.eta[(int64_t)q * nel + e];
+        T d11, d22, d33, d12, d13, d23;
+        if (a.iso) {
+            d11 = d22 = T(4) / T(3); d12 = -T(2) / T(3); d33 = T(1); d13 = d23 = T(0);
+        } else {
+            // StiffnessTensor order eta11, eta33, eta55, eta13, eta15, eta35
+            const double *C = a.C + (int64_t)q * nel + e;
+            const int64_t cstr = 6 * nel;
+            d11 = (T)C[0 * cstr]; d22 = (T)C[1 * cstr]; d33 = (T)C[2 * cstr];
+            d12 = (T)C[3 * cstr]; d13 = (T)C[4 * cstr]; d23 = (T)C[5 * cstr];
+        }
+        const T sc = om * eta;
+        d11 *= sc; d22 *= sc; d33 *= sc; d12 *= sc; d13 *= sc; d23 *= sc;
+        // D B columns of node b for the x and z dofs
+        T Dx[6][3], Dz[6][3];
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+            Dx[b][0] = d11 * dx[b] + d13 * dz[b];
+            Dx[b][1] = d12 * dx[b] + d23 * dz[b];
+            Dx[b][2] = d13 * dx[b] + d33 * dz[b];
+            Dz[b][0] = d12 * dz[b] + d13 * dx[b];
+            Dz[b][1] = d22 * dz[b] + d23 * dx[b];
+            Dz[b][2] = d23 * dz[b] + d33 * dx[b];
+        }
+#pragma unroll
+        for (int aa = 0; aa < 6; ++aa)
+#pragma unroll
+            for (int b = 0; b <= aa; ++b) {
+                const int k = lblk(aa, b) * 4;
+                sK[k + 0][tid] += dx[aa] * Dx[b][0] + dz[aa] * Dx[b][2];
+                sK[k + 1][tid] += dx[aa] * Dz[b][0] + dz[aa] * Dz[b][2];
+                sK[k + 2][tid] += dz[aa] * Dx[b][1] + dx[aa] * Dx[b][2];
+                sK[k + 3][tid] += dz[aa] * Dz[b][1] + dx[aa] * Dz[b][2];
+            }
+#pragma unroll
+        for (int aa = 0; aa < 6; ++aa)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                sG[(aa * 3 + i) * 2 + 0] += om * NP[i] * dx[aa];
+                sG[(aa * 3 + i) * 2 + 1] += om * NP[i] * dz[aa];
+            }
+        const T wm = om / eta;
+        sM[0][tid] += wm * NP[0] * NP[0]; sM[1][tid] += wm * NP[0] * NP[1]; sM[2][tid] += wm * NP[0] * NP[2];
+        sM[3][tid] += wm * NP[1] * NP[1]; sM[4][tid] += wm * NP[1] * NP[2]; sM[5][tid] += wm * NP[2] * NP[2];
+        const T rhoq = NP[0] * rhov[0] + NP[1] * rhov[1] + NP[2] * rhov[2];
+        const T cte = om * (T)a.g * rhoq;
+#pragma unroll
+        for (int aa = 0; aa < 6; ++aa) {
+            sF[2 * aa + 0] += sn * cte * N[aa];
+            sF[2 * aa + 1] += cs * cte * N[aa];
+        }
+    }
+
+    // rotate to the node frames: blk(a,b) <- Q_a blk Q_b^T, Q = [[c, -s], [s, c]]
+    if (!a.cart) {
+        T c[6], s[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            c[k] = (T)a.nodecs[2 * (int64_t)n[k]];
+            s[k] = (T)a.nodecs[2 * (int64_t)n[k] + 1];
+        }
+        for (int aa = 0; aa < 6; ++aa)
+            for (int b = 0; b <= aa; ++b) {
+                const int k = lblk(aa, b) * 4;
+                const T k00 = sK[k][tid], k01 = sK[k + 1][tid], k10 = sK[k + 2][tid], k11 = sK[k + 3][tid];
+                // t = Q_a blk
+                const T t00 = c[aa] * k00 - s[aa] * k10, t01 = c[aa] * k01 - s[aa] * k11;
+                const T t10 = s[aa] * k00 + c[aa] * k10, t11 = s[aa] * k01 + c[aa] * k11;
+                // t Q_b^T, Q_b^T = [[c, s], [-s, c]]
+                sK[k + 0][tid] = t00 * c[b] - t01 * s[b];
+                sK[k + 1][tid] = t00 * s[b] + t01 * c[b];
+                sK[k + 2][tid] = t10 * c[b] - t11 * s[b];
+                sK[k + 3][tid] = t10 * s[b] + t11 * c[b];
+            }
+#pragma unroll
+        for (int aa = 0; aa < 6; ++aa) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const int k = (aa * 3 + i) * 2;
+                const T g0 = sG[k], g1 = sG[k + 1];
+                sG[k] = c[aa] * g0 - s[aa] * g1;
+                sG[k + 1] = s[aa] * g0 + c[aa] * g1;
+            }
+            const T f0 = sF[2 * aa], f1 = sF[2 * aa + 1];
+            sF[2 * aa] = c[aa] * f0 - s[aa] * f1;
+            sF[2 * aa + 1] = s[aa] * f0 + c[aa] * f1;
+        }
+    }
+
+    uint8_t fx[12];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        fx[2 * k] = a.bc ? a.fixed[2 * (int64_t)n[k]] : 0;
+        fx[2 * k + 1] = a.bc ? a.fixed[2 * (int64_t)n[k] + 1] : 0;
+    }
+
+    // scatter K (both triangles of the symmetric operator)
+    for (int aa = 0; aa < 6; ++aa) {
+        const int64_t rowbase = a.Kptr[n[aa]];
+        for (int b = 0; b < 6; ++b) {
+            const int64_t blk = rowbase + a.sK[(aa * 6 + b) * nel + p];
+            T v[4];
+            if (b <= aa) {
+                const int k = lblk(aa, b) * 4;
+                v[0] = sK[k][tid]; v[1] = sK[k + 1][tid]; v[2] = sK[k + 2][tid]; v[3] = sK[k + 3][tid];
+            } else {
+                const int k = lblk(b, aa) * 4;
+                v[0] = sK[k][tid]; v[1] = sK[k + 2][tid]; v[2] = sK[k + 1][tid]; v[3] = sK[k + 3][tid];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rc = q >> 1, cc = q & 1;
+                if (!(fx[2 * aa + rc] | fx[2 * b + cc])) a.K[4 * blk + q] += v[q];
+            }
+        }
+    }
+    // scatter G (rows: velocity dofs) and G^T (rows: pressure vertices)
+#pragma unroll
+    for (int aa = 0; aa < 6; ++aa)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int k = (aa * 3 + i) * 2;
+            const T g0 = fx[2 * aa] ? T(0) : sG[k];
+            const T g1 = fx[2 * aa + 1] ? T(0) : sG[k + 1];
+            const int64_t bg = (int64_t)a.Gptr[n[aa]] + a.sG[(aa * 3 + i) * nel + p];
+            a.G[2 * bg] += g0;
+            a.G[2 * bg + 1] += g1;
+            const int64_t bt = (int64_t)a.GTptr[n[i]] + a.sGT[(i * 6 + aa) * nel + p];
+            a.GT[2 * bt] += g0;
+            a.GT[2 * bt + 1] += g1;
+        }
+    // scatter M (full 3x3) and F
+    {
+        const T m[3][3] = {{sM[0][tid], sM[1][tid], sM[2][tid]},
+                           {sM[1][tid], sM[3][tid], sM[4][tid]},
+                           {sM[2][tid], sM[4][tid], sM[5][tid]}};
+        for (int i = 0; i < 3; ++i)
+            for (int jj = 0; jj < 3; ++jj) {
+                const int64_t bm = (int64_t)a.Mptr[n[i]] + a.sM[(i * 3 + jj) * nel + p];
+                a.M[bm] += m[i][jj];
+            }
+    }
+#pragma unroll
+    for (int aa = 0; aa < 6; ++aa) {
+        a.F[2 * (int64_t)n[aa]] += sF[2 * aa];
+        a.F[2 * (int64_t)n[aa] + 1] += sF[2 * aa + 1];
+    }
+
+    if (a.A1 == nullptr) return;
+    // ---- P1 projection: E1 = Ph^T K Ph, Ph: 12 x 6 (rows: P2 dofs, cols: vertex dofs)
+    // vertex rows: identity; midpoint 3+k of edge (ea, eb): 0.5 R(+h) from ea, 0.5 R(-h) from eb,
+    // h = (theta_eb - theta_ea) / 2; rows / columns of fixed dofs are zero.
+    const int ea[3] = {0, 1, 2}, eb[3] = {1, 2, 0};
+    T Ph[6][3][4];   // block (fine node, coarse vertex)
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+#pragma unroll
+        for (int v = 0; v < 3; ++v)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Ph[f][v][q] = T(0);
+#pragma unroll
+    for (int v = 0; v < 3; ++v) { Ph[v][v][0] = T(1); Ph[v][v][3] = T(1); }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const T h = T(0.5) * (th[eb[k]] - th[ea[k]]);
+        T sh, ch;
+        if constexpr (sizeof(T) == 8) sincos(h, &sh, &ch);
+        else sincosf(h, &sh, &ch);
+        T *A = Ph[3 + k][ea[k]], *B = Ph[3 + k][eb[k]];
+        A[0] = T(0.5) * ch; A[1] = -T(0.5) * sh; A[2] = T(0.5) * sh; A[3] = T(0.5) * ch;
+        B[0] = T(0.5) * ch; B[1] = T(0.5) * sh; B[2] = -T(0.5) * sh; B[3] = T(0.5) * ch;
+    }
+    // masks (always applied: the multigrid acts on the constrained space)
+    uint8_t fxa[12];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        fxa[2 * k] = a.fixed[2 * (int64_t)n[k]];
+        fxa[2 * k + 1] = a.fixed[2 * (int64_t)n[k] + 1];
+    }
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+#pragma unroll
+        for (int v = 0; v < 3; ++v)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (fxa[2 * f + (q >> 1)] | fxa[2 * v + (q & 1)]) Ph[f][v][q] = T(0);
+    // T1 = K Ph (12 x 6), accumulated column by column into E1 = Ph^T T1
+    T E[6][6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int jj = 0; jj < 6; ++jj) E[i][jj] = T(0);
+    for (int f = 0; f < 6; ++f) {          // fine row node of T1
+        T t1[2][6];
+#pragma unroll
+        for (int rc = 0; rc < 2; ++rc)
+#pragma unroll
+            for (int cd = 0; cd < 6; ++cd) t1[rc][cd] = T(0);
+        for (int b = 0; b < 6; ++b) {      // fine column node of K
+            T kb[4];
+            if (b <= f) {
+                const int k = lblk(f, b) * 4;
+                kb[0] = sK[k][tid]; kb[1] = sK[k + 1][tid]; kb[2] = sK[k + 2][tid]; kb[3] = sK[k + 3][tid];
+            } else {
+                const int k = lblk(b, f) * 4;
+                kb[0] = sK[k][tid]; kb[1] = sK[k + 2][tid]; kb[2] = sK[k + 1][tid]; kb[3] = sK[k + 3][tid];
+            }
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                const T *P = Ph[b][v];
+                t1[0][2 * v + 0] += kb[0] * P[0] + kb[1] * P[2];
+                t1[0][2 * v + 1] += kb[0] * P[1] + kb[1] * P[3];
+                t1[1][2 * v + 0] += kb[2] * P[0] + kb[3] * P[2];
+                t1[1][2 * v + 1] += kb[2] * P[1] + kb[3] * P[3];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const T *P = Ph[f][u];   // rows (f, rc), cols (u, cu): Ph^T entry = P[rc*2+cu]
+#pragma unroll
+            for (int cd = 0; cd < 6; ++cd) {
+                E[2 * u + 0][cd] += P[0] * t1[0][cd] + P[2] * t1[1][cd];
+                E[2 * u + 1][cd] += P[1] * t1[0][cd] + P[3] * t1[1][cd];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int jj = 0; jj < 6; ++jj) a.E1[(int64_t)(i * 6 + jj) * nel + e] = E[i][jj];
+    for (int u = 0; u < 3; ++u)
+        for (int v = 0; v < 3; ++v) {
+            const int64_t bm = (int64_t)a.Mptr[n[u]] + a.sM[(u * 3 + v) * nel + p];
+            a.A1[4 * bm + 0] += E[2 * u][2 * v];
+            a.A1[4 * bm + 1] += E[2 * u][2 * v + 1];
+            a.A1[4 * bm + 2] += E[2 * u + 1][2 * v];
+            a.A1[4 * bm + 3] += E[2 * u + 1][2 * v + 1];
+        }
+}
+
+// ------------------------------------------------------------------------ coarse Galerkin
+// Coarse P1 element (I, J, t) of level l+1 is the union of four level-l elements.
+// Child table: (di, dj, type) of the fine cells (2I+di, 2J+dj).
+__constant__ int8_t c_child[2][4][3] = {
+    {{0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 0}},
+    {{1, 1, 0}, {1, 1, 1}, {1, 0, 1}, {0, 1, 1}},
+};
+
+template <typename T>
+struct GalArgs {
+    int32_t ncr_c, nct_c, ncr_f, nct_f;
+    int64_t nel_c, nel_f, off, cnt;
+    const int32_t *perm, *e2v;      // coarse colour order, e2v SoA by position
+    const uint8_t *slot;            // 9 x nel_c by position
+    const int32_t *ptr;             // coarse pattern rows
+    const T *Ef;                    // 36 x nel_f by element id
+    T *Ec;                          // 36 x nel_c by element id
+    T *A;                           // coarse BSR 2x2
+};
+
+template <typename T>
+__device__ __forceinline__ void tri_verts(int t, int I, int J, int vi[3], int vj[3]) {
+    if (t == 0) { vi[0] = I; vj[0] = J; vi[1] = I; vj[1] = J + 1; vi[2] = I + 1; vj[2] = J; }
+    else { vi[0] = I; vj[0] = J + 1; vi[1] = I + 1; vj[1] = J + 1; vi[2] = I + 1; vj[2] = J; }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_galerkin(GalArgs<T> a) {
+    const int64_t j = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (j >= a.cnt) return;
+    const int64_t p = a.off + j;
+    const int64_t ec = a.perm[p];
+    const int t = (int)(ec % 2);
+    const int64_t qc = ec / 2;
+    const int I = (int)(qc % a.ncr_c), J = (int)(qc / a.ncr_c);
+    int ci[3], cj[3];
+    tri_verts<T>(t, I, J, ci, cj);
+    const T dthc = T(2.0 * M_PI) / T(a.nct_c);
+    T E[6][6];
+#pragma unroll
+    for (int u = 0; u < 6; ++u)
+#pragma unroll
+        for (int v = 0; v < 6; ++v) E[u][v] = T(0);
+    for (int ch = 0; ch < 4; ++ch) {
+        const int di = c_child[t][ch][0], dj = c_child[t][ch][1], tt = c_child[t][ch][2];
+        const int fi0 = 2 * I + di, fj0 = 2 * J + dj;
+        const int64_t ef = 2 * ((int64_t)fi0 + (int64_t)a.ncr_f * fj0) + tt;
+        int fi[3], fj[3];
+        tri_verts<T>(tt, fi0, fj0, fi, fj);
+        // interpolation blocks Ph[fine vertex][coarse vertex]
+        T Ph[3][3][4];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            // normalised position in the coarse cell
+            const T rho = T(0.5) * (fi[f] - 2 * I), tau = T(0.5) * (fj[f] - 2 * J);
+            T lam[3];
+            if (t == 0) { lam[0] = T(1) - rho - tau; lam[1] = tau; lam[2] = rho; }
+            else { lam[0] = T(1) - rho; lam[1] = rho + tau - T(1); lam[2] = T(1) - tau; }
+            const bool ffix = (fi[f] == 0) || (fi[f] == a.ncr_f);
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                const T d = dthc * (T(0.5) * fj[f] - T(cj[v]));
+                T sd, cd;
+                if constexpr (sizeof(T) == 8) sincos(d, &sd, &cd);
+                else sincosf(d, &sd, &cd);
+                const bool cfix = (ci[v] == 0) || (ci[v] == a.ncr_c);
+                const T w = lam[v];
+                Ph[f][v][0] = w * cd;
+                Ph[f][v][1] = cfix ? T(0) : -w * sd;
+                Ph[f][v][2] = ffix ? T(0) : w * sd;
+                Ph[f][v][3] = (ffix || cfix) ? T(0) : w * cd;
+            }
+        }
+        // Ef (6 x 6) of the child
+        T Kf[6][6];
+#pragma unroll
+        for (int u = 0; u < 6; ++u)
+#pragma unroll
+            for (int v = 0; v < 6; ++v) Kf[u][v] = a.Ef[(int64_t)(u * 6 + v) * a.nel_f + ef];
+        // E += Ph^T Kf Ph
+        T T1[6][6];   // Kf Ph
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int v = 0; v < 3; ++v)
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc) {
+                    T s = T(0);
+#pragma unroll
+                    for (int f = 0; f < 3; ++f) s += Kf[r][2 * f] * Ph[f][v][cc] + Kf[r][2 * f + 1] * Ph[f][v][2 + cc];
+                    T1[r][2 * v + cc] = s;
+                }
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+#pragma unroll
+            for (int cu = 0; cu < 2; ++cu)
+#pragma unroll
+                for (int c2 = 0; c2 < 6; ++c2) {
+                    T s = T(0);
+#pragma unroll
+                    for (int f = 0; f < 3; ++f) s += Ph[f][u][cu] * T1[2 * f][c2] + Ph[f][u][2 + cu] * T1[2 * f + 1][c2];
+                    E[2 * u + cu][c2] += s;
+                }
+    }
+#pragma unroll
+    for (int u = 0; u < 6; ++u)
+#pragma unroll
+        for (int v = 0; v < 6; ++v) a.Ec[(int64_t)(u * 6 + v) * a.nel_c + ec] = E[u][v];
+    int32_t vid[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vid[k] = a.e2v[k * a.nel_c + p];
+    for (int u = 0; u < 3; ++u)
+        for (int v = 0; v < 3; ++v) {
+            const int64_t bm = (int64_t)a.ptr[vid[u]] + a.slot[(u * 3 + v) * a.nel_c + p];
+            a.A[4 * bm + 0] += E[2 * u][2 * v];
+            a.A[4 * bm + 1] += E[2 * u][2 * v + 1];
+            a.A[4 * bm + 2] += E[2 * u + 1][2 * v];
+            a.A[4 * bm + 3] += E[2 * u + 1][2 * v + 1];
+        }
+}
+
+// Unit diagonal on the fixed dofs of a BSR 2x2 operator (their rows/columns are zero).
+template <typename T>
+__global__ void k_fix_diag(int64_t nfix, const int64_t *__restrict__ dofs, const int32_t *__restrict__ ptr,
+                           const int32_t *__restrict__ col, T *__restrict__ A) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nfix) return;
+    const int64_t d = dofs[i], node = d / 2, c = d % 2;
+    for (int32_t k = ptr[node]; k < ptr[node + 1]; ++k)
+        if (col[k] == node) { A[4 * (int64_t)k + 3 * c] = T(1); break; }
+}
+
+}  // namespace
+
+void upload_quadrature(hipStream_t s) {
+    static bool done = false;
+    if (done) return;
+    const QuadTables q = make_quad();
+    STK_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_quad), &q, sizeof(q), 0, hipMemcpyHostToDevice, s));
+    STK_HIP(hipStreamSynchronize(s));
+    done = true;
+}
+
+template <typename T>
+void assemble_impl(Context &c, int flags, T *K, T *G, T *GT, T *M, T *F, T *A1, T *E1) {
+    AsmArgs<T> a{};
+    a.nel = c.nel;
+    a.perm = c.d_perm.p; a.e2n = c.d_e2n.p; a.th = c.d_elth.p; a.r = c.d_elr.p;
+    a.nodecs = c.d_nodecs.p; a.fixed = c.d_fixed.p;
+    a.rho = c.d_rho.p; a.eta = c.d_eta.p; a.C = c.d_C.p; a.g = c.g;
+    a.sK = c.d_sK.p; a.sG = c.d_sG.p; a.sGT = c.d_sGT.p; a.sM = c.d_sM.p;
+    a.Kptr = c.KN.dptr.p; a.Gptr = c.GN.dptr.p; a.GTptr = c.GTN.dptr.p; a.Mptr = c.MN.dptr.p;
+    a.K = K; a.G = G; a.GT = GT; a.M = M; a.F = F; a.A1 = A1; a.E1 = E1;
+    a.cart = (flags & STK_ASM_CARTESIAN) ? 1 : 0;
+    a.bc = (flags & STK_ASM_NO_BC) ? 0 : 1;
+    a.iso = ((flags & STK_ASM_ISOTROPIC) || !c.have_C) ? 1 : 0;
+    for (size_t k = 0; k + 1 < c.col_off.size(); ++k) {
+        a.off = c.col_off[k];
+        a.cnt = c.col_off[k + 1] - c.col_off[k];
+        const int nb = (int)((a.cnt + kAsmThreads - 1) / kAsmThreads);
+        k_assemble<T><<<dim3(nb), dim3(kAsmThreads), 0, c.stream>>>(a);
+        STK_HIP(hipGetLastError());
+    }
+}
+
+template <typename T>
+void galerkin_impl(Context &c, size_t l, T *Ef, T *Ec, T *A) {
+    MGLevel &F = c.mg[l], &C = c.mg[l + 1];
+    GalArgs<T> a{};
+    a.ncr_c = C.ncr; a.nct_c = C.nct; a.ncr_f = F.ncr; a.nct_f = F.nct;
+    a.nel_c = C.nel; a.nel_f = F.nel;
+    a.perm = C.perm.p; a.e2v = C.e2v.p; a.slot = C.slot.p; a.ptr = C.pat.dptr.p;
+    a.Ef = Ef; a.Ec = Ec; a.A = A;
+    for (size_t k = 0; k + 1 < C.col_off.size(); ++k) {
+        a.off = C.col_off[k];
+        a.cnt = C.col_off[k + 1] - C.col_off[k];
+        const int nb = (int)((a.cnt + 63) / 64);
+        k_galerkin<T><<<dim3(nb), dim3(64), 0, c.stream>>>(a);
+        STK_HIP(hipGetLastError());
+    }
+}
+
+template <typename T>
+void fix_diag(Context &c, const Pattern &p, int64_t nfix, const int64_t *dofs, T *A) {
+    if (nfix == 0) return;
+    const int nb = (int)((nfix + 255) / 256);
+    k_fix_diag<T><<<dim3(nb), dim3(256), 0, c.stream>>>(nfix, dofs, p.dptr.p, p.dcol.p, A);
+    STK_HIP(hipGetLastError());
+}
+
+template void assemble_impl<double>(Context &, int, double *, double *, double *, double *, double *, double *, double *);
+template void assemble_impl<float>(Context &, int, float *, float *, float *, float *, float *, float *, float *);
+template void galerkin_impl<double>(Context &, size_t, double *, double *, double *);
+template void galerkin_impl<float>(Context &, size_t, float *, float *, float *);
+template void fix_diag<double>(Context &, const Pattern &, int64_t, const int64_t *, double *);
+template void fix_diag<float>(Context &, const Pattern &, int64_t, const int64_t *, float *);
+
+}  // namespace stk

@@ -1,0 +1,161 @@
+// Device building blocks: node-block sparse products with fused epilogues,
+// deterministic two-pass reductions and elementwise maps.
+//
+// Sparse operators are stored as node-block CSR: row node i owns blocks
+// ptr[i] .. ptr[i+1]-1, block k couples row node i with column node col[k] and
+// holds BR x BC values (row-major) at val[k*BR*BC].  K is 2x2 (u_theta, u_r per
+// node), G is 2x1, G^T is 1x2, the pressure matrices are 1x1.  A group of L
+// lanes of one wavefront walks the blocks of one row node; the partial sums are
+// combined with cross-lane shuffles inside the group.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace stk {
+
+constexpr int kWave = 64;
+constexpr int kReduceBlocks = 1024;   // partial-sum slots of a reduction
+
+// Sum over the L-lane group (L a power of two dividing the wavefront).
+template <int L, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, L);
+    return v;
+}
+
+// Block-wide sum of one double per thread; result valid in thread 0.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double part[NT / kWave];
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+    if (l == 0) part[w] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < NT / kWave; ++i) s += part[i];
+    }
+    __syncthreads();
+    return s;
+}
+
+// Map the hardware block id so that consecutive logical blocks stay on one XCD
+// (blocks are dealt round-robin over the 8 XCDs, each with its own L2): logical
+// chunks of rows are then contiguous per XCD and their x gathers share that L2.
+__device__ __forceinline__ int64_t xcd_block(int64_t nblocks) {
+    const int64_t b = blockIdx.x;
+    const int64_t per = (nblocks + 7) / 8;
+    const int64_t x = b % 8, k = b / 8;
+    const int64_t full = nblocks % 8 == 0 ? 8 : nblocks % 8;  // XCDs owning `per` blocks
+    int64_t lb;
+    if (x < full) lb = x * per + k;
+    else lb = full * per + (x - full) * (per - 1) + k;
+    return lb;
+}
+
+// ------------------------------------------------------------------ node-block SpMV
+// Epilogue interface: operator()(row_dof, Ax value) -> contribution to a dot (double).
+template <typename T>
+struct EpStore {                       // y = A x
+    T *y;
+    __device__ double operator()(int64_t i, T ax) const { y[i] = ax; return 0.0; }
+};
+template <typename T>
+struct EpStoreDot {                    // y = A x, dot += p_i (A x)_i
+    T *y; const T *p;
+    __device__ double operator()(int64_t i, T ax) const { y[i] = ax; return (double)p[i] * (double)ax; }
+};
+template <typename T>
+struct EpResidual {                    // r = b - A x, dot += r_i^2
+    T *r; const T *b;
+    __device__ double operator()(int64_t i, T ax) const {
+        const T v = b[i] - ax; r[i] = v; return (double)v * (double)v;
+    }
+};
+template <typename T>
+struct EpAxpyOut {                     // y = beta*y + alpha*A x  (alpha, beta scalars)
+    T *y; T alpha, beta;
+    __device__ double operator()(int64_t i, T ax) const { y[i] = beta * y[i] + alpha * ax; return 0.0; }
+};
+// Chebyshev step with point-Jacobi: d = c1 d + c2 Dinv (b - A x); xo = x + d.
+template <typename T>
+struct EpCheb {
+    const T *b, *dinv, *x; T *d, *xo; T c1, c2;
+    __device__ double operator()(int64_t i, T ax) const {
+        const T dn = c1 * d[i] + c2 * dinv[i] * (b[i] - ax);
+        d[i] = dn; xo[i] = x[i] + dn; return 0.0;
+    }
+};
+
+template <int BR, int BC, int L, int NT, typename T, typename Ep>
+__global__ __launch_bounds__(NT) void k_spmv_block(int64_t nrows, const int32_t *__restrict__ ptr,
+                                                    const int32_t *__restrict__ col,
+                                                    const T *__restrict__ val, const T *__restrict__ x,
+                                                    Ep ep, double *__restrict__ partial, int64_t nblocks) {
+    const int64_t lb = xcd_block(nblocks);
+    const int64_t row = lb * (NT / L) + threadIdx.x / L;
+    const int lane = threadIdx.x % L;
+    double dsum = 0.0;
+    if (row < nrows) {
+        T acc[BR];
+#pragma unroll
+        for (int r = 0; r < BR; ++r) acc[r] = T(0);
+        const int32_t b0 = ptr[row], b1 = ptr[row + 1];
+        for (int32_t k = b0 + lane; k < b1; k += L) {
+            const int64_t c = col[k];
+            T xv[BC];
+#pragma unroll
+            for (int q = 0; q < BC; ++q) xv[q] = x[c * BC + q];
+#pragma unroll
+            for (int r = 0; r < BR; ++r)
+#pragma unroll
+                for (int q = 0; q < BC; ++q) acc[r] += val[(int64_t)k * (BR * BC) + r * BC + q] * xv[q];
+        }
+#pragma unroll
+        for (int r = 0; r < BR; ++r) acc[r] = group_sum<L>(acc[r]);
+        if (lane < BR) {
+            T v = acc[0];
+#pragma unroll
+            for (int r = 1; r < BR; ++r)
+                if (lane == r) v = acc[r];
+            dsum = ep(row * BR + lane, v);
+        }
+    }
+    if (partial) {
+        const double s = block_sum<NT>(dsum);
+        if (threadIdx.x == 0) partial[lb] = s;
+    }
+}
+
+// ------------------------------------------------------------------ reductions
+template <int NT, typename F>
+__global__ __launch_bounds__(NT) void k_reduce_map(int64_t n, F f, double *__restrict__ partial) {
+    double s = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * NT;
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += stride) s += f(i);
+    s = block_sum<NT>(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// Sum npart partials in a fixed order; out[0] = sum.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_reduce_final(int64_t npart, const double *__restrict__ partial,
+                                                     double *__restrict__ out) {
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < npart; i += NT) s += partial[i];
+    s = block_sum<NT>(s);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+template <int NT, typename F>
+__global__ __launch_bounds__(NT) void k_map(int64_t n, F f) {
+    const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (i < n) f(i);
+}
+
+}  // namespace stk

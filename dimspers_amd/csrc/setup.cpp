@@ -1,0 +1,405 @@
+// Host-side, once-per-mesh set-up: node-block sparsity patterns, element colouring,
+// per-element scatter slots and the multigrid hierarchy structure.
+//
+// The reference builds triplet index lists once per mesh (sparsitystokes,
+// src/Algebra/Sparsity.jl:10-103) and lets sparse(I, J, V) sort and sum them at
+// every assembly.  Here the patterns are built once as node-block CSR, and every
+// element records, for each of its node pairs, the offset of the column node in
+// the row node's pattern row.  Elements are coloured so that no two elements of
+// one colour share a node (src/Grid/coloring.jl:5-64 colours greedily by first
+// fit; the balanced variant below keeps the colours of equal size, which keeps
+// every launch large).
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "stk_internal.h"
+
+namespace stk {
+
+namespace {
+
+// Rows: for every row node, the sorted unique column nodes reachable through elements.
+void pattern_from_elements(Pattern &pat, int64_t nrows, int64_t ncols, int64_t nel,
+                           const std::vector<int32_t> &e2r, int nr_per_el,
+                           const std::vector<int32_t> &e2c, int nc_per_el) {
+    // row -> elements incidence
+    std::vector<int64_t> cnt(nrows + 1, 0);
+    for (int64_t e = 0; e < nel; ++e)
+        for (int a = 0; a < nr_per_el; ++a) cnt[e2r[e * nr_per_el + a] + 1]++;
+    for (int64_t i = 0; i < nrows; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int32_t> inc(cnt[nrows]);
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    for (int64_t e = 0; e < nel; ++e)
+        for (int a = 0; a < nr_per_el; ++a) inc[pos[e2r[e * nr_per_el + a]]++] = (int32_t)e;
+
+    std::vector<int32_t> deg(nrows);
+    std::vector<std::vector<int32_t>> rows(nrows);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nrows; ++i) {
+        std::vector<int32_t> &r = rows[i];
+        r.reserve(64);
+        for (int64_t k = cnt[i]; k < cnt[i + 1]; ++k) {
+            int64_t e = inc[k];
+            for (int b = 0; b < nc_per_el; ++b) r.push_back(e2c[e * nc_per_el + b]);
+        }
+        std::sort(r.begin(), r.end());
+        r.erase(std::unique(r.begin(), r.end()), r.end());
+        deg[i] = (int32_t)r.size();
+    }
+    pat.nrows = nrows;
+    pat.ncols = ncols;
+    pat.ptr.assign(nrows + 1, 0);
+    for (int64_t i = 0; i < nrows; ++i) pat.ptr[i + 1] = pat.ptr[i] + deg[i];
+    pat.col.resize(pat.ptr[nrows]);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < nrows; ++i)
+        std::copy(rows[i].begin(), rows[i].end(), pat.col.begin() + pat.ptr[i]);
+}
+
+inline int find_offset(const Pattern &p, int32_t row, int32_t col) {
+    const int32_t *b = p.col.data() + p.ptr[row];
+    const int32_t *e = p.col.data() + p.ptr[row + 1];
+    const int32_t *it = std::lower_bound(b, e, col);
+    if (it == e || *it != col) throw Error(STK_E_SHAPE, "pattern lookup failed");
+    int off = (int)(it - b);
+    if (off > 255) throw Error(STK_E_SHAPE, "pattern row longer than 256 blocks");
+    return off;
+}
+
+// Balanced greedy colouring: every element takes, among the colours none of its
+// nodes carries yet, the one with the fewest elements so far.
+void colour_elements(int64_t nel, int nn, const std::vector<int32_t> &e2x, int64_t nnodes,
+                     std::vector<int32_t> &perm, std::vector<int64_t> &off) {
+    std::vector<uint64_t> used(nnodes, 0);
+    std::vector<int32_t> colour(nel);
+    std::vector<int64_t> count;
+    for (int64_t e = 0; e < nel; ++e) {
+        uint64_t forbid = 0;
+        for (int a = 0; a < nn; ++a) forbid |= used[e2x[e * nn + a]];
+        int best = -1;
+        for (int k = 0; k < (int)count.size(); ++k)
+            if (!(forbid >> k & 1ull) && (best < 0 || count[k] < count[best])) best = k;
+        if (best < 0) {
+            best = (int)count.size();
+            if (best >= 64) throw Error(STK_E_SHAPE, "colouring needs more than 64 colours");
+            count.push_back(0);
+        }
+        colour[e] = best;
+        count[best]++;
+        for (int a = 0; a < nn; ++a) used[e2x[e * nn + a]] |= 1ull << best;
+    }
+    int nc = (int)count.size();
+    off.assign(nc + 1, 0);
+    for (int k = 0; k < nc; ++k) off[k + 1] = off[k] + count[k];
+    perm.resize(nel);
+    std::vector<int64_t> p(off.begin(), off.end() - 1);
+    for (int64_t e = 0; e < nel; ++e) perm[p[colour[e]]++] = (int32_t)e;
+}
+
+}  // namespace
+
+void build_patterns(Context &c) {
+    std::vector<int32_t> e2v(c.nel * 3);
+    for (int64_t e = 0; e < c.nel; ++e)
+        for (int a = 0; a < 3; ++a) e2v[e * 3 + a] = c.e2n[e * 6 + a];
+    pattern_from_elements(c.KN, c.nnod, c.nnod, c.nel, c.e2n, 6, c.e2n, 6);
+    pattern_from_elements(c.GN, c.nnod, c.nvert, c.nel, c.e2n, 6, e2v, 3);
+    pattern_from_elements(c.GTN, c.nvert, c.nnod, c.nel, e2v, 3, c.e2n, 6);
+    pattern_from_elements(c.MN, c.nvert, c.nvert, c.nel, e2v, 3, e2v, 3);
+    for (Pattern *p : {&c.KN, &c.GN, &c.GTN, &c.MN}) {
+        p->dptr.upload(p->ptr, c.stream);
+        p->dcol.upload(p->col, c.stream);
+    }
+}
+
+void build_colouring(Context &c) { colour_elements(c.nel, 6, c.e2n, c.nnod, c.perm, c.col_off); }
+
+void build_slot_maps(Context &c) {
+    const int64_t nel = c.nel;
+    std::vector<uint8_t> sK(36 * nel), sG(18 * nel), sGT(18 * nel), sM(9 * nel);
+    std::vector<int32_t> e2n_p(6 * nel);
+    std::vector<double> th_p(3 * nel), r_p(3 * nel);
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < nel; ++p) {
+        const int64_t e = c.perm[p];
+        const int32_t *n = &c.e2n[e * 6];
+        for (int a = 0; a < 6; ++a) {
+            e2n_p[a * nel + p] = n[a];
+            for (int b = 0; b < 6; ++b) sK[(a * 6 + b) * nel + p] = (uint8_t)find_offset(c.KN, n[a], n[b]);
+            for (int i = 0; i < 3; ++i) {
+                sG[(a * 3 + i) * nel + p] = (uint8_t)find_offset(c.GN, n[a], n[i]);
+                sGT[(i * 6 + a) * nel + p] = (uint8_t)find_offset(c.GTN, n[i], n[a]);
+            }
+        }
+        for (int i = 0; i < 3; ++i) {
+            th_p[i * nel + p] = c.el_th[e * 3 + i];
+            r_p[i * nel + p] = c.el_r[e * 3 + i];
+            for (int j = 0; j < 3; ++j) sM[(i * 3 + j) * nel + p] = (uint8_t)find_offset(c.MN, n[i], n[j]);
+        }
+    }
+    std::vector<int32_t> perm32(c.perm.begin(), c.perm.end());
+    c.d_perm.upload(perm32, c.stream);
+    c.d_e2n.upload(e2n_p, c.stream);
+    c.d_elth.upload(th_p, c.stream);
+    c.d_elr.upload(r_p, c.stream);
+    c.d_sK.upload(sK, c.stream);
+    c.d_sG.upload(sG, c.stream);
+    c.d_sGT.upload(sGT, c.stream);
+    c.d_sM.upload(sM, c.stream);
+    std::vector<double> cs(2 * c.nnod);
+    for (int64_t i = 0; i < c.nnod; ++i) {
+        cs[2 * i] = std::cos(c.node_th[i]);
+        cs[2 * i + 1] = std::sin(c.node_th[i]);
+    }
+    c.d_nodecs.upload(cs, c.stream);
+    c.d_noder.upload(c.node_r, c.stream);
+    c.d_fixed.upload(c.fixed, c.stream);
+    STK_HIP(hipStreamSynchronize(c.stream));
+}
+
+// The multigrid needs the structured annulus of Grid_split1 (src/Grid/mesher.jl:152-233):
+// vertex v(i, j) = i + nr*j (radial fastest), cell q = i + Nr*j, elements 2q and 2q+1 with
+// vertices [v(i,j), v(i,j+1), v(i+1,j)] and [v(i,j+1), v(i+1,j+1), v(i+1,j)].
+void detect_structured(Context &c) {
+    c.structured = false;
+    // number of radial vertices: run of equal angles at the start of the vertex list
+    int64_t nr = 1;
+    while (nr < c.nvert && c.node_th[nr] == c.node_th[0]) ++nr;
+    if (nr < 2 || c.nvert % nr != 0) return;
+    const int64_t nt = c.nvert / nr;
+    const int64_t ncr = nr - 1;
+    if (c.nel != 2 * ncr * nt) return;
+    for (int64_t q = 0; q < ncr * nt; ++q) {
+        const int64_t i = q % ncr, j = q / ncr, jp = (j + 1) % nt;
+        const int32_t v00 = (int32_t)(i + nr * j), v01 = (int32_t)(i + nr * jp);
+        const int32_t v10 = v00 + 1, v11 = v01 + 1;
+        const int32_t *a = &c.e2n[(2 * q) * 6], *b = &c.e2n[(2 * q + 1) * 6];
+        if (a[0] != v00 || a[1] != v01 || a[2] != v10) return;
+        if (b[0] != v01 || b[1] != v11 || b[2] != v10) return;
+    }
+    c.ncell_r = (int32_t)ncr;
+    c.ncell_t = (int32_t)nt;
+    c.structured = true;
+}
+
+namespace {
+
+// Local vertex lattice coordinates (i, j) of the 3 vertices of element type t in cell (I, J).
+inline void tri_vertices(int t, int I, int J, int vi[3], int vj[3]) {
+    if (t == 0) {
+        vi[0] = I;     vj[0] = J;
+        vi[1] = I;     vj[1] = J + 1;
+        vi[2] = I + 1; vj[2] = J;
+    } else {
+        vi[0] = I;     vj[0] = J + 1;
+        vi[1] = I + 1; vj[1] = J + 1;
+        vi[2] = I + 1; vj[2] = J;
+    }
+}
+
+void fill_level_geometry(MGLevel &L, hipStream_t s) {
+    const int ncr = L.ncr, nct = L.nct, nr = ncr + 1;
+    L.nvert = (int64_t)nr * nct;
+    L.nel = 2ll * ncr * nct;
+    L.h_e2v.resize(3 * L.nel);
+    for (int64_t q = 0; q < (int64_t)ncr * nct; ++q) {
+        const int I = (int)(q % ncr), J = (int)(q / ncr);
+        for (int t = 0; t < 2; ++t) {
+            int vi[3], vj[3];
+            tri_vertices(t, I, J, vi, vj);
+            for (int a = 0; a < 3; ++a) L.h_e2v[(2 * q + t) * 3 + a] = vi[a] + nr * (vj[a] % nct);
+        }
+    }
+    pattern_from_elements(L.pat, L.nvert, L.nvert, L.nel, L.h_e2v, 3, L.h_e2v, 3);
+    L.pat.dptr.upload(L.pat.ptr, s);
+    L.pat.dcol.upload(L.pat.col, s);
+    std::vector<int32_t> perm;
+    colour_elements(L.nel, 3, L.h_e2v, L.nvert, perm, L.col_off);
+    std::vector<int32_t> e2v_p(3 * L.nel);
+    std::vector<uint8_t> slot(9 * L.nel);
+    for (int64_t p = 0; p < L.nel; ++p) {
+        const int64_t e = perm[p];
+        for (int a = 0; a < 3; ++a) {
+            e2v_p[a * L.nel + p] = L.h_e2v[e * 3 + a];
+            for (int b = 0; b < 3; ++b)
+                slot[(a * 3 + b) * L.nel + p] =
+                    (uint8_t)find_offset(L.pat, L.h_e2v[e * 3 + a], L.h_e2v[e * 3 + b]);
+        }
+    }
+    L.perm.upload(perm, s);
+    L.e2v.upload(e2v_p, s);
+    L.slot.upload(slot, s);
+    std::vector<double> cs(2 * L.nvert);
+    std::vector<uint8_t> fixed(2 * L.nvert, 0);
+    const double dth = 2.0 * M_PI / nct;
+    for (int64_t v = 0; v < L.nvert; ++v) {
+        const int64_t i = v % nr, j = v / nr;
+        cs[2 * v] = std::cos(dth * j);
+        cs[2 * v + 1] = std::sin(dth * j);
+        if (i == 0 || i == ncr) fixed[2 * v + 1] = 1;
+    }
+    L.cs.upload(cs, s);
+    L.fixed.upload(fixed, s);
+}
+
+// 2x2 rotation block R(d) = [[cos d, -sin d], [sin d, cos d]] scaled by w.
+inline void rot_block(double w, double d, double *blk) {
+    const double cd = std::cos(d), sd = std::sin(d);
+    blk[0] = w * cd; blk[1] = -w * sd; blk[2] = w * sd; blk[3] = w * cd;
+}
+
+// Transfer operator as BSR 2x2 (fine node -> coarse vertex) plus its transpose.
+struct Triplet { int32_t f, c; double blk[4]; };
+
+void build_transfer(MGLevel &coarse, int64_t nfine, const std::vector<Triplet> &t,
+                    const std::vector<uint8_t> &fine_fixed, const std::vector<uint8_t> &coarse_fixed,
+                    hipStream_t s) {
+    // P rows: fine nodes
+    Pattern &P = coarse.P_pat;
+    P.nrows = nfine; P.ncols = coarse.nvert;
+    P.ptr.assign(nfine + 1, 0);
+    for (const Triplet &x : t) P.ptr[x.f + 1]++;
+    for (int64_t i = 0; i < nfine; ++i) P.ptr[i + 1] += P.ptr[i];
+    P.col.resize(t.size());
+    std::vector<double> Pv(4 * t.size());
+    {
+        std::vector<int32_t> pos(P.ptr.begin(), P.ptr.end() - 1);
+        std::vector<const Triplet *> sorted(t.size());
+        for (const Triplet &x : t) sorted[pos[x.f]++] = &x;
+        for (int64_t i = 0; i < nfine; ++i)
+            std::sort(sorted.begin() + P.ptr[i], sorted.begin() + P.ptr[i + 1],
+                      [](const Triplet *a, const Triplet *b) { return a->c < b->c; });
+        for (size_t k = 0; k < t.size(); ++k) {
+            const Triplet &x = *sorted[k];
+            P.col[k] = x.c;
+            for (int q = 0; q < 4; ++q) {
+                const int rr = q / 2, cc = q % 2;
+                const bool zero = fine_fixed[2 * x.f + rr] || coarse_fixed[2 * x.c + cc];
+                Pv[4 * k + q] = zero ? 0.0 : x.blk[q];
+            }
+        }
+    }
+    // R = P^T rows: coarse vertices
+    Pattern &R = coarse.R_pat;
+    R.nrows = coarse.nvert; R.ncols = nfine;
+    R.ptr.assign(coarse.nvert + 1, 0);
+    for (int32_t cidx : P.col) R.ptr[cidx + 1]++;
+    for (int64_t i = 0; i < coarse.nvert; ++i) R.ptr[i + 1] += R.ptr[i];
+    R.col.resize(P.col.size());
+    std::vector<double> Rv(4 * P.col.size());
+    std::vector<int32_t> pos(R.ptr.begin(), R.ptr.end() - 1);
+    for (int64_t f = 0; f < nfine; ++f)
+        for (int32_t k = P.ptr[f]; k < P.ptr[f + 1]; ++k) {
+            const int32_t cidx = P.col[k];
+            const int32_t dst = pos[cidx]++;
+            R.col[dst] = (int32_t)f;
+            // transpose of the 2x2 block
+            Rv[4 * dst + 0] = Pv[4 * k + 0];
+            Rv[4 * dst + 1] = Pv[4 * k + 2];
+            Rv[4 * dst + 2] = Pv[4 * k + 1];
+            Rv[4 * dst + 3] = Pv[4 * k + 3];
+        }
+    P.dptr.upload(P.ptr, s); P.dcol.upload(P.col, s);
+    R.dptr.upload(R.ptr, s); R.dcol.upload(R.col, s);
+    coarse.P64.upload(Pv, s);
+    coarse.R64.upload(Rv, s);
+    std::vector<float> Pf(Pv.begin(), Pv.end()), Rf(Rv.begin(), Rv.end());
+    coarse.P32.upload(Pf, s);
+    coarse.R32.upload(Rf, s);
+}
+
+}  // namespace
+
+// Levels: mg[0] = P1 on the original mesh (vertices), mg[l+1] = 2x coarser P1 lattice.
+// Transfer mg[0] -> P2 nodes interpolates midpoints from their edge ends; transfer
+// mg[l+1] -> mg[l] interpolates the P1 field of the nested coarse triangles.  Both act on
+// the rotated (u_theta, u_r) node frames, so every block is a rotation by the angle between
+// the fine and the coarse node, and rows / columns of fixed (radial boundary) dofs are zero.
+void build_mg_structure(Context &c) {
+    c.mg.clear();
+    if (!c.structured) return;
+    int ncr = c.ncell_r, nct = c.ncell_t;
+    {
+        MGLevel L;
+        L.ncr = ncr; L.nct = nct;
+        c.mg.push_back(std::move(L));
+    }
+    while (ncr % 2 == 0 && nct % 2 == 0 && ncr / 2 >= 2 && nct / 2 >= 8) {
+        ncr /= 2; nct /= 2;
+        MGLevel L;
+        L.ncr = ncr; L.nct = nct;
+        c.mg.push_back(std::move(L));
+    }
+    for (MGLevel &L : c.mg) fill_level_geometry(L, c.stream);
+
+    // P2 nodes <- vertices (level 0)
+    {
+        MGLevel &L0 = c.mg[0];
+        std::vector<Triplet> t;
+        std::vector<uint8_t> seen(c.nnod, 0);
+        for (int64_t e = 0; e < c.nel; ++e) {
+            const int32_t *n = &c.e2n[e * 6];
+            const double *th = &c.el_th[e * 3];
+            static const int ea[3] = {0, 1, 2}, eb[3] = {1, 2, 0};
+            for (int a = 0; a < 3; ++a)
+                if (!seen[n[a]]) {
+                    seen[n[a]] = 1;
+                    Triplet x{n[a], n[a], {1, 0, 0, 1}};
+                    t.push_back(x);
+                }
+            for (int k = 0; k < 3; ++k) {
+                const int32_t m = n[3 + k];
+                if (seen[m]) continue;
+                seen[m] = 1;
+                const double half = 0.5 * (th[eb[k]] - th[ea[k]]);  // theta_m - theta_a
+                Triplet xa{m, n[ea[k]], {}}, xb{m, n[eb[k]], {}};
+                rot_block(0.5, half, xa.blk);
+                rot_block(0.5, -half, xb.blk);
+                t.push_back(xa);
+                t.push_back(xb);
+            }
+        }
+        std::vector<uint8_t> cfix(2 * L0.nvert, 0);
+        for (int64_t v = 0; v < L0.nvert; ++v) {
+            const int64_t i = v % (L0.ncr + 1);
+            if (i == 0 || i == L0.ncr) cfix[2 * v + 1] = 1;
+        }
+        build_transfer(L0, c.nnod, t, c.fixed, cfix, c.stream);
+    }
+    // mg[l] <- mg[l+1]
+    for (size_t l = 0; l + 1 < c.mg.size(); ++l) {
+        MGLevel &F = c.mg[l], &C = c.mg[l + 1];
+        const int nrf = F.ncr + 1, nrc = C.ncr + 1;
+        const double dthc = 2.0 * M_PI / C.nct;
+        std::vector<Triplet> t;
+        std::vector<uint8_t> ffix(2 * F.nvert, 0), cfix(2 * C.nvert, 0);
+        for (int64_t v = 0; v < F.nvert; ++v) {
+            const int64_t i = v % nrf;
+            if (i == 0 || i == F.ncr) ffix[2 * v + 1] = 1;
+        }
+        for (int64_t v = 0; v < C.nvert; ++v) {
+            const int64_t i = v % nrc;
+            if (i == 0 || i == C.ncr) cfix[2 * v + 1] = 1;
+        }
+        for (int64_t v = 0; v < F.nvert; ++v) {
+            const int i = (int)(v % nrf), j = (int)(v / nrf);
+            const int I = i / 2, J = j / 2;
+            auto cv = [&](int ci, int cj) { return (int32_t)(ci + nrc * (cj % C.nct)); };
+            // fine angle minus coarse angle, in units of the coarse cell width
+            auto add = [&](int ci, int cj, double w) {
+                Triplet x{(int32_t)v, cv(ci, cj), {}};
+                rot_block(w, dthc * (0.5 * j - cj), x.blk);
+                t.push_back(x);
+            };
+            if (i % 2 == 0 && j % 2 == 0) add(I, J, 1.0);
+            else if (i % 2 == 1 && j % 2 == 0) { add(I, J, 0.5); add(I + 1, J, 0.5); }
+            else if (i % 2 == 0 && j % 2 == 1) { add(I, J, 0.5); add(I, J + 1, 0.5); }
+            else { add(I, J + 1, 0.5); add(I + 1, J, 0.5); }
+        }
+        build_transfer(C, F.nvert, t, ffix, cfix, c.stream);
+    }
+    STK_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace stk

@@ -1,0 +1,41 @@
+// Solver entry points and the device-scalar slots they share.
+#pragma once
+
+#include "stk_internal.h"
+
+namespace stk {
+
+// Slots of Context::d_sc (device doubles written by reductions, read by kernels).
+enum : int {
+    SC_TMP = 0,      // scratch (two slots)
+    SC_BB = 2,       // ||b||^2 of the inner CG
+    SC_PAP = 3,      // p . A p
+    SC_RZ0 = 4,      // r . z, ping-pong pair (4, 5)
+    SC_RU = 6,       // ||ru||^2 of the full residual
+    SC_RC = 7,       // ||rc||^2 of the full residual
+    SC_HIST = 8,     // recent inner residual norms (8 slots)
+    SC_VY = 16,      // v . y of the inner right-hand side (null-vector component)
+    SC_VZ = 17,      // v . z of the inner solution
+    SC_VAP = 18,     // v . (K p) inside the inner CG
+    SC_VZZ = 19,     // v . (M^-1 r) inside the inner CG
+    SC_COUNT = 32
+};
+
+// Two-pass deterministic reductions into device scalar slots.
+struct Reducer {
+    Context &c;
+    explicit Reducer(Context &c);
+    template <typename F>
+    void map_reduce(int64_t n, F f, int slot);
+    void finish(int64_t npart, int slot);
+    double fetch(int slot);
+    void fetch_many(int slot0, int n, double *out);
+};
+
+template <typename T>
+void setup_solver_data(Context &c);
+void build_coarse_inverse(Context &c, bool f64);
+int run_solve(Context &c, const stk_opts &o, stk_report &rep);
+double full_residual(Context &c, const double *U, const double *P, double *ru, double *hc, bool project_null);
+
+}  // namespace stk

@@ -1,0 +1,1069 @@
+// Solvers on the assembled operators.
+//
+//  * Schur-complement PCG (src/IterativeSolvers.jl:94-168): the reference
+//    algorithm -- initial velocity solve, residual r = -G^T U with its mean
+//    removed, lumped 1/eta-mass preconditioner (:189-240), Polak-Ribiere beta
+//    (:163), fused solution update (:246-259) -- with the direct inner solve
+//    lu(K[ifree, ifree]) (src/PardisoSolvers.jl:247-278) replaced by CG
+//    preconditioned with a geometric multigrid V-cycle.
+//  * Mixed precision: FP64 iterative refinement on the full saddle-point
+//    residual around an FP32 Schur PCG whose inner solves are FP32 too.
+//  * Learned: the mixed solver with the assembled local-Schur blocks predicted
+//    by the MLP as the pressure preconditioner, falling back to the plain FP64
+//    method when a refinement sweep stalls.
+//
+// The rigid rotation of the annulus (free slip on both walls) is a numerical
+// null vector of K; the inner solves project it out of their right-hand side,
+// and the caller removes the net rotation afterwards (updatevelocity2,
+// src/Stokes/StokesSolver.jl:170-184), exactly as the reference does.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <type_traits>
+
+#include "device_ops.h"
+#include "solver.h"
+
+namespace stk {
+
+namespace {
+constexpr int NT = 256;
+
+// The FP64 or FP32 member of a pair, chosen by the value type.
+template <typename T, typename A, typename B>
+auto &pick(A &a64, B &a32) {
+    if constexpr (std::is_same_v<T, double>) return a64;
+    else return a32;
+}
+
+inline int nblk(int64_t n, int per = NT) { return (int)((n + per - 1) / per); }
+
+// STK_TRACE=1 in the environment prints solver progress to stderr.
+bool trace_on() {
+    static const int on = [] {
+        const char *e = std::getenv("STK_TRACE");
+        return (e && *e && *e != '0') ? 1 : 0;
+    }();
+    return on != 0;
+}
+}  // namespace
+
+template <typename T>
+void DBuf<T>::zero(hipStream_t s) {
+    if (n) STK_HIP(hipMemsetAsync(p, 0, n * sizeof(T), s));
+}
+template struct DBuf<double>;
+template struct DBuf<float>;
+template struct DBuf<int32_t>;
+template struct DBuf<uint8_t>;
+template struct DBuf<int64_t>;
+
+// ------------------------------------------------------------------ launch helpers
+Reducer::Reducer(Context &c) : c(c) {}
+
+double Reducer::fetch(int slot) {
+    STK_HIP(hipMemcpyAsync(c.h_pinned + slot, c.d_sc.p + slot, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    STK_HIP(hipStreamSynchronize(c.stream));
+    return c.h_pinned[slot];
+}
+
+void Reducer::fetch_many(int slot0, int n, double *out) {
+    STK_HIP(hipMemcpyAsync(c.h_pinned + slot0, c.d_sc.p + slot0, n * sizeof(double), hipMemcpyDeviceToHost,
+                           c.stream));
+    STK_HIP(hipStreamSynchronize(c.stream));
+    for (int i = 0; i < n; ++i) out[i] = c.h_pinned[slot0 + i];
+}
+
+void Reducer::finish(int64_t npart, int slot) {
+    k_reduce_final<NT><<<dim3(1), dim3(NT), 0, c.stream>>>(npart, c.red64.p, c.d_sc.p + slot);
+    STK_HIP(hipGetLastError());
+}
+
+template <typename F>
+void Reducer::map_reduce(int64_t n, F f, int slot) {
+    const int nb = std::min<int64_t>(kReduceBlocks, std::max<int64_t>(1, nblk(n)));
+    k_reduce_map<NT, F><<<dim3(nb), dim3(NT), 0, c.stream>>>(n, f, c.red64.p);
+    STK_HIP(hipGetLastError());
+    finish(nb, slot);
+}
+
+template <typename F>
+void launch_map(Context &c, int64_t n, F f) {
+    if (n == 0) return;
+    k_map<NT, F><<<dim3(nblk(n)), dim3(NT), 0, c.stream>>>(n, f);
+    STK_HIP(hipGetLastError());
+}
+
+template <typename S, typename D>
+void convert_vec(Context &c, const S *src, D *dst, int64_t n) {
+    launch_map(c, n, [=] __device__(int64_t i) { dst[i] = (D)src[i]; });
+}
+
+// Node-block SpMV y = A x (plus epilogue), L lanes per row node.
+template <int BR, int BC, int L, typename T, typename Ep>
+int64_t spmv(Context &c, const Pattern &p, const T *val, const T *x, Ep ep, double *partial) {
+    const int64_t rows_per_block = NT / L;
+    const int64_t nb = (p.nrows + rows_per_block - 1) / rows_per_block;
+    if (partial && (int64_t)c.red64.n < nb) throw Error(STK_E_STATE, "reduction buffer too small");
+    k_spmv_block<BR, BC, L, NT, T, Ep><<<dim3(nb), dim3(NT), 0, c.stream>>>(p.nrows, p.dptr.p, p.dcol.p, val, x, ep, partial, nb);
+    STK_HIP(hipGetLastError());
+    return nb;
+}
+
+// ------------------------------------------------------------------ operator wrappers
+template <typename T>
+struct Op22 {   // BSR 2x2 operator
+    const Pattern *pat;
+    const T *val;
+};
+
+template <int L, typename T, typename Ep>
+int64_t apply22(Context &c, const Op22<T> &A, const T *x, Ep ep, double *partial = nullptr) {
+    return spmv<2, 2, L, T, Ep>(c, *A.pat, A.val, x, ep, partial);
+}
+
+// lanes per row: P2 rows carry up to 19 blocks, P1 rows 7, transfer rows 1-2 / up to 13.
+template <typename T, typename Ep>
+int64_t apply22_auto(Context &c, const Op22<T> &A, const T *x, Ep ep, double *partial = nullptr) {
+    const double avg = (double)A.pat->nnz() / std::max<int64_t>(1, A.pat->nrows);
+    if (avg > 10) return apply22<8>(c, A, x, ep, partial);
+    if (avg > 3) return apply22<4>(c, A, x, ep, partial);
+    return apply22<2>(c, A, x, ep, partial);
+}
+
+// ------------------------------------------------------------------ multigrid
+template <typename T>
+struct MGView {
+    struct Lv {
+        Op22<T> A;            // operator
+        const T *dinv;        // inverse diagonal
+        double lmax;          // upper eigenvalue estimate of D^-1 A
+        int64_t n;            // dofs
+        T *b, *x, *xa, *r, *d;
+        Op22<T> P, R;         // transfer to / from the next coarser level (P: this <- coarser)
+    };
+    std::vector<Lv> lv;
+    const T *coarse_inv = nullptr;
+    int64_t coarse_n = 0;
+};
+
+template <typename T>
+MGLevelBufs<T> &level_bufs(MGLevel &L) { return pick<T>(L.w64, L.w32); }
+
+template <typename T>
+MGView<T> make_view(Context &c) {
+    MGView<T> v;
+    const bool f64 = sizeof(T) == 8;
+    // level 0: the P2 operator K
+    typename MGView<T>::Lv l0{};
+    l0.A = {&c.KN, (const T *)pick<T>(c.K64, c.K32).p};
+    l0.dinv = (const T *)pick<T>(c.Kdinv64, c.Kdinv32).p;
+    l0.lmax = f64 ? c.Klmax64 : c.Klmax32;
+    l0.n = c.nU;
+    MGLevelBufs<T> &w0 = pick<T>(c.kw64, c.kw32);
+    l0.b = w0.b.p; l0.x = w0.x.p; l0.xa = w0.xa.p; l0.r = w0.r.p; l0.d = w0.d.p;
+    if (!c.mg.empty()) {
+        l0.P = {&c.mg[0].P_pat, f64 ? (const T *)c.mg[0].P64.p : (const T *)c.mg[0].P32.p};
+        l0.R = {&c.mg[0].R_pat, f64 ? (const T *)c.mg[0].R64.p : (const T *)c.mg[0].R32.p};
+    }
+    v.lv.push_back(l0);
+    for (size_t i = 0; i < c.mg.size(); ++i) {
+        MGLevel &L = c.mg[i];
+        typename MGView<T>::Lv l{};
+        l.A = {&L.pat, (const T *)pick<T>(L.A64, L.A32).p};
+        l.dinv = (const T *)pick<T>(L.dinv64, L.dinv32).p;
+        l.lmax = f64 ? L.lmax64 : L.lmax32;
+        l.n = 2 * L.nvert;
+        MGLevelBufs<T> &w = level_bufs<T>(L);
+        l.b = w.b.p; l.x = w.x.p; l.xa = w.xa.p; l.r = w.r.p; l.d = w.d.p;
+        if (i + 1 < c.mg.size()) {
+            l.P = {&c.mg[i + 1].P_pat, f64 ? (const T *)c.mg[i + 1].P64.p : (const T *)c.mg[i + 1].P32.p};
+            l.R = {&c.mg[i + 1].R_pat, f64 ? (const T *)c.mg[i + 1].R64.p : (const T *)c.mg[i + 1].R32.p};
+        }
+        v.lv.push_back(l);
+    }
+    v.coarse_inv = (const T *)pick<T>(c.coarse_inv64, c.coarse_inv32).p;
+    v.coarse_n = c.coarse_n;
+    return v;
+}
+
+// Chebyshev smoothing of A x = b with point Jacobi on [lo, hi] = [0.3, 1.1] * lmax.
+// x0 is zero when `zero_start`; the result is written to *xout (one of x / xa).
+template <typename T>
+void chebyshev(Context &c, const typename MGView<T>::Lv &l, const T *b, T *x, T *xa, T *d, int deg,
+               bool zero_start, T **xout) {
+    const double hi = 1.1 * l.lmax, lo = 0.3 * hi;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
+    const double sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    T *cur = x, *nxt = xa;
+    const T *dinv = l.dinv;
+    if (zero_start) {
+        const T it = (T)(1.0 / theta);
+        launch_map(c, l.n, [=] __device__(int64_t i) {
+            const T v = it * dinv[i] * b[i];
+            d[i] = v;
+            cur[i] = v;
+        });
+    } else {
+        // d = D^-1 (b - A x) / theta ; x += d
+        EpCheb<T> ep{b, dinv, cur, d, nxt, T(0), (T)(1.0 / theta)};
+        apply22_auto<T>(c, l.A, cur, ep);
+        std::swap(cur, nxt);
+    }
+    for (int k = 1; k < deg; ++k) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        EpCheb<T> ep{b, dinv, cur, d, nxt, (T)(rn * rho), (T)(2.0 * rn / delta)};
+        apply22_auto<T>(c, l.A, cur, ep);
+        std::swap(cur, nxt);
+        rho = rn;
+    }
+    *xout = cur;
+}
+
+// x_l = V(b_l); result pointer returned (lives in lv[l].x or lv[l].xa).
+template <typename T>
+T *vcycle(Context &c, MGView<T> &v, size_t l, const T *b, int deg) {
+    auto &L = v.lv[l];
+    if (l + 1 == v.lv.size()) {
+        // coarsest: dense inverse times b
+        const T *Ainv = v.coarse_inv;
+        const int64_t n = v.coarse_n;
+        T *x = L.x;
+        launch_map(c, n, [=] __device__(int64_t i) {
+            T s = T(0);
+            for (int64_t k = 0; k < n; ++k) s += Ainv[i * n + k] * b[k];
+            x[i] = s;
+        });
+        return x;
+    }
+    T *x;
+    chebyshev<T>(c, L, b, L.x, L.xa, L.d, deg, true, &x);
+    // r = b - A x
+    EpResidual<T> er{L.r, b};
+    apply22_auto<T>(c, L.A, x, er);
+    // b_c = R r
+    auto &C = v.lv[l + 1];
+    EpStore<T> es{C.b};
+    apply22_auto<T>(c, L.R, L.r, es);
+    T *xc = vcycle<T>(c, v, l + 1, C.b, deg);
+    // x += P xc
+    EpAxpyOut<T> ea{x, T(1), T(1)};
+    apply22_auto<T>(c, L.P, xc, ea);
+    T *other = (x == L.x) ? L.xa : L.x;
+    T *xo;
+    chebyshev<T>(c, L, b, x, other, L.d, deg, false, &xo);
+    return xo;
+}
+
+// ------------------------------------------------------------------ inner CG
+// Solve K z = y (BC operator) to relative tolerance rtol on the complement of the
+// near-null vector v: y is overwritten by y - (v.y) v and CG runs on the projected
+// operator P K P (P = I - v v^T; K p and the preconditioned residual are projected
+// every iteration), so it never resolves the near-null direction whatever the
+// tolerance; z comes back orthogonal to v.  With `deflate` the exact null-space
+// term (v.y / lambda) v is added to z.
+// Scalars live in device memory (slots of d_sc); the host reads the residual
+// norms every `check_every` iterations.
+template <typename T>
+int64_t inner_cg(Context &c, MGView<T> &v, bool use_mg, T *y, T *z, double rtol, int itmax, int deg,
+                 bool *converged, bool deflate = false) {
+    Reducer red(c);
+    const int64_t n = c.nU;
+    const bool f64 = sizeof(T) == 8;
+    MGLevelBufs<T> &w = pick<T>(c.cg64, c.cg32);
+    T *r = w.r.p, *p = w.d.p, *Ap = w.b.p;
+    const T *rot = (const T *)pick<T>(c.rot64, c.rot32).p;
+    const T *kdinv = (const T *)pick<T>(c.Kdinv64, c.Kdinv32).p;
+    double *sc = c.d_sc.p;
+    Op22<T> K{&c.KN, (const T *)pick<T>(c.K64, c.K32).p};
+
+    // project the near-null component out of y; r = y; z = 0
+    red.map_reduce(n, [=] __device__(int64_t i) { return (double)rot[i] * (double)y[i]; }, SC_VY);
+    launch_map(c, n, [=] __device__(int64_t i) {
+        const T v = y[i] - (T)sc[SC_VY] * rot[i];
+        y[i] = v;
+        r[i] = v;
+        z[i] = T(0);
+    });
+    const double lam = c.null_lambda;
+    auto finish_null = [&]() {
+        // z <- z - (v.z) v  (+ (v.y / lambda) v when deflating)
+        red.map_reduce(n, [=] __device__(int64_t i) { return (double)rot[i] * (double)z[i]; }, SC_VZ);
+        const double dl = deflate ? 1.0 / lam : 0.0;
+        launch_map(c, n, [=] __device__(int64_t i) {
+            z[i] += (T)(dl * sc[SC_VY] - sc[SC_VZ]) * rot[i];
+        });
+    };
+    red.map_reduce(n, [=] __device__(int64_t i) { return (double)r[i] * (double)r[i]; }, SC_BB);
+    const double bb = red.fetch(SC_BB);
+    if (!(bb > 0)) {
+        *converged = std::isfinite(bb);
+        finish_null();
+        return 0;
+    }
+    const double tol2 = rtol * rtol * bb;
+    auto precond = [&](const T *rin, T *zout) {
+        if (use_mg) {
+            T *res = vcycle<T>(c, v, 0, rin, deg);
+            launch_map(c, n, [=] __device__(int64_t i) { zout[i] = res[i]; });
+        } else {
+            launch_map(c, n, [=] __device__(int64_t i) { zout[i] = kdinv[i] * rin[i]; });
+        }
+    };
+    T *zz = w.x.p;   // preconditioned residual
+    precond(r, zz);
+    red.map_reduce(n, [=] __device__(int64_t i) { return (double)rot[i] * (double)zz[i]; }, SC_VZZ);
+    launch_map(c, n, [=] __device__(int64_t i) {
+        const T v = zz[i] - (T)sc[SC_VZZ] * rot[i];
+        zz[i] = v;
+        p[i] = v;
+    });
+    red.map_reduce(n, [=] __device__(int64_t i) { return (double)r[i] * (double)zz[i]; }, SC_RZ0);
+    int par = 0;
+    const int check_every = 2;
+    int64_t it = 0;
+    *converged = false;
+    double hist[8];
+    while (it < itmax) {
+        const int rz_cur = SC_RZ0 + par, rz_new = SC_RZ0 + (1 - par);
+        // Ap = K p, pAp
+        EpStoreDot<T> ed{Ap, p};
+        const int64_t nb = apply22<8>(c, K, p, ed, c.red64.p);
+        red.finish(nb, SC_PAP);
+        red.map_reduce(n, [=] __device__(int64_t i) { return (double)rot[i] * (double)Ap[i]; }, SC_VAP);
+        const int slot_rr = SC_HIST + (int)(it % check_every);
+        launch_map(c, n, [=] __device__(int64_t i) {
+            const T al = (T)(sc[rz_cur] / sc[SC_PAP]);
+            z[i] += al * p[i];
+            r[i] -= al * (Ap[i] - (T)sc[SC_VAP] * rot[i]);
+        });
+        red.map_reduce(n, [=] __device__(int64_t i) { return (double)r[i] * (double)r[i]; }, slot_rr);
+        ++it;
+        if (it % check_every == 0 || it == itmax) {
+            if (trace_on() && it == itmax) std::fprintf(stderr, "[stk] inner CG hit its cap %d\n", itmax);
+            const int m = (it % check_every == 0) ? check_every : (int)(it % check_every);
+            red.fetch_many(SC_HIST, m, hist);
+            bool bad = false, done = false;
+            for (int k = 0; k < m; ++k) {
+                if (!std::isfinite(hist[k])) bad = true;
+                if (hist[k] <= tol2) done = true;
+            }
+            if (bad) throw Error(STK_E_NAN, "non-finite residual in the inner CG");
+            if (done) { *converged = true; break; }
+            bool grew = false;   // rounding-dominated right-hand side: stop instead of diverging
+            for (int k = 0; k < m; ++k) grew |= hist[k] > 1e6 * bb;
+            if (grew) break;
+        }
+        precond(r, zz);
+        red.map_reduce(n, [=] __device__(int64_t i) { return (double)r[i] * (double)zz[i]; }, rz_new);
+        red.map_reduce(n, [=] __device__(int64_t i) { return (double)rot[i] * (double)zz[i]; }, SC_VZZ);
+        launch_map(c, n, [=] __device__(int64_t i) {
+            const T be = (T)(sc[rz_new] / sc[rz_cur]);
+            p[i] = zz[i] - (T)sc[SC_VZZ] * rot[i] + be * p[i];
+        });
+        par = 1 - par;
+    }
+    finish_null();
+    if (trace_on()) {
+        // true residual of the returned solution on the complement: |y - K z| / |y|
+        T *t = Ap;
+        EpResidual<T> er{t, y};
+        const int64_t nb = apply22<8>(c, K, z, er, c.red64.p);
+        red.finish(nb, SC_TMP);
+        const double tr = red.fetch(SC_TMP);
+        std::fprintf(stderr, "[stk] inner CG %lld its, recursive |r|/|b| %.3e, true %.3e\n", (long long)it,
+                     std::sqrt(hist[0] / bb), std::sqrt(tr / bb));
+    }
+    return it;
+}
+
+// ------------------------------------------------------------------ set-up of the solver data
+template <typename T>
+void power_lmax(Context &c, const Op22<T> &A, const T *dinv, int64_t n, T *v, T *w, double *out) {
+    Reducer red(c);
+    launch_map(c, n, [=] __device__(int64_t i) {
+        uint64_t h = (uint64_t)i * 0x9E3779B97F4A7C15ull + 0x12345ull;
+        h ^= h >> 31; h *= 0xBF58476D1CE4E5B9ull; h ^= h >> 29;
+        v[i] = (T)((double)(h >> 11) * 0x1.0p-53 - 0.5) * (dinv[i] != T(0) ? T(1) : T(0));
+    });
+    double lam = 0;
+    for (int k = 0; k < 20; ++k) {
+        EpStore<T> es{w};
+        apply22_auto<T>(c, A, v, es);
+        // Rayleigh quotient in the D inner product: (v, A v) / (v, D v)
+        red.map_reduce(n, [=] __device__(int64_t i) { return (double)v[i] * (double)w[i]; }, SC_TMP);
+        red.map_reduce(n, [=] __device__(int64_t i) {
+            return dinv[i] != T(0) ? (double)v[i] * (double)v[i] / (double)dinv[i] : 0.0;
+        }, SC_TMP + 1);
+        double num, den;
+        num = red.fetch(SC_TMP);
+        den = red.fetch(SC_TMP + 1);
+        lam = num / den;
+        // v <- D^-1 A v, normalised in the D norm
+        const double s = 1.0 / std::sqrt(std::max(den, 1e-300));
+        launch_map(c, n, [=] __device__(int64_t i) { v[i] = (T)s * dinv[i] * w[i]; });
+    }
+    *out = lam;
+}
+
+template <typename T>
+void diag_inverse(Context &c, const Pattern &p, const T *A, T *dinv) {
+    const int32_t *ptr = p.dptr.p, *col = p.dcol.p;
+    launch_map(c, p.nrows, [=] __device__(int64_t node) {
+        for (int32_t k = ptr[node]; k < ptr[node + 1]; ++k)
+            if (col[k] == node) {
+                const T a0 = A[4 * (int64_t)k], a3 = A[4 * (int64_t)k + 3];
+                dinv[2 * node] = a0 != T(0) ? T(1) / a0 : T(0);
+                dinv[2 * node + 1] = a3 != T(0) ? T(1) / a3 : T(0);
+                break;
+            }
+    });
+}
+
+// Dense inverse (Gauss-Jordan, partial pivoting) of the coarsest operator, on the host.
+static void dense_inverse(std::vector<double> &A, int64_t n) {
+    std::vector<double> I(n * n, 0.0);
+    for (int64_t i = 0; i < n; ++i) I[i * n + i] = 1.0;
+    for (int64_t k = 0; k < n; ++k) {
+        int64_t piv = k;
+        for (int64_t i = k + 1; i < n; ++i)
+            if (std::fabs(A[i * n + k]) > std::fabs(A[piv * n + k])) piv = i;
+        if (A[piv * n + k] == 0.0) throw Error(STK_E_SHAPE, "singular coarsest multigrid operator");
+        if (piv != k)
+            for (int64_t j = 0; j < n; ++j) {
+                std::swap(A[k * n + j], A[piv * n + j]);
+                std::swap(I[k * n + j], I[piv * n + j]);
+            }
+        const double inv = 1.0 / A[k * n + k];
+        for (int64_t j = 0; j < n; ++j) { A[k * n + j] *= inv; I[k * n + j] *= inv; }
+        for (int64_t i = 0; i < n; ++i) {
+            if (i == k) continue;
+            const double f = A[i * n + k];
+            if (f == 0.0) continue;
+            for (int64_t j = 0; j < n; ++j) { A[i * n + j] -= f * A[k * n + j]; I[i * n + j] -= f * I[k * n + j]; }
+        }
+    }
+    A.swap(I);
+}
+
+void build_coarse_inverse(Context &c, bool f64) {
+    MGLevel &L = c.mg.back();
+    const int64_t nb = L.pat.nnz();
+    std::vector<double> blocks(4 * nb);
+    if (f64) {
+        L.A64.download(blocks.data(), 4 * nb, c.stream);
+    } else {
+        std::vector<float> bf(4 * nb);
+        L.A32.download(bf.data(), 4 * nb, c.stream);
+        STK_HIP(hipStreamSynchronize(c.stream));
+        std::copy(bf.begin(), bf.end(), blocks.begin());
+    }
+    STK_HIP(hipStreamSynchronize(c.stream));
+    const int64_t n = 2 * L.nvert;
+    std::vector<double> A(n * n, 0.0);
+    for (int64_t r = 0; r < L.nvert; ++r)
+        for (int32_t k = L.pat.ptr[r]; k < L.pat.ptr[r + 1]; ++k) {
+            const int64_t cl = L.pat.col[k];
+            A[(2 * r) * n + 2 * cl] = blocks[4 * k];
+            A[(2 * r) * n + 2 * cl + 1] = blocks[4 * k + 1];
+            A[(2 * r + 1) * n + 2 * cl] = blocks[4 * k + 2];
+            A[(2 * r + 1) * n + 2 * cl + 1] = blocks[4 * k + 3];
+        }
+    dense_inverse(A, n);
+    c.coarse_n = n;
+    if (f64) {
+        c.coarse_inv64.upload(A, c.stream);
+    } else {
+        std::vector<float> Af(A.begin(), A.end());
+        c.coarse_inv32.upload(Af, c.stream);
+        STK_HIP(hipStreamSynchronize(c.stream));
+    }
+    STK_HIP(hipStreamSynchronize(c.stream));
+}
+
+template <typename T>
+void alloc_bufs(MGLevelBufs<T> &w, int64_t n) {
+    w.b.alloc(n); w.x.alloc(n); w.xa.alloc(n); w.r.alloc(n); w.d.alloc(n);
+}
+
+// After assembly: inverse diagonals, eigenvalue bounds, coarse inverse, rotation vector.
+template <typename T>
+void setup_solver_data(Context &c) {
+    const bool f64 = sizeof(T) == 8;
+    MGLevelBufs<T> &kw = pick<T>(c.kw64, c.kw32);
+    MGLevelBufs<T> &cg = pick<T>(c.cg64, c.cg32);
+    alloc_bufs(kw, c.nU);
+    alloc_bufs(cg, c.nU);
+    DBuf<T> &kd = pick<T>(c.Kdinv64, c.Kdinv32);
+    kd.alloc(c.nU);
+    const T *K = (const T *)pick<T>(c.K64, c.K32).p;
+    diag_inverse<T>(c, c.KN, K, kd.p);
+    // rigid rotation in the node frames: u_theta = r, u_r = 0 (zero on fixed dofs), unit norm
+    DBuf<T> &rot = pick<T>(c.rot64, c.rot32);
+    rot.alloc(c.nU);
+    {
+        const double *nr = c.d_noder.p;
+        const uint8_t *fx = c.d_fixed.p;
+        T *rp = rot.p;
+        double s2 = 0;
+        for (double r : c.node_r) s2 += r * r;
+        const double inv = 1.0 / std::sqrt(s2);
+        launch_map(c, c.nU, [=] __device__(int64_t i) {
+            rp[i] = (i % 2 == 0 && !fx[i]) ? (T)(nr[i / 2] * inv) : T(0);
+        });
+    }
+    if (!c.mg.empty()) {
+        for (MGLevel &L : c.mg) {
+            alloc_bufs(level_bufs<T>(L), 2 * L.nvert);
+            DBuf<T> &di = pick<T>(L.dinv64, L.dinv32);
+            di.alloc(2 * L.nvert);
+            diag_inverse<T>(c, L.pat, (const T *)pick<T>(L.A64, L.A32).p, di.p);
+        }
+        // eigenvalue bounds (level 0 = K, then the P1 levels except the coarsest)
+        Op22<T> K0{&c.KN, K};
+        double lm;
+        power_lmax<T>(c, K0, kd.p, c.nU, kw.r.p, kw.d.p, &lm);
+        (f64 ? c.Klmax64 : c.Klmax32) = lm;
+        for (size_t i = 0; i + 1 < c.mg.size(); ++i) {
+            MGLevel &L = c.mg[i];
+            MGLevelBufs<T> &w = level_bufs<T>(L);
+            Op22<T> A{&L.pat, (const T *)pick<T>(L.A64, L.A32).p};
+            power_lmax<T>(c, A, (const T *)pick<T>(L.dinv64, L.dinv32).p, 2 * L.nvert, w.r.p, w.d.p, &lm);
+            (f64 ? L.lmax64 : L.lmax32) = lm;
+            if (trace_on()) std::fprintf(stderr, "[stk] level %zu lmax %.4e\n", i + 1, lm);
+        }
+        build_coarse_inverse(c, f64);
+    } else {
+        Op22<T> K0{&c.KN, K};
+        double lm;
+        power_lmax<T>(c, K0, kd.p, c.nU, kw.r.p, kw.d.p, &lm);
+        (f64 ? c.Klmax64 : c.Klmax32) = lm;
+    }
+    // Near-null vector: the nodal rigid rotation z0 corrected by one solve on its
+    // complement, v = z0 - Pi K^-1 Pi K z0 (Pi = I - z0 z0^T), which is the lowest
+    // eigenvector of K to rounding at the mesh sizes where it can be checked.
+    if (!f64 && c.rot64) {
+        convert_vec(c, c.rot64.p, rot.p, c.nU);
+    } else {
+        // Jacobi-Davidson steps v <- v - P (P K P)^-1 P K v, quadratically convergent
+        MGView<T> view = make_view<T>(c);
+        DBuf<T> y(c.nU), d(c.nU);
+        Op22<T> K0{&c.KN, K};
+        bool conv = false;
+        double inv = 1.0, dn = 1.0;
+        T *rp = rot.p;
+        const T *dp = d.p;
+        Reducer red(c);
+        DBuf<T> keep(c.nU);
+        T *kp = keep.p;
+        double prev = 1.0;
+        for (int step = 0; step < 4; ++step) {
+            launch_map(c, c.nU, [=] __device__(int64_t i) { kp[i] = rp[i]; });
+            EpStore<T> es{y.p};
+            apply22<8>(c, K0, rot.p, es);
+            inner_cg<T>(c, view, c.mg_built, y.p, d.p, f64 ? 1e-12 : 1e-6, 4000, 3, &conv);
+            red.map_reduce(c.nU, [=] __device__(int64_t i) { return (double)dp[i] * (double)dp[i]; }, SC_TMP);
+            dn = std::sqrt(red.fetch(SC_TMP));
+            if (trace_on()) std::fprintf(stderr, "[stk] null vector step %d: |correction| %.3e conv %d\n", step, dn, (int)conv);
+            // a step is kept only while the (quadratically convergent) correction still shrinks
+            if (step > 0 && (!conv || !std::isfinite(dn) || dn > 0.1 * prev)) {
+                launch_map(c, c.nU, [=] __device__(int64_t i) { rp[i] = kp[i]; });
+                break;
+            }
+            launch_map(c, c.nU, [=] __device__(int64_t i) { rp[i] -= dp[i]; });
+            red.map_reduce(c.nU, [=] __device__(int64_t i) { return (double)rp[i] * (double)rp[i]; }, SC_TMP);
+            inv = 1.0 / std::sqrt(red.fetch(SC_TMP));
+            launch_map(c, c.nU, [=] __device__(int64_t i) { rp[i] = (T)(inv * (double)rp[i]); });
+            prev = dn;
+            if (dn < (f64 ? 1e-13 : 1e-6)) break;
+        }
+        if (f64) {
+            EpStoreDot<T> ed{y.p, rp};
+            const int64_t nb = apply22<8>(c, K0, rp, ed, c.red64.p);
+            red.finish(nb, SC_TMP);
+            c.null_lambda = red.fetch(SC_TMP);
+        }
+        if (trace_on())
+            std::fprintf(stderr, "[stk] null vector: correction solve conv %d, |v|^-1 %.6e, lambda %.6e, Klmax %.4e\n",
+                         (int)conv, inv, c.null_lambda, f64 ? c.Klmax64 : c.Klmax32);
+    }
+    STK_HIP(hipStreamSynchronize(c.stream));
+}
+
+template void setup_solver_data<double>(Context &);
+template void setup_solver_data<float>(Context &);
+
+// ------------------------------------------------------------------ Schur PCG
+// Pressure-space vectors (nP) and the velocity work vectors of one Schur solve.
+template <typename T>
+struct SchurWork {
+    T *y, *z, *du;        // nU
+    T *r, *d, *q, *Sq, *rl, *pc, *dp;   // nP
+};
+
+template <typename T>
+void vmean_remove(Context &c, T *a, int64_t n) {
+    Reducer red(c);
+    red.map_reduce(n, [=] __device__(int64_t i) { return (double)a[i]; }, SC_TMP);
+    const double *sc = c.d_sc.p;
+    launch_map(c, n, [=] __device__(int64_t i) { a[i] -= (T)(sc[SC_TMP] / (double)n); });
+}
+
+template <typename T>
+double vdot(Context &c, const T *a, const T *b, int64_t n) {
+    Reducer red(c);
+    red.map_reduce(n, [=] __device__(int64_t i) { return (double)a[i] * (double)b[i]; }, SC_TMP);
+    return red.fetch(SC_TMP);
+}
+
+// Generalised StokesPcCG: solve K du - G dp = fu, G^T du = h (h may be null = 0)
+// with du, dp starting at zero.  Returns outer iterations; inner iterations accumulate.
+template <typename T>
+int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *h, SchurWork<T> &w,
+              double rtol, int itmax, int itmin, double inner_rtol, int64_t *inner_its, double *schur_res,
+              bool *conv, const T *pcvec, bool learned, bool deflate = false) {
+    const bool f64 = sizeof(T) == 8;
+    MGView<T> v = make_view<T>(c);
+    const int64_t nU = c.nU, nP = c.nP;
+    const T *G = (const T *)pick<T>(c.G64, c.G32).p;
+    const T *GT = (const T *)pick<T>(c.GT64, c.GT32).p;
+    const int deg = o.cheb_degree;
+    bool ic;
+    // du = K^-1 fu
+    launch_map(c, nU, [=] __device__(int64_t i) { w.y[i] = fu[i]; });
+    *inner_its += inner_cg<T>(c, v, use_mg, w.y, w.du, inner_rtol, o.inner_itmax, deg, &ic, deflate);
+    // r = h - G^T du ; mean removed
+    {
+        EpStore<T> es{w.r};
+        spmv<1, 2, 8, T>(c, c.GTN, GT, w.du, es, nullptr);
+        T *r = w.r;
+        launch_map(c, nP, [=] __device__(int64_t i) { r[i] = (h ? h[i] : T(0)) - r[i]; });
+    }
+    vmean_remove<T>(c, w.r, nP);
+    launch_map(c, nP, [=] __device__(int64_t i) { w.dp[i] = T(0); });
+    const double r0 = std::sqrt(vdot<T>(c, w.r, w.r, nP));
+    const double tol = rtol * r0;
+    auto apply_pc = [&](const T *rin, T *dout) {
+        if (learned) {
+            // three damped Jacobi sweeps on the assembled learned Schur operator
+            const T *S = (const T *)pick<T>(c.S64, c.S32).p;
+            const T *sd = pcvec;   // inverse diagonal of S
+            T *tmp = w.Sq;
+            launch_map(c, nP, [=] __device__(int64_t i) { dout[i] = sd[i] * rin[i]; });
+            for (int s = 0; s < 3; ++s) {
+                EpStore<T> es{tmp};
+                spmv<1, 1, 8, T>(c, c.MN, S, dout, es, nullptr);
+                launch_map(c, nP, [=] __device__(int64_t i) { dout[i] += T(0.7) * sd[i] * (rin[i] - tmp[i]); });
+            }
+        } else {
+            launch_map(c, nP, [=] __device__(int64_t i) { dout[i] = pcvec[i] * rin[i]; });
+        }
+    };
+    *conv = false;
+    *schur_res = 1.0;
+    if (!(r0 > 0)) {
+        *conv = std::isfinite(r0);
+        *schur_res = 0.0;
+        return 0;
+    }
+    apply_pc(w.r, w.d);
+    launch_map(c, nP, [=] __device__(int64_t i) { w.q[i] = w.d[i]; });
+    int it = 0;
+    for (it = 1; it <= itmax; ++it) {
+        vmean_remove<T>(c, w.q, nP);
+        const double rd = vdot<T>(c, w.r, w.d, nP);
+        // y = G q ; z = K^-1 y ; Sq = G^T z
+        EpStore<T> ey{w.y};
+        spmv<2, 1, 8, T>(c, c.GN, G, w.q, ey, nullptr);
+        *inner_its += inner_cg<T>(c, v, use_mg, w.y, w.z, inner_rtol, o.inner_itmax, deg, &ic, deflate);
+        EpStore<T> es{w.Sq};
+        spmv<1, 2, 8, T>(c, c.GTN, GT, w.z, es, nullptr);
+        const double qSq = vdot<T>(c, w.q, w.Sq, nP);
+        const double alpha = rd / qSq;
+        if (!std::isfinite(alpha)) throw Error(STK_E_NAN, "non-finite step in the Schur PCG");
+        const T al = (T)alpha;
+        launch_map(c, nP, [=] __device__(int64_t i) {
+            w.rl[i] = w.r[i];
+            w.dp[i] += al * w.q[i];
+            w.r[i] -= al * w.Sq[i];
+        });
+        launch_map(c, nU, [=] __device__(int64_t i) { w.du[i] += al * w.z[i]; });
+        vmean_remove<T>(c, w.r, nP);
+        const double rn = std::sqrt(vdot<T>(c, w.r, w.r, nP));
+        *schur_res = rn / r0;
+        if (trace_on())
+            std::fprintf(stderr, "[stk] schur it %d  |r|/|r0| %.3e  alpha %.3e  inner its %lld\n", it, rn / r0, alpha,
+                         (long long)*inner_its);
+        if (!std::isfinite(rn)) throw Error(STK_E_NAN, "non-finite residual in the Schur PCG");
+        if (rn < tol && it >= itmin) { *conv = true; break; }
+        apply_pc(w.r, w.d);
+        Reducer red(c);
+        red.map_reduce(nP, [=] __device__(int64_t i) {
+            return ((double)w.r[i] - (double)w.rl[i]) * (double)w.d[i];
+        }, SC_TMP);
+        const double beta = red.fetch(SC_TMP) / rd;
+        const T be = (T)beta;
+        launch_map(c, nP, [=] __device__(int64_t i) { w.q[i] = w.d[i] + be * w.q[i]; });
+    }
+    return std::min(it, itmax);
+}
+
+// FP64 full-system residual: ru = F - K U + G P (free dofs; fixed rows are zero), rc = G^T U.
+// Returns ||[ru; rc]|| / ||F_free||, optionally keeping ru and -rc.
+double full_residual(Context &c, const double *U, const double *P, double *ru, double *hc, bool project_null) {
+    Reducer red(c);
+    const int64_t nU = c.nU, nP = c.nP;
+    double *sc = c.d_sc.p;
+    const uint8_t *fx = c.d_fixed.p;
+    const double *F = c.F64.p;
+    // ru = K U
+    EpStore<double> ek{ru};
+    spmv<2, 2, 8, double>(c, c.KN, c.K64.p, U, ek, nullptr);
+    // t = G P ; ru = F + t - ru  (masked)
+    double *t = c.work64.p;
+    EpStore<double> eg{t};
+    spmv<2, 1, 8, double>(c, c.GN, c.G64.p, P, eg, nullptr);
+    launch_map(c, nU, [=] __device__(int64_t i) { ru[i] = fx[i] ? 0.0 : F[i] + t[i] - ru[i]; });
+    if (project_null) {
+        // the near-null component is outside the solved space (STK_NULL_PROJECT)
+        const double *v = c.rot64.p;
+        red.map_reduce(nU, [=] __device__(int64_t i) { return v[i] * ru[i]; }, SC_TMP + 1);
+        launch_map(c, nU, [=] __device__(int64_t i) { ru[i] -= sc[SC_TMP + 1] * v[i]; });
+    }
+    red.map_reduce(nU, [=] __device__(int64_t i) { return ru[i] * ru[i]; }, SC_RU);
+    // hc = -G^T U with its mean removed: the Schur iteration works modulo constants
+    // (rm0space!, src/IterativeSolvers.jl:131,153), so the continuity residual does too.
+    EpStore<double> ec{hc};
+    spmv<1, 2, 8, double>(c, c.GTN, c.GT64.p, U, ec, nullptr);
+    red.map_reduce(nP, [=] __device__(int64_t i) { return hc[i]; }, SC_TMP);
+    launch_map(c, nP, [=] __device__(int64_t i) { hc[i] = sc[SC_TMP] / (double)nP - hc[i]; });
+    red.map_reduce(nP, [=] __device__(int64_t i) { return hc[i] * hc[i]; }, SC_RC);
+    double v[2];
+    red.fetch_many(SC_RU, 2, v);
+    if (trace_on())
+        std::fprintf(stderr, "[stk] full residual: momentum %.3e continuity %.3e (relative to |f|)\n",
+                     std::sqrt(v[0]) / c.fnorm, std::sqrt(v[1]) / c.fnorm);
+    return std::sqrt(v[0] + v[1]) / c.fnorm;
+}
+
+template <typename T>
+SchurWork<T> schur_work(Context &c) {
+    const bool f64 = sizeof(T) == 8;
+    SchurBufs<T> &b = pick<T>(c.sw64, c.sw32);
+    b.v.alloc(3 * c.nU + 7 * c.nP);
+    T *base = b.v.p;
+    SchurWork<T> w;
+    w.y = base; w.z = base + c.nU; w.du = base + 2 * c.nU;
+    T *pp = base + 3 * c.nU;
+    w.r = pp; w.d = pp + c.nP; w.q = pp + 2 * c.nP; w.Sq = pp + 3 * c.nP; w.rl = pp + 4 * c.nP;
+    w.pc = pp + 5 * c.nP; w.dp = pp + 6 * c.nP;
+    return w;
+}
+
+int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
+    std::memset(&rep, 0, sizeof(rep));
+    const bool use_mg = (o.inner == STK_INNER_MG) && c.mg_built && !c.loaded;
+    if (o.inner == STK_INNER_MG && !use_mg) throw Error(STK_E_NOMESH, "multigrid not available for this system");
+    const int64_t nU = c.nU, nP = c.nP;
+    c.U.alloc(nU);
+    c.P.alloc(nP);
+    c.U.zero(c.stream);
+    c.P.zero(c.stream);
+    c.work64.alloc(std::max(nU, nP));
+    hipEvent_t e0, e1;
+    STK_HIP(hipEventCreate(&e0));
+    STK_HIP(hipEventCreate(&e1));
+    STK_HIP(hipEventRecord(e0, c.stream));
+    // masked right-hand side
+    DBuf<double> Fm(nU);
+    {
+        const uint8_t *fx = c.d_fixed.p;
+        const double *F = c.F64.p;
+        double *f = Fm.p;
+        launch_map(c, nU, [=] __device__(int64_t i) { f[i] = fx[i] ? 0.0 : F[i]; });
+        c.fnorm = std::sqrt(vdot<double>(c, Fm.p, Fm.p, nU));
+        if (!(c.fnorm > 0)) c.fnorm = 1.0;
+    }
+    int status = STK_OK;
+    const bool deflate = (o.null_mode == STK_NULL_DEFLATE);
+    const bool project = !deflate;
+    if (deflate && (o.method != STK_SOLVE_PLAIN || !(c.null_lambda > 0)))
+        throw Error(STK_E_ARG, "null-space deflation needs the plain FP64 method");
+    if (o.method == STK_SOLVE_PLAIN) {
+        SchurWork<double> w = schur_work<double>(c);
+        bool conv;
+        double sres;
+        int64_t iits = 0;
+        // the plain method checks the full residual by running to the Schur tolerance,
+        // then (STK_STOP_FULL) continuing in restarted sweeps until the full residual is met.
+        const double *pc = c.pc64.p;
+        int its = schur_pcg<double>(c, o, use_mg, Fm.p, nullptr, w, o.rtol, o.itmax, o.itmin, o.inner_rtol,
+                                    &iits, &sres, &conv, pc, false, deflate);
+        {
+            double *U = c.U.p, *P = c.P.p;
+            launch_map(c, nU, [=] __device__(int64_t i) { U[i] = w.du[i]; });
+            launch_map(c, nP, [=] __device__(int64_t i) { P[i] = w.dp[i]; });
+        }
+        rep.outer_iterations = its;
+        rep.inner_iterations = iits;
+        rep.schur_residual = sres;
+        rep.refinements = 0;
+        DBuf<double> ru(nU), hc(nP);
+        rep.full_residual = full_residual(c, c.U.p, c.P.p, ru.p, hc.p, project);
+        if (trace_on()) std::fprintf(stderr, "[stk] plain first pass: %d its, full residual %.3e\n", its, rep.full_residual);
+        if (o.stop == STK_STOP_FULL) {
+            // restarted sweeps of the same FP64 method on the residual system
+            while (rep.full_residual >= o.rtol && rep.outer_iterations < o.itmax) {
+                int k = schur_pcg<double>(c, o, use_mg, ru.p, hc.p, w, o.rtol, o.itmax - rep.outer_iterations,
+                                          1, o.inner_rtol, &iits, &sres, &conv, pc, false, deflate);
+                double *U = c.U.p, *P = c.P.p;
+                launch_map(c, nU, [=] __device__(int64_t i) { U[i] += w.du[i]; });
+                launch_map(c, nP, [=] __device__(int64_t i) { P[i] += w.dp[i]; });
+                rep.outer_iterations += k;
+                rep.inner_iterations = iits;
+                rep.refinements++;
+                rep.full_residual = full_residual(c, c.U.p, c.P.p, ru.p, hc.p, project);
+                if (trace_on()) std::fprintf(stderr, "[stk] plain sweep %d full residual %.3e\n", rep.refinements, rep.full_residual);
+                if (k == 0) break;
+            }
+            conv = rep.full_residual < o.rtol;
+        }
+        rep.converged = conv ? 1 : 0;
+    } else {
+        // FP64 refinement around FP32 Schur PCG corrections
+        if (!c.K32) throw Error(STK_E_STATE, "mixed solve needs the FP32 operators (STK_ASM_FP32)");
+        const bool learned = (o.method == STK_SOLVE_LEARNED);
+        if (learned && !c.S32) throw Error(STK_E_STATE, "learned solve needs the learned Schur operator");
+        SchurWork<float> w = schur_work<float>(c);
+        DBuf<double> ru(nU), hc(nP);
+        DBuf<float> ru32(nU), hc32(nP);
+        int64_t iits = 0;
+        double res = 1.0, prev = 1.0;
+        bool use_learned = learned;
+        rep.full_residual = 1.0;
+        for (int sweep = 0; sweep < o.refine_max; ++sweep) {
+            res = full_residual(c, c.U.p, c.P.p, ru.p, hc.p, project);
+            rep.full_residual = res;
+            if (trace_on()) std::fprintf(stderr, "[stk] mixed sweep %d full residual %.3e\n", sweep, res);
+            if (res < o.rtol) break;
+            if (sweep > 0 && use_learned && res > o.stall_ratio * prev) {
+                use_learned = false;
+                rep.fallbacks++;
+            }
+            prev = res;
+            // scale the correction problem to O(1) for FP32
+            const double scale = 1.0 / (res * c.fnorm);
+            {
+                double *a = ru.p, *b = hc.p;
+                float *a32 = ru32.p, *b32 = hc32.p;
+                launch_map(c, nU, [=] __device__(int64_t i) { a32[i] = (float)(a[i] * scale); });
+                launch_map(c, nP, [=] __device__(int64_t i) { b32[i] = (float)(b[i] * scale); });
+            }
+            bool conv;
+            double sres;
+            const float *pc = use_learned ? c.Sdinv32.p : c.pc32.p;
+            int k = schur_pcg<float>(c, o, use_mg, ru32.p, hc32.p, w, o.mixed_schur_rtol, o.itmax, 1,
+                                     o.mixed_inner_rtol, &iits, &sres, &conv, pc, use_learned);
+            rep.outer_iterations += k;
+            rep.schur_residual = sres;
+            const double inv = 1.0 / scale;
+            double *U = c.U.p, *P = c.P.p;
+            launch_map(c, nU, [=] __device__(int64_t i) { U[i] += inv * (double)w.du[i]; });
+            launch_map(c, nP, [=] __device__(int64_t i) { P[i] += inv * (double)w.dp[i]; });
+            rep.refinements++;
+        }
+        rep.inner_iterations = iits;
+        rep.full_residual = full_residual(c, c.U.p, c.P.p, ru.p, hc.p, project);
+        // fallback: finish with the plain FP64 method when the mixed sweeps did not get there
+        if (rep.full_residual >= o.rtol) {
+            rep.fallbacks++;
+            SchurWork<double> w64 = schur_work<double>(c);
+            bool conv;
+            double sres;
+            int64_t it64 = 0;
+            const bool mg64 = use_mg && c.K64 && c.mg_built && c.have_mg64;
+            int k = schur_pcg<double>(c, o, mg64, ru.p, hc.p, w64, o.rtol, o.itmax, 1, o.inner_rtol, &it64, &sres,
+                                      &conv, c.pc64.p, false);
+            double *U = c.U.p, *P = c.P.p;
+            launch_map(c, nU, [=] __device__(int64_t i) { U[i] += w64.du[i]; });
+            launch_map(c, nP, [=] __device__(int64_t i) { P[i] += w64.dp[i]; });
+            rep.outer_iterations += k;
+            rep.inner_iterations += it64;
+            rep.full_residual = full_residual(c, c.U.p, c.P.p, ru.p, hc.p, project);
+        }
+        rep.converged = rep.full_residual < o.rtol ? 1 : 0;
+    }
+    STK_HIP(hipEventRecord(e1, c.stream));
+    STK_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    STK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    rep.solve_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!rep.converged) status = STK_E_NOTCONV;
+    return status;
+}
+
+}  // namespace stk
+
+// ------------------------------------------------------------------ post-assembly helpers
+namespace stk {
+
+template <typename S, typename D>
+void convert(Context &c, const S *src, D *dst, int64_t n) {
+    launch_map(c, n, [=] __device__(int64_t i) { dst[i] = (D)src[i]; });
+}
+template void convert<double, float>(Context &, const double *, float *, int64_t);
+template void convert<float, double>(Context &, const float *, double *, int64_t);
+
+// pc = 1 / rowsum(M) (lumpedpreconditioner, src/IterativeSolvers.jl:238-240)
+template <typename T>
+void lumped_inverse(Context &c, const Pattern &p, const T *M, T *pc) {
+    const int32_t *ptr = p.dptr.p;
+    launch_map(c, p.nrows, [=] __device__(int64_t i) {
+        double s = 0;
+        for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) s += (double)M[k];
+        pc[i] = (T)(1.0 / s);
+    });
+}
+template void lumped_inverse<double>(Context &, const Pattern &, const double *, double *);
+template void lumped_inverse<float>(Context &, const Pattern &, const float *, float *);
+
+// Inverse diagonal of a scalar CSR operator.
+template <typename T>
+void csr_diag_inverse(Context &c, const Pattern &p, const T *A, T *dinv) {
+    const int32_t *ptr = p.dptr.p, *col = p.dcol.p;
+    launch_map(c, p.nrows, [=] __device__(int64_t i) {
+        T d = T(0);
+        for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k)
+            if (col[k] == i) d = A[k];
+        dinv[i] = d != T(0) ? T(1) / d : T(0);
+    });
+}
+template void csr_diag_inverse<double>(Context &, const Pattern &, const double *, double *);
+template void csr_diag_inverse<float>(Context &, const Pattern &, const float *, float *);
+
+// y = A x for an assembled operator, host FP64 vectors.
+void host_spmv(Context &c, int which, int prec, const double *xh, double *yh) {
+    const Pattern *p = nullptr;
+    int64_t nx = 0, ny = 0;
+    switch (which) {
+        case STK_MAT_K: p = &c.KN; nx = ny = c.nU; break;
+        case STK_MAT_G: p = &c.GN; nx = c.nP; ny = c.nU; break;
+        case STK_MAT_GT: p = &c.GTN; nx = c.nU; ny = c.nP; break;
+        case STK_MAT_M: case STK_MAT_S: p = &c.MN; nx = ny = c.nP; break;
+        default: throw Error(STK_E_ARG, "unknown operator");
+    }
+    DBuf<double> x64(nx), y64(ny);
+    x64.upload(xh, nx, c.stream);
+    auto run = [&](auto tag, const auto *val, auto *x, auto *y) {
+        using T = decltype(tag);
+        EpStore<T> es{y};
+        switch (which) {
+            case STK_MAT_K: spmv<2, 2, 8, T>(c, *p, val, x, es, nullptr); break;
+            case STK_MAT_G: spmv<2, 1, 8, T>(c, *p, val, x, es, nullptr); break;
+            case STK_MAT_GT: spmv<1, 2, 8, T>(c, *p, val, x, es, nullptr); break;
+            default: spmv<1, 1, 8, T>(c, *p, val, x, es, nullptr); break;
+        }
+    };
+    auto pick64 = [&]() -> const double * {
+        switch (which) {
+            case STK_MAT_K: return c.K64.p;
+            case STK_MAT_G: return c.G64.p;
+            case STK_MAT_GT: return c.GT64.p;
+            case STK_MAT_M: return c.M64.p;
+            default: return c.S64.p;
+        }
+    };
+    auto pick32 = [&]() -> const float * {
+        switch (which) {
+            case STK_MAT_K: return c.K32.p;
+            case STK_MAT_G: return c.G32.p;
+            case STK_MAT_GT: return c.GT32.p;
+            case STK_MAT_M: return c.M32.p;
+            default: return c.S32.p;
+        }
+    };
+    if (prec == 64) {
+        const double *v = pick64();
+        if (!v) throw Error(STK_E_STATE, "FP64 operator not assembled");
+        run(double{}, v, (const double *)x64.p, y64.p);
+    } else if (prec == 32) {
+        const float *v = pick32();
+        if (!v) throw Error(STK_E_STATE, "FP32 operator not assembled");
+        DBuf<float> x32(nx), y32(ny);
+        convert<double, float>(c, x64.p, x32.p, nx);
+        run(float{}, v, (const float *)x32.p, y32.p);
+        convert<float, double>(c, y32.p, y64.p, ny);
+    } else {
+        throw Error(STK_E_ARG, "precision must be 64 or 32");
+    }
+    y64.download(yh, ny, c.stream);
+    STK_HIP(hipStreamSynchronize(c.stream));
+}
+
+// Net-rotation removal (updatevelocity2, src/Stokes/StokesSolver.jl:170-184):
+// omega = int rho u_theta / int rho r with volume_integral (src/Stress/FSE.jl:412-464),
+// u_theta -= omega r at every node, Ucart = TT U.
+struct Quad6 {
+    double xi[6], eta[6], w[6];
+};
+
+double update_velocity(Context &c, double *Ucart_dev) {
+    Quad6 q;
+    {
+        const double s = std::sqrt(38.0 - 44.0 * std::sqrt(2.0 / 5.0));
+        const double g1 = (8.0 - std::sqrt(10.0) + s) / 18.0, g2 = (8.0 - std::sqrt(10.0) - s) / 18.0;
+        const double w1 = (620.0 + std::sqrt(213125.0 - 53320.0 * std::sqrt(10.0))) / 3720.0;
+        const double w2 = (620.0 - std::sqrt(213125.0 - 53320.0 * std::sqrt(10.0))) / 3720.0;
+        const double x[6][2] = {{1 - 2 * g1, g1}, {g1, 1 - 2 * g1}, {g1, g1}, {1 - 2 * g2, g2}, {g2, 1 - 2 * g2}, {g2, g2}};
+        for (int i = 0; i < 6; ++i) { q.xi[i] = x[i][0]; q.eta[i] = x[i][1]; q.w[i] = 0.5 * (i < 3 ? w1 : w2); }
+    }
+    const int64_t nel = c.nel;
+    const int32_t *e2n = c.d_e2n.p;
+    const double *th = c.d_elth.p, *rr = c.d_elr.p, *rho = c.d_rho.p, *U = c.U.p, *nr = c.d_noder.p;
+    auto integral = [=] __device__(int64_t p, int which) {
+        const double r1 = rr[p], r2 = rr[nel + p], r3 = rr[2 * nel + p];
+        const double t1 = th[p], t2 = th[nel + p], t3 = th[2 * nel + p];
+        const double detJ = (t2 - t1) * (r3 - r1) - (t3 - t1) * (r2 - r1);
+        double V[6];
+        for (int k = 0; k < 6; ++k) {
+            const int64_t nd = e2n[k * nel + p];
+            V[k] = rho[nd] * (which == 0 ? U[2 * nd] : nr[nd]);
+        }
+        double s = 0;
+        for (int i = 0; i < 6; ++i) {
+            const double a = q.xi[i], b = q.eta[i], t = 1.0 - a - b;
+            const double N[6] = {t * (2 * t - 1), a * (2 * a - 1), b * (2 * b - 1), 4 * a * t, 4 * a * b, 4 * b * t};
+            const double rq = t * r1 + a * r2 + b * r3;
+            double v = 0;
+            for (int k = 0; k < 6; ++k) v += N[k] * V[k];
+            s += v * rq * detJ * q.w[i];
+        }
+        return s;
+    };
+    Reducer red(c);
+    red.map_reduce(nel, [=] __device__(int64_t p) { return integral(p, 0); }, SC_TMP);
+    red.map_reduce(nel, [=] __device__(int64_t p) { return integral(p, 1); }, SC_TMP + 1);
+    double v[2];
+    red.fetch_many(SC_TMP, 2, v);
+    const double omega = v[0] / v[1];
+    double *Um = c.U.p;
+    const double *cs = c.d_nodecs.p;
+    launch_map(c, c.nnod, [=] __device__(int64_t n) {
+        Um[2 * n] -= omega * nr[n];
+        if (Ucart_dev) {
+            const double co = cs[2 * n], si = cs[2 * n + 1];
+            const double a = Um[2 * n], b = Um[2 * n + 1];
+            Ucart_dev[2 * n] = co * a + si * b;
+            Ucart_dev[2 * n + 1] = -si * a + co * b;
+        }
+    });
+    STK_HIP(hipStreamSynchronize(c.stream));
+    return omega;
+}
+
+}  // namespace stk

@@ -1,0 +1,202 @@
+// Internal declarations shared by the host set-up, the kernels and the solvers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dimspers_stokes.h"
+
+namespace stk {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define STK_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            throw ::stk::Error(STK_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define STK_CHECK(cond, code, msg)                           \
+    do {                                                     \
+        if (!(cond)) throw ::stk::Error((code), (msg));      \
+    } while (0)
+
+// Owning device buffer.
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf &operator=(DBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t count) {
+        if (count == n && p) return;
+        release();
+        n = count;
+        if (count) STK_HIP(hipMalloc(&p, count * sizeof(T)));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T *h, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) STK_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T> &h, hipStream_t s) { upload(h.data(), h.size(), s); }
+    void download(T *h, size_t count, hipStream_t s) const {
+        if (count) STK_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+    void zero(hipStream_t s);
+    explicit operator bool() const { return p != nullptr; }
+};
+
+// Node-level sparsity pattern: row i lists the sorted column nodes of its blocks.
+struct Pattern {
+    int64_t nrows = 0, ncols = 0;
+    std::vector<int32_t> ptr, col;   // host copies
+    DBuf<int32_t> dptr, dcol;        // device copies
+    int64_t nnz() const { return (int64_t)col.size(); }
+};
+
+// Work vectors of one multigrid level (or of the inner CG).
+template <typename T>
+struct MGLevelBufs {
+    DBuf<T> b, x, xa, r, d;
+};
+
+template <typename T>
+struct SchurBufs {
+    DBuf<T> v;   // y, z, du (nU) then r, d, q, Sq, rl, pc, dp (nP)
+};
+
+// One multigrid level (P1 vertex lattice of the structured annulus, 2 dofs per vertex).
+struct MGLevel {
+    int32_t ncr = 0, nct = 0;        // cells radial / azimuthal
+    int64_t nvert = 0, nel = 0;
+    Pattern pat;                     // vertex -> vertex (BSR 2x2)
+    std::vector<int32_t> h_e2v;      // 3 x nel vertex ids (element-major per local vertex)
+    DBuf<int32_t> e2v;               // 3 x nel, SoA
+    DBuf<uint8_t> slot;              // 9 x nel, offset of b in row of a
+    DBuf<int32_t> perm;              // elements grouped by colour
+    std::vector<int64_t> col_off;    // colour offsets into perm
+    DBuf<double> cs;                 // vertex cos/sin (2 x nvert)
+    DBuf<uint8_t> fixed;             // per dof
+    DBuf<double> E64;                // 36 x nel element matrices (rotated frame, masked)
+    DBuf<float> E32;
+    DBuf<double> A64, dinv64;        // 4 x nnz blocks, inverse diagonal (2 nvert)
+    DBuf<float> A32, dinv32;
+    double lmax64 = 0, lmax32 = 0;   // Chebyshev upper bounds of D^-1 A
+    // transfer from this level (coarse) to the next finer level:
+    Pattern P_pat;                   // fine node -> coarse vertex (BSR 2x2)
+    DBuf<double> P64;  DBuf<float> P32;
+    Pattern R_pat;                   // coarse vertex -> fine node (transpose)
+    DBuf<double> R64;  DBuf<float> R32;
+    DBuf<int64_t> fixlist;           // fixed dofs of this level
+    int64_t nfix = 0;
+    MGLevelBufs<double> w64;
+    MGLevelBufs<float> w32;
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    int64_t nel = 0, nnod = 0, nvert = 0, nU = 0, nP = 0;
+    // host mesh
+    std::vector<int32_t> e2n;        // 6 x nel, 0-based, element-major (e*6 + a)
+    std::vector<double> el_th, el_r; // 3 x nel, element-major
+    std::vector<double> node_th, node_r;
+    std::vector<uint8_t> fixed;      // per dof
+    std::vector<int64_t> fixed_list; // 0-based
+
+    // colouring
+    std::vector<int32_t> perm;       // elements sorted by colour
+    std::vector<int64_t> col_off;    // ncolour + 1
+
+    // patterns
+    Pattern KN, GN, GTN, MN;
+
+    // device mesh data (colour-permuted, SoA: [j * nel + p])
+    DBuf<int32_t> d_perm, d_e2n;
+    DBuf<double> d_elth, d_elr, d_nodecs, d_noder;
+    DBuf<uint8_t> d_sK, d_sG, d_sGT, d_sM, d_fixed;
+
+    // fields
+    bool have_fields = false;
+    double g = 1.0;
+    DBuf<double> d_rho, d_eta, d_C;  // eta: 6 x nel (ip-major), C: 6 comps x 6 ips x nel
+    bool have_C = false;
+
+    // operators
+    int asm_flags = 0;
+    bool assembled = false, loaded = false, have_mg64 = false, have_mg32 = false;
+    DBuf<double> K64, G64, GT64, M64, F64, pc64;
+    DBuf<float> K32, G32, GT32, M32, F32, pc32;
+    DBuf<double> S64; DBuf<float> S32;        // learned Schur sum (pattern MN)
+    DBuf<double> Sdinv64; DBuf<float> Sdinv32;
+    DBuf<double> Kdinv64; DBuf<float> Kdinv32; // inverse diagonal of K (per dof)
+    DBuf<double> rot64; DBuf<float> rot32;     // unit near-null vector of K (discrete rigid rotation)
+    double null_lambda = 0;                    // its Rayleigh quotient v^T K v
+    DBuf<int64_t> d_fixlist;
+    double Klmax64 = 0, Klmax32 = 0;
+    double fnorm = 1.0;
+
+    // multigrid
+    bool structured = false;
+    int32_t ncell_r = 0, ncell_t = 0;
+    std::vector<MGLevel> mg;                  // mg[0] = P1 on the original mesh
+    bool mg_built = false;
+    DBuf<double> coarse_inv64;                // dense inverse of the coarsest operator
+    DBuf<float> coarse_inv32;
+    int64_t coarse_n = 0;
+
+    // solution and work
+    DBuf<double> U, P, work64;
+    DBuf<double> red64;                       // reduction partials
+    DBuf<double> d_sc;                        // device scalars
+    double *h_pinned = nullptr;               // pinned staging for scalars
+    MGLevelBufs<double> kw64, cg64;
+    MGLevelBufs<float> kw32, cg32;
+    SchurBufs<double> sw64;
+    SchurBufs<float> sw32;
+
+    ~Context();
+};
+
+// ---------------------------------------------------------------- set-up (setup.cpp)
+void build_patterns(Context &c);
+void build_colouring(Context &c);
+void build_slot_maps(Context &c);
+void detect_structured(Context &c);
+void build_mg_structure(Context &c);
+
+// ---------------------------------------------------------------- kernels (assembly.hip)
+void upload_quadrature(hipStream_t s);
+template <typename T>
+void assemble_impl(Context &c, int flags, T *K, T *G, T *GT, T *M, T *F, T *A1, T *E1);
+template <typename T>
+void galerkin_impl(Context &c, size_t l, T *Ef, T *Ec, T *A);
+template <typename T>
+void fix_diag(Context &c, const Pattern &p, int64_t nfix, const int64_t *dofs, T *A);
+
+}  // namespace stk

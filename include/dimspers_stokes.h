@@ -1,0 +1,184 @@
+/*
+ * dimspers_stokes.h -- C ABI of the MI355X Stokes hot path.
+ *
+ * The reference has no FFI: its boundary is a set of Julia functions.  Each
+ * entry point below names the reference function it stands in for
+ * (file:line in the DimsPers tree).  Conventions follow the reference:
+ *   - index arrays are int64, 1-based, column-major (Julia layout);
+ *   - floating-point inputs are FP64, column-major;
+ *   - sparse exports are CSC with 1-based colptr/rowval;
+ *   - host pointers are only read/written during the call.
+ * Every function returns STK_OK (0) or a negative STK_E_* code; the message
+ * is available from stk_last_error(ctx) (or stk_last_error(NULL) when
+ * stk_create itself fails).  A context owns device memory and one HIP
+ * stream on the device that was current at stk_create; it is not thread-safe.
+ */
+#ifndef DIMSPERS_STOKES_H
+#define DIMSPERS_STOKES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct stk_ctx stk_ctx;
+
+enum {
+    STK_OK = 0,
+    STK_E_HIP = -1,       /* HIP runtime error (message carries hipGetErrorString)     */
+    STK_E_SHAPE = -2,     /* inconsistent sizes / indices                              */
+    STK_E_STATE = -3,     /* call out of order (e.g. solve before assemble)            */
+    STK_E_NOTCONV = -4,   /* solver stopped at its iteration cap                       */
+    STK_E_NAN = -5,       /* non-finite value met in the solve                         */
+    STK_E_ARG = -6,       /* bad option value                                          */
+    STK_E_NOMESH = -7     /* mesh is not the structured annulus the multigrid needs   */
+};
+
+/* Assembly flags (stk_assemble). */
+enum {
+    STK_ASM_FP64 = 1,        /* compute in FP64, write FP64 operators                    */
+    STK_ASM_FP32 = 2,        /* FP32 operators for the inner solves (with FP64: rounded) */
+    STK_ASM_CARTESIAN = 4,   /* skip the TT^T . TT rotation (assembly_stokes_cylindric)  */
+    STK_ASM_NO_BC = 8,       /* keep rows/columns of fixed dofs (no _apply_bcs)          */
+    STK_ASM_ISOTROPIC = 16,  /* isotropic D (StokesSolver.jl:688-742), C6 ignored       */
+    STK_ASM_NO_MG = 32       /* do not build the multigrid hierarchy                     */
+};
+
+/* Which operator stk_export_csc / stk_spmv address. */
+enum {
+    STK_MAT_K = 0,    /* velocity block, nU x nU                 */
+    STK_MAT_G = 1,    /* gradient, nU x nP                       */
+    STK_MAT_GT = 2,   /* divergence G^T, nP x nU                 */
+    STK_MAT_M = 3,    /* 1/eta pressure mass, nP x nP            */
+    STK_MAT_S = 4     /* learned/exact local-Schur sum, nP x nP  */
+};
+
+/* Solver methods (stk_opts.method). */
+enum {
+    STK_SOLVE_PLAIN = 0,    /* FP64 Schur PCG, lumped 1/eta-mass preconditioner (reference) */
+    STK_SOLVE_MIXED = 1,    /* FP64 refinement around FP32 Schur PCG                        */
+    STK_SOLVE_LEARNED = 2   /* MIXED with the learned local-Schur preconditioner            */
+};
+
+/* Inner velocity solver (stk_opts.inner). */
+enum {
+    STK_INNER_MG = 0,       /* CG + geometric multigrid V-cycle   */
+    STK_INNER_JACOBI = 1    /* CG + point Jacobi                  */
+};
+
+/* Treatment of the discrete rigid rotation (stk_opts.null_mode).  With free slip
+ * on both walls the lowest eigenvector v of K is a slightly compressible rigid
+ * rotation whose eigenvalue falls like h^6 (about 1e-16 of the largest at 64 x 512
+ * cells), so K is numerically singular. */
+enum {
+    STK_NULL_PROJECT = 0,   /* velocities orthogonal to v (well posed at every size)        */
+    STK_NULL_DEFLATE = 1    /* also add (v.y / lambda) v to every inner solve: the exact    */
+                            /* K^-1 of lu(K[ifree, ifree]) where FP64 still resolves lambda */
+};
+
+/* Stopping rule (stk_opts.stop). */
+enum {
+    STK_STOP_SCHUR = 0,     /* ||r_k|| < rtol ||r_0|| on the Schur residual (IterativeSolvers.jl:118,156) */
+    STK_STOP_FULL = 1       /* ||[f + G p - K u ; -G^T u]|| < rtol ||[f ; 0]|| in FP64, with the  */
+                            /* constant removed from -G^T u and, for STK_NULL_PROJECT, the v      */
+                            /* component removed from f + G p - K u                              */
+};
+
+typedef struct stk_opts {
+    int32_t method;         /* STK_SOLVE_*                                         */
+    int32_t inner;          /* STK_INNER_*                                         */
+    int32_t stop;           /* STK_STOP_*                                          */
+    int32_t itmax;          /* outer (Schur) iteration cap; reference: 200         */
+    int32_t itmin;          /* minimum outer iterations; reference: 10             */
+    int32_t inner_itmax;    /* inner CG cap                                        */
+    int32_t refine_max;     /* mixed/learned: FP64 refinement sweeps cap           */
+    int32_t cheb_degree;    /* multigrid Chebyshev smoothing degree                */
+    int32_t null_mode;      /* STK_NULL_*                                          */
+    int32_t reserved;       /* zero                                                */
+    double rtol;            /* outer tolerance; reference: 1e-8                    */
+    double inner_rtol;      /* inner CG relative tolerance (plain method)          */
+    double mixed_inner_rtol;/* inner CG tolerance inside the FP32 Schur PCG         */
+    double mixed_schur_rtol;/* Schur PCG tolerance of one FP32 correction           */
+    double stall_ratio;     /* learned: fall back when a sweep reduces less than this */
+} stk_opts;
+
+typedef struct stk_report {
+    int32_t converged;          /* 1 if the stopping rule was met               */
+    int32_t outer_iterations;   /* Schur PCG iterations (all sweeps)            */
+    int64_t inner_iterations;   /* inner CG iterations (all)                    */
+    int32_t refinements;        /* FP64 refinement sweeps (mixed / learned)     */
+    int32_t fallbacks;          /* learned -> plain FP64 fallbacks              */
+    double schur_residual;      /* final ||r_k|| / ||r_0|| of the last Schur PCG */
+    double full_residual;       /* FP64 full-system relative residual           */
+    double solve_ms;            /* wall time of the solve on the stream         */
+} stk_report;
+
+/* Fill *opts with the defaults of the reference StokesPcCG. */
+void stk_default_opts(stk_opts *opts);
+
+/* Mesh set-up.  Replaces sparsitystokes (src/Algebra/Sparsity.jl:10-29),
+ * rotation_matrix (src/Stokes/StokesSolver.jl:203-245) and the element
+ * coordinates of MAIN.jl:62-66; fixed_dofs is UBC.Omega
+ * (src/Setup/boundary_conditions.jl:15-37), 1-based.
+ * e2n: 6 x nel (P2, local order v1 v2 v3 m12 m23 m31); el_theta/el_r: 3 x nel
+ * vertex polar coordinates after fixangles!; node_theta/node_r: gr.theta, gr.r. */
+int stk_create(stk_ctx **ctx, int64_t nel, int64_t nnod, int64_t nvert,
+               const int64_t *e2n, const double *el_theta, const double *el_r,
+               const double *node_theta, const double *node_r,
+               const int64_t *fixed_dofs, int64_t nfixed);
+
+void stk_destroy(stk_ctx *ctx);
+
+const char *stk_last_error(const stk_ctx *ctx);
+
+/* Sizes: out[0..6] = nel, nnod, nvert, nU, nP, colours, multigrid levels. */
+int stk_info(const stk_ctx *ctx, int64_t *out7);
+
+/* Upload the per-step fields assembly_stokes_cylindric reads
+ * (StokesSolver.jl:247-249): g, rho.rho (nnod), eta.ip (nel x nip column-major,
+ * nip >= 6, first 6 columns used, :608) and the six StiffnessTensor arrays
+ * (src/Rheology/anisotropy.jl:1-8) in the order eta11, eta33, eta55, eta13,
+ * eta15, eta35, each nel x nip.  C may be NULL for the isotropic law. */
+int stk_set_fields(stk_ctx *ctx, double g, const double *rho_node,
+                   const double *eta_ip, int64_t nip, const double *const C[6]);
+
+/* Element assembly on the GPU.  Replaces assembly_stokes_cylindric
+ * (StokesSolver.jl:247-404) and, unless STK_ASM_CARTESIAN / STK_ASM_NO_BC,
+ * _prepare_matrices (:192-201) and _apply_bcs (:186-190).  flags: STK_ASM_*. */
+int stk_assemble(stk_ctx *ctx, int32_t flags);
+
+/* Export an assembled operator as 1-based CSC (FP64).  Call with
+ * colptr/rowval/nzval NULL to get *nnz (and *nrows, *ncols) first.
+ * lower_only = 1 gives the lower triangle (KK, MM as the reference returns them). */
+int stk_export_csc(stk_ctx *ctx, int32_t which, int32_t lower_only,
+                   int64_t *nrows, int64_t *ncols, int64_t *nnz,
+                   int64_t *colptr, int64_t *rowval, double *nzval);
+
+/* Export the assembled right-hand side (Fb, nU), FP64. */
+int stk_export_rhs(stk_ctx *ctx, double *F);
+
+/* y = A x with an assembled operator (parity checks of the SpMV kernels).
+ * precision 64 or 32 selects the FP64 or FP32 copy; x, y are host FP64. */
+int stk_spmv(stk_ctx *ctx, int32_t which, int32_t precision, const double *x, double *y);
+
+/* Schur-complement PCG.  Replaces StokesPcCG (src/IterativeSolvers.jl:94-168),
+ * its _CholeskyFactorizationSolve inner solve (src/PardisoSolvers.jl:247-278)
+ * and StokesPcCG_gpu (src/src/GPUsolver.jl:2-86).  U and P start at zero as
+ * in the reference (:101-102).  Returns STK_E_NOTCONV when the cap is hit
+ * (the reference returns silently); the report is filled either way. */
+int stk_solve(stk_ctx *ctx, const stk_opts *opts, stk_report *report);
+
+/* Copy the last solution to the host (U: nU, P: nP; polar dof frame). */
+int stk_get_solution(stk_ctx *ctx, double *U, double *P);
+
+/* Net-rotation removal and Cartesian velocity.  Replaces updatevelocity2
+ * (StokesSolver.jl:170-184) with volume_integral (src/Stress/FSE.jl:412-464).
+ * Operates on the device solution; U (polar, nU) and Ucart (nU) may be NULL. */
+int stk_updatevelocity(stk_ctx *ctx, double *U, double *Ucart, double *omega);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIMSPERS_STOKES_H */

@@ -1,0 +1,397 @@
+"""FP64 CPU oracle of the Stokes hot path -- TEST INFRASTRUCTURE ONLY.
+
+This module is the checker for the GPU library.  Only ``tests/``, the smoke
+check and the CPU-baseline leg of ``bench.py`` may import it; the product
+path never calls it.
+
+It restates, in vectorised numpy/scipy, the reference Julia path:
+
+* ``element_matrices``      -- src/Stokes/StokesSolver.jl:247-404 (blocked assembly,
+  6-point rule), :522-541 (_fill_R_J!), :543-639 (_ip_loop!), :641-686
+  (_derivative_weights!), src/src/Stokes/StokesSolver.jl:744-808 (unclamped
+  anisotropic _fill_Kblk!), :812-845 (_fill_Gblk!, _fill_Mblk!, _fill_Fbblk!),
+  :1017-1022 (_element_density).
+* ``assembly_stokes_cylindric`` -- StokesSolver.jl:488-520 with the triplet
+  layout of src/Algebra/Sparsity.jl:31-103 (lower-triangular K and M, full G)
+  and ``accumarray`` (src/Utilities.jl:1-8).
+* ``rotation_matrix``       -- StokesSolver.jl:203-245.
+* ``prepare_matrices``      -- StokesSolver.jl:192-201.
+* ``stokes_pcg``            -- src/IterativeSolvers.jl:94-168 with
+  ``_preconditioner(:jacobi)`` = lumped inverse (:189-217, :238-240),
+  ``rm0space!`` (:172-186), ``_updatesolution!`` (:246-259), Polak-Ribiere beta
+  (:163) and the direct inner solve ``lu(K[ifree, ifree])``
+  (src/PardisoSolvers.jl:247-278; SuperLU here in place of UMFPACK).
+* ``volume_integral``/``updatevelocity2`` -- src/Stress/FSE.jl:412-464,
+  StokesSolver.jl:170-184.
+
+Pinning (tests/test_oracle_*.py): quadrature exactness and partition of
+unity, exact annulus area, element-matrix symmetry / rank / translation
+null spaces, isotropic-tensor equivalence with the isotropic _fill_Kblk!
+(StokesSolver.jl:688-742), tensor rotation against an independent fourth-order
+tensor rotation, closed-form force sums, a dense LAPACK solve of the whole
+saddle-point system, and golden fixtures.  The reference ships no tests or
+vectors and Julia is absent, so nothing is compared against a run of the
+reference itself.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from dimspers_amd.quadrature import ip_triangle6, sf_dN_tri6, sf_N_tri3, sf_N_tri6
+
+
+# ----------------------------------------------------------------------------- element level
+def geometry(el_theta, el_r):
+    """_fill_R_J! (StokesSolver.jl:522-541): returns R21, R31, Th21, Th31, detJ."""
+    R21 = el_r[1] - el_r[0]
+    R31 = el_r[2] - el_r[0]
+    Th21 = el_theta[1] - el_theta[0]
+    Th31 = el_theta[2] - el_theta[0]
+    detJ = Th21 * R31 - Th31 * R21
+    return R21, R31, Th21, Th31, detJ
+
+
+def ip_derivatives(el_theta, el_r, q):
+    """_derivative_weights! (StokesSolver.jl:641-686) at integration point q.
+
+    Returns (dNdx, dNdz) of shape (nel, 6), omega (nel,), theta_q, r_q.
+    """
+    x_ip, w_ip = ip_triangle6()
+    N3 = sf_N_tri3(x_ip[q, 0], x_ip[q, 1])
+    dNds = sf_dN_tri6(x_ip[q, 0], x_ip[q, 1])      # (2, 6)
+    R21, R31, Th21, Th31, detJ = geometry(el_theta, el_r)
+    th = N3 @ el_theta
+    r = N3 @ el_r
+    s, c = np.sin(th), np.cos(th)
+    inv = 1.0 / r / detJ
+    jx0 = (R31 * c - Th31 * s * r) * inv
+    jx1 = (-R21 * c + Th21 * s * r) * inv
+    jz0 = (-R31 * s - Th31 * c * r) * inv
+    jz1 = (R21 * s + Th21 * c * r) * inv
+    dNdx = jx0[:, None] * dNds[0][None, :] + jx1[:, None] * dNds[1][None, :]
+    dNdz = jz0[:, None] * dNds[0][None, :] + jz1[:, None] * dNds[1][None, :]
+    omega = r * detJ * w_ip[q]
+    return dNdx, dNdz, omega, th, r
+
+
+def element_matrices(inp, isotropic: bool = False):
+    """Per-element K_e (12x12), G_e (12x3), M_e (3x3), f_e (12) in the Cartesian frame.
+
+    Local velocity dofs are x1 z1 x2 z2 ... x6 z6.  K_e = sum_q omega_q B^T D_q B
+    with B_j = [[dN_j/dx, 0], [0, dN_j/dz], [dN_j/dz, dN_j/dx]] and
+    D_q = eta_ip[e, q] * [[C11, C13, C15], [C13, C33, C35], [C15, C35, C55]]
+    (the tensor naming of src/src/Stokes/StokesSolver.jl:749-754).
+    ``isotropic`` uses D = eta * [[4/3, -2/3, 0], [-2/3, 4/3, 0], [0, 0, 1]]
+    (StokesSolver.jl:688-742).
+    """
+    mesh = inp.mesh
+    nel = mesh.nel
+    x_ip, _ = ip_triangle6()
+    K = np.zeros((nel, 12, 12))
+    G = np.zeros((nel, 12, 3))
+    M = np.zeros((nel, 3, 3))
+    f = np.zeros((nel, 12))
+    rho_v = inp.rho[mesh.e2n[:3]]                   # (3, nel) vertex densities
+    for q in range(6):
+        dNdx, dNdz, om, th, r = ip_derivatives(mesh.el_theta, mesh.el_r, q)
+        NP = sf_N_tri3(x_ip[q, 0], x_ip[q, 1])      # (3,)
+        Nq = sf_N_tri6(x_ip[q, 0], x_ip[q, 1])      # (6,)
+        eta = inp.eta_ip[:, q]
+        B = np.zeros((nel, 3, 12))
+        B[:, 0, 0::2] = dNdx
+        B[:, 1, 1::2] = dNdz
+        B[:, 2, 0::2] = dNdz
+        B[:, 2, 1::2] = dNdx
+        if isotropic:
+            D = np.zeros((nel, 3, 3))
+            D[:, 0, 0] = D[:, 1, 1] = 4.0 / 3.0
+            D[:, 0, 1] = D[:, 1, 0] = -2.0 / 3.0
+            D[:, 2, 2] = 1.0
+        else:
+            c11, c33, c55, c13, c15, c35 = (a[:, q] for a in inp.C6)
+            D = np.stack([
+                np.stack([c11, c13, c15], -1),
+                np.stack([c13, c33, c35], -1),
+                np.stack([c15, c35, c55], -1),
+            ], -2)
+        D = D * (eta * om)[:, None, None]
+        K += np.einsum("eia,eij,ejb->eab", B, D, B)
+        # G_e[(x_j | z_j), i] += omega NP_i dN_j/d(x | z)   (StokesSolver.jl:812-823)
+        G[:, 0::2, :] += (om[:, None, None] * dNdx[:, :, None]) * NP[None, None, :]
+        G[:, 1::2, :] += (om[:, None, None] * dNdz[:, :, None]) * NP[None, None, :]
+        # M_e += omega/eta NP NP^T                        (StokesSolver.jl:825-836)
+        M += (om / eta)[:, None, None] * np.outer(NP, NP)[None]
+        # f_e[x_j | z_j] += omega g rho_q N_j (sin | cos)   (StokesSolver.jl:838-845)
+        rho_q = NP @ rho_v
+        cte = (om * inp.g * rho_q)[:, None] * Nq[None, :]
+        f[:, 0::2] += np.sin(th)[:, None] * cte
+        f[:, 1::2] += np.cos(th)[:, None] * cte
+    return K, G, M, f
+
+
+def el2dof(e2n):
+    """EL2DOF (Sparsity.jl:46-48), 0-based: x dof 2n, z dof 2n+1."""
+    d = np.empty((12, e2n.shape[1]), dtype=np.int64)
+    d[0::2] = 2 * e2n
+    d[1::2] = 2 * e2n + 1
+    return d
+
+
+# ----------------------------------------------------------------------------- global level
+def assembly_stokes_cylindric(inp, isotropic: bool = False):
+    """Global KK (lower triangle), GG, MM (lower triangle), Fb; Cartesian frame.
+
+    Returns scipy CSC matrices (0-based) and Fb, like StokesSolver.jl:247-404.
+    """
+    mesh = inp.mesh
+    nU, nP = mesh.nU, mesh.nP
+    Ke, Ge, Me, fe = element_matrices(inp, isotropic)
+    dof = el2dof(mesh.e2n)                           # (12, nel)
+    I = np.broadcast_to(dof[:, None, :], (12, 12, mesh.nel))
+    J = np.broadcast_to(dof[None, :, :], (12, 12, mesh.nel))
+    Kfull = sp.coo_matrix((np.transpose(Ke, (1, 2, 0)).ravel(), (I.ravel(), J.ravel())),
+                          shape=(nU, nU)).tocsc()
+    KK = sp.tril(Kfull, format="csc")
+    pI = np.broadcast_to(dof[:, None, :], (12, 3, mesh.nel))
+    pJ = np.broadcast_to(mesh.e2nP[None, :, :], (12, 3, mesh.nel))
+    GG = sp.coo_matrix((np.transpose(Ge, (1, 2, 0)).ravel(), (pI.ravel(), pJ.ravel())),
+                       shape=(nU, nP)).tocsc()
+    mI = np.broadcast_to(mesh.e2nP[:, None, :], (3, 3, mesh.nel))
+    mJ = np.broadcast_to(mesh.e2nP[None, :, :], (3, 3, mesh.nel))
+    Mfull = sp.coo_matrix((np.transpose(Me, (1, 2, 0)).ravel(), (mI.ravel(), mJ.ravel())),
+                          shape=(nP, nP)).tocsc()
+    MM = sp.tril(Mfull, format="csc")
+    Fb = np.zeros(nU)
+    np.add.at(Fb, dof.ravel(), fe.T.ravel())
+    for A in (KK, GG, MM):
+        A.sum_duplicates()
+        A.sort_indices()
+    return KK, GG, MM, Fb
+
+
+def rotation_matrix(theta):
+    """Block-diagonal TT, per node [[cos, sin], [-sin, cos]] (StokesSolver.jl:203-245)."""
+    n = theta.shape[0]
+    c, s = np.cos(theta), np.sin(theta)
+    rows = np.concatenate([2 * np.arange(n), 2 * np.arange(n) + 1, 2 * np.arange(n), 2 * np.arange(n) + 1])
+    cols = np.concatenate([2 * np.arange(n), 2 * np.arange(n) + 1, 2 * np.arange(n) + 1, 2 * np.arange(n)])
+    vals = np.concatenate([c, c, s, -s])
+    return sp.csc_matrix((vals, (rows, cols)), shape=(2 * n, 2 * n))
+
+
+def symmetric_from_lower(L):
+    """``SparseMatrixCSC(Symmetric(L, :L))``: full matrix from the lower triangle."""
+    L = sp.tril(L, format="csc")
+    return (L + sp.tril(L, k=-1, format="csc").T).tocsc()
+
+
+def prepare_matrices(KK, GG, Fb, TT):
+    """_prepare_matrices (StokesSolver.jl:192-201): rotate to (u_theta, u_r) node frames."""
+    K = symmetric_from_lower(KK)
+    K.eliminate_zeros()
+    GG = GG.copy()
+    GG.eliminate_zeros()
+    K = (TT.T @ K @ TT).tocsc()
+    G = (TT.T @ GG).tocsc()
+    F = TT.T @ Fb
+    K = symmetric_from_lower(K)
+    K.sort_indices()
+    G.sort_indices()
+    return K, G, F
+
+
+def lumped_preconditioner(MM):
+    """_preconditioner(:jacobi) -> lumpedpreconditioner (IterativeSolvers.jl:189-240)."""
+    Mfull = symmetric_from_lower(MM)
+    return 1.0 / np.asarray(Mfull.sum(axis=1)).ravel()
+
+
+def rm0space(a):
+    """rm0space! (IterativeSolvers.jl:172-186)."""
+    a -= a.mean()
+    return a
+
+
+@dataclass
+class PCGInfo:
+    iterations: int = 0
+    converged: bool = False
+    residuals: list = field(default_factory=list)
+
+
+def inner_direct(K, ifree):
+    """``lu(K[ifree, ifree])`` (PardisoSolvers.jl:247-256) as a solve callable."""
+    Kff = K[ifree][:, ifree].tocsc()
+    lu = spla.splu(Kff, permc_spec="COLAMD")
+    n = K.shape[0]
+
+    def solve(y):
+        z = np.zeros(n)
+        z[ifree] = lu.solve(y[ifree])
+        return z
+
+    return solve
+
+
+def null_vector(K, ifree, mesh, iters=3):
+    """Lowest eigenvector of K[ifree, ifree] (the discrete rigid rotation) and its eigenvalue.
+
+    Inverse iteration with the sparse LU, started from the nodal rigid rotation
+    u_theta = r, u_r = 0.  With free slip on both walls this eigenvalue falls
+    like h^6; the reference never forms the vector but its LU solve contains it.
+    Not from the reference: used for the projected (null-space) formulation.
+    """
+    n = K.shape[0]
+    Kff = K[ifree][:, ifree].tocsc()
+    lu = spla.splu(Kff, permc_spec="COLAMD")
+    z = np.zeros(n)
+    z[0::2] = mesh.r
+    z = z[ifree]
+    z /= np.linalg.norm(z)
+    v = z.copy()
+    for _ in range(iters):
+        v = lu.solve(v)
+        v /= np.linalg.norm(v)
+    if v @ z < 0:
+        v = -v
+    out = np.zeros(n)
+    out[ifree] = v
+    return out, float(out @ (K @ out))
+
+
+def inner_projected(K, ifree, v):
+    """Direct inner solve on the complement of the null vector v: z = Pi K^-1 Pi y."""
+    Kff = K[ifree][:, ifree].tocsc()
+    lu = spla.splu(Kff, permc_spec="COLAMD")
+    n = K.shape[0]
+    vf = v[ifree]
+
+    def solve(y):
+        yf = y[ifree] - (vf @ y[ifree]) * vf
+        x = lu.solve(yf)
+        x -= (vf @ x) * vf
+        z = np.zeros(n)
+        z[ifree] = x
+        return z
+
+    return solve
+
+
+def stokes_pcg(K, MM, G, F, ifree, rtol=1e-8, itmax=200, itmin=10, inner=None, pc=None):
+    """StokesPcCG (IterativeSolvers.jl:94-168).  Returns U, P, PCGInfo."""
+    nU, nP = G.shape
+    U = np.zeros(nU)
+    P = np.zeros(nP)
+    Rhs = F + G @ P
+    solve = inner if inner is not None else inner_direct(K, ifree)
+    U = solve(Rhs)
+    GT = G.T.tocsr()
+    r = -(GT @ U)
+    rm0space(r)
+    tol = np.linalg.norm(r) * rtol
+    if pc is None:
+        pc = lumped_preconditioner(MM)
+    d = r * pc
+    q = d.copy()
+    info = PCGInfo(residuals=[np.linalg.norm(r)])
+    for it in range(1, itmax + 1):
+        info.iterations = it
+        rm0space(q)
+        rd = r @ d
+        y = G @ q
+        z = solve(y)
+        Sq = GT @ z
+        qSq = q @ Sq
+        rlast = r.copy()
+        alpha = rd / qSq
+        P += alpha * q
+        U += alpha * z
+        r -= alpha * Sq
+        rm0space(r)
+        nr = np.linalg.norm(r)
+        info.residuals.append(nr)
+        if nr < tol and it >= itmin:
+            info.converged = True
+            break
+        d = r * pc
+        beta = ((r - rlast) @ d) / rd
+        q = d + beta * q
+    return U, P, info
+
+
+def volume_integral(V, mesh):
+    """volume_integral (FSE.jl:412-464): sum_e sum_q (N.V_e) r_q detJ w_q."""
+    x_ip, w_ip = ip_triangle6()
+    _, _, _, _, detJ = geometry(mesh.el_theta, mesh.el_r)
+    Ve = V[mesh.e2n]                                  # (6, nel)
+    total = np.zeros(mesh.nel)
+    for q in range(6):
+        N3 = sf_N_tri3(x_ip[q, 0], x_ip[q, 1])
+        N = sf_N_tri6(x_ip[q, 0], x_ip[q, 1])
+        r = N3 @ mesh.el_r
+        total += (N @ Ve) * r * detJ * w_ip[q]
+    return total.sum()
+
+
+def updatevelocity2(U, rho, mesh):
+    """Net-rotation removal (StokesSolver.jl:170-184); returns (U, Ucart)."""
+    U = U.copy()
+    ut = U[0::2]
+    omega = volume_integral(ut * rho, mesh) / volume_integral(mesh.r * rho, mesh)
+    U[0::2] -= omega * mesh.r
+    TT = rotation_matrix(mesh.theta)
+    return U, TT @ U
+
+
+def full_residual(K, G, F, U, P, ifree, null=None):
+    """||[f + G p - K u ; -G^T u - c]|| / ||[f ; 0]|| on free velocity dofs.
+
+    c is the mean of -G^T u: the reference removes it from the continuity
+    residual at every iteration (rm0space!, IterativeSolvers.jl:131,153), so
+    its solutions satisfy continuity up to a constant.  With ``null`` (the
+    near-null vector) the momentum residual is taken on its complement.
+    """
+    ru = (F + G @ P - K @ U)
+    if null is not None:
+        ru = ru - (null @ ru) * null
+    ru = ru[ifree]
+    rp = -(G.T @ U)
+    rp = rp - rp.mean()
+    return np.sqrt(ru @ ru + rp @ rp) / np.linalg.norm(F[ifree])
+
+
+@dataclass
+class OracleSolution:
+    U: np.ndarray
+    P: np.ndarray
+    Ucart: np.ndarray
+    info: PCGInfo
+    K: object
+    G: object
+    MM: object
+    F: np.ndarray
+    ifree: np.ndarray
+
+
+def solve_stokes(inp, rtol=1e-8, itmax=200, itmin=10, null_mode="literal"):
+    """solveStokes (StokesSolver.jl:7-54) with the :suitesparse inner solve.
+
+    null_mode "literal" is the reference algorithm (lu(K[ifree, ifree]));
+    "project" solves on the complement of the near-null vector of K.
+    """
+    mesh = inp.mesh
+    KK, GG, MM, Fb = assembly_stokes_cylindric(inp)
+    TT = rotation_matrix(mesh.theta)
+    K, G, F = prepare_matrices(KK, GG, Fb, TT)
+    fixed, ifree = mesh.free_slip()
+    inner = None
+    if null_mode == "project":
+        v, _ = null_vector(K, ifree, mesh)
+        inner = inner_projected(K, ifree, v)
+    U, P, info = stokes_pcg(K, MM, G, F, ifree, rtol=rtol, itmax=itmax, itmin=itmin, inner=inner)
+    Urot, Ucart = updatevelocity2(U, inp.rho, mesh)
+    return OracleSolution(Urot, P, Ucart, info, K, G, MM, F, ifree), U
