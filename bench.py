@@ -1,0 +1,173 @@
+"""Stokes solve benchmark on one MI355X.
+
+Workload: the learned / mixed-precision configuration at 256 x 2048 cells
+(1,048,576 quadratic triangles, 4,202,496 velocity + 526,336 pressure dofs):
+viscosity contrast 1e4, anisotropy ratio uniform in [1, 100], seeded fields.
+One step = one Schur-complement solve, from zero, of the assembled system to a
+1e-8 relative full-system residual (FP64, evaluated on the device).  The
+operators are assembled and resident in HBM before the timed region; assembly
+is timed separately.  The plain FP64 method is timed on the same system as the
+comparison point, and the errors of the fast solve are measured against it.
+
+Prints one JSON line (see the repository documentation for the fields).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline():
+    """The FP64 oracle (reference algorithm, SuperLU inner solve) on the 32 x 256 workload."""
+    import numpy as np
+
+    from dimspers_amd.inputs import make_inputs
+    from oracle import stokes_oracle as so
+
+    inp = make_inputs(1)
+    t0 = time.perf_counter()
+    sol, _ = so.solve_stokes(inp, rtol=1e-8)
+    dt = time.perf_counter() - t0
+    ndof = inp.mesh.nU + inp.mesh.nP
+    return dt, ndof, sol.info.iterations
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", type=int, default=4)
+    ap.add_argument("--method", default="mixed", choices=["mixed", "learned"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path is not sharded across GPUs.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from dimspers_amd import _lib
+    from dimspers_amd.inputs import WORKLOADS, make_inputs
+
+    w = WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    inp = make_inputs(w)
+    m = inp.mesh
+    log(f"workload {w.index}: {w.ncell_r}x{w.ncell_t} cells, nel {m.nel}, nU {m.nU}, nP {m.nP}, "
+        f"inputs {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    ctx = _lib.Context.from_mesh(m)
+    log(f"context (patterns, colouring, multigrid structure) {time.perf_counter() - t0:.1f}s, "
+        f"{ctx.ncolours} colours, {ctx.mg_levels} P1 levels")
+    ctx.set_fields(inp.g, inp.rho, inp.eta_ip, inp.C6)
+    t0 = time.perf_counter()
+    ctx.assemble(_lib.ASM_FP64 | _lib.ASM_FP32)
+    log(f"assembly + solver set-up {time.perf_counter() - t0:.2f}s")
+    asm_ms, asm_bytes = ctx.time_kernel(_lib.KERNEL_ASSEMBLY, reps=3)
+    log(f"element assembly kernels {asm_ms:.2f} ms ({asm_bytes / asm_ms / 1e6:.0f} GB/s compulsory)")
+
+    method = _lib.SOLVE_LEARNED if args.method == "learned" else _lib.SOLVE_MIXED
+    fast_opts = _lib.default_opts(method=method, stop=_lib.STOP_FULL, rtol=1e-8)
+    plain_opts = _lib.default_opts(method=_lib.SOLVE_PLAIN, stop=_lib.STOP_FULL, rtol=1e-8, itmax=400)
+
+    for k in range(args.warmup):
+        rep = ctx.solve(fast_opts)
+        log(f"warmup {k}: {rep.solve_ms:.1f} ms, {rep.outer_iterations} outer, {rep.inner_iterations} inner, "
+            f"residual {rep.full_residual:.2e}")
+    times, reps = [], []
+    t_all = time.perf_counter()
+    for k in range(args.steps):
+        rep = ctx.solve(fast_opts)
+        times.append(rep.solve_ms)
+        reps.append(rep)
+        log(f"step {k}: {rep.solve_ms:.1f} ms, {rep.outer_iterations} outer, {rep.inner_iterations} inner, "
+            f"{rep.refinements} sweeps, residual {rep.full_residual:.2e}")
+    wall = (time.perf_counter() - t_all) * 1000.0 / args.steps
+    U_fast, P_fast = ctx.solution()
+    ms = float(np.mean(times))
+
+    plain = None
+    if not args.no_plain:
+        prep = ctx.solve(plain_opts)
+        U_pl, P_pl = ctx.solution()
+        eu = float(np.linalg.norm(U_fast - U_pl) / np.linalg.norm(U_pl))
+        ep = float(np.linalg.norm((P_fast - P_fast.mean()) - (P_pl - P_pl.mean())) / np.linalg.norm(P_pl - P_pl.mean()))
+        plain = dict(ms=prep.solve_ms, outer=prep.outer_iterations, inner=int(prep.inner_iterations),
+                     residual=prep.full_residual, errU=eu, errP=ep)
+        log(f"plain FP64: {prep.solve_ms:.1f} ms, {prep.outer_iterations} outer, {prep.inner_iterations} inner, "
+            f"residual {prep.full_residual:.2e}; fast-vs-plain errU {eu:.2e} errP {ep:.2e}")
+
+    kms, kbytes = ctx.time_kernel(_lib.KERNEL_CHEB_K32, reps=50)
+    achieved = kbytes / (kms * 1e-3) / 1e9
+    sms, sbytes = ctx.time_kernel(_lib.KERNEL_SPMV_K32, reps=50)
+    log(f"dominant kernel (fused Chebyshev SpMV, FP32): {kms * 1e3:.1f} us, {achieved:.0f} GB/s; "
+        f"plain FP32 SpMV {sms * 1e3:.1f} us, {sbytes / sms / 1e6:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        dt, ndof, its = cpu_baseline()
+        scaled_ms = dt * 1000.0 * (m.nU + m.nP) / ndof
+        cpu = {"value": round(scaled_ms, 1), "unit": "ms (scaled)", "cores": 1, "kind": "port",
+               "sample": f"FP64 oracle (reference StokesPcCG: SuperLU inner solve, lumped Schur PCG, {its} its) on "
+                         f"the 32x256 workload ({ndof} dofs) took {dt:.2f} s on one host core; value scales it "
+                         f"linearly in dofs to this workload (a lower bound: LU grows faster than linearly)"}
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_cheb_spmv_k32_256x2048.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    ndof = m.nU + m.nP
+    out = {
+        "metric": "Stokes solve time to 1e-8 residual (learned+mixed vs plain FP64 PCG), 1 GPU",
+        "value": round(ms, 2),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 inner / fp64 refinement",
+        "data": "synthetic (seeded fields of the workload definition)",
+        "config": {"workload": f"annulus {w.ncell_r}x{w.ncell_t} cells, {m.nel} P2-P1 triangles, "
+                               f"contrast {w.contrast:g}, anisotropy {w.ratio}", "method": args.method,
+                   "nU": m.nU, "nP": m.nP},
+        "dofs_per_s": round(ndof / (ms * 1e-3)),
+        "outer_iterations": reps[-1].outer_iterations,
+        "inner_iterations": int(reps[-1].inner_iterations),
+        "refinements": reps[-1].refinements,
+        "fallbacks": reps[-1].fallbacks,
+        "full_residual": reps[-1].full_residual,
+        "assembly_ms": round(asm_ms, 3),
+        "plain_fp64": plain,
+        "speedup_vs_plain": round(plain["ms"] / ms, 2) if plain else None,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "kernel": "k_spmv_block<2,2,8,256,float,EpCheb<float>> (fused Chebyshev SpMV on K)",
+                     "avg_us": round(kms * 1e3, 2), "bytes_per_launch": kbytes},
+        "cpu_baseline": cpu,
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -24,8 +24,9 @@ NULL_PROJECT, NULL_DEFLATE = 0, 1
 EXPORTED = [
     "stk_default_opts", "stk_create", "stk_destroy", "stk_last_error", "stk_info", "stk_set_fields",
     "stk_assemble", "stk_export_csc", "stk_export_rhs", "stk_spmv", "stk_solve", "stk_get_solution",
-    "stk_updatevelocity",
+    "stk_updatevelocity", "stk_time_kernel",
 ]
+KERNEL_SPMV_K32, KERNEL_SPMV_K64, KERNEL_CHEB_K32, KERNEL_CHEB_K64, KERNEL_ASSEMBLY = 0, 1, 2, 3, 4
 
 
 class StkOpts(C.Structure):
@@ -87,6 +88,7 @@ def load(path: Path | str | None = None):
     lib.stk_solve.argtypes = [vp, C.POINTER(StkOpts), C.POINTER(StkReport)]
     lib.stk_get_solution.argtypes = [vp, dp, dp]
     lib.stk_updatevelocity.argtypes = [vp, dp, dp, dp]
+    lib.stk_time_kernel.argtypes = [vp, C.c_int32, C.c_int32, dp, dp]
     for name in EXPORTED:
         if name not in ("stk_default_opts", "stk_destroy", "stk_last_error"):
             getattr(lib, name).restype = C.c_int
@@ -210,6 +212,12 @@ class Context:
         P = np.zeros(self.nP)
         self._check(self.lib.stk_get_solution(self.h, _d(U), _d(P)))
         return U, P
+
+    def time_kernel(self, kernel, reps=20):
+        """(mean ms per launch, algorithmic bytes per launch) of one hot kernel."""
+        ms, by = C.c_double(), C.c_double()
+        self._check(self.lib.stk_time_kernel(self.h, kernel, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
 
     def updatevelocity(self):
         U = np.zeros(self.nU)

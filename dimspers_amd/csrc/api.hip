@@ -397,6 +397,44 @@ int stk_get_solution(stk_ctx *ctx, double *U, double *P) {
     });
 }
 
+int stk_time_kernel(stk_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms, double *bytes) {
+    if (!ctx || !avg_ms || !bytes) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK(c.assembled, STK_E_STATE, "nothing assembled");
+        if (kernel == STK_KERNEL_ASSEMBLY) {
+            STK_CHECK(c.have_fields && c.K64, STK_E_STATE, "FP64 assembly needs fields");
+            const int flags = c.asm_flags;
+            hipEvent_t e0, e1;
+            STK_HIP(hipEventCreate(&e0));
+            STK_HIP(hipEventCreate(&e1));
+            // element kernels only (all colours), into the FP64 operators of the last assembly
+            const bool mg = c.mg_built;
+            STK_HIP(hipEventRecord(e0, c.stream));
+            for (int k = 0; k < reps; ++k)
+                assemble_impl<double>(c, flags, c.K64.p, c.G64.p, c.GT64.p, c.M64.p, c.F64.p,
+                                      mg ? c.mg[0].A64.p : nullptr, mg ? c.mg[0].E64.p : nullptr);
+            STK_HIP(hipEventRecord(e1, c.stream));
+            STK_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            STK_HIP(hipEventElapsedTime(&ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            *avg_ms = ms / reps;
+            // compulsory bytes: per-element inputs, per-node inputs, one write of every operator
+            const double in = (double)c.nel * (6 * 4 + 6 * 8 + 6 * 8 + 36 * 8 + 81) + (double)c.nnod * (8 + 16);
+            const double out = 8.0 * (4 * c.KN.nnz() + 4 * c.GN.nnz() + c.MN.nnz() + c.nU) +
+                                (mg ? 8.0 * (4 * c.mg[0].pat.nnz() + 36 * c.nel) : 0.0);
+            *bytes = in + out;
+            // the timed launches accumulated into the operators: re-assemble cleanly
+            STK_HIP(hipStreamSynchronize(c.stream));
+            return stk_assemble(ctx, flags);
+        }
+        *avg_ms = time_kernel(c, kernel, reps, bytes);
+        return (int)STK_OK;
+    });
+}
+
 int stk_updatevelocity(stk_ctx *ctx, double *U, double *Ucart, double *omega) {
     if (!ctx) return STK_E_ARG;
     return guarded(ctx, [&]() {

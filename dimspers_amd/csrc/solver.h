@@ -37,5 +37,6 @@ void setup_solver_data(Context &c);
 void build_coarse_inverse(Context &c, bool f64);
 int run_solve(Context &c, const stk_opts &o, stk_report &rep);
 double full_residual(Context &c, const double *U, const double *P, double *ru, double *hc, bool project_null);
+double time_kernel(Context &c, int kernel, int reps, double *bytes);
 
 }  // namespace stk

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
 #include <type_traits>
 
 #include "device_ops.h"
@@ -299,8 +300,9 @@ int64_t inner_cg(Context &c, MGView<T> &v, bool use_mg, T *y, T *z, double rtol,
     };
     red.map_reduce(n, [=] __device__(int64_t i) { return (double)r[i] * (double)r[i]; }, SC_BB);
     const double bb = red.fetch(SC_BB);
+    if (!std::isfinite(bb)) throw Error(STK_E_NAN, "non-finite right-hand side in the inner CG");
     if (!(bb > 0)) {
-        *converged = std::isfinite(bb);
+        *converged = true;
         finish_null();
         return 0;
     }
@@ -664,8 +666,9 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
     };
     *conv = false;
     *schur_res = 1.0;
+    if (!std::isfinite(r0)) throw Error(STK_E_NAN, "non-finite initial Schur residual");
     if (!(r0 > 0)) {
-        *conv = std::isfinite(r0);
+        *conv = true;
         *schur_res = 0.0;
         return 0;
     }
@@ -896,6 +899,15 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
         }
         rep.converged = rep.full_residual < o.rtol ? 1 : 0;
     }
+    if (project) {
+        // the solved space excludes the near-null direction: drop whatever component
+        // of it the FP32 corrections (orthogonal to the rounded vector) left in U
+        Reducer red(c);
+        const double *v = c.rot64.p;
+        double *U = c.U.p, *sc = c.d_sc.p;
+        red.map_reduce(nU, [=] __device__(int64_t i) { return v[i] * U[i]; }, SC_TMP);
+        launch_map(c, nU, [=] __device__(int64_t i) { U[i] -= sc[SC_TMP] * v[i]; });
+    }
     STK_HIP(hipEventRecord(e1, c.stream));
     STK_HIP(hipEventSynchronize(e1));
     float ms = 0;
@@ -1064,6 +1076,77 @@ double update_velocity(Context &c, double *Ucart_dev) {
     });
     STK_HIP(hipStreamSynchronize(c.stream));
     return omega;
+}
+
+}  // namespace stk
+
+// ------------------------------------------------------------------ kernel timing
+namespace stk {
+
+double time_kernel(Context &c, int kernel, int reps, double *bytes) {
+    STK_CHECK(reps > 0, STK_E_ARG, "reps must be positive");
+    hipEvent_t e0, e1;
+    STK_HIP(hipEventCreate(&e0));
+    STK_HIP(hipEventCreate(&e1));
+    const int64_t nU = c.nU, nb = c.KN.nnz(), nn = c.nnod;
+    auto spmv_bytes = [&](int sv) { return (double)nb * (4 * sv + 4) + (double)(nn + 1) * 4 + 2.0 * nU * sv; };
+    std::function<void()> launch;
+    DBuf<double> x64, y64, b64, d64, o64;
+    DBuf<float> x32, y32, b32, d32, o32;
+    switch (kernel) {
+        case STK_KERNEL_SPMV_K32: case STK_KERNEL_CHEB_K32: {
+            STK_CHECK(c.K32 && c.Kdinv32, STK_E_STATE, "FP32 operators not assembled");
+            x32.alloc(nU); y32.alloc(nU); b32.alloc(nU); d32.alloc(nU); o32.alloc(nU);
+            float *x = x32.p, *y = y32.p, *b = b32.p, *d = d32.p;
+            launch_map(c, nU, [=] __device__(int64_t i) { x[i] = 1.0f + 1e-3f * (float)(i % 17); b[i] = 1.0f; d[i] = 0.0f; y[i] = 0.0f; });
+            Op22<float> K{&c.KN, c.K32.p};
+            if (kernel == STK_KERNEL_SPMV_K32) {
+                launch = [&c, K, x, y]() { EpStore<float> es{y}; apply22<8>(c, K, x, es); };
+                *bytes = spmv_bytes(4);
+            } else {
+                float *o = o32.p;
+                const float *dinv = c.Kdinv32.p;
+                launch = [&c, K, x, b, d, o, dinv]() {
+                    EpCheb<float> ep{b, dinv, x, d, o, 0.5f, 0.25f};
+                    apply22<8>(c, K, x, ep);
+                };
+                *bytes = spmv_bytes(4) + 5.0 * nU * 4;
+            }
+            break;
+        }
+        case STK_KERNEL_SPMV_K64: case STK_KERNEL_CHEB_K64: {
+            STK_CHECK(c.K64 && c.Kdinv64, STK_E_STATE, "FP64 operators not assembled");
+            x64.alloc(nU); y64.alloc(nU); b64.alloc(nU); d64.alloc(nU); o64.alloc(nU);
+            double *x = x64.p, *y = y64.p, *b = b64.p, *d = d64.p;
+            launch_map(c, nU, [=] __device__(int64_t i) { x[i] = 1.0 + 1e-3 * (double)(i % 17); b[i] = 1.0; d[i] = 0.0; y[i] = 0.0; });
+            Op22<double> K{&c.KN, c.K64.p};
+            if (kernel == STK_KERNEL_SPMV_K64) {
+                launch = [&c, K, x, y]() { EpStore<double> es{y}; apply22<8>(c, K, x, es); };
+                *bytes = spmv_bytes(8);
+            } else {
+                double *o = o64.p;
+                const double *dinv = c.Kdinv64.p;
+                launch = [&c, K, x, b, d, o, dinv]() {
+                    EpCheb<double> ep{b, dinv, x, d, o, 0.5, 0.25};
+                    apply22<8>(c, K, x, ep);
+                };
+                *bytes = spmv_bytes(8) + 5.0 * nU * 8;
+            }
+            break;
+        }
+        default:
+            throw Error(STK_E_ARG, "unknown kernel id");
+    }
+    launch();   // warm-up
+    STK_HIP(hipEventRecord(e0, c.stream));
+    for (int k = 0; k < reps; ++k) launch();
+    STK_HIP(hipEventRecord(e1, c.stream));
+    STK_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    STK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms / reps;
 }
 
 }  // namespace stk

@@ -177,6 +177,22 @@ int stk_get_solution(stk_ctx *ctx, double *U, double *P);
  * Operates on the device solution; U (polar, nU) and Ucart (nU) may be NULL. */
 int stk_updatevelocity(stk_ctx *ctx, double *U, double *Ucart, double *omega);
 
+/* Kernels stk_time_kernel can time (on the assembled operators). */
+enum {
+    STK_KERNEL_SPMV_K32 = 0,    /* y = K x, FP32 BSR 2x2                                  */
+    STK_KERNEL_SPMV_K64 = 1,    /* y = K x, FP64 BSR 2x2                                  */
+    STK_KERNEL_CHEB_K32 = 2,    /* fused Chebyshev step x' = x + c1 d + c2 D^-1 (b - K x) */
+    STK_KERNEL_CHEB_K64 = 3,
+    STK_KERNEL_ASSEMBLY = 4     /* one full element assembly (all colours), FP64          */
+};
+
+/* Measurement hook with no counterpart in the reference: runs one of the hot
+ * kernels `reps` times on the context stream between HIP events and returns
+ * the mean duration of one launch (ms) and its algorithmic HBM bytes (values,
+ * indices and vectors each moved once).  The operands are the context's own
+ * assembled operators and solver work vectors, at their real sizes. */
+int stk_time_kernel(stk_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms, double *bytes);
+
 #ifdef __cplusplus
 }
 #endif
