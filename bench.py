@@ -50,7 +50,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", type=int, default=4)
-    ap.add_argument("--method", default="mixed", choices=["mixed", "learned"])
+    ap.add_argument("--method", default="learned", choices=["mixed", "learned"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
@@ -83,7 +83,25 @@ def main():
 
     method = _lib.SOLVE_LEARNED if args.method == "learned" else _lib.SOLVE_MIXED
     fast_opts = _lib.default_opts(method=method, stop=_lib.STOP_FULL, rtol=1e-8)
+    mixed_opts = _lib.default_opts(method=_lib.SOLVE_MIXED, stop=_lib.STOP_FULL, rtol=1e-8)
     plain_opts = _lib.default_opts(method=_lib.SOLVE_PLAIN, stop=_lib.STOP_FULL, rtol=1e-8, itmax=400)
+
+    train = None
+    setup_ms = 0.0
+    if method == _lib.SOLVE_LEARNED:
+        ctx.set_fabric(inp.phi, inp.delta)
+        trep = ctx.mlp_train(_lib.default_train_opts())
+        train = {"ms": round(trep.ms, 1), "samples": trep.samples, "steps": trep.steps,
+                 "initial_loss": trep.initial_loss, "final_loss": trep.final_loss,
+                 "validation_loss": trep.validation_loss}
+        log(f"MLP training on the GPU: {trep.ms:.0f} ms, {trep.samples} samples, {trep.steps} steps, "
+            f"loss {trep.initial_loss:.3f} -> {trep.final_loss:.4f} (validation {trep.validation_loss:.4f})")
+        ctx.learned_setup()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            ctx.learned_setup()
+        setup_ms = (time.perf_counter() - t0) * 1000.0 / 3
+        log(f"learned preconditioner set-up (features, MLP inference, assembly of S): {setup_ms:.2f} ms")
 
     for k in range(args.warmup):
         rep = ctx.solve(fast_opts)
@@ -97,9 +115,17 @@ def main():
         reps.append(rep)
         log(f"step {k}: {rep.solve_ms:.1f} ms, {rep.outer_iterations} outer, {rep.inner_iterations} inner, "
             f"{rep.refinements} sweeps, residual {rep.full_residual:.2e}")
-    wall = (time.perf_counter() - t_all) * 1000.0 / args.steps
+    wall = (time.perf_counter() - t_all) * 1000.0 / args.steps + setup_ms
     U_fast, P_fast = ctx.solution()
-    ms = float(np.mean(times))
+    ms = float(np.mean(times)) + setup_ms
+
+    mixed = None
+    if method == _lib.SOLVE_LEARNED:
+        mrep = ctx.solve(mixed_opts)
+        mixed = {"ms": round(mrep.solve_ms, 2), "outer": mrep.outer_iterations, "inner": int(mrep.inner_iterations),
+                 "residual": mrep.full_residual}
+        log(f"mixed without the MLP: {mrep.solve_ms:.1f} ms, {mrep.outer_iterations} outer, "
+            f"{mrep.inner_iterations} inner, residual {mrep.full_residual:.2e}")
 
     plain = None
     if not args.no_plain:
@@ -157,6 +183,9 @@ def main():
         "fallbacks": reps[-1].fallbacks,
         "full_residual": reps[-1].full_residual,
         "assembly_ms": round(asm_ms, 3),
+        "learned_setup_ms": round(setup_ms, 3),
+        "mlp_training": train,
+        "mixed_without_mlp": mixed,
         "plain_fp64": plain,
         "speedup_vs_plain": round(plain["ms"] / ms, 2) if plain else None,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",

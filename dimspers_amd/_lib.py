@@ -24,8 +24,10 @@ NULL_PROJECT, NULL_DEFLATE = 0, 1
 EXPORTED = [
     "stk_default_opts", "stk_create", "stk_destroy", "stk_last_error", "stk_info", "stk_set_fields",
     "stk_assemble", "stk_export_csc", "stk_export_rhs", "stk_spmv", "stk_solve", "stk_get_solution",
-    "stk_updatevelocity", "stk_time_kernel",
+    "stk_updatevelocity", "stk_time_kernel", "stk_default_train_opts", "stk_set_fabric", "stk_mlp_train",
+    "stk_mlp_params", "stk_mlp_forward", "stk_mlp_gradient", "stk_local_schur", "stk_learned_setup",
 ]
+N_PARAM = 9094
 KERNEL_SPMV_K32, KERNEL_SPMV_K64, KERNEL_CHEB_K32, KERNEL_CHEB_K64, KERNEL_ASSEMBLY = 0, 1, 2, 3, 4
 
 
@@ -44,6 +46,24 @@ class StkReport(C.Structure):
         ("converged", C.c_int32), ("outer_iterations", C.c_int32), ("inner_iterations", C.c_int64),
         ("refinements", C.c_int32), ("fallbacks", C.c_int32), ("schur_residual", C.c_double),
         ("full_residual", C.c_double), ("solve_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class StkTrainOpts(C.Structure):
+    _fields_ = [
+        ("snapshots", C.c_int32), ("per_snapshot", C.c_int32), ("epochs", C.c_int32), ("batch", C.c_int32),
+        ("ratio_kind", C.c_int32), ("reserved", C.c_int32), ("lr", C.c_double), ("lr_final", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+class StkTrainReport(C.Structure):
+    _fields_ = [
+        ("initial_loss", C.c_double), ("final_loss", C.c_double), ("validation_loss", C.c_double),
+        ("ms", C.c_double), ("samples", C.c_int64), ("steps", C.c_int32), ("reserved", C.c_int32),
     ]
 
     def as_dict(self):
@@ -89,8 +109,18 @@ def load(path: Path | str | None = None):
     lib.stk_get_solution.argtypes = [vp, dp, dp]
     lib.stk_updatevelocity.argtypes = [vp, dp, dp, dp]
     lib.stk_time_kernel.argtypes = [vp, C.c_int32, C.c_int32, dp, dp]
+    fp = C.POINTER(C.c_float)
+    lib.stk_default_train_opts.argtypes = [C.POINTER(StkTrainOpts)]
+    lib.stk_default_train_opts.restype = None
+    lib.stk_set_fabric.argtypes = [vp, dp, dp]
+    lib.stk_mlp_train.argtypes = [vp, C.POINTER(StkTrainOpts), C.POINTER(StkTrainReport)]
+    lib.stk_mlp_params.argtypes = [vp, fp, fp, C.c_int32]
+    lib.stk_mlp_forward.argtypes = [vp, C.c_int64, fp, fp]
+    lib.stk_mlp_gradient.argtypes = [vp, C.c_int64, fp, fp, fp, dp]
+    lib.stk_local_schur.argtypes = [vp, dp, fp]
+    lib.stk_learned_setup.argtypes = [vp]
     for name in EXPORTED:
-        if name not in ("stk_default_opts", "stk_destroy", "stk_last_error"):
+        if name not in ("stk_default_opts", "stk_destroy", "stk_last_error", "stk_default_train_opts"):
             getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -102,6 +132,20 @@ def _d(a: np.ndarray):
 
 def _i(a: np.ndarray):
     return a.ctypes.data_as(i64p)
+
+
+def _f(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def default_train_opts(**kw) -> StkTrainOpts:
+    o = StkTrainOpts()
+    load().stk_default_train_opts(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(k)
+        setattr(o, k, v)
+    return o
 
 
 def default_opts(**kw) -> StkOpts:
@@ -218,6 +262,52 @@ class Context:
         ms, by = C.c_double(), C.c_double()
         self._check(self.lib.stk_time_kernel(self.h, kernel, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+    # ---- learned local-Schur preconditioner
+    def set_fabric(self, phi, delta):
+        self._fab = [np.ascontiguousarray(phi, dtype=np.float64), np.ascontiguousarray(delta, dtype=np.float64)]
+        self._check(self.lib.stk_set_fabric(self.h, _d(self._fab[0]), _d(self._fab[1])))
+
+    def mlp_train(self, opts: StkTrainOpts | None = None):
+        rep = StkTrainReport()
+        self._check(self.lib.stk_mlp_train(self.h, C.byref(opts if opts is not None else default_train_opts()),
+                                           C.byref(rep)))
+        return rep
+
+    def mlp_params(self, params=None, norm=None):
+        """Get (no arguments) or set the FP32 parameters and normalisation."""
+        if params is None:
+            p = np.zeros(N_PARAM, dtype=np.float32)
+            nm = np.zeros(22, dtype=np.float32)
+            self._check(self.lib.stk_mlp_params(self.h, _f(p), _f(nm), 0))
+            return p, nm
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        nm = np.ascontiguousarray(norm, dtype=np.float32)
+        self._check(self.lib.stk_mlp_params(self.h, _f(p), _f(nm), 1))
+        return p, nm
+
+    def mlp_forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], 6), dtype=np.float32)
+        self._check(self.lib.stk_mlp_forward(self.h, X.shape[0], _f(X), _f(Y)))
+        return Y
+
+    def mlp_gradient(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(N_PARAM, dtype=np.float32)
+        loss = C.c_double()
+        self._check(self.lib.stk_mlp_gradient(self.h, X.shape[0], _f(X), _f(T), _f(g), C.byref(loss)))
+        return loss.value, g
+
+    def local_schur(self):
+        S = np.zeros((self.nel, 6))
+        X = np.zeros((self.nel, 5), dtype=np.float32)
+        self._check(self.lib.stk_local_schur(self.h, _d(S), _f(X)))
+        return S, X
+
+    def learned_setup(self):
+        self._check(self.lib.stk_learned_setup(self.h))
 
     def updatevelocity(self):
         U = np.zeros(self.nU)

@@ -4,6 +4,7 @@
 #include <numeric>
 
 #include "device_ops.h"
+#include "mlp.h"
 #include "solver.h"
 
 namespace stk {
@@ -14,6 +15,7 @@ void host_spmv(Context &c, int which, int prec, const double *xh, double *yh);
 double update_velocity(Context &c, double *Ucart_dev);
 
 Context::~Context() {
+    delete mlp;
     if (h_pinned) (void)hipHostFree(h_pinned);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -431,6 +433,128 @@ int stk_time_kernel(stk_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms, 
             return stk_assemble(ctx, flags);
         }
         *avg_ms = time_kernel(c, kernel, reps, bytes);
+        return (int)STK_OK;
+    });
+}
+
+void stk_default_train_opts(stk_train_opts *o) {
+    o->snapshots = 2000;
+    o->per_snapshot = 512;
+    o->epochs = 4;
+    o->batch = 4096;
+    o->ratio_kind = 1;
+    o->reserved = 0;
+    o->lr = 3e-3;
+    o->lr_final = 1e-4;
+    o->seed = 0x5EED2000ull;
+}
+
+int stk_set_fabric(stk_ctx *ctx, const double *phi, const double *delta) {
+    if (!ctx || !phi || !delta) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        for (int64_t e = 0; e < c.nel; ++e)
+            STK_CHECK(std::isfinite(phi[e]) && delta[e] > 0, STK_E_ARG, "bad fabric value");
+        c.d_phi.upload(phi, c.nel, c.stream);
+        c.d_delta.upload(delta, c.nel, c.stream);
+        STK_HIP(hipStreamSynchronize(c.stream));
+        c.have_fabric = true;
+        return (int)STK_OK;
+    });
+}
+
+int stk_mlp_train(stk_ctx *ctx, const stk_train_opts *opts, stk_train_report *report) {
+    if (!ctx) return STK_E_ARG;
+    stk_train_report local{};
+    stk_train_report &rep = report ? *report : local;
+    std::memset(&rep, 0, sizeof(rep));
+    return guarded(ctx, [&]() {
+        stk_train_opts o;
+        stk_default_train_opts(&o);
+        if (opts) o = *opts;
+        mlp_train(ctx->c, o, rep);
+        return (int)STK_OK;
+    });
+}
+
+int stk_mlp_params(stk_ctx *ctx, float *params, float *norm, int32_t set) {
+    if (!ctx || !params || !norm) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        if (!c.mlp) {
+            STK_CHECK(set, STK_E_STATE, "no model");
+            c.mlp = new MLP();
+            c.mlp->init(1, c.stream);
+        }
+        MLP &net = *c.mlp;
+        if (set) {
+            net.w.upload(params, mlp::kNParam, c.stream);
+            std::memcpy(net.norm, norm, sizeof(net.norm));
+            net.dnorm.upload(net.norm, 22, c.stream);
+            net.pack(c.stream);
+            net.trained = true;
+        } else {
+            net.w.download(params, mlp::kNParam, c.stream);
+            std::memcpy(norm, net.norm, sizeof(net.norm));
+        }
+        STK_HIP(hipStreamSynchronize(c.stream));
+        return (int)STK_OK;
+    });
+}
+
+int stk_mlp_forward(stk_ctx *ctx, int64_t n, const float *X, float *Y) {
+    if (!ctx || !X || !Y || n <= 0) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK(c.mlp, STK_E_STATE, "no model");
+        DBuf<float> x(5 * n), y(6 * n);
+        x.upload(X, 5 * n, c.stream);
+        c.mlp->forward(x.p, n, y.p, true, c.stream);
+        y.download(Y, 6 * n, c.stream);
+        STK_HIP(hipStreamSynchronize(c.stream));
+        return (int)STK_OK;
+    });
+}
+
+int stk_mlp_gradient(stk_ctx *ctx, int64_t n, const float *X, const float *T, float *grad, double *loss) {
+    if (!ctx || !X || !T || !grad || n <= 0) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK(c.mlp, STK_E_STATE, "no model");
+        DBuf<float> x(5 * n), t(6 * n);
+        DBuf<double> lb;
+        x.upload(X, 5 * n, c.stream);
+        t.upload(T, 6 * n, c.stream);
+        const double l = c.mlp->gradient(x.p, t.p, n, c.stream, lb);
+        if (loss) *loss = l;
+        c.mlp->grad.download(grad, mlp::kNParam, c.stream);
+        STK_HIP(hipStreamSynchronize(c.stream));
+        return (int)STK_OK;
+    });
+}
+
+int stk_local_schur(stk_ctx *ctx, double *S, float *X) {
+    if (!ctx) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK(c.have_fields && c.have_fabric, STK_E_STATE, "fields and fabric needed");
+        STK_CHECK(c.structured, STK_E_NOMESH, "the learned model needs the structured annulus");
+        DBuf<double> s(6 * c.nel);
+        DBuf<float> x(5 * c.nel);
+        SchurSampleHost h;
+        h.nsamp = c.nel; h.mode = 0; h.X = x.p; h.S = s.p;
+        launch_schur_samples(c, h);
+        if (S) s.download(S, 6 * c.nel, c.stream);
+        if (X) x.download(X, 5 * c.nel, c.stream);
+        STK_HIP(hipStreamSynchronize(c.stream));
+        return (int)STK_OK;
+    });
+}
+
+int stk_learned_setup(stk_ctx *ctx) {
+    if (!ctx) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        learned_setup(ctx->c);
         return (int)STK_OK;
     });
 }

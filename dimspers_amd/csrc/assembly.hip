@@ -86,33 +86,19 @@ struct AsmArgs {
     int cart, bc, iso;
 };
 
-template <typename T>
-__global__ __launch_bounds__(kAsmThreads) void k_assemble(AsmArgs<T> a) {
-    __shared__ T sK[84][kAsmThreads];
-    __shared__ T sM[6][kAsmThreads];
-    T sG[36], sF[12];
-    const int tid = threadIdx.x;
-    const int64_t j = (int64_t)blockIdx.x * kAsmThreads + tid;
-    if (j >= a.cnt) return;   // no block-level synchronisation below: early exit is safe
-    const int64_t p = a.off + j;
-    const int64_t nel = a.nel;
-    const int64_t e = a.perm[p];
-
-    int32_t n[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) n[k] = a.e2n[k * nel + p];
-    T th[3], rr[3], rhov[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        th[k] = (T)a.th[k * nel + p];
-        rr[k] = (T)a.r[k * nel + p];
-        rhov[k] = (T)a.rho[n[k]];
-    }
+// Integrates one element with the 6-point rule (StokesSolver.jl:543-845): the lower
+// node-pair 2x2 blocks of K_e into sK[.][tid] (LDS), M_e (upper triangle, 6) into
+// sM[.][tid] (LDS), G_e (node a, pressure i, component) and f_e into registers.
+// eta_at(q) gives eta.ip at integration point q, tensor_at(q, d) the six viscous
+// tensor components (eta11, eta33, eta55, eta13, eta15, eta35) there.
+template <typename T, typename EtaF, typename TensF>
+__device__ __forceinline__ void integrate_element(const T th[3], const T rr[3], const T rhov[3], T g, bool iso,
+                                                  EtaF eta_at, TensF tensor_at, T (*sK)[kAsmThreads],
+                                                  T (*sM)[kAsmThreads], int tid, T sG[36], T sF[12]) {
     // _fill_R_J!
     const T R21 = rr[1] - rr[0], R31 = rr[2] - rr[0];
     const T Th21 = th[1] - th[0], Th31 = th[2] - th[0];
     const T detJ = Th21 * R31 - Th31 * R21;
-
     for (int k = 0; k < 84; ++k) sK[k][tid] = T(0);
 #pragma unroll
     for (int k = 0; k < 36; ++k) sG[k] = T(0);
@@ -142,16 +128,14 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(AsmArgs<T> a) {
             dx[b] = jx0 * dr[b] + jx1 * ds[b];
             dz[b] = jz0 * dr[b] + jz1 * ds[b];
         }
-        const T eta = (T)a.eta[(int64_t)q * nel + e];
+        const T eta = eta_at(q);
         T d11, d22, d33, d12, d13, d23;
-        if (a.iso) {
+        if (iso) {
             d11 = d22 = T(4) / T(3); d12 = -T(2) / T(3); d33 = T(1); d13 = d23 = T(0);
         } else {
-            // StiffnessTensor order eta11, eta33, eta55, eta13, eta15, eta35
-            const double *C = a.C + (int64_t)q * nel + e;
-            const int64_t cstr = 6 * nel;
-            d11 = (T)C[0 * cstr]; d22 = (T)C[1 * cstr]; d33 = (T)C[2 * cstr];
-            d12 = (T)C[3 * cstr]; d13 = (T)C[4 * cstr]; d23 = (T)C[5 * cstr];
+            T d[6];
+            tensor_at(q, d);
+            d11 = d[0]; d22 = d[1]; d33 = d[2]; d12 = d[3]; d13 = d[4]; d23 = d[5];
         }
         const T sc = om * eta;
         d11 *= sc; d22 *= sc; d33 *= sc; d12 *= sc; d13 *= sc; d23 *= sc;
@@ -187,13 +171,48 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(AsmArgs<T> a) {
         sM[0][tid] += wm * NP[0] * NP[0]; sM[1][tid] += wm * NP[0] * NP[1]; sM[2][tid] += wm * NP[0] * NP[2];
         sM[3][tid] += wm * NP[1] * NP[1]; sM[4][tid] += wm * NP[1] * NP[2]; sM[5][tid] += wm * NP[2] * NP[2];
         const T rhoq = NP[0] * rhov[0] + NP[1] * rhov[1] + NP[2] * rhov[2];
-        const T cte = om * (T)a.g * rhoq;
+        const T cte = om * g * rhoq;
 #pragma unroll
         for (int aa = 0; aa < 6; ++aa) {
             sF[2 * aa + 0] += sn * cte * N[aa];
             sF[2 * aa + 1] += cs * cte * N[aa];
         }
     }
+
+}
+
+template <typename T>
+__global__ __launch_bounds__(kAsmThreads) void k_assemble(AsmArgs<T> a) {
+    __shared__ T sK[84][kAsmThreads];
+    __shared__ T sM[6][kAsmThreads];
+    T sG[36], sF[12];
+    const int tid = threadIdx.x;
+    const int64_t j = (int64_t)blockIdx.x * kAsmThreads + tid;
+    if (j >= a.cnt) return;   // no block-level synchronisation below: early exit is safe
+    const int64_t p = a.off + j;
+    const int64_t nel = a.nel;
+    const int64_t e = a.perm[p];
+
+    int32_t n[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) n[k] = a.e2n[k * nel + p];
+    T th[3], rr[3], rhov[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        th[k] = (T)a.th[k * nel + p];
+        rr[k] = (T)a.r[k * nel + p];
+        rhov[k] = (T)a.rho[n[k]];
+    }
+    // _fill_R_J! geometry is recomputed inside integrate_element
+    auto eta_at = [&](int q) { return (T)a.eta[(int64_t)q * nel + e]; };
+    auto tensor_at = [&](int q, T d[6]) {
+        // StiffnessTensor order eta11, eta33, eta55, eta13, eta15, eta35
+        const double *C = a.C + (int64_t)q * nel + e;
+        const int64_t cstr = 6 * nel;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) d[k] = (T)C[k * cstr];
+    };
+    integrate_element<T>(th, rr, rhov, (T)a.g, a.iso != 0, eta_at, tensor_at, sK, sM, tid, sG, sF);
 
     // rotate to the node frames: blk(a,b) <- Q_a blk Q_b^T, Q = [[c, -s], [s, c]]
     if (!a.cart) {
@@ -382,6 +401,244 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(AsmArgs<T> a) {
         }
 }
 
+// ------------------------------------------------------------------------ local Schur blocks
+// Scalar (i >= j) entry of the 12x12 element matrix held as lower node-pair blocks.
+__device__ __forceinline__ int kidx(int i, int j) { return lblk(i >> 1, j >> 1) * 4 + (i & 1) * 2 + (j & 1); }
+
+// Exact local Schur block of one element, frame independent:
+//   S_e = G_e^T (K_e + mu N N^T)^-1 G_e,
+// N = orthonormal Cartesian rigid motions of the six nodes (two translations and the
+// rotation about the node centroid), mu = trace(K_e) / 12.  K_e vanishes on these
+// motions up to the curvature of the polar map and G_e^T annihilates them (rigid
+// motions are divergence free), so S_e is the pressure Schur complement of the
+// element on its deformation modes.  Uses sK (destroyed) and the G_e registers.
+template <typename T>
+__device__ void local_schur(T (*sK)[kAsmThreads], int tid, const T sG[36], const T xn[6], const T zn[6], T S[3][3]) {
+    T nv[3][12];
+    T xc = 0, zc = 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) { xc += xn[a]; zc += zn[a]; }
+    xc /= T(6); zc /= T(6);
+    T nr = 0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+        nv[0][2 * a] = T(1); nv[0][2 * a + 1] = T(0);
+        nv[1][2 * a] = T(0); nv[1][2 * a + 1] = T(1);
+        nv[2][2 * a] = zn[a] - zc; nv[2][2 * a + 1] = -(xn[a] - xc);
+        nr += (zn[a] - zc) * (zn[a] - zc) + (xn[a] - xc) * (xn[a] - xc);
+    }
+    const T s6 = T(1) / sqrt(T(6)), sr = T(1) / sqrt(nr);
+#pragma unroll
+    for (int i = 0; i < 12; ++i) { nv[0][i] *= s6; nv[1][i] *= s6; nv[2][i] *= sr; }
+    T tr = 0;
+    for (int i = 0; i < 12; ++i) tr += sK[kidx(i, i)][tid];
+    const T mu = tr / T(12);
+    for (int i = 0; i < 12; ++i)
+        for (int j = 0; j <= i; ++j)
+            sK[kidx(i, j)][tid] += mu * (nv[0][i] * nv[0][j] + nv[1][i] * nv[1][j] + nv[2][i] * nv[2][j]);
+    // Cholesky in place
+    for (int j = 0; j < 12; ++j) {
+        T d = sK[kidx(j, j)][tid];
+        for (int k = 0; k < j; ++k) { const T l = sK[kidx(j, k)][tid]; d -= l * l; }
+        const T ljj = sqrt(d);
+        sK[kidx(j, j)][tid] = ljj;
+        const T inv = T(1) / ljj;
+        for (int i = j + 1; i < 12; ++i) {
+            T v = sK[kidx(i, j)][tid];
+            for (int k = 0; k < j; ++k) v -= sK[kidx(i, k)][tid] * sK[kidx(j, k)][tid];
+            sK[kidx(i, j)][tid] = v * inv;
+        }
+    }
+    // X = Kr^-1 G (three columns), S = G^T X
+    for (int c = 0; c < 3; ++c) {
+        T x[12];
+        for (int i = 0; i < 12; ++i) {
+            T v = sG[((i >> 1) * 3 + c) * 2 + (i & 1)];
+            for (int k = 0; k < i; ++k) v -= sK[kidx(i, k)][tid] * x[k];
+            x[i] = v / sK[kidx(i, i)][tid];
+        }
+        for (int i = 11; i >= 0; --i) {
+            T v = x[i];
+            for (int k = i + 1; k < 12; ++k) v -= sK[kidx(k, i)][tid] * x[k];
+            x[i] = v / sK[kidx(i, i)][tid];
+        }
+        for (int c2 = 0; c2 < 3; ++c2) {
+            T s = 0;
+            for (int i = 0; i < 12; ++i) s += sG[((i >> 1) * 3 + c2) * 2 + (i & 1)] * x[i];
+            S[c2][c] = s;
+        }
+    }
+}
+
+// directRotation2D (src/Rheology/anisotropy.jl:503-536) of the fabric-frame tensor
+// a = c = 4/3, h = -2/3, e = 1/delta with rotmatrix(cos phi, sin phi) (:287-292).
+__device__ __forceinline__ void fabric_tensor(double phi, double delta, double d[6]) {
+    const double x1 = cos(phi), x2 = sin(phi), y1 = -x2, y2 = x1;
+    const double a = 4.0 / 3.0, c = 4.0 / 3.0, h = -2.0 / 3.0, e = 1.0 / delta;
+    const double x1x1 = x1 * x1, x2x2 = x2 * x2, y1y1 = y1 * y1, y2y2 = y2 * y2, s = x1 * y2 + x2 * y1;
+    d[0] = (c * x2x2 + h * x1x1 + 4.0 * e * x1x1) * x2x2 + (a * x1x1 + h * x2x2) * x1x1;
+    d[1] = (c * y2y2 + h * y1y1 + 4.0 * e * y1y1) * y2y2 + (a * y1y1 + h * y2y2) * y1y1;
+    d[2] = (c * x2 * y2 + h * x1 * y1) * x2 * y2 + s * s * e + (a * x1 * y1 + h * x2 * y2) * x1 * y1;
+    d[3] = ((c * x2x2 + h * x1x1) * y2 + 4.0 * e * x1 * x2 * y1) * y2 + (a * x1x1 + h * x2x2) * y1y1;
+    d[4] = ((c * x2x2 + h * x1x1) * y2 + 2.0 * s * e * x1) * x2 + (a * x1x1 + h * x2x2) * x1 * y1;
+    d[5] = ((c * y2y2 + h * y1y1) * x2 + 2.0 * s * e * y1) * y2 + (a * y1y1 + h * y2y2) * x1 * y1;
+}
+
+__device__ __forceinline__ double splitmix_u(uint64_t seed, uint64_t k) {
+    uint64_t z = seed + (k + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * 0x1.0p-53;
+}
+
+struct SchurSampleArgs {
+    int64_t nel, nsamp, per_snap, first_snap;
+    int mode;                    // 0: the current fields of every element; 1: seeded training draws
+    int ratio_kind;              // 0: delta = 10, 1: U[1, 100], 2: log-uniform [1, 1e3]
+    double mean_area;
+    const int32_t *iperm;        // element -> colour position
+    const double *th, *r;        // 3 x nel, by position
+    const double *eta;           // 6 x nel by element (ip-major)
+    const double *phi, *delta;   // by element (mode 0)
+    float *X;                    // 5 x nsamp features (sample-major)
+    float *Tg;                   // 6 x nsamp log-Cholesky targets (mode 1, may be null)
+    double *S;                   // 6 x nsamp canonical S_e / s_e (may be null)
+    double *scale;               // s_e = area / eta_bar (may be null)
+};
+
+// Features (and, for training, exact targets) of the learned local-Schur model:
+// mean log eta, cos / sin 2(phi - theta_c), log delta and the element area relative
+// to the mean area of its cell (the two triangles of a cell differ by the curvature
+// of the polar map, which is the largest variation of the normalised blocks).
+// Canonical vertex order: right-angle vertex, its azimuthal neighbour, its radial
+// neighbour -- (0, 1, 2) for the first triangle of a cell, (1, 2, 0) for the second.
+__global__ __launch_bounds__(kAsmThreads) void k_schur_samples(SchurSampleArgs a) {
+    __shared__ double sK[84][kAsmThreads];
+    __shared__ double sM[6][kAsmThreads];
+    const int tid = threadIdx.x;
+    const int64_t sidx = (int64_t)blockIdx.x * kAsmThreads + tid;
+    if (sidx >= a.nsamp) return;
+    int64_t e;
+    double fscale = 1.0, phi, delta;
+    if (a.mode == 0) {
+        e = sidx;
+        phi = a.phi[e];
+        delta = a.delta[e];
+    } else {
+        const int64_t snap = a.first_snap + sidx / a.per_snap, k = sidx % a.per_snap;
+        const uint64_t seed = 0x5EED1000ull + (uint64_t)snap;
+        e = (int64_t)(splitmix_u(seed, 8 * k) * a.nel);
+        if (e >= a.nel) e = a.nel - 1;
+        fscale = exp((2.0 * splitmix_u(seed, 8 * k + 1) - 1.0) * 2.302585092994046);
+        phi = M_PI * splitmix_u(seed, 8 * k + 2);
+        const double u = splitmix_u(seed, 8 * k + 3);
+        delta = a.ratio_kind == 0 ? 10.0 : a.ratio_kind == 1 ? 1.0 + 99.0 * u : pow(10.0, 3.0 * u);
+    }
+    const int64_t pos = a.iperm[e];
+    double th[3], rr[3], rhov[3] = {0, 0, 0};
+    for (int k = 0; k < 3; ++k) { th[k] = a.th[k * a.nel + pos]; rr[k] = a.r[k * a.nel + pos]; }
+    double tens[6];
+    fabric_tensor(phi, delta, tens);
+    double logeta = 0;
+    for (int q = 0; q < 6; ++q) logeta += log(a.eta[(int64_t)q * a.nel + e] * fscale);
+    logeta /= 6.0;
+    auto eta_at = [&](int q) { return a.eta[(int64_t)q * a.nel + e] * fscale; };
+    auto tensor_at = [&](int, double d[6]) { for (int k = 0; k < 6; ++k) d[k] = tens[k]; };
+    double sG[36], sF[12];
+    integrate_element<double>(th, rr, rhov, 0.0, false, eta_at, tensor_at, sK, sM, tid, sG, sF);
+    const double detJ = (th[1] - th[0]) * (rr[2] - rr[0]) - (th[2] - th[0]) * (rr[1] - rr[0]);
+    const double area = detJ * (rr[0] + rr[1] + rr[2]) / 6.0;
+    const double thc = (th[0] + th[1] + th[2]) / 3.0;
+    // area relative to the mean of the two triangles of its cell (elements e and e ^ 1)
+    double parea;
+    {
+        const int64_t pp = a.iperm[e ^ 1];
+        double pt[3], pr[3];
+        for (int k = 0; k < 3; ++k) { pt[k] = a.th[k * a.nel + pp]; pr[k] = a.r[k * a.nel + pp]; }
+        parea = ((pt[1] - pt[0]) * (pr[2] - pr[0]) - (pt[2] - pt[0]) * (pr[1] - pr[0])) * (pr[0] + pr[1] + pr[2]) / 6.0;
+    }
+    float *X = a.X + 5 * sidx;
+    X[0] = (float)logeta;
+    X[1] = (float)cos(2.0 * (phi - thc));
+    X[2] = (float)sin(2.0 * (phi - thc));
+    X[3] = (float)log(delta);
+    X[4] = (float)(area / (0.5 * (area + parea)));
+    const double se = area / exp(logeta);
+    if (a.scale) a.scale[sidx] = se;
+    if (!a.Tg && !a.S) return;
+    double xn[6], zn[6];
+    const int ea[3] = {0, 1, 2}, eb[3] = {1, 2, 0};
+    for (int k = 0; k < 3; ++k) { xn[k] = rr[k] * sin(th[k]); zn[k] = rr[k] * cos(th[k]); }
+    for (int k = 0; k < 3; ++k) {
+        const double tm = 0.5 * (th[ea[k]] + th[eb[k]]), rm = 0.5 * (rr[ea[k]] + rr[eb[k]]);
+        xn[3 + k] = rm * sin(tm); zn[3 + k] = rm * cos(tm);
+    }
+    double Sl[3][3];
+    local_schur<double>(sK, tid, sG, xn, zn, Sl);
+    const int pm[2][3] = {{0, 1, 2}, {1, 2, 0}};
+    const int *pp = pm[e & 1];
+    double Sc[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Sc[i][j] = Sl[pp[i]][pp[j]] / se;
+    if (a.S) {
+        double *S = a.S + 6 * sidx;
+        S[0] = Sc[0][0]; S[1] = Sc[1][0]; S[2] = Sc[1][1]; S[3] = Sc[2][0]; S[4] = Sc[2][1]; S[5] = Sc[2][2];
+    }
+    if (a.Tg) {
+        // log-Cholesky parameters: (log L00, L10, log L11, L20, L21, log L22)
+        const double l00 = sqrt(Sc[0][0]);
+        const double l10 = Sc[1][0] / l00, l20 = Sc[2][0] / l00;
+        const double l11 = sqrt(Sc[1][1] - l10 * l10);
+        const double l21 = (Sc[2][1] - l20 * l10) / l11;
+        const double l22 = sqrt(Sc[2][2] - l20 * l20 - l21 * l21);
+        float *Tg = a.Tg + 6 * sidx;
+        Tg[0] = (float)log(l00); Tg[1] = (float)l10; Tg[2] = (float)log(l11);
+        Tg[3] = (float)l20; Tg[4] = (float)l21; Tg[5] = (float)log(l22);
+    }
+}
+
+// Predicted element blocks -> assembled S (pattern of M), colour by colour.
+struct LearnedScatterArgs {
+    int64_t nel, off, cnt;
+    const int32_t *perm, *e2n;      // e2n by position
+    const uint8_t *sM;
+    const int32_t *Mptr;
+    const float *Y;                 // 6 x nel log-Cholesky parameters (denormalised), element order
+    const double *scale;            // s_e
+    double *S64;
+};
+
+__global__ void k_learned_scatter(LearnedScatterArgs a) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.cnt) return;
+    const int64_t p = a.off + j, nel = a.nel;
+    const int64_t e = a.perm[p];
+    const float *y = a.Y + 6 * e;
+    const double l00 = exp((double)y[0]), l10 = y[1], l11 = exp((double)y[2]);
+    const double l20 = y[3], l21 = y[4], l22 = exp((double)y[5]);
+    const double se = a.scale[e];
+    double Sc[3][3];
+    Sc[0][0] = l00 * l00;
+    Sc[1][0] = Sc[0][1] = l10 * l00;
+    Sc[1][1] = l10 * l10 + l11 * l11;
+    Sc[2][0] = Sc[0][2] = l20 * l00;
+    Sc[2][1] = Sc[1][2] = l20 * l10 + l21 * l11;
+    Sc[2][2] = l20 * l20 + l21 * l21 + l22 * l22;
+    const int pm[2][3] = {{0, 1, 2}, {1, 2, 0}};
+    const int *pp = pm[e & 1];
+    double Sl[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) Sl[pp[i]][pp[k]] = Sc[i][k] * se;
+    int32_t n[3];
+    for (int k = 0; k < 3; ++k) n[k] = a.e2n[k * nel + p];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const int64_t bm = (int64_t)a.Mptr[n[i]] + a.sM[(i * 3 + k) * nel + p];
+            a.S64[bm] += Sl[i][k];
+        }
+}
+
 // ------------------------------------------------------------------------ coarse Galerkin
 // Coarse P1 element (I, J, t) of level l+1 is the union of four level-l elements.
 // Child table: (di, dj, type) of the fine cells (2I+di, 2J+dj).
@@ -515,6 +772,31 @@ __global__ void k_fix_diag(int64_t nfix, const int64_t *__restrict__ dofs, const
 }
 
 }  // namespace
+
+void launch_schur_samples(Context &c, const SchurSampleHost &h) {
+    SchurSampleArgs a{};
+    a.nel = c.nel; a.nsamp = h.nsamp; a.per_snap = h.per_snap; a.first_snap = h.first_snap;
+    a.mode = h.mode; a.ratio_kind = h.ratio_kind; a.mean_area = c.mean_area;
+    a.iperm = c.d_iperm.p; a.th = c.d_elth.p; a.r = c.d_elr.p; a.eta = c.d_eta.p;
+    a.phi = c.d_phi.p; a.delta = c.d_delta.p;
+    a.X = h.X; a.Tg = h.Tg; a.S = h.S; a.scale = h.scale;
+    const int nb = (int)((h.nsamp + kAsmThreads - 1) / kAsmThreads);
+    k_schur_samples<<<dim3(nb), dim3(kAsmThreads), 0, c.stream>>>(a);
+    STK_HIP(hipGetLastError());
+}
+
+void launch_learned_scatter(Context &c, const float *Y, const double *scale, double *S64) {
+    LearnedScatterArgs a{};
+    a.nel = c.nel; a.perm = c.d_perm.p; a.e2n = c.d_e2n.p; a.sM = c.d_sM.p; a.Mptr = c.MN.dptr.p;
+    a.Y = Y; a.scale = scale; a.S64 = S64;
+    for (size_t k = 0; k + 1 < c.col_off.size(); ++k) {
+        a.off = c.col_off[k];
+        a.cnt = c.col_off[k + 1] - c.col_off[k];
+        const int nb = (int)((a.cnt + 255) / 256);
+        k_learned_scatter<<<dim3(nb), dim3(256), 0, c.stream>>>(a);
+        STK_HIP(hipGetLastError());
+    }
+}
 
 void upload_quadrature(hipStream_t s) {
     static bool done = false;

@@ -140,6 +140,15 @@ void build_slot_maps(Context &c) {
     }
     std::vector<int32_t> perm32(c.perm.begin(), c.perm.end());
     c.d_perm.upload(perm32, c.stream);
+    std::vector<int32_t> iperm(nel);
+    double area = 0;
+    for (int64_t p = 0; p < nel; ++p) iperm[c.perm[p]] = (int32_t)p;
+    for (int64_t e = 0; e < nel; ++e) {
+        const double *t = &c.el_th[3 * e], *r = &c.el_r[3 * e];
+        area += ((t[1] - t[0]) * (r[2] - r[0]) - (t[2] - t[0]) * (r[1] - r[0])) * (r[0] + r[1] + r[2]) / 6.0;
+    }
+    c.mean_area = area / (double)nel;
+    c.d_iperm.upload(iperm, c.stream);
     c.d_e2n.upload(e2n_p, c.stream);
     c.d_elth.upload(th_p, c.stream);
     c.d_elr.upload(r_p, c.stream);

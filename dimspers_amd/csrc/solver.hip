@@ -60,6 +60,7 @@ template struct DBuf<float>;
 template struct DBuf<int32_t>;
 template struct DBuf<uint8_t>;
 template struct DBuf<int64_t>;
+template struct DBuf<uint16_t>;
 
 // ------------------------------------------------------------------ launch helpers
 Reducer::Reducer(Context &c) : c(c) {}
@@ -626,7 +627,7 @@ double vdot(Context &c, const T *a, const T *b, int64_t n) {
 template <typename T>
 int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *h, SchurWork<T> &w,
               double rtol, int itmax, int itmin, double inner_rtol, int64_t *inner_its, double *schur_res,
-              bool *conv, const T *pcvec, bool learned, bool deflate = false) {
+              bool *conv, const T *pcvec, bool learned, bool deflate = false, bool *stalled = nullptr) {
     const bool f64 = sizeof(T) == 8;
     MGView<T> v = make_view<T>(c);
     const int64_t nU = c.nU, nP = c.nP;
@@ -675,9 +676,14 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
     apply_pc(w.r, w.d);
     launch_map(c, nP, [=] __device__(int64_t i) { w.q[i] = w.d[i]; });
     int it = 0;
+    if (stalled) *stalled = false;
     for (it = 1; it <= itmax; ++it) {
         vmean_remove<T>(c, w.q, nP);
         const double rd = vdot<T>(c, w.r, w.d, nP);
+        if (stalled && !(rd > 0)) {   // the learned preconditioner is not positive definite here
+            *stalled = true;
+            return it - 1;
+        }
         // y = G q ; z = K^-1 y ; Sq = G^T z
         EpStore<T> ey{w.y};
         spmv<2, 1, 8, T>(c, c.GN, G, w.q, ey, nullptr);
@@ -702,6 +708,11 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
                          (long long)*inner_its);
         if (!std::isfinite(rn)) throw Error(STK_E_NAN, "non-finite residual in the Schur PCG");
         if (rn < tol && it >= itmin) { *conv = true; break; }
+        // learned preconditioner stall: less than a tenfold reduction after 30 iterations
+        if (stalled && it == 30 && rn > 0.1 * r0) {
+            *stalled = true;
+            break;
+        }
         apply_pc(w.r, w.d);
         Reducer red(c);
         red.map_reduce(nP, [=] __device__(int64_t i) {
@@ -865,11 +876,18 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
                 launch_map(c, nU, [=] __device__(int64_t i) { a32[i] = (float)(a[i] * scale); });
                 launch_map(c, nP, [=] __device__(int64_t i) { b32[i] = (float)(b[i] * scale); });
             }
-            bool conv;
+            bool conv, stalled = false;
             double sres;
             const float *pc = use_learned ? c.Sdinv32.p : c.pc32.p;
             int k = schur_pcg<float>(c, o, use_mg, ru32.p, hc32.p, w, o.mixed_schur_rtol, o.itmax, 1,
-                                     o.mixed_inner_rtol, &iits, &sres, &conv, pc, use_learned);
+                                     o.mixed_inner_rtol, &iits, &sres, &conv, pc, use_learned, false,
+                                     use_learned ? &stalled : nullptr);
+            if (stalled) {
+                // keep the partial correction (a valid Krylov step) and continue without the model
+                use_learned = false;
+                rep.fallbacks++;
+                if (trace_on()) std::fprintf(stderr, "[stk] learned preconditioner stalled: falling back\n");
+            }
             rep.outer_iterations += k;
             rep.schur_residual = sres;
             const double inv = 1.0 / scale;
@@ -957,6 +975,14 @@ void csr_diag_inverse(Context &c, const Pattern &p, const T *A, T *dinv) {
 }
 template void csr_diag_inverse<double>(Context &, const Pattern &, const double *, double *);
 template void csr_diag_inverse<float>(Context &, const Pattern &, const float *, float *);
+
+// FP32 copy and inverse diagonal of the learned Schur operator.
+void finish_learned_operator(Context &c) {
+    const int64_t nnz = c.MN.nnz();
+    convert<double, float>(c, c.S64.p, c.S32.p, nnz);
+    csr_diag_inverse<double>(c, c.MN, c.S64.p, c.Sdinv64.p);
+    convert<double, float>(c, c.Sdinv64.p, c.Sdinv32.p, c.nP);
+}
 
 // y = A x for an assembled operator, host FP64 vectors.
 void host_spmv(Context &c, int which, int prec, const double *xh, double *yh) {

@@ -141,6 +141,15 @@ struct Context {
     DBuf<double> d_elth, d_elr, d_nodecs, d_noder;
     DBuf<uint8_t> d_sK, d_sG, d_sGT, d_sM, d_fixed;
 
+    // fabric (learned preconditioner inputs) and element data
+    bool have_fabric = false;
+    DBuf<double> d_phi, d_delta;     // director angle, normal/shear ratio per element
+    DBuf<int32_t> d_iperm;           // element -> colour position
+    double mean_area = 1.0;
+
+    // learned model
+    struct MLP *mlp = nullptr;
+
     // fields
     bool have_fields = false;
     double g = 1.0;
@@ -192,6 +201,14 @@ void build_mg_structure(Context &c);
 
 // ---------------------------------------------------------------- kernels (assembly.hip)
 void upload_quadrature(hipStream_t s);
+struct SchurSampleHost {
+    int64_t nsamp = 0, per_snap = 1, first_snap = 0;
+    int mode = 0, ratio_kind = 1;
+    float *X = nullptr, *Tg = nullptr;
+    double *S = nullptr, *scale = nullptr;
+};
+void launch_schur_samples(Context &c, const SchurSampleHost &h);
+void launch_learned_scatter(Context &c, const float *Y, const double *scale, double *S64);
 template <typename T>
 void assemble_impl(Context &c, int flags, T *K, T *G, T *GT, T *M, T *F, T *A1, T *E1);
 template <typename T>

@@ -177,6 +177,66 @@ int stk_get_solution(stk_ctx *ctx, double *U, double *P);
  * Operates on the device solution; U (polar, nU) and Ucart (nU) may be NULL. */
 int stk_updatevelocity(stk_ctx *ctx, double *U, double *Ucart, double *omega);
 
+/* ---------------------------------------------------------------------------
+ * Learned local-Schur preconditioner (no counterpart in the reference).  A
+ * per-element MLP (5 -> 64 -> 64 -> 64 -> 6, GELU, BF16 weights, FP32
+ * accumulation on MFMA) maps (mean log eta, cos 2(phi - theta_c),
+ * sin 2(phi - theta_c), log delta, area / mean area of the element's cell) to the log-Cholesky factor
+ * of the element's local Schur block S_e = G_e^T (K_e + mu N N^T)^-1 G_e
+ * divided by area / eta; the assembled sum of the predicted blocks is the
+ * pressure preconditioner of STK_SOLVE_LEARNED.
+ * ------------------------------------------------------------------------- */
+typedef struct stk_train_opts {
+    int32_t snapshots;      /* seeded snapshots of exact local Schur blocks (2000)      */
+    int32_t per_snapshot;   /* elements drawn per snapshot                              */
+    int32_t epochs;
+    int32_t batch;          /* minibatch size (multiple of 32)                          */
+    int32_t ratio_kind;     /* delta law of the draws: 0 = 10, 1 = U[1,100], 2 = logU[1,1e3] */
+    int32_t reserved;
+    double lr, lr_final;    /* Adam step, cosine-decayed to lr_final                    */
+    uint64_t seed;
+} stk_train_opts;
+
+typedef struct stk_train_report {
+    double initial_loss, final_loss, validation_loss;   /* MSE in normalised output units */
+    double ms;
+    int64_t samples;
+    int32_t steps;
+    int32_t reserved;
+} stk_train_report;
+
+void stk_default_train_opts(stk_train_opts *opts);
+
+/* Per-element fabric director angle phi (radians, Cartesian x-z frame) and
+ * normal/shear viscosity ratio delta: the inputs the reference's fabric model
+ * turns into the viscous tensor (src/Rheology/anisotropy.jl:140-285). */
+int stk_set_fabric(stk_ctx *ctx, const double *phi, const double *delta);
+
+/* Generate the training snapshots from the current mesh and fields with the
+ * plain FP64 element matrices and train the MLP on the GPU. */
+int stk_mlp_train(stk_ctx *ctx, const stk_train_opts *opts, stk_train_report *report);
+
+/* Read (set = 0) or write (set = 1) the FP32 master parameters (9094 floats:
+ * W1 5x64, b1, W2 64x64, b2, W3 64x64, b3, W4 64x6, b4; weights [in][out]) and
+ * the normalisation (22 floats: feature means, stds, output means, stds). */
+int stk_mlp_params(stk_ctx *ctx, float *params, float *norm, int32_t set);
+
+/* Forward pass on host features (n x 5) -> denormalised outputs (n x 6). */
+int stk_mlp_forward(stk_ctx *ctx, int64_t n, const float *X, float *Y);
+
+/* Loss (mean squared error of the normalised outputs) and its gradient with
+ * respect to the 9094 parameters for host features / raw targets (n x 6). */
+int stk_mlp_gradient(stk_ctx *ctx, int64_t n, const float *X, const float *T, float *grad, double *loss);
+
+/* Exact local Schur blocks of every element for the current fields and fabric:
+ * S (nel x 6: S00 S10 S11 S20 S21 S22 in canonical vertex order, divided by
+ * area / eta) and the model features X (nel x 5).  Either may be NULL. */
+int stk_local_schur(stk_ctx *ctx, double *S, float *X);
+
+/* Run the model on every element and assemble the learned Schur operator used
+ * by STK_SOLVE_LEARNED (after stk_assemble). */
+int stk_learned_setup(stk_ctx *ctx);
+
 /* Kernels stk_time_kernel can time (on the assembled operators). */
 enum {
     STK_KERNEL_SPMV_K32 = 0,    /* y = K x, FP32 BSR 2x2                                  */
