@@ -1,0 +1,125 @@
+# Julia binding of libdimspers_stokes.so for Persephone (DimsPers).
+#
+# Keeps the reference call signatures:
+#   assembly_stokes_cylindric(gr, coordinates, g, ρ, η, 𝓒, εII, T, P, PhaseID, KKidx, GGidx, MMidx)
+#       (src/Stokes/StokesSolver.jl:247-404)  -> (KK, GG, MM, Fb, η)
+#   StokesPcCG(U, P, KK, MM, GG, Rhs, ifree)  (src/IterativeSolvers.jl:94-168), here on the
+#       device-resident operators of the last `solveStokes` call
+#   solveStokes(U, P, T, gr, Ucartesian, Upolar, g, ρ, η, 𝓒, εII, coordinates, RotationMatrices,
+#               PhaseID, UBC, KKidx, GGidx, MMidx, to; solver=:mi355x)
+#       (src/Stokes/StokesSolver.jl:7-54)
+# Every call passes plain pointers (GC.@preserve) and turns a non-zero status into an error.
+module StokesMI355X
+
+using SparseArrays
+
+const LIB = get(ENV, "DIMSPERS_STOKES_LIB", joinpath(@__DIR__, "..", "libdimspers_stokes.so"))
+
+# ---- C structs (include/dimspers_stokes.h)
+mutable struct Opts
+    method::Int32; inner::Int32; stop::Int32; itmax::Int32; itmin::Int32; inner_itmax::Int32
+    refine_max::Int32; cheb_degree::Int32; null_mode::Int32; reserved::Int32
+    rtol::Float64; inner_rtol::Float64; mixed_inner_rtol::Float64; mixed_schur_rtol::Float64; stall_ratio::Float64
+end
+Opts() = (o = Opts(zeros(Int32, 10)..., zeros(5)...); ccall((:stk_default_opts, LIB), Cvoid, (Ref{Opts},), o); o)
+
+mutable struct Report
+    converged::Int32; outer_iterations::Int32; inner_iterations::Int64; refinements::Int32; fallbacks::Int32
+    schur_residual::Float64; full_residual::Float64; solve_ms::Float64
+end
+Report() = Report(0, 0, 0, 0, 0, 0.0, 0.0, 0.0)
+
+const SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED = Int32(0), Int32(1), Int32(2)
+const STOP_SCHUR, STOP_FULL = Int32(0), Int32(1)
+const ASM_FP64, ASM_FP32, ASM_CARTESIAN, ASM_NO_BC = Int32(1), Int32(2), Int32(4), Int32(8)
+const MAT_K, MAT_G, MAT_GT, MAT_M = Int32(0), Int32(1), Int32(2), Int32(3)
+
+mutable struct Context
+    h::Ptr{Cvoid}
+    nU::Int
+    nP::Int
+end
+
+check(ctx::Union{Context,Nothing}, rc) = rc == 0 ? nothing :
+    error("libdimspers_stokes: ", unsafe_string(ccall((:stk_last_error, LIB), Cstring, (Ptr{Cvoid},),
+                                                      ctx === nothing ? C_NULL : ctx.h)), " (", rc, ")")
+
+const CONTEXTS = IdDict{Any,Context}()
+
+"One device context per `Grid` (patterns, colouring and multigrid structure are built once)."
+function context(gr, coordinates, UBC)
+    get!(CONTEXTS, gr) do
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        e2n = Matrix{Int64}(gr.e2n)                 # 6 x nel, 1-based, column-major
+        th = Matrix{Float64}(coordinates.θ); r = Matrix{Float64}(coordinates.r)
+        fixed = Vector{Int64}(UBC.Ω)
+        GC.@preserve e2n th r fixed gr begin
+            rc = ccall((:stk_create, LIB), Cint,
+                       (Ref{Ptr{Cvoid}}, Int64, Int64, Int64, Ptr{Int64}, Ptr{Float64}, Ptr{Float64},
+                        Ptr{Float64}, Ptr{Float64}, Ptr{Int64}, Int64),
+                       h, size(e2n, 2), length(gr.θ), maximum(view(e2n, 1:3, :)), e2n, th, r,
+                       gr.θ, gr.r, fixed, length(fixed))
+        end
+        check(nothing, rc)
+        ctx = Context(h[], 2 * length(gr.θ), maximum(view(e2n, 1:3, :)))
+        finalizer(c -> ccall((:stk_destroy, LIB), Cvoid, (Ptr{Cvoid},), c.h), ctx)
+        ctx
+    end
+end
+
+function set_fields!(ctx, g, ρ, η, 𝓒)
+    ip = Matrix{Float64}(η.ip)
+    C = [Matrix{Float64}(getfield(𝓒, f)) for f in (:η11, :η33, :η55, :η13, :η15, :η35)]
+    ptrs = [pointer(c) for c in C]
+    GC.@preserve ip C ptrs ρ begin
+        check(ctx, ccall((:stk_set_fields, LIB), Cint,
+                         (Ptr{Cvoid}, Float64, Ptr{Float64}, Ptr{Float64}, Int64, Ptr{Ptr{Float64}}),
+                         ctx.h, g, ρ.ρ, ip, size(ip, 2), ptrs))
+    end
+end
+
+function export_csc(ctx, which; lower=false)
+    nr, nc, nz = Ref{Int64}(0), Ref{Int64}(0), Ref{Int64}(0)
+    check(ctx, ccall((:stk_export_csc, LIB), Cint,
+                     (Ptr{Cvoid}, Int32, Int32, Ref{Int64}, Ref{Int64}, Ref{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}),
+                     ctx.h, which, lower, nr, nc, nz, C_NULL, C_NULL, C_NULL))
+    colptr = Vector{Int64}(undef, nc[] + 1); rowval = Vector{Int64}(undef, nz[]); nzval = Vector{Float64}(undef, nz[])
+    check(ctx, ccall((:stk_export_csc, LIB), Cint,
+                     (Ptr{Cvoid}, Int32, Int32, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}),
+                     ctx.h, which, lower, C_NULL, C_NULL, C_NULL, colptr, rowval, nzval))
+    SparseMatrixCSC(nr[], nc[], colptr, rowval, nzval)
+end
+
+"GPU `assembly_stokes_cylindric`: lower-triangular KK and MM, GG, Fb in the Cartesian frame."
+function assembly_stokes_cylindric(gr, coordinates, g, ρ, η, 𝓒, εII, T, P, PhaseID, KKidx, GGidx, MMidx; UBC)
+    ctx = context(gr, coordinates, UBC)
+    set_fields!(ctx, g, ρ, η, 𝓒)
+    check(ctx, ccall((:stk_assemble, LIB), Cint, (Ptr{Cvoid}, Int32), ctx.h, ASM_FP64 | ASM_CARTESIAN | ASM_NO_BC))
+    Fb = Vector{Float64}(undef, ctx.nU)
+    check(ctx, ccall((:stk_export_rhs, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}), ctx.h, Fb))
+    return export_csc(ctx, MAT_K; lower=true), export_csc(ctx, MAT_G), export_csc(ctx, MAT_M; lower=true), Fb, η
+end
+
+"GPU `solveStokes`: assemble (rotated, free slip), Schur PCG, net-rotation removal."
+function solveStokes(U, P, T, gr, Ucartesian, Upolar, g, ρ, η, 𝓒, εII, coordinates, RotationMatrices,
+                     PhaseID, UBC, KKidx, GGidx, MMidx, to; solver=:mixed, rtol=1e-8)
+    ctx = context(gr, coordinates, UBC)
+    set_fields!(ctx, g, ρ, η, 𝓒)
+    check(ctx, ccall((:stk_assemble, LIB), Cint, (Ptr{Cvoid}, Int32), ctx.h, ASM_FP64 | ASM_FP32))
+    o = Opts()
+    o.method = solver === :plain ? SOLVE_PLAIN : solver === :learned ? SOLVE_LEARNED : SOLVE_MIXED
+    o.stop = STOP_FULL; o.rtol = rtol
+    rep = Report()
+    check(ctx, ccall((:stk_solve, LIB), Cint, (Ptr{Cvoid}, Ref{Opts}, Ref{Report}), ctx.h, o, rep))
+    Ucart = similar(U); ω = Ref(0.0)
+    check(ctx, ccall((:stk_updatevelocity, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ref{Float64}),
+                     ctx.h, U, Ucart, ω))
+    check(ctx, ccall((:stk_get_solution, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), ctx.h, C_NULL, P))
+    @inbounds for i in 1:2:length(U)
+        Upolar[div(i, 2) + 1].x = U[i];     Upolar[div(i, 2) + 1].z = U[i + 1]
+        Ucartesian[div(i, 2) + 1].x = Ucart[i]; Ucartesian[div(i, 2) + 1].z = Ucart[i + 1]
+    end
+    return Ucartesian, Upolar, U, Ucart, P, to
+end
+
+end # module
