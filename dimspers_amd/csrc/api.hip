@@ -188,7 +188,7 @@ namespace {
 template <typename T>
 void assemble_prec(Context &c, int flags, bool mg, DBuf<T> &K, DBuf<T> &G, DBuf<T> &GT, DBuf<T> &M, DBuf<T> &F,
                    bool f64) {
-    K.alloc(4 * c.KN.nnz()); K.zero(c.stream);
+    K.alloc(4 * c.KN.nstore()); K.zero(c.stream);
     G.alloc(2 * c.GN.nnz()); G.zero(c.stream);
     GT.alloc(2 * c.GTN.nnz()); GT.zero(c.stream);
     M.alloc(c.MN.nnz()); M.zero(c.stream);
@@ -303,7 +303,7 @@ int stk_export_csc(stk_ctx *ctx, int32_t which, int32_t lower_only, int64_t *nro
             default: throw Error(STK_E_ARG, "unknown operator");
         }
         STK_CHECK(v64 || v32, STK_E_STATE, "operator not available");
-        const int64_t nvals = p->nnz() * br * bc;
+        const int64_t nvals = p->nstore() * br * bc;
         std::vector<double> vals(nvals);
         if (v64) {
             STK_HIP(hipMemcpyAsync(vals.data(), v64, nvals * sizeof(double), hipMemcpyDeviceToHost, c.stream));
@@ -318,13 +318,15 @@ int stk_export_csc(stk_ctx *ctx, int32_t which, int32_t lower_only, int64_t *nro
         std::vector<int64_t> cnt(nc + 1, 0);
         auto visit = [&](auto fn) {
             for (int64_t i = 0; i < p->nrows; ++i)
-                for (int32_t k = p->ptr[i]; k < p->ptr[i + 1]; ++k)
+                for (int32_t k = p->ptr[i]; k < p->ptr[i + 1]; ++k) {
+                    const int64_t q0 = p->pos(i, k - p->ptr[i]);
                     for (int r = 0; r < br; ++r)
                         for (int q = 0; q < bc; ++q) {
                             const int64_t row = i * br + r, col = (int64_t)p->col[k] * bc + q;
                             if (lower_only && row < col) continue;
-                            fn(row, col, vals[(int64_t)k * br * bc + r * bc + q]);
+                            fn(row, col, vals[q0 * br * bc + r * bc + q]);
                         }
+                }
         };
         visit([&](int64_t, int64_t col, double) { cnt[col + 1]++; });
         for (int64_t j = 0; j < nc; ++j) cnt[j + 1] += cnt[j];
@@ -425,7 +427,7 @@ int stk_time_kernel(stk_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms, 
             *avg_ms = ms / reps;
             // compulsory bytes: per-element inputs, per-node inputs, one write of every operator
             const double in = (double)c.nel * (6 * 4 + 6 * 8 + 6 * 8 + 36 * 8 + 81) + (double)c.nnod * (8 + 16);
-            const double out = 8.0 * (4 * c.KN.nnz() + 4 * c.GN.nnz() + c.MN.nnz() + c.nU) +
+            const double out = 8.0 * (4 * c.KN.nstore() + 4 * c.GN.nnz() + c.MN.nnz() + c.nU) +
                                 (mg ? 8.0 * (4 * c.mg[0].pat.nnz() + 36 * c.nel) : 0.0);
             *bytes = in + out;
             // the timed launches accumulated into the operators: re-assemble cleanly

@@ -80,6 +80,7 @@ struct AsmArgs {
     double g;
     const uint8_t *sK, *sG, *sGT, *sM;  // slot offsets, SoA by position
     const int32_t *Kptr, *Gptr, *GTptr, *Mptr;
+    const int64_t *Ksptr;               // sliced layout of K (null: CSR order)
     T *K, *G, *GT, *M, *F;
     // finest multigrid level (vertex space)
     T *A1, *E1;                         // A1: BSR 2x2 over the M pattern; E1: 36 x nel by element id
@@ -259,9 +260,11 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(AsmArgs<T> a) {
 
     // scatter K (both triangles of the symmetric operator)
     for (int aa = 0; aa < 6; ++aa) {
-        const int64_t rowbase = a.Kptr[n[aa]];
+        const int64_t na = n[aa];
+        const int64_t rowbase = a.Ksptr ? a.Ksptr[na / Pattern::kSlice] + na % Pattern::kSlice : (int64_t)a.Kptr[na];
+        const int64_t kstride = a.Ksptr ? Pattern::kSlice : 1;
         for (int b = 0; b < 6; ++b) {
-            const int64_t blk = rowbase + a.sK[(aa * 6 + b) * nel + p];
+            const int64_t blk = rowbase + kstride * a.sK[(aa * 6 + b) * nel + p];
             T v[4];
             if (b <= aa) {
                 const int k = lblk(aa, b) * 4;
@@ -763,12 +766,17 @@ __global__ __launch_bounds__(64) void k_galerkin(GalArgs<T> a) {
 // Unit diagonal on the fixed dofs of a BSR 2x2 operator (their rows/columns are zero).
 template <typename T>
 __global__ void k_fix_diag(int64_t nfix, const int64_t *__restrict__ dofs, const int32_t *__restrict__ ptr,
-                           const int32_t *__restrict__ col, T *__restrict__ A) {
+                           const int32_t *__restrict__ col, const int64_t *__restrict__ sptr,
+                           const int32_t *__restrict__ scol, T *__restrict__ A) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nfix) return;
     const int64_t d = dofs[i], node = d / 2, c = d % 2;
-    for (int32_t k = ptr[node]; k < ptr[node + 1]; ++k)
-        if (col[k] == node) { A[4 * (int64_t)k + 3 * c] = T(1); break; }
+    const int32_t len = ptr[node + 1] - ptr[node];
+    for (int32_t k = 0; k < len; ++k) {
+        const int64_t q = sptr ? sptr[node / Pattern::kSlice] + (int64_t)k * Pattern::kSlice + node % Pattern::kSlice
+                               : (int64_t)ptr[node] + k;
+        if ((sptr ? scol[q] : col[q]) == node) { A[4 * q + 3 * c] = T(1); break; }
+    }
 }
 
 }  // namespace
@@ -816,6 +824,7 @@ void assemble_impl(Context &c, int flags, T *K, T *G, T *GT, T *M, T *F, T *A1, 
     a.rho = c.d_rho.p; a.eta = c.d_eta.p; a.C = c.d_C.p; a.g = c.g;
     a.sK = c.d_sK.p; a.sG = c.d_sG.p; a.sGT = c.d_sGT.p; a.sM = c.d_sM.p;
     a.Kptr = c.KN.dptr.p; a.Gptr = c.GN.dptr.p; a.GTptr = c.GTN.dptr.p; a.Mptr = c.MN.dptr.p;
+    a.Ksptr = c.KN.sell ? c.KN.dsptr.p : nullptr;
     a.K = K; a.G = G; a.GT = GT; a.M = M; a.F = F; a.A1 = A1; a.E1 = E1;
     a.cart = (flags & STK_ASM_CARTESIAN) ? 1 : 0;
     a.bc = (flags & STK_ASM_NO_BC) ? 0 : 1;
@@ -850,7 +859,8 @@ template <typename T>
 void fix_diag(Context &c, const Pattern &p, int64_t nfix, const int64_t *dofs, T *A) {
     if (nfix == 0) return;
     const int nb = (int)((nfix + 255) / 256);
-    k_fix_diag<T><<<dim3(nb), dim3(256), 0, c.stream>>>(nfix, dofs, p.dptr.p, p.dcol.p, A);
+    k_fix_diag<T><<<dim3(nb), dim3(256), 0, c.stream>>>(nfix, dofs, p.dptr.p, p.dcol.p,
+                                                        p.sell ? p.dsptr.p : nullptr, p.sell ? p.dscol.p : nullptr, A);
     STK_HIP(hipGetLastError());
 }
 

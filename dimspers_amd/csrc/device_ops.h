@@ -132,6 +132,51 @@ __global__ __launch_bounds__(NT) void k_spmv_block(int64_t nrows, const int32_t 
     }
 }
 
+// 2x2-block SpMV on the sliced-ELL layout: one lane per row node, 64 rows per slice,
+// block k of the slice's rows stored contiguously (values as one 16/32-byte vector
+// per lane), no cross-lane reduction.
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct Vec4<double> { typedef double type __attribute__((ext_vector_type(4))); };
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct Vec2<double> { typedef double type __attribute__((ext_vector_type(2))); };
+
+template <int NT, typename T, typename Ep>
+__global__ __launch_bounds__(NT) void k_spmv_sell22(int64_t nrows, const int64_t *__restrict__ sptr,
+                                                     const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                     const T *__restrict__ x, Ep ep, double *__restrict__ partial,
+                                                     int64_t nblocks) {
+    typedef typename Vec4<T>::type V4;
+    typedef typename Vec2<T>::type V2;
+    constexpr int C = 64;
+    const int64_t lb = xcd_block(nblocks);
+    const int64_t slice = lb * (NT / C) + threadIdx.x / C;
+    const int lane = threadIdx.x % C;
+    const int64_t row = slice * C + lane;
+    double dsum = 0.0;
+    if (row < nrows) {
+        const int64_t base = sptr[slice];
+        const int len = (int)((sptr[slice + 1] - base) / C);
+        const V4 *__restrict__ vp = reinterpret_cast<const V4 *>(val) + base + lane;
+        const int32_t *__restrict__ cp = col + base + lane;
+        const V2 *__restrict__ xp = reinterpret_cast<const V2 *>(x);
+        T a0 = T(0), a1 = T(0);
+#pragma unroll 4
+        for (int k = 0; k < len; ++k) {
+            const V4 v = vp[(int64_t)k * C];
+            const V2 xv = xp[cp[(int64_t)k * C]];
+            a0 += v.x * xv.x + v.y * xv.y;
+            a1 += v.z * xv.x + v.w * xv.y;
+        }
+        dsum = ep(2 * row, a0) + ep(2 * row + 1, a1);
+    }
+    if (partial) {
+        const double s = block_sum<NT>(dsum);
+        if (threadIdx.x == 0) partial[lb] = s;
+    }
+}
+
 // ------------------------------------------------------------------ reductions
 template <int NT, typename F>
 __global__ __launch_bounds__(NT) void k_reduce_map(int64_t n, F f, double *__restrict__ partial) {

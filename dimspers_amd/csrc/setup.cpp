@@ -111,6 +111,34 @@ void build_patterns(Context &c) {
         p->dptr.upload(p->ptr, c.stream);
         p->dcol.upload(p->col, c.stream);
     }
+    make_sliced(c.KN, c.stream);
+}
+
+void make_sliced(Pattern &p, hipStream_t s) {
+    const int64_t C = Pattern::kSlice;
+    const int64_t ns = (p.nrows + C - 1) / C;
+    p.sptr.assign(ns + 1, 0);
+    for (int64_t sl = 0; sl < ns; ++sl) {
+        int64_t len = 0;
+        for (int64_t r = sl * C; r < std::min<int64_t>(p.nrows, (sl + 1) * C); ++r)
+            len = std::max<int64_t>(len, p.ptr[r + 1] - p.ptr[r]);
+        p.sptr[sl + 1] = p.sptr[sl] + len * C;
+    }
+    p.scol.assign(p.sptr[ns], 0);
+    for (int64_t sl = 0; sl < ns; ++sl) {
+        const int64_t len = (p.sptr[sl + 1] - p.sptr[sl]) / C;
+        for (int64_t l = 0; l < C; ++l) {
+            const int64_t r = sl * C + l;
+            for (int64_t k = 0; k < len; ++k) {
+                const int64_t q = p.sptr[sl] + k * C + l;
+                if (r < p.nrows && k < p.ptr[r + 1] - p.ptr[r]) p.scol[q] = p.col[p.ptr[r] + k];
+                else p.scol[q] = (int32_t)std::min<int64_t>(r, p.nrows - 1);   // padding: zero block on the diagonal
+            }
+        }
+    }
+    p.sell = true;
+    p.dsptr.upload(p.sptr, s);
+    p.dscol.upload(p.scol, s);
 }
 
 void build_colouring(Context &c) { colour_elements(c.nel, 6, c.e2n, c.nnod, c.perm, c.col_off); }

@@ -121,8 +121,21 @@ struct Op22 {   // BSR 2x2 operator
     const T *val;
 };
 
+template <typename T, typename Ep>
+int64_t spmv_sell(Context &c, const Pattern &p, const T *val, const T *x, Ep ep, double *partial) {
+    const int64_t slices_per_block = NT / Pattern::kSlice;
+    const int64_t nslice = (p.nrows + Pattern::kSlice - 1) / Pattern::kSlice;
+    const int64_t nb = (nslice + slices_per_block - 1) / slices_per_block;
+    if (partial && (int64_t)c.red64.n < nb) throw Error(STK_E_STATE, "reduction buffer too small");
+    k_spmv_sell22<NT, T, Ep><<<dim3(nb), dim3(NT), 0, c.stream>>>(p.nrows, p.dsptr.p, p.dscol.p, val, x, ep,
+                                                                   partial, nb);
+    STK_HIP(hipGetLastError());
+    return nb;
+}
+
 template <int L, typename T, typename Ep>
 int64_t apply22(Context &c, const Op22<T> &A, const T *x, Ep ep, double *partial = nullptr) {
+    if (A.pat->sell) return spmv_sell<T, Ep>(c, *A.pat, A.val, x, ep, partial);
     return spmv<2, 2, L, T, Ep>(c, *A.pat, A.val, x, ep, partial);
 }
 
@@ -414,15 +427,20 @@ void power_lmax(Context &c, const Op22<T> &A, const T *dinv, int64_t n, T *v, T 
 
 template <typename T>
 void diag_inverse(Context &c, const Pattern &p, const T *A, T *dinv) {
-    const int32_t *ptr = p.dptr.p, *col = p.dcol.p;
+    const int32_t *ptr = p.dptr.p, *col = p.dcol.p, *scol = p.sell ? p.dscol.p : nullptr;
+    const int64_t *sptr = p.sell ? p.dsptr.p : nullptr;
     launch_map(c, p.nrows, [=] __device__(int64_t node) {
-        for (int32_t k = ptr[node]; k < ptr[node + 1]; ++k)
-            if (col[k] == node) {
-                const T a0 = A[4 * (int64_t)k], a3 = A[4 * (int64_t)k + 3];
+        const int32_t len = ptr[node + 1] - ptr[node];
+        for (int32_t k = 0; k < len; ++k) {
+            const int64_t q = sptr ? sptr[node / Pattern::kSlice] + (int64_t)k * Pattern::kSlice + node % Pattern::kSlice
+                                   : (int64_t)ptr[node] + k;
+            if ((sptr ? scol[q] : col[q]) == node) {
+                const T a0 = A[4 * q], a3 = A[4 * q + 3];
                 dinv[2 * node] = a0 != T(0) ? T(1) / a0 : T(0);
                 dinv[2 * node + 1] = a3 != T(0) ? T(1) / a3 : T(0);
                 break;
             }
+        }
     });
 }
 
@@ -735,7 +753,7 @@ double full_residual(Context &c, const double *U, const double *P, double *ru, d
     const double *F = c.F64.p;
     // ru = K U
     EpStore<double> ek{ru};
-    spmv<2, 2, 8, double>(c, c.KN, c.K64.p, U, ek, nullptr);
+    apply22<8>(c, Op22<double>{&c.KN, c.K64.p}, U, ek);
     // t = G P ; ru = F + t - ru  (masked)
     double *t = c.work64.p;
     EpStore<double> eg{t};
@@ -1001,7 +1019,7 @@ void host_spmv(Context &c, int which, int prec, const double *xh, double *yh) {
         using T = decltype(tag);
         EpStore<T> es{y};
         switch (which) {
-            case STK_MAT_K: spmv<2, 2, 8, T>(c, *p, val, x, es, nullptr); break;
+            case STK_MAT_K: apply22<8>(c, Op22<T>{p, val}, x, es); break;
             case STK_MAT_G: spmv<2, 1, 8, T>(c, *p, val, x, es, nullptr); break;
             case STK_MAT_GT: spmv<1, 2, 8, T>(c, *p, val, x, es, nullptr); break;
             default: spmv<1, 1, 8, T>(c, *p, val, x, es, nullptr); break;
@@ -1114,7 +1132,7 @@ double time_kernel(Context &c, int kernel, int reps, double *bytes) {
     hipEvent_t e0, e1;
     STK_HIP(hipEventCreate(&e0));
     STK_HIP(hipEventCreate(&e1));
-    const int64_t nU = c.nU, nb = c.KN.nnz(), nn = c.nnod;
+    const int64_t nU = c.nU, nb = c.KN.nnz(), nn = c.nnod;   // compulsory: structural blocks only
     auto spmv_bytes = [&](int sv) { return (double)nb * (4 * sv + 4) + (double)(nn + 1) * 4 + 2.0 * nU * sv; };
     std::function<void()> launch;
     DBuf<double> x64, y64, b64, d64, o64;

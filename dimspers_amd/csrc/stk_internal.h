@@ -70,11 +70,26 @@ struct DBuf {
 };
 
 // Node-level sparsity pattern: row i lists the sorted column nodes of its blocks.
+// Optionally the values live in a sliced-ELL order: rows are grouped in slices of
+// kSlice consecutive rows, each slice padded to its longest row, and block k of row
+// n sits at sptr[n / kSlice] + k * kSlice + n % kSlice (padding: column = row, zero
+// values).  One lane per row then reads consecutive blocks with its neighbours.
 struct Pattern {
+    static constexpr int kSlice = 64;
     int64_t nrows = 0, ncols = 0;
-    std::vector<int32_t> ptr, col;   // host copies
+    std::vector<int32_t> ptr, col;   // host copies (CSR order)
     DBuf<int32_t> dptr, dcol;        // device copies
+    bool sell = false;
+    std::vector<int64_t> sptr;       // slice block offsets (nslices + 1)
+    std::vector<int32_t> scol;       // columns in sliced order, padded
+    DBuf<int64_t> dsptr;
+    DBuf<int32_t> dscol;
     int64_t nnz() const { return (int64_t)col.size(); }
+    // number of stored blocks (value array length / block size)
+    int64_t nstore() const { return sell ? sptr.back() : nnz(); }
+    int64_t pos(int64_t row, int64_t k) const {
+        return sell ? sptr[row / kSlice] + k * kSlice + row % kSlice : ptr[row] + k;
+    }
 };
 
 // Work vectors of one multigrid level (or of the inner CG).
@@ -194,6 +209,7 @@ struct Context {
 
 // ---------------------------------------------------------------- set-up (setup.cpp)
 void build_patterns(Context &c);
+void make_sliced(Pattern &p, hipStream_t s);
 void build_colouring(Context &c);
 void build_slot_maps(Context &c);
 void detect_structured(Context &c);
