@@ -84,7 +84,10 @@ def main():
     method = _lib.SOLVE_LEARNED if args.method == "learned" else _lib.SOLVE_MIXED
     fast_opts = _lib.default_opts(method=method, stop=_lib.STOP_FULL, rtol=1e-8)
     mixed_opts = _lib.default_opts(method=_lib.SOLVE_MIXED, stop=_lib.STOP_FULL, rtol=1e-8)
-    plain_opts = _lib.default_opts(method=_lib.SOLVE_PLAIN, stop=_lib.STOP_FULL, rtol=1e-8, itmax=400)
+    # plain FP64 at its fastest setting that keeps its outer convergence (inner 1e-10:
+    # same 53 Schur iterations and residual as 1e-11 at 256 x 2048, 10% less time)
+    plain_opts = _lib.default_opts(method=_lib.SOLVE_PLAIN, stop=_lib.STOP_FULL, rtol=1e-8, itmax=400,
+                                   inner_rtol=1e-10)
 
     train = None
     setup_ms = 0.0

@@ -35,7 +35,7 @@ class StkOpts(C.Structure):
     _fields_ = [
         ("method", C.c_int32), ("inner", C.c_int32), ("stop", C.c_int32), ("itmax", C.c_int32),
         ("itmin", C.c_int32), ("inner_itmax", C.c_int32), ("refine_max", C.c_int32), ("cheb_degree", C.c_int32),
-        ("null_mode", C.c_int32), ("reserved", C.c_int32),
+        ("null_mode", C.c_int32), ("pc_sweeps", C.c_int32),
         ("rtol", C.c_double), ("inner_rtol", C.c_double), ("mixed_inner_rtol", C.c_double),
         ("mixed_schur_rtol", C.c_double), ("stall_ratio", C.c_double),
     ]

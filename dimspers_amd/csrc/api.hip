@@ -1,6 +1,7 @@
 // extern "C" entry points of libdimspers_stokes.so (declared in include/dimspers_stokes.h).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 
 #include "device_ops.h"
@@ -49,19 +50,23 @@ static int guarded(stk_ctx *ctx, F f) {
 extern "C" {
 
 void stk_default_opts(stk_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
     o->method = STK_SOLVE_PLAIN;
     o->inner = STK_INNER_MG;
     o->stop = STK_STOP_SCHUR;
     o->itmax = 200;
     o->itmin = 10;
     o->inner_itmax = 2000;
-    o->refine_max = 12;
-    o->cheb_degree = 3;
+    o->refine_max = 30;
+    o->cheb_degree = 2;
     o->rtol = 1e-8;
     o->inner_rtol = 1e-11;
-    o->mixed_inner_rtol = 1e-3;
-    o->mixed_schur_rtol = 1e-3;
+    o->mixed_inner_rtol = 0.1;
+    o->mixed_schur_rtol = 0.1;
     o->stall_ratio = 0.5;
+    o->null_mode = STK_NULL_PROJECT;
+    o->pc_sweeps = 3;
 }
 
 const char *stk_last_error(const stk_ctx *ctx) { return ctx ? ctx->c.err.c_str() : g_err.c_str(); }
@@ -440,6 +445,8 @@ int stk_time_kernel(stk_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms, 
 }
 
 void stk_default_train_opts(stk_train_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
     o->snapshots = 2000;
     o->per_snapshot = 512;
     o->epochs = 4;

@@ -645,7 +645,8 @@ double vdot(Context &c, const T *a, const T *b, int64_t n) {
 template <typename T>
 int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *h, SchurWork<T> &w,
               double rtol, int itmax, int itmin, double inner_rtol, int64_t *inner_its, double *schur_res,
-              bool *conv, const T *pcvec, bool learned, bool deflate = false, bool *stalled = nullptr) {
+              bool *conv, const T *pcvec, bool learned, bool deflate = false, bool *stalled = nullptr,
+              double abs_tol = 0.0) {
     const bool f64 = sizeof(T) == 8;
     MGView<T> v = make_view<T>(c);
     const int64_t nU = c.nU, nP = c.nP;
@@ -666,15 +667,16 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
     vmean_remove<T>(c, w.r, nP);
     launch_map(c, nP, [=] __device__(int64_t i) { w.dp[i] = T(0); });
     const double r0 = std::sqrt(vdot<T>(c, w.r, w.r, nP));
-    const double tol = rtol * r0;
+    const double tol = std::max(rtol * r0, abs_tol);
     auto apply_pc = [&](const T *rin, T *dout) {
         if (learned) {
-            // three damped Jacobi sweeps on the assembled learned Schur operator
+            // damped Jacobi sweeps on the assembled learned Schur operator
             const T *S = (const T *)pick<T>(c.S64, c.S32).p;
             const T *sd = pcvec;   // inverse diagonal of S
             T *tmp = w.Sq;
+            const int nsweep = o.pc_sweeps > 0 ? o.pc_sweeps : 3;
             launch_map(c, nP, [=] __device__(int64_t i) { dout[i] = sd[i] * rin[i]; });
-            for (int s = 0; s < 3; ++s) {
+            for (int s = 0; s < nsweep; ++s) {
                 EpStore<T> es{tmp};
                 spmv<1, 1, 8, T>(c, c.MN, S, dout, es, nullptr);
                 launch_map(c, nP, [=] __device__(int64_t i) { dout[i] += T(0.7) * sd[i] * (rin[i] - tmp[i]); });
@@ -795,6 +797,9 @@ SchurWork<T> schur_work(Context &c) {
     return w;
 }
 
+// Smallest relative Schur reduction asked of one FP32 correction sweep.
+constexpr double kSweepFloor = 1e-5;
+
 int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
     std::memset(&rep, 0, sizeof(rep));
     const bool use_mg = (o.inner == STK_INNER_MG) && c.mg_built && !c.loaded;
@@ -829,11 +834,14 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
         bool conv;
         double sres;
         int64_t iits = 0;
-        // the plain method checks the full residual by running to the Schur tolerance,
-        // then (STK_STOP_FULL) continuing in restarted sweeps until the full residual is met.
+        // the plain method runs to the Schur tolerance and checks the full residual;
+        // with STK_STOP_FULL it stops as soon as the continuity residual |r| is below
+        // half the full-residual target (the momentum part is the inner-solve error),
+        // and continues in restarted sweeps while the full residual is not met.
         const double *pc = c.pc64.p;
+        const double abs_tol = (o.stop == STK_STOP_FULL) ? 0.5 * o.rtol * c.fnorm : 0.0;
         int its = schur_pcg<double>(c, o, use_mg, Fm.p, nullptr, w, o.rtol, o.itmax, o.itmin, o.inner_rtol,
-                                    &iits, &sres, &conv, pc, false, deflate);
+                                    &iits, &sres, &conv, pc, false, deflate, nullptr, abs_tol);
         {
             double *U = c.U.p, *P = c.P.p;
             launch_map(c, nU, [=] __device__(int64_t i) { U[i] = w.du[i]; });
@@ -850,7 +858,8 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
             // restarted sweeps of the same FP64 method on the residual system
             while (rep.full_residual >= o.rtol && rep.outer_iterations < o.itmax) {
                 int k = schur_pcg<double>(c, o, use_mg, ru.p, hc.p, w, o.rtol, o.itmax - rep.outer_iterations,
-                                          1, o.inner_rtol, &iits, &sres, &conv, pc, false, deflate);
+                                          1, o.inner_rtol, &iits, &sres, &conv, pc, false, deflate, nullptr,
+                                          abs_tol);
                 double *U = c.U.p, *P = c.P.p;
                 launch_map(c, nU, [=] __device__(int64_t i) { U[i] += w.du[i]; });
                 launch_map(c, nP, [=] __device__(int64_t i) { P[i] += w.dp[i]; });
@@ -874,14 +883,17 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
         DBuf<float> ru32(nU), hc32(nP);
         int64_t iits = 0;
         double res = 1.0, prev = 1.0;
-        bool use_learned = learned;
+        bool use_learned = learned, last_conv = true;
         rep.full_residual = 1.0;
         for (int sweep = 0; sweep < o.refine_max; ++sweep) {
             res = full_residual(c, c.U.p, c.P.p, ru.p, hc.p, project);
             rep.full_residual = res;
             if (trace_on()) std::fprintf(stderr, "[stk] mixed sweep %d full residual %.3e\n", sweep, res);
             if (res < o.rtol) break;
-            if (sweep > 0 && use_learned && res > o.stall_ratio * prev) {
+            // a learned sweep whose Schur PCG missed its target and that barely reduced
+            // the full residual: drop the model (a converged Schur PCG whose sweep is
+            // limited by the inner-solve error is not the model's fault)
+            if (sweep > 0 && use_learned && !last_conv && res > o.stall_ratio * prev) {
                 use_learned = false;
                 rep.fallbacks++;
             }
@@ -897,9 +909,16 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
             bool conv, stalled = false;
             double sres;
             const float *pc = use_learned ? c.Sdinv32.p : c.pc32.p;
-            int k = schur_pcg<float>(c, o, use_mg, ru32.p, hc32.p, w, o.mixed_schur_rtol, o.itmax, 1,
+            // The scaled correction problem has unit full residual; its Schur residual is
+            // the continuity part of the next full residual.  Target the reduction this
+            // sweep needs (mixed_schur_rtol, or less on the last sweep), but never ask FP32
+            // for more than kSweepFloor relative to the sweep's initial Schur residual
+            // (which may be far above 1: continuity and momentum carry different units).
+            const double target = std::max(o.mixed_schur_rtol, 0.5 * o.rtol / res);
+            int k = schur_pcg<float>(c, o, use_mg, ru32.p, hc32.p, w, kSweepFloor, o.itmax, 1,
                                      o.mixed_inner_rtol, &iits, &sres, &conv, pc, use_learned, false,
-                                     use_learned ? &stalled : nullptr);
+                                     use_learned ? &stalled : nullptr, target);
+            last_conv = conv;
             if (stalled) {
                 // keep the partial correction (a valid Krylov step) and continue without the model
                 use_learned = false;

@@ -95,12 +95,14 @@ typedef struct stk_opts {
     int32_t refine_max;     /* mixed/learned: FP64 refinement sweeps cap           */
     int32_t cheb_degree;    /* multigrid Chebyshev smoothing degree                */
     int32_t null_mode;      /* STK_NULL_*                                          */
-    int32_t reserved;       /* zero                                                */
+    int32_t pc_sweeps;      /* learned: damped Jacobi sweeps on the learned Schur  */
+                            /* operator per preconditioner application (0 = 3)     */
     double rtol;            /* outer tolerance; reference: 1e-8                    */
     double inner_rtol;      /* inner CG relative tolerance (plain method)          */
     double mixed_inner_rtol;/* inner CG tolerance inside the FP32 Schur PCG         */
-    double mixed_schur_rtol;/* Schur PCG tolerance of one FP32 correction           */
-    double stall_ratio;     /* learned: fall back when a sweep reduces less than this */
+    double mixed_schur_rtol;/* full-residual reduction asked of one FP32 correction */
+    double stall_ratio;     /* learned: fall back when a sweep's Schur PCG misses its */
+                            /* target and the full residual falls by less than this */
 } stk_opts;
 
 typedef struct stk_report {
