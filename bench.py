@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--method", default="learned", choices=["mixed", "learned"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--kernels", action="store_true",
+                    help="only time the solver's dominant kernels and the copy reference (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path is not sharded across GPUs.",
@@ -80,6 +82,17 @@ def main():
     log(f"assembly + solver set-up {time.perf_counter() - t0:.2f}s")
     asm_ms, asm_bytes = ctx.time_kernel(_lib.KERNEL_ASSEMBLY, reps=3)
     log(f"element assembly kernels {asm_ms:.2f} ms ({asm_bytes / asm_ms / 1e6:.0f} GB/s compulsory)")
+
+    if args.kernels:
+        res = {}
+        for name, kid in (("cheb_k32", _lib.KERNEL_CHEB_K32), ("spmv_k32", _lib.KERNEL_SPMV_K32),
+                          ("copy_256MiB", _lib.KERNEL_COPY)):
+            kms, kb = ctx.time_kernel(kid, reps=20)
+            res[name] = {"avg_us": round(kms * 1e3, 2), "bytes": kb, "GB/s": round(kb / kms / 1e6, 1)}
+            log(f"{name}: {kms * 1e3:.1f} us, {kb / kms / 1e6:.0f} GB/s (21 launches: 1 warm-up + 20 timed)")
+        ctx.close()
+        print(json.dumps({"kernels": res}), flush=True)
+        return
 
     method = _lib.SOLVE_LEARNED if args.method == "learned" else _lib.SOLVE_MIXED
     fast_opts = _lib.default_opts(method=method, stop=_lib.STOP_FULL, rtol=1e-8)

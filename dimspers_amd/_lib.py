@@ -15,6 +15,7 @@ STK_OK = 0
 STK_E_HIP, STK_E_SHAPE, STK_E_STATE, STK_E_NOTCONV, STK_E_NAN, STK_E_ARG, STK_E_NOMESH = -1, -2, -3, -4, -5, -6, -7
 
 ASM_FP64, ASM_FP32, ASM_CARTESIAN, ASM_NO_BC, ASM_ISOTROPIC, ASM_NO_MG = 1, 2, 4, 8, 16, 32
+ASM_FP16_SMOOTHER = 64
 MAT_K, MAT_G, MAT_GT, MAT_M, MAT_S = 0, 1, 2, 3, 4
 SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED = 0, 1, 2
 INNER_MG, INNER_JACOBI = 0, 1
@@ -28,7 +29,8 @@ EXPORTED = [
     "stk_mlp_params", "stk_mlp_forward", "stk_mlp_gradient", "stk_local_schur", "stk_learned_setup",
 ]
 N_PARAM = 9094
-KERNEL_SPMV_K32, KERNEL_SPMV_K64, KERNEL_CHEB_K32, KERNEL_CHEB_K64, KERNEL_ASSEMBLY = 0, 1, 2, 3, 4
+KERNEL_SPMV_K32, KERNEL_SPMV_K64, KERNEL_CHEB_K32, KERNEL_CHEB_K64, KERNEL_ASSEMBLY, KERNEL_CHEB_KH16 = 0, 1, 2, 3, 4, 5
+KERNEL_COPY = 6
 
 
 class StkOpts(C.Structure):

@@ -207,7 +207,7 @@ void assemble_prec(Context &c, int flags, bool mg, DBuf<T> &K, DBuf<T> &G, DBuf<
     };
     if (mg) {
         for (MGLevel &L : c.mg) {
-            Abuf(L).alloc(4 * L.pat.nnz());
+            Abuf(L).alloc(4 * L.pat.nstore());
             Abuf(L).zero(c.stream);
             Ebuf(L).alloc(36 * L.nel);
         }
@@ -271,6 +271,7 @@ int stk_assemble(stk_ctx *ctx, int32_t flags) {
         }
         STK_HIP(hipStreamSynchronize(c.stream));
         c.mg_built = mg;
+        c.asm_flags = flags;
         if (solve_form) {
             if (fp64) {
                 setup_solver_data<double>(c);

@@ -81,6 +81,7 @@ struct AsmArgs {
     const uint8_t *sK, *sG, *sGT, *sM;  // slot offsets, SoA by position
     const int32_t *Kptr, *Gptr, *GTptr, *Mptr;
     const int64_t *Ksptr;               // sliced layout of K (null: CSR order)
+    const int64_t *A1sptr;              // sliced layout of the P1 level operator (null: CSR)
     T *K, *G, *GT, *M, *F;
     // finest multigrid level (vertex space)
     T *A1, *E1;                         // A1: BSR 2x2 over the M pattern; E1: 36 x nel by element id
@@ -396,7 +397,9 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(AsmArgs<T> a) {
         for (int jj = 0; jj < 6; ++jj) a.E1[(int64_t)(i * 6 + jj) * nel + e] = E[i][jj];
     for (int u = 0; u < 3; ++u)
         for (int v = 0; v < 3; ++v) {
-            const int64_t bm = (int64_t)a.Mptr[n[u]] + a.sM[(u * 3 + v) * nel + p];
+            const int64_t sl = a.sM[(u * 3 + v) * nel + p];
+            const int64_t bm = a.A1sptr ? a.A1sptr[n[u] / Pattern::kSlice] + sl * Pattern::kSlice + n[u] % Pattern::kSlice
+                                        : (int64_t)a.Mptr[n[u]] + sl;
             a.A1[4 * bm + 0] += E[2 * u][2 * v];
             a.A1[4 * bm + 1] += E[2 * u][2 * v + 1];
             a.A1[4 * bm + 2] += E[2 * u + 1][2 * v];
@@ -657,6 +660,7 @@ struct GalArgs {
     const int32_t *perm, *e2v;      // coarse colour order, e2v SoA by position
     const uint8_t *slot;            // 9 x nel_c by position
     const int32_t *ptr;             // coarse pattern rows
+    const int64_t *sptr;            // coarse sliced layout (null: CSR order)
     const T *Ef;                    // 36 x nel_f by element id
     T *Ec;                          // 36 x nel_c by element id
     T *A;                           // coarse BSR 2x2
@@ -755,7 +759,9 @@ __global__ __launch_bounds__(64) void k_galerkin(GalArgs<T> a) {
     for (int k = 0; k < 3; ++k) vid[k] = a.e2v[k * a.nel_c + p];
     for (int u = 0; u < 3; ++u)
         for (int v = 0; v < 3; ++v) {
-            const int64_t bm = (int64_t)a.ptr[vid[u]] + a.slot[(u * 3 + v) * a.nel_c + p];
+            const int64_t sl = a.slot[(u * 3 + v) * a.nel_c + p];
+            const int64_t bm = a.sptr ? a.sptr[vid[u] / Pattern::kSlice] + sl * Pattern::kSlice + vid[u] % Pattern::kSlice
+                                      : (int64_t)a.ptr[vid[u]] + sl;
             a.A[4 * bm + 0] += E[2 * u][2 * v];
             a.A[4 * bm + 1] += E[2 * u][2 * v + 1];
             a.A[4 * bm + 2] += E[2 * u + 1][2 * v];
@@ -825,6 +831,7 @@ void assemble_impl(Context &c, int flags, T *K, T *G, T *GT, T *M, T *F, T *A1, 
     a.sK = c.d_sK.p; a.sG = c.d_sG.p; a.sGT = c.d_sGT.p; a.sM = c.d_sM.p;
     a.Kptr = c.KN.dptr.p; a.Gptr = c.GN.dptr.p; a.GTptr = c.GTN.dptr.p; a.Mptr = c.MN.dptr.p;
     a.Ksptr = c.KN.sell ? c.KN.dsptr.p : nullptr;
+    a.A1sptr = (A1 && c.mg[0].pat.sell) ? c.mg[0].pat.dsptr.p : nullptr;
     a.K = K; a.G = G; a.GT = GT; a.M = M; a.F = F; a.A1 = A1; a.E1 = E1;
     a.cart = (flags & STK_ASM_CARTESIAN) ? 1 : 0;
     a.bc = (flags & STK_ASM_NO_BC) ? 0 : 1;
@@ -845,6 +852,7 @@ void galerkin_impl(Context &c, size_t l, T *Ef, T *Ec, T *A) {
     a.ncr_c = C.ncr; a.nct_c = C.nct; a.ncr_f = F.ncr; a.nct_f = F.nct;
     a.nel_c = C.nel; a.nel_f = F.nel;
     a.perm = C.perm.p; a.e2v = C.e2v.p; a.slot = C.slot.p; a.ptr = C.pat.dptr.p;
+    a.sptr = C.pat.sell ? C.pat.dsptr.p : nullptr;
     a.Ef = Ef; a.Ec = Ec; a.A = A;
     for (size_t k = 0; k + 1 < C.col_off.size(); ++k) {
         a.off = C.col_off[k];

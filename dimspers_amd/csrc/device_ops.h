@@ -92,6 +92,31 @@ struct EpCheb {
     }
 };
 
+// Chebyshev step in the symmetrically scaled variables of A^ = S^-1 A S^-1 (S^2 = diag A),
+// where point Jacobi is the identity: d = c1 d + c2 (b^ - A^ x^); xo^ = x^ + d; on the last
+// step also the unscaled xu = S^-1 xo^.
+struct EpChebScaled {
+    const float *b, *x; float *d, *xo; float c1, c2; float *xu; const float *sinv;
+    __device__ double operator()(int64_t i, float ax) const {
+        const float dn = c1 * d[i] + c2 * (b[i] - ax);
+        d[i] = dn;
+        const float v = x[i] + dn;
+        xo[i] = v;
+        if (xu) xu[i] = v * sinv[i];
+        return 0.0;
+    }
+};
+// xu += A x (prolongation) and its scaled copy xh = S xu.
+struct EpAxpyScaled {
+    float *xu, *xh; const float *s;
+    __device__ double operator()(int64_t i, float ax) const {
+        const float v = xu[i] + ax;
+        xu[i] = v;
+        xh[i] = v * s[i];
+        return 0.0;
+    }
+};
+
 template <int BR, int BC, int L, int NT, typename T, typename Ep>
 __global__ __launch_bounds__(NT) void k_spmv_block(int64_t nrows, const int32_t *__restrict__ ptr,
                                                     const int32_t *__restrict__ col,
@@ -138,16 +163,18 @@ __global__ __launch_bounds__(NT) void k_spmv_block(int64_t nrows, const int32_t 
 template <typename T> struct Vec4;
 template <> struct Vec4<float> { typedef float type __attribute__((ext_vector_type(4))); };
 template <> struct Vec4<double> { typedef double type __attribute__((ext_vector_type(4))); };
+template <> struct Vec4<_Float16> { typedef _Float16 type __attribute__((ext_vector_type(4))); };
 template <typename T> struct Vec2;
 template <> struct Vec2<float> { typedef float type __attribute__((ext_vector_type(2))); };
 template <> struct Vec2<double> { typedef double type __attribute__((ext_vector_type(2))); };
 
-template <int NT, typename T, typename Ep>
+// VT is the storage type of the values (T, or _Float16 for the scaled smoother copy).
+template <int NT, typename VT, typename T, typename Ep>
 __global__ __launch_bounds__(NT) void k_spmv_sell22(int64_t nrows, const int64_t *__restrict__ sptr,
-                                                     const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                     const int32_t *__restrict__ col, const VT *__restrict__ val,
                                                      const T *__restrict__ x, Ep ep, double *__restrict__ partial,
                                                      int64_t nblocks) {
-    typedef typename Vec4<T>::type V4;
+    typedef typename Vec4<VT>::type V4;
     typedef typename Vec2<T>::type V2;
     constexpr int C = 64;
     const int64_t lb = xcd_block(nblocks);
@@ -166,8 +193,8 @@ __global__ __launch_bounds__(NT) void k_spmv_sell22(int64_t nrows, const int64_t
         for (int k = 0; k < len; ++k) {
             const V4 v = vp[(int64_t)k * C];
             const V2 xv = xp[cp[(int64_t)k * C]];
-            a0 += v.x * xv.x + v.y * xv.y;
-            a1 += v.z * xv.x + v.w * xv.y;
+            a0 += (T)v.x * xv.x + (T)v.y * xv.y;
+            a1 += (T)v.z * xv.x + (T)v.w * xv.y;
         }
         dsum = ep(2 * row, a0) + ep(2 * row + 1, a1);
     }

@@ -270,18 +270,19 @@ __global__ __launch_bounds__(NTW) void k_mlp_wgrad(WgradArgs a) {
     }
 }
 
-// Biases: db[j] = sum_s dZ[s][j], written into every slice-0 partial slot; other slices zero.
-__global__ void k_mlp_bgrad(int64_t n, const float *dZ1, const float *dZ2, const float *dZ3, const float *dY,
-                            float *part) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= 3 * 64 + 6) return;
+// Biases: db[j] = sum_s dZ[s][j] into the slice-0 partial slots (other slices hold zero
+// there).  One block per bias entry (3 * 64 + 6 blocks), fixed-order block reduction.
+__global__ __launch_bounds__(256) void k_mlp_bgrad(int64_t n, const float *dZ1, const float *dZ2,
+                                                   const float *dZ3, const float *dY, float *part) {
+    const int j = blockIdx.x;
     const int layer = j / 64, jj = j % 64;
     const float *D = layer == 0 ? dZ1 : layer == 1 ? dZ2 : layer == 2 ? dZ3 : dY;
     const int ld = layer == 3 ? 8 : 64;
     double s = 0.0;
-    for (int64_t k = 0; k < n; ++k) s += D[ld * k + jj];
+    for (int64_t k = threadIdx.x; k < n; k += 256) s += D[ld * k + jj];
+    s = block_sum<256>(s);
     const int boff = layer == 0 ? mlp::oB1 : layer == 1 ? mlp::oB2 : layer == 2 ? mlp::oB3 : mlp::oB4;
-    part[boff + jj] = (float)s;
+    if (threadIdx.x == 0) part[boff + jj] = (float)s;
 }
 
 __global__ void k_sum_slices(int64_t nslice, const float *part, float *grad) {
@@ -431,7 +432,7 @@ double MLP::gradient(const float *X, const float *T, int64_t n, hipStream_t s, D
     g.dZ1 = dZ1.p; g.dZ2 = dZ2.p; g.dZ3 = dZ3.p; g.dY = dY.p; g.part = part.p;
     k_mlp_wgrad<<<dim3((int)(12 * nslice)), dim3(NTW), 0, s>>>(g);
     STK_HIP(hipGetLastError());
-    k_mlp_bgrad<<<dim3(1), dim3(256), 0, s>>>(n, dZ1.p, dZ2.p, dZ3.p, dY.p, part.p);
+    k_mlp_bgrad<<<dim3(3 * 64 + 6), dim3(256), 0, s>>>(n, dZ1.p, dZ2.p, dZ3.p, dY.p, part.p);
     STK_HIP(hipGetLastError());
     k_sum_slices<<<dim3((mlp::kNParam + 255) / 256), dim3(256), 0, s>>>(nslice, part.p, grad.p);
     STK_HIP(hipGetLastError());

@@ -131,11 +131,16 @@ void make_sliced(Pattern &p, hipStream_t s) {
             const int64_t r = sl * C + l;
             for (int64_t k = 0; k < len; ++k) {
                 const int64_t q = p.sptr[sl] + k * C + l;
-                if (r < p.nrows && k < p.ptr[r + 1] - p.ptr[r]) p.scol[q] = p.col[p.ptr[r] + k];
-                else p.scol[q] = (int32_t)std::min<int64_t>(r, p.nrows - 1);   // padding: zero block on the diagonal
+                // padding (zero block): repeat the row's last column, which is a valid
+                // index of x for any pattern shape (rectangular transfers included)
+                const int64_t len_r = r < p.nrows ? p.ptr[r + 1] - p.ptr[r] : 0;
+                if (k < len_r) p.scol[q] = p.col[p.ptr[r] + k];
+                else p.scol[q] = len_r > 0 ? p.col[p.ptr[r + 1] - 1] : 0;
             }
         }
     }
+    for (int32_t cidx : p.scol)
+        if (cidx < 0 || cidx >= p.ncols) throw Error(STK_E_STATE, "sliced pattern column out of range");
     p.sell = true;
     p.dsptr.upload(p.sptr, s);
     p.dscol.upload(p.scol, s);
@@ -251,6 +256,7 @@ void fill_level_geometry(MGLevel &L, hipStream_t s) {
     pattern_from_elements(L.pat, L.nvert, L.nvert, L.nel, L.h_e2v, 3, L.h_e2v, 3);
     L.pat.dptr.upload(L.pat.ptr, s);
     L.pat.dcol.upload(L.pat.col, s);
+    if (L.nvert >= kSellMinRows) make_sliced(L.pat, s);
     std::vector<int32_t> perm;
     colour_elements(L.nel, 3, L.h_e2v, L.nvert, perm, L.col_off);
     std::vector<int32_t> e2v_p(3 * L.nel);
@@ -284,6 +290,17 @@ void fill_level_geometry(MGLevel &L, hipStream_t s) {
 inline void rot_block(double w, double d, double *blk) {
     const double cd = std::cos(d), sd = std::sin(d);
     blk[0] = w * cd; blk[1] = -w * sd; blk[2] = w * sd; blk[3] = w * cd;
+}
+
+// Values of a block pattern given in CSR order, moved to the sliced positions (padding zero).
+std::vector<double> sliced_values(const Pattern &p, const std::vector<double> &v, int bs) {
+    std::vector<double> out((size_t)bs * p.nstore(), 0.0);
+    for (int64_t r = 0; r < p.nrows; ++r)
+        for (int32_t k = p.ptr[r]; k < p.ptr[r + 1]; ++k) {
+            const int64_t q = p.pos(r, k - p.ptr[r]);
+            for (int b = 0; b < bs; ++b) out[(size_t)bs * q + b] = v[(size_t)bs * k + b];
+        }
+    return out;
 }
 
 // Transfer operator as BSR 2x2 (fine node -> coarse vertex) plus its transpose.
@@ -339,6 +356,8 @@ void build_transfer(MGLevel &coarse, int64_t nfine, const std::vector<Triplet> &
         }
     P.dptr.upload(P.ptr, s); P.dcol.upload(P.col, s);
     R.dptr.upload(R.ptr, s); R.dcol.upload(R.col, s);
+    if (P.nrows >= kSellMinRows) { make_sliced(P, s); Pv = sliced_values(P, Pv, 4); }
+    if (R.nrows >= kSellMinRows) { make_sliced(R, s); Rv = sliced_values(R, Rv, 4); }
     coarse.P64.upload(Pv, s);
     coarse.R64.upload(Rv, s);
     std::vector<float> Pf(Pv.begin(), Pv.end()), Rf(Rv.begin(), Rv.end());

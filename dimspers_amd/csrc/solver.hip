@@ -121,13 +121,13 @@ struct Op22 {   // BSR 2x2 operator
     const T *val;
 };
 
-template <typename T, typename Ep>
-int64_t spmv_sell(Context &c, const Pattern &p, const T *val, const T *x, Ep ep, double *partial) {
+template <typename VT, typename T, typename Ep>
+int64_t spmv_sell(Context &c, const Pattern &p, const VT *val, const T *x, Ep ep, double *partial) {
     const int64_t slices_per_block = NT / Pattern::kSlice;
     const int64_t nslice = (p.nrows + Pattern::kSlice - 1) / Pattern::kSlice;
     const int64_t nb = (nslice + slices_per_block - 1) / slices_per_block;
     if (partial && (int64_t)c.red64.n < nb) throw Error(STK_E_STATE, "reduction buffer too small");
-    k_spmv_sell22<NT, T, Ep><<<dim3(nb), dim3(NT), 0, c.stream>>>(p.nrows, p.dsptr.p, p.dscol.p, val, x, ep,
+    k_spmv_sell22<NT, VT, T, Ep><<<dim3(nb), dim3(NT), 0, c.stream>>>(p.nrows, p.dsptr.p, p.dscol.p, val, x, ep,
                                                                    partial, nb);
     STK_HIP(hipGetLastError());
     return nb;
@@ -135,7 +135,7 @@ int64_t spmv_sell(Context &c, const Pattern &p, const T *val, const T *x, Ep ep,
 
 template <int L, typename T, typename Ep>
 int64_t apply22(Context &c, const Op22<T> &A, const T *x, Ep ep, double *partial = nullptr) {
-    if (A.pat->sell) return spmv_sell<T, Ep>(c, *A.pat, A.val, x, ep, partial);
+    if (A.pat->sell) return spmv_sell<T, T, Ep>(c, *A.pat, A.val, x, ep, partial);
     return spmv<2, 2, L, T, Ep>(c, *A.pat, A.val, x, ep, partial);
 }
 
@@ -162,6 +162,7 @@ struct MGView {
     std::vector<Lv> lv;
     const T *coarse_inv = nullptr;
     int64_t coarse_n = 0;
+    bool half0 = false;       // level 0 smooths with the FP16 scaled copy of K (FP32 only)
 };
 
 template <typename T>
@@ -201,6 +202,7 @@ MGView<T> make_view(Context &c) {
     }
     v.coarse_inv = (const T *)pick<T>(c.coarse_inv64, c.coarse_inv32).p;
     v.coarse_n = c.coarse_n;
+    v.half0 = !f64 && c.ksm.on && v.lv.size() > 1;
     return v;
 }
 
@@ -238,9 +240,75 @@ void chebyshev(Context &c, const typename MGView<T>::Lv &l, const T *b, T *x, T 
     *xout = cur;
 }
 
+template <typename T>
+T *vcycle(Context &c, MGView<T> &v, size_t l, const T *b, int deg);
+
+// Level 0 of an FP32 V-cycle with the smoother on the FP16 scaled copy K^ = S^-1 K S^-1:
+// Jacobi-Chebyshev on K is Chebyshev on K^ in the variables x^ = S x, b^ = S^-1 b (same
+// coefficients, same spectrum).  The residual passed to the coarse level is formed with
+// the FP32 K on the unscaled iterate: smooth error components cancel to ~1e-4 of |K||x|,
+// below FP16 resolution but well inside FP32.
+float *vcycle0_half(Context &c, MGView<float> &v, const float *b, int deg) {
+    auto &L = v.lv[0];
+    Context::HalfSmoother &H = c.ksm;
+    const _Float16 *Ah = reinterpret_cast<const _Float16 *>(H.A.p);
+    const double hi = 1.1 * L.lmax, lo = 0.3 * hi;
+    const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
+    const double sigma = theta / delta;
+    float *bh = H.bh.p, *xu = H.xu.p, *d = L.d;
+    const float *sinv = H.sinv.p, *s = H.s.p;
+    float *cur = H.xh.p, *nxt = H.xha.p;
+    auto step = [&](double c1, double c2, bool last) {
+        EpChebScaled ep{bh, cur, d, nxt, (float)c1, (float)c2, last ? xu : nullptr, sinv};
+        spmv_sell<_Float16, float>(c, c.KN, Ah, cur, ep, nullptr);
+        std::swap(cur, nxt);
+    };
+    // pre-smoothing from zero: x^ = d = b^ / theta
+    {
+        const float it = (float)(1.0 / theta);
+        float *xh = cur;
+        const bool one = deg == 1;
+        launch_map(c, L.n, [=] __device__(int64_t i) {
+            const float bv = b[i] * sinv[i];
+            const float v0 = it * bv;
+            bh[i] = bv;
+            d[i] = v0;
+            xh[i] = v0;
+            if (one) xu[i] = v0 * sinv[i];
+        });
+    }
+    double rho = 1.0 / sigma;
+    for (int k = 1; k < deg; ++k) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        step(rn * rho, 2.0 * rn / delta, k == deg - 1);
+        rho = rn;
+    }
+    // coarse-grid correction on the FP32 residual
+    EpResidual<float> er{L.r, b};
+    apply22<8>(c, L.A, xu, er);
+    auto &C = v.lv[1];
+    EpStore<float> es{C.b};
+    apply22_auto<float>(c, L.R, L.r, es);
+    float *xc = vcycle<float>(c, v, 1, C.b, deg);
+    EpAxpyScaled ea{xu, cur, s};
+    apply22_auto<float>(c, L.P, xc, ea);
+    // post-smoothing from the corrected iterate
+    rho = 1.0 / sigma;
+    step(0.0, 1.0 / theta, deg == 1);
+    for (int k = 1; k < deg; ++k) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        step(rn * rho, 2.0 * rn / delta, k == deg - 1);
+        rho = rn;
+    }
+    return xu;
+}
+
 // x_l = V(b_l); result pointer returned (lives in lv[l].x or lv[l].xa).
 template <typename T>
 T *vcycle(Context &c, MGView<T> &v, size_t l, const T *b, int deg) {
+    if constexpr (sizeof(T) == 4) {
+        if (l == 0 && v.half0) return vcycle0_half(c, v, b, deg);
+    }
     auto &L = v.lv[l];
     if (l + 1 == v.lv.size()) {
         // coarsest: dense inverse times b
@@ -472,7 +540,7 @@ static void dense_inverse(std::vector<double> &A, int64_t n) {
 
 void build_coarse_inverse(Context &c, bool f64) {
     MGLevel &L = c.mg.back();
-    const int64_t nb = L.pat.nnz();
+    const int64_t nb = L.pat.nstore();
     std::vector<double> blocks(4 * nb);
     if (f64) {
         L.A64.download(blocks.data(), 4 * nb, c.stream);
@@ -487,11 +555,11 @@ void build_coarse_inverse(Context &c, bool f64) {
     std::vector<double> A(n * n, 0.0);
     for (int64_t r = 0; r < L.nvert; ++r)
         for (int32_t k = L.pat.ptr[r]; k < L.pat.ptr[r + 1]; ++k) {
-            const int64_t cl = L.pat.col[k];
-            A[(2 * r) * n + 2 * cl] = blocks[4 * k];
-            A[(2 * r) * n + 2 * cl + 1] = blocks[4 * k + 1];
-            A[(2 * r + 1) * n + 2 * cl] = blocks[4 * k + 2];
-            A[(2 * r + 1) * n + 2 * cl + 1] = blocks[4 * k + 3];
+            const int64_t cl = L.pat.col[k], q = L.pat.pos(r, k - L.pat.ptr[r]);
+            A[(2 * r) * n + 2 * cl] = blocks[4 * q];
+            A[(2 * r) * n + 2 * cl + 1] = blocks[4 * q + 1];
+            A[(2 * r + 1) * n + 2 * cl] = blocks[4 * q + 2];
+            A[(2 * r + 1) * n + 2 * cl + 1] = blocks[4 * q + 3];
         }
     dense_inverse(A, n);
     c.coarse_n = n;
@@ -508,6 +576,48 @@ void build_coarse_inverse(Context &c, bool f64) {
 template <typename T>
 void alloc_bufs(MGLevelBufs<T> &w, int64_t n) {
     w.b.alloc(n); w.x.alloc(n); w.xa.alloc(n); w.r.alloc(n); w.d.alloc(n);
+}
+
+// FP16 copy of K^ = S^-1 K S^-1 (S^2 = diag K) in the sliced layout of K (STK_ASM_FP16_SMOOTHER).
+// For SPD K every entry satisfies |K_ij| <= sqrt(K_ii K_jj), so |K^_ij| <= 1 and FP16 holds it
+// with 2^-11 relative rounding (entries below 2^-24 flush to zero).  It streams 12 instead of
+// 20 bytes per block, but its rounding (summed over the 38 terms of a row) perturbs the
+// smoothed iterate by ~1e-2 relative, which costs more Krylov iterations than it saves at
+// 256 x 2048, so the FP32 smoother is the default.
+void build_half_smoother(Context &c) {
+    Context::HalfSmoother &H = c.ksm;
+    H.on = false;
+    if (!(c.asm_flags & STK_ASM_FP16_SMOOTHER)) return;
+    const int64_t nU = c.nU, nrows = c.KN.nrows;
+    H.A.alloc(4 * c.KN.nstore());
+    for (DBuf<float> *b : {&H.s, &H.sinv, &H.bh, &H.xh, &H.xha, &H.xu}) b->alloc(nU);
+    float *s = H.s.p, *sinv = H.sinv.p;
+    const float *dinv = c.Kdinv32.p;
+    launch_map(c, nU, [=] __device__(int64_t i) {
+        const float di = dinv[i];
+        const float si = di > 0.0f ? sqrtf(di) : 1.0f;
+        sinv[i] = si;
+        s[i] = 1.0f / si;
+    });
+    const int64_t *sptr = c.KN.dsptr.p;
+    const int32_t *scol = c.KN.dscol.p;
+    const float *K = c.K32.p;
+    _Float16 *A = reinterpret_cast<_Float16 *>(H.A.p);
+    const int64_t npad = (int64_t)(c.KN.sptr.size() - 1) * Pattern::kSlice;
+    launch_map(c, npad, [=] __device__(int64_t i) {
+        const int64_t sl = i / Pattern::kSlice, lane = i % Pattern::kSlice;
+        const int64_t base = sptr[sl], len = (sptr[sl + 1] - base) / Pattern::kSlice;
+        for (int64_t k = 0; k < len; ++k) {
+            const int64_t q = base + k * Pattern::kSlice + lane;
+            const int64_t j = scol[q];
+            for (int a = 0; a < 2; ++a)
+                for (int bb = 0; bb < 2; ++bb) {
+                    const float v = i < nrows ? K[4 * q + 2 * a + bb] * sinv[2 * i + a] * sinv[2 * j + bb] : 0.0f;
+                    A[4 * q + 2 * a + bb] = (_Float16)v;
+                }
+        }
+    });
+    H.on = true;
 }
 
 // After assembly: inverse diagonals, eigenvalue bounds, coarse inverse, rotation vector.
@@ -557,6 +667,7 @@ void setup_solver_data(Context &c) {
             if (trace_on()) std::fprintf(stderr, "[stk] level %zu lmax %.4e\n", i + 1, lm);
         }
         build_coarse_inverse(c, f64);
+        if (!f64) build_half_smoother(c);
     } else {
         Op22<T> K0{&c.KN, K};
         double lm;
@@ -1175,6 +1286,32 @@ double time_kernel(Context &c, int kernel, int reps, double *bytes) {
                 };
                 *bytes = spmv_bytes(4) + 5.0 * nU * 4;
             }
+            break;
+        }
+        case STK_KERNEL_CHEB_KH16: {
+            STK_CHECK(c.ksm.on, STK_E_STATE, "FP16 smoother copy not built");
+            x32.alloc(nU); b32.alloc(nU); d32.alloc(nU); o32.alloc(nU); y32.alloc(nU);
+            float *x = x32.p, *b = b32.p, *d = d32.p, *o = o32.p, *xu = y32.p;
+            launch_map(c, nU, [=] __device__(int64_t i) { x[i] = 1.0f + 1e-3f * (float)(i % 17); b[i] = 1.0f; d[i] = 0.0f; });
+            const _Float16 *Ah = reinterpret_cast<const _Float16 *>(c.ksm.A.p);
+            const float *sinv = c.ksm.sinv.p;
+            (void)xu; (void)sinv;
+            launch = [&c, Ah, x, b, d, o]() {
+                EpChebScaled ep{b, x, d, o, 0.5f, 0.25f, nullptr, nullptr};
+                spmv_sell<_Float16, float>(c, c.KN, Ah, x, ep, nullptr);
+            };
+            // values 8 B + column 4 B per structural block; x^ gathered once, b^ and d read,
+            // d and xo^ written
+            *bytes = (double)nb * 12 + 5.0 * nU * 4;
+            break;
+        }
+        case STK_KERNEL_COPY: {
+            const int64_t n = (int64_t)1 << 26;   // 256 MiB of FP32
+            x32.alloc(n); y32.alloc(n);
+            float *x = x32.p, *y = y32.p;
+            launch_map(c, n, [=] __device__(int64_t i) { x[i] = (float)(i % 1021); });
+            launch = [&c, x, y, n]() { launch_map(c, n, [=] __device__(int64_t i) { y[i] = x[i]; }); };
+            *bytes = 2.0 * n * 4;
             break;
         }
         case STK_KERNEL_SPMV_K64: case STK_KERNEL_CHEB_K64: {

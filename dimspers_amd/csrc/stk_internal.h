@@ -72,7 +72,7 @@ struct DBuf {
 // Node-level sparsity pattern: row i lists the sorted column nodes of its blocks.
 // Optionally the values live in a sliced-ELL order: rows are grouped in slices of
 // kSlice consecutive rows, each slice padded to its longest row, and block k of row
-// n sits at sptr[n / kSlice] + k * kSlice + n % kSlice (padding: column = row, zero
+// n sits at sptr[n / kSlice] + k * kSlice + n % kSlice (padding: the row's last column, zero
 // values).  One lane per row then reads consecutive blocks with its neighbours.
 struct Pattern {
     static constexpr int kSlice = 64;
@@ -179,6 +179,13 @@ struct Context {
     DBuf<double> S64; DBuf<float> S32;        // learned Schur sum (pattern MN)
     DBuf<double> Sdinv64; DBuf<float> Sdinv32;
     DBuf<double> Kdinv64; DBuf<float> Kdinv32; // inverse diagonal of K (per dof)
+    // FP16 copy of the scaled K^ = S^-1 K S^-1 (S^2 = diag K, layout of K) for the
+    // fine-level multigrid smoother, with S, S^-1 and its scaled-space work vectors
+    struct HalfSmoother {
+        bool on = false;
+        DBuf<uint16_t> A;
+        DBuf<float> s, sinv, bh, xh, xha, xu;
+    } ksm;
     DBuf<double> rot64; DBuf<float> rot32;     // unit near-null vector of K (discrete rigid rotation)
     double null_lambda = 0;                    // its Rayleigh quotient v^T K v
     DBuf<int64_t> d_fixlist;
@@ -210,6 +217,8 @@ struct Context {
 // ---------------------------------------------------------------- set-up (setup.cpp)
 void build_patterns(Context &c);
 void make_sliced(Pattern &p, hipStream_t s);
+// 2x2 operators with at least this many block rows use the sliced layout.
+constexpr int64_t kSellMinRows = 16384;
 void build_colouring(Context &c);
 void build_slot_maps(Context &c);
 void detect_structured(Context &c);

@@ -42,7 +42,8 @@ enum {
     STK_ASM_CARTESIAN = 4,   /* skip the TT^T . TT rotation (assembly_stokes_cylindric)  */
     STK_ASM_NO_BC = 8,       /* keep rows/columns of fixed dofs (no _apply_bcs)          */
     STK_ASM_ISOTROPIC = 16,  /* isotropic D (StokesSolver.jl:688-742), C6 ignored       */
-    STK_ASM_NO_MG = 32       /* do not build the multigrid hierarchy                     */
+    STK_ASM_NO_MG = 32,      /* do not build the multigrid hierarchy                     */
+    STK_ASM_FP16_SMOOTHER = 64 /* fine-level smoother on an FP16 copy of S^-1 K S^-1       */
 };
 
 /* Which operator stk_export_csc / stk_spmv address. */
@@ -245,7 +246,9 @@ enum {
     STK_KERNEL_SPMV_K64 = 1,    /* y = K x, FP64 BSR 2x2                                  */
     STK_KERNEL_CHEB_K32 = 2,    /* fused Chebyshev step x' = x + c1 d + c2 D^-1 (b - K x) */
     STK_KERNEL_CHEB_K64 = 3,
-    STK_KERNEL_ASSEMBLY = 4     /* one full element assembly (all colours), FP64          */
+    STK_KERNEL_ASSEMBLY = 4,    /* one full element assembly (all colours), FP64          */
+    STK_KERNEL_CHEB_KH16 = 5,   /* Chebyshev step on the FP16 scaled copy of K (smoother) */
+    STK_KERNEL_COPY = 6         /* plain copy of 256 MiB (FP32): traffic-counter reference */
 };
 
 /* Measurement hook with no counterpart in the reference: runs one of the hot

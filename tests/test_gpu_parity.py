@@ -145,6 +145,32 @@ def test_solve_matches_oracle_projected(mesh, method):
     ctx.close()
 
 
+def test_fp16_smoother_matches_fp32_smoother():
+    """The FP16 scaled copy of K only changes the fine-level smoother: the mixed solve
+    reaches the same full residual, the same solution (1e-7 of the plain FP64 one) and
+    about the same number of Schur iterations as with the FP32 smoother."""
+    inp = make_inputs(scaled_workload(3, 64, 512))
+    out = {}
+    for name, flags in (("fp16", _lib.ASM_FP16_SMOOTHER), ("fp32", 0)):
+        ctx = _ctx(inp, _lib.ASM_FP64 | _lib.ASM_FP32 | flags)
+        rep = ctx.solve(_lib.default_opts(method=_lib.SOLVE_MIXED, stop=_lib.STOP_FULL, rtol=1e-8))
+        assert rep.converged and rep.full_residual < 1e-8 and rep.fallbacks == 0
+        U, P = ctx.solution()
+        if name == "fp16":
+            prep = ctx.solve(_lib.default_opts(method=_lib.SOLVE_PLAIN, stop=_lib.STOP_FULL, rtol=1e-10, itmax=400))
+            assert prep.converged
+            out["plain"] = ctx.solution()
+        out[name] = (U, P, rep)
+        print(f"[result] {name} smoother: outer {rep.outer_iterations} inner {rep.inner_iterations} "
+              f"residual {rep.full_residual:.2e}")
+        ctx.close()
+    Upl, Ppl = out["plain"]
+    for name in ("fp16", "fp32"):
+        eu, ep = _errs(out[name][0], out[name][1], Upl, Ppl)
+        assert eu < 1e-7 and ep < 1e-7, (name, eu, ep)
+    assert out["fp16"][2].outer_iterations <= 1.5 * out["fp32"][2].outer_iterations + 2
+
+
 @pytest.mark.parametrize("mesh", [(4, 16, 3), (32, 256, 1)])
 def test_deflated_solve_matches_reference_lu(mesh):
     """STK_NULL_DEFLATE reproduces lu(K[ifree, ifree]) of the reference (IterativeSolvers.jl:94-168)."""
