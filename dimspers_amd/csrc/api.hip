@@ -132,7 +132,7 @@ int stk_create(stk_ctx **out, int64_t nel, int64_t nnod, int64_t nvert, const in
         }
         // reduction partials: the largest node-row grid of any SpMV with a fused dot
         const int64_t maxrows = std::max<int64_t>(c.nnod, c.nvert);
-        c.red64.alloc(std::max<int64_t>(kReduceBlocks, maxrows / 32 + 64));
+        c.red64.alloc(std::max<int64_t>(2 * kReduceBlocks, 2 * (maxrows / 32 + 64)));   // up to 2 dots
         c.d_sc.alloc(SC_COUNT);
         c.d_sc.zero(c.stream);
         STK_HIP(hipHostMalloc(&c.h_pinned, SC_COUNT * sizeof(double), hipHostMallocDefault));

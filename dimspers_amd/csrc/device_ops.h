@@ -44,6 +44,42 @@ __device__ __forceinline__ double block_sum(double v) {
     return s;
 }
 
+// Several dot products accumulated together (epilogues with kDots > 1 return one).
+template <int ND>
+struct Dots {
+    double v[ND];
+};
+__device__ __forceinline__ void add_dots(double (&s)[1], double v) { s[0] += v; }
+template <int ND>
+__device__ __forceinline__ void add_dots(double (&s)[ND], const Dots<ND> &v) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) s[d] += v.v[d];
+}
+
+// Block-wide sums of ND doubles per thread; results valid in thread 0.
+template <int NT, int ND>
+__device__ __forceinline__ void block_sum_n(double (&v)[ND]) {
+    __shared__ double part[ND][NT / kWave];
+    const int w = threadIdx.x / kWave, l = threadIdx.x % kWave;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) v[d] += __shfl_xor(v[d], o, kWave);
+        if (l == 0) part[d][w] = v[d];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            double s = 0;
+#pragma unroll
+            for (int i = 0; i < NT / kWave; ++i) s += part[d][i];
+            v[d] = s;
+        }
+    }
+    __syncthreads();
+}
+
 // Map the hardware block id so that consecutive logical blocks stay on one XCD
 // (blocks are dealt round-robin over the 8 XCDs, each with its own L2): logical
 // chunks of rows are then contiguous per XCD and their x gathers share that L2.
@@ -63,16 +99,19 @@ __device__ __forceinline__ int64_t xcd_block(int64_t nblocks) {
 template <typename T>
 struct EpStore {                       // y = A x
     T *y;
+    static constexpr int kDots = 1;
     __device__ double operator()(int64_t i, T ax) const { y[i] = ax; return 0.0; }
 };
 template <typename T>
 struct EpStoreDot {                    // y = A x, dot += p_i (A x)_i
     T *y; const T *p;
+    static constexpr int kDots = 1;
     __device__ double operator()(int64_t i, T ax) const { y[i] = ax; return (double)p[i] * (double)ax; }
 };
 template <typename T>
 struct EpResidual {                    // r = b - A x, dot += r_i^2
     T *r; const T *b;
+    static constexpr int kDots = 1;
     __device__ double operator()(int64_t i, T ax) const {
         const T v = b[i] - ax; r[i] = v; return (double)v * (double)v;
     }
@@ -80,12 +119,36 @@ struct EpResidual {                    // r = b - A x, dot += r_i^2
 template <typename T>
 struct EpAxpyOut {                     // y = beta*y + alpha*A x  (alpha, beta scalars)
     T *y; T alpha, beta;
+    static constexpr int kDots = 1;
     __device__ double operator()(int64_t i, T ax) const { y[i] = beta * y[i] + alpha * ax; return 0.0; }
+};
+template <typename T>
+struct EpStoreDot2 {                   // y = A x, dots (p . A x, w . A x)
+    static constexpr int kDots = 2;
+    T *y; const T *p, *w;
+    __device__ Dots<2> operator()(int64_t i, T ax) const {
+        y[i] = ax;
+        return Dots<2>{{(double)p[i] * (double)ax, (double)w[i] * (double)ax}};
+    }
+};
+// Chebyshev step that also returns the dots (b . xo, w . xo) of its result.
+template <typename T>
+struct EpChebDot2 {
+    static constexpr int kDots = 2;
+    const T *b, *dinv, *x; T *d, *xo; T c1, c2; const T *w;
+    __device__ Dots<2> operator()(int64_t i, T ax) const {
+        const T dn = c1 * d[i] + c2 * dinv[i] * (b[i] - ax);
+        d[i] = dn;
+        const T v = x[i] + dn;
+        xo[i] = v;
+        return Dots<2>{{(double)b[i] * (double)v, (double)w[i] * (double)v}};
+    }
 };
 // Chebyshev step with point-Jacobi: d = c1 d + c2 Dinv (b - A x); xo = x + d.
 template <typename T>
 struct EpCheb {
     const T *b, *dinv, *x; T *d, *xo; T c1, c2;
+    static constexpr int kDots = 1;
     __device__ double operator()(int64_t i, T ax) const {
         const T dn = c1 * d[i] + c2 * dinv[i] * (b[i] - ax);
         d[i] = dn; xo[i] = x[i] + dn; return 0.0;
@@ -96,6 +159,7 @@ struct EpCheb {
 // where point Jacobi is the identity: d = c1 d + c2 (b^ - A^ x^); xo^ = x^ + d; on the last
 // step also the unscaled xu = S^-1 xo^.
 struct EpChebScaled {
+    static constexpr int kDots = 1;
     const float *b, *x; float *d, *xo; float c1, c2; float *xu; const float *sinv;
     __device__ double operator()(int64_t i, float ax) const {
         const float dn = c1 * d[i] + c2 * (b[i] - ax);
@@ -108,6 +172,7 @@ struct EpChebScaled {
 };
 // xu += A x (prolongation) and its scaled copy xh = S xu.
 struct EpAxpyScaled {
+    static constexpr int kDots = 1;
     float *xu, *xh; const float *s;
     __device__ double operator()(int64_t i, float ax) const {
         const float v = xu[i] + ax;
@@ -122,10 +187,11 @@ __global__ __launch_bounds__(NT) void k_spmv_block(int64_t nrows, const int32_t 
                                                     const int32_t *__restrict__ col,
                                                     const T *__restrict__ val, const T *__restrict__ x,
                                                     Ep ep, double *__restrict__ partial, int64_t nblocks) {
+    constexpr int ND = Ep::kDots;
     const int64_t lb = xcd_block(nblocks);
     const int64_t row = lb * (NT / L) + threadIdx.x / L;
     const int lane = threadIdx.x % L;
-    double dsum = 0.0;
+    double dsum[ND] = {};
     if (row < nrows) {
         T acc[BR];
 #pragma unroll
@@ -148,12 +214,14 @@ __global__ __launch_bounds__(NT) void k_spmv_block(int64_t nrows, const int32_t 
 #pragma unroll
             for (int r = 1; r < BR; ++r)
                 if (lane == r) v = acc[r];
-            dsum = ep(row * BR + lane, v);
+            add_dots(dsum, ep(row * BR + lane, v));
         }
     }
-    if (partial) {
-        const double s = block_sum<NT>(dsum);
-        if (threadIdx.x == 0) partial[lb] = s;
+    if (partial) {   // partial[d * nblocks + block]
+        block_sum_n<NT, ND>(dsum);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) partial[d * nblocks + lb] = dsum[d];
     }
 }
 
@@ -177,11 +245,12 @@ __global__ __launch_bounds__(NT) void k_spmv_sell22(int64_t nrows, const int64_t
     typedef typename Vec4<VT>::type V4;
     typedef typename Vec2<T>::type V2;
     constexpr int C = 64;
+    constexpr int ND = Ep::kDots;
     const int64_t lb = xcd_block(nblocks);
     const int64_t slice = lb * (NT / C) + threadIdx.x / C;
     const int lane = threadIdx.x % C;
     const int64_t row = slice * C + lane;
-    double dsum = 0.0;
+    double dsum[ND] = {};
     if (row < nrows) {
         const int64_t base = sptr[slice];
         const int len = (int)((sptr[slice + 1] - base) / C);
@@ -196,32 +265,40 @@ __global__ __launch_bounds__(NT) void k_spmv_sell22(int64_t nrows, const int64_t
             a0 += (T)v.x * xv.x + (T)v.y * xv.y;
             a1 += (T)v.z * xv.x + (T)v.w * xv.y;
         }
-        dsum = ep(2 * row, a0) + ep(2 * row + 1, a1);
+        add_dots(dsum, ep(2 * row, a0));
+        add_dots(dsum, ep(2 * row + 1, a1));
     }
-    if (partial) {
-        const double s = block_sum<NT>(dsum);
-        if (threadIdx.x == 0) partial[lb] = s;
+    if (partial) {   // partial[d * nblocks + block]
+        block_sum_n<NT, ND>(dsum);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) partial[d * nblocks + lb] = dsum[d];
     }
 }
 
 // ------------------------------------------------------------------ reductions
-template <int NT, typename F>
+// f(i) returns a double (ND = 1) or Dots<ND>; it may also write (fused update + dot).
+template <int NT, int ND, typename F>
 __global__ __launch_bounds__(NT) void k_reduce_map(int64_t n, F f, double *__restrict__ partial) {
-    double s = 0.0;
+    double s[ND] = {};
     const int64_t stride = (int64_t)gridDim.x * NT;
-    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += stride) s += f(i);
-    s = block_sum<NT>(s);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += stride) add_dots(s, f(i));
+    block_sum_n<NT, ND>(s);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) partial[d * gridDim.x + blockIdx.x] = s[d];
 }
 
-// Sum npart partials in a fixed order; out[0] = sum.
+// Sum nd columns of npart partials (partial[d * npart + i]) in a fixed order: out[d].
 template <int NT>
-__global__ __launch_bounds__(NT) void k_reduce_final(int64_t npart, const double *__restrict__ partial,
+__global__ __launch_bounds__(NT) void k_reduce_final(int64_t npart, int nd, const double *__restrict__ partial,
                                                      double *__restrict__ out) {
-    double s = 0.0;
-    for (int64_t i = threadIdx.x; i < npart; i += NT) s += partial[i];
-    s = block_sum<NT>(s);
-    if (threadIdx.x == 0) out[0] = s;
+    for (int d = 0; d < nd; ++d) {
+        double s = 0.0;
+        for (int64_t i = threadIdx.x; i < npart; i += NT) s += partial[d * npart + i];
+        s = block_sum<NT>(s);
+        if (threadIdx.x == 0) out[d] = s;
+    }
 }
 
 template <int NT, typename F>

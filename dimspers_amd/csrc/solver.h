@@ -9,25 +9,24 @@ namespace stk {
 enum : int {
     SC_TMP = 0,      // scratch (two slots)
     SC_BB = 2,       // ||b||^2 of the inner CG
-    SC_PAP = 3,      // p . A p
-    SC_RZ0 = 4,      // r . z, ping-pong pair (4, 5)
     SC_RU = 6,       // ||ru||^2 of the full residual
     SC_RC = 7,       // ||rc||^2 of the full residual
     SC_HIST = 8,     // recent inner residual norms (8 slots)
     SC_VY = 16,      // v . y of the inner right-hand side (null-vector component)
     SC_VZ = 17,      // v . z of the inner solution
-    SC_VAP = 18,     // v . (K p) inside the inner CG
-    SC_VZZ = 19,     // v . (M^-1 r) inside the inner CG
+    SC_PAP2 = 20,    // (p . K p, v . K p) of the inner CG (20, 21)
+    SC_RZV = 22,     // (r . z, v . z) of the inner CG, ping-pong pairs (22, 23) and (24, 25)
     SC_COUNT = 32
 };
 
-// Two-pass deterministic reductions into device scalar slots.
+// Two-pass deterministic reductions into device scalar slots.  map_reduce<ND> takes a
+// device function returning a double (ND = 1) or Dots<ND> (slots slot .. slot+ND-1).
 struct Reducer {
     Context &c;
     explicit Reducer(Context &c);
-    template <typename F>
+    template <int ND = 1, typename F>
     void map_reduce(int64_t n, F f, int slot);
-    void finish(int64_t npart, int slot);
+    void finish(int64_t npart, int slot, int nd = 1);
     double fetch(int slot);
     void fetch_many(int slot0, int n, double *out);
 };

@@ -78,17 +78,18 @@ void Reducer::fetch_many(int slot0, int n, double *out) {
     for (int i = 0; i < n; ++i) out[i] = c.h_pinned[slot0 + i];
 }
 
-void Reducer::finish(int64_t npart, int slot) {
-    k_reduce_final<NT><<<dim3(1), dim3(NT), 0, c.stream>>>(npart, c.red64.p, c.d_sc.p + slot);
+void Reducer::finish(int64_t npart, int slot, int nd) {
+    k_reduce_final<NT><<<dim3(1), dim3(NT), 0, c.stream>>>(npart, nd, c.red64.p, c.d_sc.p + slot);
     STK_HIP(hipGetLastError());
 }
 
-template <typename F>
+template <int ND, typename F>
 void Reducer::map_reduce(int64_t n, F f, int slot) {
     const int nb = std::min<int64_t>(kReduceBlocks, std::max<int64_t>(1, nblk(n)));
-    k_reduce_map<NT, F><<<dim3(nb), dim3(NT), 0, c.stream>>>(n, f, c.red64.p);
+    if ((int64_t)c.red64.n < (int64_t)nb * ND) throw Error(STK_E_STATE, "reduction buffer too small");
+    k_reduce_map<NT, ND, F><<<dim3(nb), dim3(NT), 0, c.stream>>>(n, f, c.red64.p);
     STK_HIP(hipGetLastError());
-    finish(nb, slot);
+    finish(nb, slot, ND);
 }
 
 template <typename F>
@@ -108,7 +109,7 @@ template <int BR, int BC, int L, typename T, typename Ep>
 int64_t spmv(Context &c, const Pattern &p, const T *val, const T *x, Ep ep, double *partial) {
     const int64_t rows_per_block = NT / L;
     const int64_t nb = (p.nrows + rows_per_block - 1) / rows_per_block;
-    if (partial && (int64_t)c.red64.n < nb) throw Error(STK_E_STATE, "reduction buffer too small");
+    if (partial && (int64_t)c.red64.n < nb * Ep::kDots) throw Error(STK_E_STATE, "reduction buffer too small");
     k_spmv_block<BR, BC, L, NT, T, Ep><<<dim3(nb), dim3(NT), 0, c.stream>>>(p.nrows, p.dptr.p, p.dcol.p, val, x, ep, partial, nb);
     STK_HIP(hipGetLastError());
     return nb;
@@ -126,7 +127,7 @@ int64_t spmv_sell(Context &c, const Pattern &p, const VT *val, const T *x, Ep ep
     const int64_t slices_per_block = NT / Pattern::kSlice;
     const int64_t nslice = (p.nrows + Pattern::kSlice - 1) / Pattern::kSlice;
     const int64_t nb = (nslice + slices_per_block - 1) / slices_per_block;
-    if (partial && (int64_t)c.red64.n < nb) throw Error(STK_E_STATE, "reduction buffer too small");
+    if (partial && (int64_t)c.red64.n < nb * Ep::kDots) throw Error(STK_E_STATE, "reduction buffer too small");
     k_spmv_sell22<NT, VT, T, Ep><<<dim3(nb), dim3(NT), 0, c.stream>>>(p.nrows, p.dsptr.p, p.dscol.p, val, x, ep,
                                                                    partial, nb);
     STK_HIP(hipGetLastError());
@@ -208,9 +209,11 @@ MGView<T> make_view(Context &c) {
 
 // Chebyshev smoothing of A x = b with point Jacobi on [lo, hi] = [0.3, 1.1] * lmax.
 // x0 is zero when `zero_start`; the result is written to *xout (one of x / xa).
+// With dot_w (non-zero start only) the last step also accumulates (b . x, dot_w . x) of its
+// result into block partials; *dot_nb receives their count.
 template <typename T>
 void chebyshev(Context &c, const typename MGView<T>::Lv &l, const T *b, T *x, T *xa, T *d, int deg,
-               bool zero_start, T **xout) {
+               bool zero_start, T **xout, const T *dot_w = nullptr, int64_t *dot_nb = nullptr) {
     const double hi = 1.1 * l.lmax, lo = 0.3 * hi;
     const double theta = 0.5 * (hi + lo), delta = 0.5 * (hi - lo);
     const double sigma = theta / delta;
@@ -224,24 +227,29 @@ void chebyshev(Context &c, const typename MGView<T>::Lv &l, const T *b, T *x, T 
             d[i] = v;
             cur[i] = v;
         });
-    } else {
-        // d = D^-1 (b - A x) / theta ; x += d
-        EpCheb<T> ep{b, dinv, cur, d, nxt, T(0), (T)(1.0 / theta)};
-        apply22_auto<T>(c, l.A, cur, ep);
-        std::swap(cur, nxt);
     }
+    auto step = [&](double c1, double c2, bool last) {
+        if (last && dot_w) {
+            EpChebDot2<T> ep{b, dinv, cur, d, nxt, (T)c1, (T)c2, dot_w};
+            *dot_nb = apply22_auto<T>(c, l.A, cur, ep, c.red64.p);
+        } else {
+            EpCheb<T> ep{b, dinv, cur, d, nxt, (T)c1, (T)c2};
+            apply22_auto<T>(c, l.A, cur, ep);
+        }
+        std::swap(cur, nxt);
+    };
+    // d = D^-1 (b - A x) / theta ; x += d
+    if (!zero_start) step(0.0, 1.0 / theta, deg == 1);
     for (int k = 1; k < deg; ++k) {
         const double rn = 1.0 / (2.0 * sigma - rho);
-        EpCheb<T> ep{b, dinv, cur, d, nxt, (T)(rn * rho), (T)(2.0 * rn / delta)};
-        apply22_auto<T>(c, l.A, cur, ep);
-        std::swap(cur, nxt);
+        step(rn * rho, 2.0 * rn / delta, k == deg - 1);
         rho = rn;
     }
     *xout = cur;
 }
 
 template <typename T>
-T *vcycle(Context &c, MGView<T> &v, size_t l, const T *b, int deg);
+T *vcycle(Context &c, MGView<T> &v, size_t l, const T *b, int deg, const T *dot_w, int64_t *dot_nb);
 
 // Level 0 of an FP32 V-cycle with the smoother on the FP16 scaled copy K^ = S^-1 K S^-1:
 // Jacobi-Chebyshev on K is Chebyshev on K^ in the variables x^ = S x, b^ = S^-1 b (same
@@ -289,7 +297,7 @@ float *vcycle0_half(Context &c, MGView<float> &v, const float *b, int deg) {
     auto &C = v.lv[1];
     EpStore<float> es{C.b};
     apply22_auto<float>(c, L.R, L.r, es);
-    float *xc = vcycle<float>(c, v, 1, C.b, deg);
+    float *xc = vcycle<float>(c, v, 1, C.b, deg, nullptr, nullptr);
     EpAxpyScaled ea{xu, cur, s};
     apply22_auto<float>(c, L.P, xc, ea);
     // post-smoothing from the corrected iterate
@@ -304,8 +312,11 @@ float *vcycle0_half(Context &c, MGView<float> &v, const float *b, int deg) {
 }
 
 // x_l = V(b_l); result pointer returned (lives in lv[l].x or lv[l].xa).
+// With dot_w, the level-0 result x also gets its dots (b . x, dot_w . x) accumulated
+// into block partials by the last smoothing step; *dot_nb = their count (0: not done).
 template <typename T>
-T *vcycle(Context &c, MGView<T> &v, size_t l, const T *b, int deg) {
+T *vcycle(Context &c, MGView<T> &v, size_t l, const T *b, int deg, const T *dot_w, int64_t *dot_nb) {
+    if (dot_nb) *dot_nb = 0;
     if constexpr (sizeof(T) == 4) {
         if (l == 0 && v.half0) return vcycle0_half(c, v, b, deg);
     }
@@ -331,13 +342,13 @@ T *vcycle(Context &c, MGView<T> &v, size_t l, const T *b, int deg) {
     auto &C = v.lv[l + 1];
     EpStore<T> es{C.b};
     apply22_auto<T>(c, L.R, L.r, es);
-    T *xc = vcycle<T>(c, v, l + 1, C.b, deg);
+    T *xc = vcycle<T>(c, v, l + 1, C.b, deg, nullptr, nullptr);
     // x += P xc
     EpAxpyOut<T> ea{x, T(1), T(1)};
     apply22_auto<T>(c, L.P, xc, ea);
     T *other = (x == L.x) ? L.xa : L.x;
     T *xo;
-    chebyshev<T>(c, L, b, x, other, L.d, deg, false, &xo);
+    chebyshev<T>(c, L, b, x, other, L.d, deg, false, &xo, dot_w, dot_nb);
     return xo;
 }
 
@@ -363,14 +374,15 @@ int64_t inner_cg(Context &c, MGView<T> &v, bool use_mg, T *y, T *z, double rtol,
     double *sc = c.d_sc.p;
     Op22<T> K{&c.KN, (const T *)pick<T>(c.K64, c.K32).p};
 
-    // project the near-null component out of y; r = y; z = 0
+    // project the near-null component out of y; r = y; z = 0; |r|^2
     red.map_reduce(n, [=] __device__(int64_t i) { return (double)rot[i] * (double)y[i]; }, SC_VY);
-    launch_map(c, n, [=] __device__(int64_t i) {
+    red.map_reduce(n, [=] __device__(int64_t i) {
         const T v = y[i] - (T)sc[SC_VY] * rot[i];
         y[i] = v;
         r[i] = v;
         z[i] = T(0);
-    });
+        return (double)v * (double)v;
+    }, SC_BB);
     const double lam = c.null_lambda;
     auto finish_null = [&]() {
         // z <- z - (v.z) v  (+ (v.y / lambda) v when deflating)
@@ -380,7 +392,6 @@ int64_t inner_cg(Context &c, MGView<T> &v, bool use_mg, T *y, T *z, double rtol,
             z[i] += (T)(dl * sc[SC_VY] - sc[SC_VZ]) * rot[i];
         });
     };
-    red.map_reduce(n, [=] __device__(int64_t i) { return (double)r[i] * (double)r[i]; }, SC_BB);
     const double bb = red.fetch(SC_BB);
     if (!std::isfinite(bb)) throw Error(STK_E_NAN, "non-finite right-hand side in the inner CG");
     if (!(bb > 0)) {
@@ -389,42 +400,56 @@ int64_t inner_cg(Context &c, MGView<T> &v, bool use_mg, T *y, T *z, double rtol,
         return 0;
     }
     const double tol2 = rtol * rtol * bb;
-    auto precond = [&](const T *rin, T *zout) {
+    // zz = M^-1 rin (the V-cycle's own output buffer, or w.x for Jacobi) together with
+    // (rin . zz, v . zz) in slots (slot, slot + 1): from the V-cycle's last smoothing
+    // step when it can return them, else from one fused pass.
+    auto precond = [&](const T *rin, int slot) -> T * {
         if (use_mg) {
-            T *res = vcycle<T>(c, v, 0, rin, deg);
-            launch_map(c, n, [=] __device__(int64_t i) { zout[i] = res[i]; });
-        } else {
-            launch_map(c, n, [=] __device__(int64_t i) { zout[i] = kdinv[i] * rin[i]; });
+            int64_t nbd = 0;
+            T *res = vcycle<T>(c, v, 0, rin, deg, rot, &nbd);
+            if (nbd > 0) {
+                red.finish(nbd, slot, 2);
+            } else {
+                red.map_reduce<2>(n, [=] __device__(int64_t i) {
+                    return Dots<2>{{(double)rin[i] * (double)res[i], (double)rot[i] * (double)res[i]}};
+                }, slot);
+            }
+            return res;
         }
+        T *zo = w.x.p;
+        red.map_reduce<2>(n, [=] __device__(int64_t i) {
+            const T zv = kdinv[i] * rin[i];
+            zo[i] = zv;
+            return Dots<2>{{(double)rin[i] * (double)zv, (double)rot[i] * (double)zv}};
+        }, slot);
+        return zo;
     };
-    T *zz = w.x.p;   // preconditioned residual
-    precond(r, zz);
-    red.map_reduce(n, [=] __device__(int64_t i) { return (double)rot[i] * (double)zz[i]; }, SC_VZZ);
-    launch_map(c, n, [=] __device__(int64_t i) {
-        const T v = zz[i] - (T)sc[SC_VZZ] * rot[i];
-        zz[i] = v;
-        p[i] = v;
-    });
-    red.map_reduce(n, [=] __device__(int64_t i) { return (double)r[i] * (double)zz[i]; }, SC_RZ0);
+    // first direction: p = zz - (v . zz) v, r . z taken with the projected zz
+    {
+        T *zz = precond(r, SC_RZV);
+        launch_map(c, n, [=] __device__(int64_t i) { p[i] = zz[i] - (T)sc[SC_RZV + 1] * rot[i]; });
+        red.map_reduce(n, [=] __device__(int64_t i) { return (double)r[i] * (double)p[i]; }, SC_RZV);
+    }
     int par = 0;
     const int check_every = 2;
     int64_t it = 0;
     *converged = false;
     double hist[8];
     while (it < itmax) {
-        const int rz_cur = SC_RZ0 + par, rz_new = SC_RZ0 + (1 - par);
-        // Ap = K p, pAp
-        EpStoreDot<T> ed{Ap, p};
+        const int rz_cur = SC_RZV + 2 * par, rz_new = SC_RZV + 2 * (1 - par);
+        // Ap = K p with (p . Ap, v . Ap)
+        EpStoreDot2<T> ed{Ap, p, rot};
         const int64_t nb = apply22<8>(c, K, p, ed, c.red64.p);
-        red.finish(nb, SC_PAP);
-        red.map_reduce(n, [=] __device__(int64_t i) { return (double)rot[i] * (double)Ap[i]; }, SC_VAP);
+        red.finish(nb, SC_PAP2, 2);
+        // z += al p, r -= al P Ap, with |r|^2
         const int slot_rr = SC_HIST + (int)(it % check_every);
-        launch_map(c, n, [=] __device__(int64_t i) {
-            const T al = (T)(sc[rz_cur] / sc[SC_PAP]);
+        red.map_reduce(n, [=] __device__(int64_t i) {
+            const T al = (T)(sc[rz_cur] / sc[SC_PAP2]);
             z[i] += al * p[i];
-            r[i] -= al * (Ap[i] - (T)sc[SC_VAP] * rot[i]);
-        });
-        red.map_reduce(n, [=] __device__(int64_t i) { return (double)r[i] * (double)r[i]; }, slot_rr);
+            const T rv = r[i] - al * (Ap[i] - (T)sc[SC_PAP2 + 1] * rot[i]);
+            r[i] = rv;
+            return (double)rv * (double)rv;
+        }, slot_rr);
         ++it;
         if (it % check_every == 0 || it == itmax) {
             if (trace_on() && it == itmax) std::fprintf(stderr, "[stk] inner CG hit its cap %d\n", itmax);
@@ -441,12 +466,11 @@ int64_t inner_cg(Context &c, MGView<T> &v, bool use_mg, T *y, T *z, double rtol,
             for (int k = 0; k < m; ++k) grew |= hist[k] > 1e6 * bb;
             if (grew) break;
         }
-        precond(r, zz);
-        red.map_reduce(n, [=] __device__(int64_t i) { return (double)r[i] * (double)zz[i]; }, rz_new);
-        red.map_reduce(n, [=] __device__(int64_t i) { return (double)rot[i] * (double)zz[i]; }, SC_VZZ);
+        // zz = M^-1 r with (r . zz, v . zz); p = P zz + be p
+        T *zz = precond(r, rz_new);
         launch_map(c, n, [=] __device__(int64_t i) {
             const T be = (T)(sc[rz_new] / sc[rz_cur]);
-            p[i] = zz[i] - (T)sc[SC_VZZ] * rot[i] + be * p[i];
+            p[i] = zz[i] - (T)sc[rz_new + 1] * rot[i] + be * p[i];
         });
         par = 1 - par;
     }
