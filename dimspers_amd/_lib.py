@@ -30,7 +30,7 @@ EXPORTED = [
 ]
 N_PARAM = 9094
 KERNEL_SPMV_K32, KERNEL_SPMV_K64, KERNEL_CHEB_K32, KERNEL_CHEB_K64, KERNEL_ASSEMBLY, KERNEL_CHEB_KH16 = 0, 1, 2, 3, 4, 5
-KERNEL_COPY = 6
+KERNEL_COPY, KERNEL_VCYCLE32, KERNEL_INNER_IT32 = 6, 7, 8
 
 
 class StkOpts(C.Structure):

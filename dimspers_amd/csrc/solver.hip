@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
+#include <memory>
 #include <type_traits>
 
 #include "device_ops.h"
@@ -1329,12 +1330,41 @@ double time_kernel(Context &c, int kernel, int reps, double *bytes) {
             *bytes = (double)nb * 12 + 5.0 * nU * 4;
             break;
         }
+        case STK_KERNEL_VCYCLE32: {
+            STK_CHECK(c.K32 && c.have_mg32, STK_E_STATE, "FP32 multigrid not set up");
+            b32.alloc(nU);
+            float *b = b32.p;
+            launch_map(c, nU, [=] __device__(int64_t i) { b[i] = 1.0f + 1e-3f * (float)(i % 17); });
+            auto view = std::make_shared<MGView<float>>(make_view<float>(c));
+            launch = [&c, view, b]() { vcycle<float>(c, *view, 0, b, 2, nullptr, nullptr); };
+            *bytes = 0.0;
+            break;
+        }
+        case STK_KERNEL_INNER_IT32: {
+            // a fixed number of inner CG iterations (tolerance unreachable), per iteration
+            STK_CHECK(c.K32 && c.have_mg32, STK_E_STATE, "FP32 multigrid not set up");
+            y32.alloc(nU); x32.alloc(nU);
+            float *y = y32.p, *z = x32.p;
+            auto view = std::make_shared<MGView<float>>(make_view<float>(c));
+            launch = [&c, view, y, z, nU]() {
+                launch_map(c, nU, [=] __device__(int64_t i) { y[i] = 1.0f + 1e-3f * (float)(i % 17); });
+                bool conv;
+                inner_cg<float>(c, *view, true, y, z, 1e-30, 20, 2, &conv);
+            };
+            *bytes = 0.0;
+            break;
+        }
         case STK_KERNEL_COPY: {
             const int64_t n = (int64_t)1 << 26;   // 256 MiB of FP32
             x32.alloc(n); y32.alloc(n);
             float *x = x32.p, *y = y32.p;
             launch_map(c, n, [=] __device__(int64_t i) { x[i] = (float)(i % 1021); });
-            launch = [&c, x, y, n]() { launch_map(c, n, [=] __device__(int64_t i) { y[i] = x[i]; }); };
+            // 16-byte accesses: the attainable streaming rate of the part
+            launch = [&c, x, y, n]() {
+                const float4 *xs = reinterpret_cast<const float4 *>(x);
+                float4 *ys = reinterpret_cast<float4 *>(y);
+                launch_map(c, n / 4, [=] __device__(int64_t i) { ys[i] = xs[i]; });
+            };
             *bytes = 2.0 * n * 4;
             break;
         }
@@ -1370,7 +1400,7 @@ double time_kernel(Context &c, int kernel, int reps, double *bytes) {
     STK_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    return ms / reps;
+    return kernel == STK_KERNEL_INNER_IT32 ? ms / reps / 20 : ms / reps;
 }
 
 }  // namespace stk

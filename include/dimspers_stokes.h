@@ -248,7 +248,9 @@ enum {
     STK_KERNEL_CHEB_K64 = 3,
     STK_KERNEL_ASSEMBLY = 4,    /* one full element assembly (all colours), FP64          */
     STK_KERNEL_CHEB_KH16 = 5,   /* Chebyshev step on the FP16 scaled copy of K (smoother) */
-    STK_KERNEL_COPY = 6         /* plain copy of 256 MiB (FP32): traffic-counter reference */
+    STK_KERNEL_COPY = 6,        /* plain copy of 256 MiB (FP32): traffic-counter reference */
+    STK_KERNEL_VCYCLE32 = 7,    /* one FP32 multigrid V-cycle (the inner preconditioner)  */
+    STK_KERNEL_INNER_IT32 = 8   /* one FP32 inner CG iteration (V-cycle, K p, updates)    */
 };
 
 /* Measurement hook with no counterpart in the reference: runs one of the hot
