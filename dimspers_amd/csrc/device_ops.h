@@ -296,6 +296,20 @@ __global__ __launch_bounds__(NT) void k_reduce_final(int64_t npart, int nd, cons
     }
 }
 
+// Coarsest-level solve x = Ainv b: one wavefront per row, coalesced over the row.
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_matvec(int64_t n, const T *__restrict__ Ainv,
+                                                      const T *__restrict__ b, T *__restrict__ x) {
+    const int64_t row = (int64_t)blockIdx.x * 4 + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (row >= n) return;
+    const T *a = Ainv + row * n;
+    T s = T(0);
+    for (int64_t k = lane; k < n; k += kWave) s += a[k] * b[k];
+    s = group_sum<kWave>(s);
+    if (lane == 0) x[row] = s;
+}
+
 template <int NT, typename F>
 __global__ __launch_bounds__(NT) void k_map(int64_t n, F f) {
     const int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x;

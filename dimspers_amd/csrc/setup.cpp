@@ -381,7 +381,9 @@ void build_mg_structure(Context &c) {
         L.ncr = ncr; L.nct = nct;
         c.mg.push_back(std::move(L));
     }
-    while (ncr % 2 == 0 && nct % 2 == 0 && ncr / 2 >= 2 && nct / 2 >= 8) {
+    // halve the lattice while it is above the size solved by a dense inverse
+    while (ncr % 2 == 0 && nct % 2 == 0 && ncr / 2 >= 2 && nct / 2 >= 8 &&
+           (int64_t)(ncr + 1) * nct > kDenseMaxVerts) {
         ncr /= 2; nct /= 2;
         MGLevel L;
         L.ncr = ncr; L.nct = nct;

@@ -324,14 +324,10 @@ T *vcycle(Context &c, MGView<T> &v, size_t l, const T *b, int deg, const T *dot_
     auto &L = v.lv[l];
     if (l + 1 == v.lv.size()) {
         // coarsest: dense inverse times b
-        const T *Ainv = v.coarse_inv;
-        const int64_t n = v.coarse_n;
         T *x = L.x;
-        launch_map(c, n, [=] __device__(int64_t i) {
-            T s = T(0);
-            for (int64_t k = 0; k < n; ++k) s += Ainv[i * n + k] * b[k];
-            x[i] = s;
-        });
+        k_dense_matvec<T><<<dim3((unsigned)((v.coarse_n + 3) / 4)), dim3(256), 0, c.stream>>>(v.coarse_n, v.coarse_inv,
+                                                                                           b, x);
+        STK_HIP(hipGetLastError());
         return x;
     }
     T *x;
@@ -538,33 +534,50 @@ void diag_inverse(Context &c, const Pattern &p, const T *A, T *dinv) {
 }
 
 // Dense inverse (Gauss-Jordan, partial pivoting) of the coarsest operator, on the host.
-static void dense_inverse(std::vector<double> &A, int64_t n) {
-    std::vector<double> I(n * n, 0.0);
-    for (int64_t i = 0; i < n; ++i) I[i * n + i] = 1.0;
-    for (int64_t k = 0; k < n; ++k) {
-        int64_t piv = k;
-        for (int64_t i = k + 1; i < n; ++i)
-            if (std::fabs(A[i * n + k]) > std::fabs(A[piv * n + k])) piv = i;
-        if (A[piv * n + k] == 0.0) throw Error(STK_E_SHAPE, "singular coarsest multigrid operator");
-        if (piv != k)
-            for (int64_t j = 0; j < n; ++j) {
-                std::swap(A[k * n + j], A[piv * n + j]);
-                std::swap(I[k * n + j], I[piv * n + j]);
-            }
-        const double inv = 1.0 / A[k * n + k];
-        for (int64_t j = 0; j < n; ++j) { A[k * n + j] *= inv; I[k * n + j] *= inv; }
-        for (int64_t i = 0; i < n; ++i) {
-            if (i == k) continue;
-            const double f = A[i * n + k];
-            if (f == 0.0) continue;
-            for (int64_t j = 0; j < n; ++j) { A[i * n + j] -= f * A[k * n + j]; I[i * n + j] -= f * I[k * n + j]; }
-        }
+// In-place Gauss-Jordan inverse of an SPD matrix (no pivoting), step k: with c = column
+// k and r = row k of the current matrix (copied first, so no element is read after it is
+// overwritten) and p = A_kk:
+//   A_kj = r_j / p (j != k), A_kk = 1 / p, A_ik = -c_i / p (i != k),
+//   A_ij -= c_i r_j / p (i, j != k).
+__global__ void k_gj_copy(int64_t n, int64_t k, const double *__restrict__ A, double *__restrict__ cv,
+                          double *__restrict__ rv, int *__restrict__ bad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    cv[i] = A[i * n + k];
+    rv[i] = A[k * n + i];
+    if (i == k) {
+        const double p = A[k * n + k];
+        if (!isfinite(p)) bad[0] = 2;
+        else if (!(p > 0.0)) bad[0] = 1;
     }
-    A.swap(I);
 }
 
+__global__ void k_gj_update(int64_t n, int64_t k, double *__restrict__ A, const double *__restrict__ cv,
+                            const double *__restrict__ rv) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = blockIdx.y;
+    if (j >= n) return;
+    const double p = cv[k];
+    double v;
+    if (i == k) v = (j == k) ? 1.0 / p : rv[j] / p;
+    else v = (j == k) ? -cv[i] / p : A[i * n + j] - cv[i] * (rv[j] / p);
+    A[i * n + j] = v;
+}
+
+// Dense inverse of the coarsest multigrid operator, computed on the device in FP64 from
+// the operator of the given precision and stored in that precision.
 void build_coarse_inverse(Context &c, bool f64) {
     MGLevel &L = c.mg.back();
+    const int64_t n = 2 * L.nvert;
+    if (!f64 && (c.asm_flags & STK_ASM_FP64) && c.coarse_inv64.n == (size_t)(n * n)) {
+        // the FP64 set-up of this assembly (it runs first) already inverted the exact
+        // operator: its rounding serves the FP32 preconditioner as well
+        c.coarse_n = n;
+        c.coarse_inv32.alloc(n * n);
+        convert_vec(c, c.coarse_inv64.p, c.coarse_inv32.p, n * n);
+        STK_HIP(hipStreamSynchronize(c.stream));
+        return;
+    }
     const int64_t nb = L.pat.nstore();
     std::vector<double> blocks(4 * nb);
     if (f64) {
@@ -576,7 +589,6 @@ void build_coarse_inverse(Context &c, bool f64) {
         std::copy(bf.begin(), bf.end(), blocks.begin());
     }
     STK_HIP(hipStreamSynchronize(c.stream));
-    const int64_t n = 2 * L.nvert;
     std::vector<double> A(n * n, 0.0);
     for (int64_t r = 0; r < L.nvert; ++r)
         for (int32_t k = L.pat.ptr[r]; k < L.pat.ptr[r + 1]; ++k) {
@@ -586,16 +598,29 @@ void build_coarse_inverse(Context &c, bool f64) {
             A[(2 * r + 1) * n + 2 * cl] = blocks[4 * q + 2];
             A[(2 * r + 1) * n + 2 * cl + 1] = blocks[4 * q + 3];
         }
-    dense_inverse(A, n);
+    DBuf<double> dA, cv(n), rv(n);
+    DBuf<int> bad(1);
+    dA.upload(A, c.stream);
+    bad.zero(c.stream);
+    const int gx = (int)((n + 255) / 256);
+    for (int64_t k = 0; k < n; ++k) {
+        k_gj_copy<<<dim3(gx), dim3(256), 0, c.stream>>>(n, k, dA.p, cv.p, rv.p, bad.p);
+        k_gj_update<<<dim3(gx, (unsigned)n), dim3(256), 0, c.stream>>>(n, k, dA.p, cv.p, rv.p);
+    }
+    STK_HIP(hipGetLastError());
+    int hbad = 0;
+    bad.download(&hbad, 1, c.stream);
+    STK_HIP(hipStreamSynchronize(c.stream));
+    if (hbad == 2) throw Error(STK_E_NAN, "non-finite coarsest multigrid operator");
+    if (hbad) throw Error(STK_E_SHAPE, "coarsest multigrid operator is not positive definite");
     c.coarse_n = n;
     if (f64) {
-        c.coarse_inv64.upload(A, c.stream);
+        c.coarse_inv64 = std::move(dA);
     } else {
-        std::vector<float> Af(A.begin(), A.end());
-        c.coarse_inv32.upload(Af, c.stream);
+        c.coarse_inv32.alloc(n * n);
+        convert_vec(c, dA.p, c.coarse_inv32.p, n * n);
         STK_HIP(hipStreamSynchronize(c.stream));
     }
-    STK_HIP(hipStreamSynchronize(c.stream));
 }
 
 template <typename T>
@@ -1360,7 +1385,7 @@ double time_kernel(Context &c, int kernel, int reps, double *bytes) {
             float *x = x32.p, *y = y32.p;
             launch_map(c, n, [=] __device__(int64_t i) { x[i] = (float)(i % 1021); });
             // 16-byte accesses: the attainable streaming rate of the part
-            launch = [&c, x, y, n]() {
+            launch = [&c, x, y]() {
                 const float4 *xs = reinterpret_cast<const float4 *>(x);
                 float4 *ys = reinterpret_cast<float4 *>(y);
                 launch_map(c, n / 4, [=] __device__(int64_t i) { ys[i] = xs[i]; });

@@ -219,6 +219,9 @@ void build_patterns(Context &c);
 void make_sliced(Pattern &p, hipStream_t s);
 // 2x2 operators with at least this many block rows use the sliced layout.
 constexpr int64_t kSellMinRows = 16384;
+// Multigrid coarsening stops at the first P1 lattice with at most this many vertices; that
+// level is solved with a dense inverse (2 dofs per vertex).
+constexpr int64_t kDenseMaxVerts = 2300;
 void build_colouring(Context &c);
 void build_slot_maps(Context &c);
 void detect_structured(Context &c);
