@@ -23,6 +23,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+# FP64 flop per element of the assembly kernel, counted from its loops: per integration point
+# (6) the derivative weights (~50), D B (108 FMA), the lower triangle of B^T (D B) (234 FMA), G (36 FMA),
+# M (~9) and F (~12 FMA); then the node-frame rotation of the 21 K blocks (~336 FMA) and the P1
+# projection of the element matrix for the multigrid (~650 FMA).
+ASM_FLOP_PER_ELEMENT = 6 * (50 + 2 * (108 + 234 + 36 + 12) + 9) + 2 * (336 + 650)
 
 
 def log(*a):
@@ -202,7 +207,11 @@ def main():
         "refinements": reps[-1].refinements,
         "fallbacks": reps[-1].fallbacks,
         "full_residual": reps[-1].full_residual,
+        "schur_residual_last_sweep": reps[-1].schur_residual,
         "assembly_ms": round(asm_ms, 3),
+        "assembly": {"ms": round(asm_ms, 3), "compulsory_GB_per_s": round(asm_bytes / asm_ms / 1e6, 1),
+                     "GFLOP_per_s_fp64": round(ASM_FLOP_PER_ELEMENT * m.nel / asm_ms / 1e6, 1),
+                     "flop_per_element": ASM_FLOP_PER_ELEMENT},
         "learned_setup_ms": round(setup_ms, 3),
         "mlp_training": train,
         "mixed_without_mlp": mixed,
