@@ -162,6 +162,15 @@ def main():
         log(f"plain FP64: {prep.solve_ms:.1f} ms, {prep.outer_iterations} outer, {prep.inner_iterations} inner, "
             f"residual {prep.full_residual:.2e}; fast-vs-plain errU {eu:.2e} errP {ep:.2e}")
 
+    # stress post-processor on the same mesh (7-point rule, anisotropic law)
+    eta7 = np.asfortranarray(np.repeat(np.asarray(inp.eta_ip)[:, :1], 7, axis=1))
+    C7 = [np.asfortranarray(np.repeat(np.asarray(c)[:, :1], 7, axis=1)) for c in inp.C6]
+    ctx.set_stress_fields(eta7, C7, nip=7)
+    st_ms, st_bytes = ctx.time_kernel(_lib.KERNEL_STRESS, reps=20)
+    fg_ms, fg_bytes = ctx.time_kernel(_lib.KERNEL_DEFGRAD, reps=20)
+    log(f"stress!: {st_ms * 1e3:.1f} us ({st_bytes / st_ms / 1e6:.0f} GB/s compulsory); "
+        f"deformation_gradient!: {fg_ms * 1e3:.1f} us ({fg_bytes / fg_ms / 1e6:.0f} GB/s)")
+
     kms, kbytes = ctx.time_kernel(_lib.KERNEL_CHEB_K32, reps=50)
     achieved = kbytes / (kms * 1e-3) / 1e9
     sms, sbytes = ctx.time_kernel(_lib.KERNEL_SPMV_K32, reps=50)
@@ -222,6 +231,10 @@ def main():
                      "kernel": "k_spmv_sell22<256,float,float,EpCheb<float>> (fused Chebyshev SpMV on K, sliced layout)",
                      "avg_us": round(kms * 1e3, 2), "bytes_per_launch": kbytes},
         "cpu_baseline": cpu,
+        "stress_postprocessor": {
+            "stress_us": round(st_ms * 1e3, 1), "stress_GB_per_s": round(st_bytes / st_ms / 1e6, 1),
+            "deformation_gradient_us": round(fg_ms * 1e3, 1),
+            "deformation_gradient_GB_per_s": round(fg_bytes / fg_ms / 1e6, 1)},
     }
     ctx.close()
     print(json.dumps(out), flush=True)

@@ -27,10 +27,12 @@ EXPORTED = [
     "stk_assemble", "stk_export_csc", "stk_export_rhs", "stk_spmv", "stk_solve", "stk_get_solution",
     "stk_updatevelocity", "stk_time_kernel", "stk_default_train_opts", "stk_set_fabric", "stk_mlp_train",
     "stk_mlp_params", "stk_mlp_forward", "stk_mlp_gradient", "stk_local_schur", "stk_learned_setup",
+    "stk_set_stress_fields", "stk_stress", "stk_deformation_gradient",
 ]
 N_PARAM = 9094
 KERNEL_SPMV_K32, KERNEL_SPMV_K64, KERNEL_CHEB_K32, KERNEL_CHEB_K64, KERNEL_ASSEMBLY, KERNEL_CHEB_KH16 = 0, 1, 2, 3, 4, 5
-KERNEL_COPY, KERNEL_VCYCLE32, KERNEL_INNER_IT32 = 6, 7, 8
+KERNEL_COPY, KERNEL_VCYCLE32, KERNEL_INNER_IT32, KERNEL_STRESS, KERNEL_DEFGRAD = 6, 7, 8, 9, 10
+STRESS_FIELDS = ("exx", "ezz", "exz", "eII", "txx", "tzz", "txz", "tII")
 
 
 class StkOpts(C.Structure):
@@ -121,6 +123,9 @@ def load(path: Path | str | None = None):
     lib.stk_mlp_gradient.argtypes = [vp, C.c_int64, fp, fp, fp, dp]
     lib.stk_local_schur.argtypes = [vp, dp, fp]
     lib.stk_learned_setup.argtypes = [vp]
+    lib.stk_set_stress_fields.argtypes = [vp, dp, C.c_int32, C.POINTER(dp)]
+    lib.stk_stress.argtypes = [vp, dp, C.POINTER(dp)]
+    lib.stk_deformation_gradient.argtypes = [vp, dp, C.c_double, dp]
     for name in EXPORTED:
         if name not in ("stk_default_opts", "stk_destroy", "stk_last_error", "stk_default_train_opts"):
             getattr(lib, name).restype = C.c_int
@@ -317,3 +322,40 @@ class Context:
         w = C.c_double()
         self._check(self.lib.stk_updatevelocity(self.h, _d(U), _d(Uc), C.byref(w)))
         return U, Uc, w.value
+
+    def set_stress_fields(self, eta_ip, C6=None, nip=7):
+        """eta.ip and the tensor arrays (nel x ncol, ncol >= nip) at the nip-point stress rule."""
+        eta = np.asfortranarray(eta_ip, dtype=np.float64)
+        if eta.ndim != 2 or eta.shape[0] != self.nel or eta.shape[1] < nip:
+            raise ValueError(f"eta.ip must be nel x >= {nip}")
+        self._sfields = [eta]
+        arr = None
+        if C6 is not None:
+            comps = [np.asfortranarray(c, dtype=np.float64) for c in C6]
+            for c in comps:
+                if c.shape != eta.shape:
+                    raise ValueError("tensor components must have the shape of eta.ip")
+            self._sfields += comps
+            arr = (dp * 6)(*[_d(c) for c in comps])
+        self._check(self.lib.stk_set_stress_fields(self.h, _d(eta), int(nip), arr))
+        self.s_nip = int(nip)
+
+    def stress(self, Ucart=None):
+        """stress!: dict of the eight nel x nip arrays (Fortran order), see STRESS_FIELDS."""
+        outs = {k: np.zeros((self.nel, self.s_nip), order="F") for k in STRESS_FIELDS}
+        arr = (dp * 8)(*[_d(outs[k]) for k in STRESS_FIELDS])
+        u = None if Ucart is None else np.ascontiguousarray(Ucart, dtype=np.float64)
+        self._check(self.lib.stk_stress(self.h, None if u is None else _d(u), arr))
+        return outs
+
+    def deformation_gradient(self, F, dt, U=None):
+        """One RK4 step of F (nel x nip x 2 x 2 as float64[nel, nip, 2, 2] with F[e, q] the 2x2 matrix);
+        returns the updated copy."""
+        F = np.asarray(F, dtype=np.float64)
+        if F.shape != (self.nel, self.s_nip, 2, 2):
+            raise ValueError("F must be nel x nip x 2 x 2")
+        # Julia Matrix{SMatrix{2,2}}: points column-major, each 2x2 column-major
+        buf = np.ascontiguousarray(np.transpose(F, (1, 0, 3, 2))).reshape(-1)
+        u = None if U is None else np.ascontiguousarray(U, dtype=np.float64)
+        self._check(self.lib.stk_deformation_gradient(self.h, None if u is None else _d(u), float(dt), _d(buf)))
+        return np.transpose(buf.reshape(self.s_nip, self.nel, 2, 2), (1, 0, 3, 2)).copy()

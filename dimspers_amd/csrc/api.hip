@@ -7,6 +7,7 @@
 #include "device_ops.h"
 #include "mlp.h"
 #include "solver.h"
+#include "stress.h"
 
 namespace stk {
 template <typename S, typename D> void convert(Context &c, const S *src, D *dst, int64_t n);
@@ -574,11 +575,86 @@ int stk_updatevelocity(stk_ctx *ctx, double *U, double *Ucart, double *omega) {
     return guarded(ctx, [&]() {
         Context &c = ctx->c;
         STK_CHECK(c.U && c.have_fields, STK_E_STATE, "no solution");
-        DBuf<double> uc(c.nU);
-        const double w = update_velocity(c, uc.p);
+        c.d_Ucart.alloc(c.nU);
+        const double w = update_velocity(c, c.d_Ucart.p);
         if (omega) *omega = w;
         if (U) c.U.download(U, c.nU, c.stream);
-        if (Ucart) uc.download(Ucart, c.nU, c.stream);
+        if (Ucart) c.d_Ucart.download(Ucart, c.nU, c.stream);
+        STK_HIP(hipStreamSynchronize(c.stream));
+        return STK_OK;
+    });
+}
+
+int stk_set_stress_fields(stk_ctx *ctx, const double *eta_ip, int32_t nip, const double *const C[6]) {
+    if (!ctx) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK(eta_ip, STK_E_ARG, "NULL eta.ip");
+        stress_set_rule(c, nip);
+        stress_prepare(c);
+        const int64_t plane = c.nel * (int64_t)nip;
+        // Julia nel x ncol column-major: the first nip columns are contiguous
+        c.d_seta.upload(eta_ip, plane, c.stream);
+        if (C) {
+            std::vector<double> buf(6 * plane);
+            for (int k = 0; k < 6; ++k) {
+                STK_CHECK(C[k], STK_E_ARG, "NULL tensor component");
+                std::copy(C[k], C[k] + plane, buf.begin() + k * plane);
+            }
+            c.d_sC.upload(buf, c.stream);
+            STK_HIP(hipStreamSynchronize(c.stream));
+            c.s_aniso = true;
+        } else {
+            c.s_aniso = false;
+        }
+        STK_HIP(hipStreamSynchronize(c.stream));
+        c.have_sfields = true;
+        return STK_OK;
+    });
+}
+
+int stk_stress(stk_ctx *ctx, const double *Ucart, double *const out[8]) {
+    if (!ctx) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK(c.have_sfields, STK_E_STATE, "stress fields not set");
+        DBuf<double> u;
+        const double *Ud = c.d_Ucart.p;
+        if (Ucart) {
+            u.upload(Ucart, c.nU, c.stream);
+            Ud = u.p;
+        }
+        STK_CHECK(Ud, STK_E_STATE, "no device velocity: call stk_updatevelocity or pass Ucart");
+        const int64_t plane = c.nel * (int64_t)c.s_nip;
+        DBuf<double> o(8 * plane);
+        launch_stress(c, Ud, o.p);
+        if (out)
+            for (int k = 0; k < 8; ++k)
+                if (out[k]) STK_HIP(hipMemcpyAsync(out[k], o.p + k * plane, plane * sizeof(double),
+                                                   hipMemcpyDeviceToHost, c.stream));
+        STK_HIP(hipStreamSynchronize(c.stream));
+        return STK_OK;
+    });
+}
+
+int stk_deformation_gradient(stk_ctx *ctx, const double *U, double dt, double *F) {
+    if (!ctx || !F) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK(c.s_nip > 0, STK_E_STATE, "stress rule not set (stk_set_stress_fields)");
+        stress_prepare(c);
+        DBuf<double> u;
+        const double *Ud = c.U.p;
+        if (U) {
+            u.upload(U, c.nU, c.stream);
+            Ud = u.p;
+        }
+        STK_CHECK(Ud, STK_E_STATE, "no device solution: solve first or pass U");
+        const int64_t n = 4 * c.nel * (int64_t)c.s_nip;
+        DBuf<double> f;
+        f.upload(F, n, c.stream);
+        launch_deformation_gradient(c, Ud, dt, f.p);
+        f.download(F, n, c.stream);
         STK_HIP(hipStreamSynchronize(c.stream));
         return STK_OK;
     });

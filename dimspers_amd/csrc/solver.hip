@@ -27,6 +27,7 @@
 
 #include "device_ops.h"
 #include "solver.h"
+#include "stress.h"
 
 namespace stk {
 
@@ -1377,6 +1378,33 @@ double time_kernel(Context &c, int kernel, int reps, double *bytes) {
                 inner_cg<float>(c, *view, true, y, z, 1e-30, 20, 2, &conv);
             };
             *bytes = 0.0;
+            break;
+        }
+        case STK_KERNEL_STRESS: case STK_KERNEL_DEFGRAD: {
+            STK_CHECK(c.have_sfields, STK_E_STATE, "stress fields not set");
+            x64.alloc(nU);
+            double *u = x64.p;
+            const double *nr = c.d_noder.p;
+            // synthetic velocity: smooth in r, varying with the node index
+            launch_map(c, nU, [=] __device__(int64_t i) {
+                u[i] = (i % 2 ? 0.3 : 1.0) * nr[i / 2] + 1e-3 * (double)(i % 7);
+            });
+            const int64_t plane = c.nel * (int64_t)c.s_nip;
+            const int64_t nel = c.nel;
+            // compulsory: element connectivity and vertices, the velocity once, eta (+ tensor)
+            const double in = (double)nel * (6 * 4 + 6 * 8) + (double)nU * 8 + (double)plane * 8;
+            if (kernel == STK_KERNEL_STRESS) {
+                o64.alloc(8 * plane);
+                double *o = o64.p;
+                launch = [&c, u, o]() { launch_stress(c, u, o); };
+                *bytes = in + (c.s_aniso ? 6.0 * plane * 8 : 0.0) + 8.0 * plane * 8;
+            } else {
+                o64.alloc(4 * plane);
+                double *F = o64.p;
+                launch_map(c, 4 * plane, [=] __device__(int64_t i) { F[i] = (i % 4 == 0 || i % 4 == 3) ? 1.0 : 0.0; });
+                launch = [&c, u, F]() { launch_deformation_gradient(c, u, 1e-3, F); };
+                *bytes = (double)nel * (6 * 4 + 6 * 8) + (double)nU * 8 + 2.0 * 4 * plane * 8;
+            }
             break;
         }
         case STK_KERNEL_COPY: {

@@ -201,6 +201,15 @@ struct Context {
     DBuf<float> coarse_inv32;
     int64_t coarse_n = 0;
 
+    // stress post-processor: fields at the stress rule, natural-order element data,
+    // device Cartesian velocity of the last stk_updatevelocity
+    int s_nip = 0;
+    bool s_aniso = false, have_sfields = false;
+    DBuf<double> d_seta, d_sC;               // nel x nip; 6 components x nel x nip
+    DBuf<int32_t> d_e2n_nat;                 // 6 x nel
+    DBuf<double> d_th_nat, d_r_nat;          // 3 x nel
+    DBuf<double> d_Ucart;                    // nU
+
     // solution and work
     DBuf<double> U, P, work64;
     DBuf<double> red64;                       // reduction partials

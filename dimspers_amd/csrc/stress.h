@@ -1,0 +1,15 @@
+// Stress post-processor and deformation gradient (stress.hip).
+#pragma once
+
+#include "stk_internal.h"
+
+namespace stk {
+
+void stress_prepare(Context &c);
+void stress_set_rule(Context &c, int nip);
+// out: 8 planes of nel x nip (exx, ezz, exz, eII, txx, tzz, txz, tII)
+void launch_stress(Context &c, const double *U, double *out);
+// F: 4 x (nel x nip), column-major 2x2 per point, advanced in place by one RK4 step
+void launch_deformation_gradient(Context &c, const double *U, double dt, double *F);
+
+}  // namespace stk

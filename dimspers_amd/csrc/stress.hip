@@ -1,0 +1,252 @@
+// Stress post-processor and deformation gradient on the solved velocity.
+//
+// stress! (src/Stress/StressCalculation.jl:23-127 isotropic, :129-242 anisotropic,
+// driver :244-249) evaluates, at every point of the stress rule, the strain rate from the
+// Cartesian velocity, its second invariant, the deviatoric stress and its invariant.
+// deformation_gradient! (:251-349) advances the 2x2 deformation gradient F at every point
+// by one RK4 step of F' = (grad U)^T F.  Both map the P2 shape-function derivatives from the
+// reference triangle to (theta, r) and then to Cartesian (x, z) with the "double Jacobian":
+// d/dx = sin(theta) d/dr + cos(theta)/r d/dtheta, d/dz = cos(theta) d/dr - sin(theta)/r d/dtheta.
+//
+// One thread per element in natural element order: the eta / tensor inputs and all outputs
+// are nel x nip column-major (element fastest), so every one of them is read or written
+// coalesced; the 12 velocity values of an element are gathered (they are shared with the
+// neighbours and mostly hit in cache).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "device_ops.h"
+#include "stress.h"
+
+namespace stk {
+
+namespace {
+
+struct StressRule {
+    int nip;
+    double xi[7], eta[7];
+};
+__constant__ StressRule c_srule;
+
+// Points of ip_triangle6 / ip_triangle7 (src/Algebra/Quadrature.jl:80-138), in their order.
+StressRule make_rule(int nip) {
+    StressRule q{};
+    q.nip = nip;
+    if (nip == 7) {
+        const double g1 = (6.0 - std::sqrt(15.0)) / 21.0, g2 = (6.0 + std::sqrt(15.0)) / 21.0;
+        const double x[7] = {1.0 / 3.0, 1.0 - 2.0 * g1, g1, g1, 1.0 - 2.0 * g2, g2, g2};
+        const double y[7] = {1.0 / 3.0, g1, 1.0 - 2.0 * g1, g1, g2, 1.0 - 2.0 * g2, g2};
+        for (int i = 0; i < 7; ++i) { q.xi[i] = x[i]; q.eta[i] = y[i]; }
+    } else {
+        const double s = std::sqrt(38.0 - 44.0 * std::sqrt(2.0 / 5.0));
+        const double g1 = (8.0 - std::sqrt(10.0) + s) / 18.0, g2 = (8.0 - std::sqrt(10.0) - s) / 18.0;
+        const double x[6] = {1.0 - 2.0 * g1, g1, g1, 1.0 - 2.0 * g2, g2, g2};
+        const double y[6] = {g1, 1.0 - 2.0 * g1, g1, g2, 1.0 - 2.0 * g2, g2};
+        for (int i = 0; i < 6; ++i) { q.xi[i] = x[i]; q.eta[i] = y[i]; }
+    }
+    return q;
+}
+
+// Cartesian derivatives of the six P2 shape functions at reference point (xi, eta) of an
+// element with vertex coordinates th[3], r[3] (StressCalculation.jl:145-205).
+__device__ __forceinline__ void cartesian_gradients(const double th[3], const double r[3], double xi, double et,
+                                                    double dx[6], double dz[6]) {
+    const double R21 = r[1] - r[0], R31 = r[2] - r[0];
+    const double Th31 = th[2] - th[0], Th21 = th[1] - th[0];
+    const double invdet = 1.0 / (Th21 * R31 - R21 * Th31);
+    const double t = 1.0 - xi - et;
+    // P2 derivatives w.r.t. (xi, eta); node order: vertices 1,2,3, midpoints 12, 23, 31
+    const double dr[6] = {-(4.0 * t - 1.0), 4.0 * xi - 1.0, 0.0, 4.0 * (t - xi), 4.0 * et, -4.0 * et};
+    const double ds[6] = {-(4.0 * t - 1.0), 0.0, 4.0 * et - 1.0, -4.0 * xi, 4.0 * xi, 4.0 * (t - et)};
+    const double thq = th[0] * t + th[1] * xi + th[2] * et;
+    const double rq = r[0] * t + r[1] * xi + r[2] * et;
+    double sn, cs;
+    sincos(thq, &sn, &cs);
+    const double cr = cs / rq, sr = sn / rq;
+    const double j00 = R31 * cr - Th31 * sn, j01 = -R21 * cr + Th21 * sn;
+    const double j10 = -R31 * sr - Th31 * cs, j11 = R21 * sr + Th21 * cs;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        dx[b] = (j00 * dr[b] + j01 * ds[b]) * invdet;
+        dz[b] = (j10 * dr[b] + j11 * ds[b]) * invdet;
+    }
+}
+
+struct StressArgs {
+    int64_t nel;
+    const int32_t *e2n;             // 6 x nel, natural element order (SoA)
+    const double *th, *r;           // 3 x nel (SoA)
+    const double *U;                // Cartesian velocity (x, z) per node
+    const double *eta;              // nel x nip
+    const double *C;                // 6 components x (nel x nip), or null (isotropic)
+    double *out;                    // 8 x (nel x nip): exx, ezz, exz, eII, txx, tzz, txz, tII
+};
+
+__global__ __launch_bounds__(256) void k_stress(StressArgs a) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nel) return;
+    const int64_t nel = a.nel;
+    double th[3], r[3], ux[6], uz[6];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) { th[j] = a.th[j * nel + e]; r[j] = a.r[j * nel + e]; }
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const int64_t n = a.e2n[b * nel + e];
+        ux[b] = a.U[2 * n];
+        uz[b] = a.U[2 * n + 1];
+    }
+    const int nip = c_srule.nip;
+    const int64_t plane = nel * nip;
+    for (int q = 0; q < nip; ++q) {
+        double dx[6], dz[6];
+        cartesian_gradients(th, r, c_srule.xi[q], c_srule.eta[q], dx, dz);
+        double exx = 0, ezz = 0, duxdz = 0, duzdx = 0;
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+            exx += dx[b] * ux[b];
+            ezz += dz[b] * uz[b];
+            duxdz += dz[b] * ux[b];
+            duzdx += dx[b] * uz[b];
+        }
+        const double exz = 0.5 * (duxdz + duzdx);
+        const int64_t k = q * nel + e;
+        const double eta = a.eta[k];
+        double txx, tzz, txz;
+        if (a.C) {
+            // anisotropic law of :223-235, with the reference's naming of the tensor components
+            const double h11 = a.C[0 * plane + k] * eta, h22 = a.C[1 * plane + k] * eta;
+            const double h33 = a.C[2 * plane + k] * eta, h12 = a.C[3 * plane + k] * eta;
+            const double h13 = a.C[4 * plane + k] * eta, h23 = a.C[5 * plane + k] * eta;
+            const double u = -exx / 3.0 + 2.0 * ezz / 3.0, v = 2.0 * exx / 3.0 - ezz / 3.0;
+            txx = h12 * u + h11 * v + h13 * exz;
+            tzz = h22 * u + h12 * v + h23 * 2.0 * exz;
+            txz = h23 * u + h13 * v + h33 * exz;
+        } else {
+            // isotropic law of :117-120
+            txx = eta * (4.0 / 3.0 * exx - 2.0 / 3.0 * ezz);
+            tzz = eta * (-2.0 / 3.0 * exx + 4.0 / 3.0 * ezz);
+            txz = eta * 2.0 * exz;
+        }
+        a.out[0 * plane + k] = exx;
+        a.out[1 * plane + k] = ezz;
+        a.out[2 * plane + k] = exz;
+        a.out[3 * plane + k] = sqrt(0.5 * (exx * exx + ezz * ezz) + exz * exz);
+        a.out[4 * plane + k] = txx;
+        a.out[5 * plane + k] = tzz;
+        a.out[6 * plane + k] = txz;
+        a.out[7 * plane + k] = sqrt(0.5 * (txx * txx + tzz * tzz) + txz * txz);
+    }
+}
+
+struct DefGradArgs {
+    int64_t nel;
+    const int32_t *e2n;
+    const double *th, *r;
+    const double *U;                // velocity as the caller passes it (x, z per node)
+    double dt;
+    double *F;                      // 4 x (nel x nip): column-major 2x2 per point
+};
+
+__global__ __launch_bounds__(256) void k_deformation_gradient(DefGradArgs a) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.nel) return;
+    const int64_t nel = a.nel;
+    double th[3], r[3], ux[6], uz[6];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) { th[j] = a.th[j * nel + e]; r[j] = a.r[j * nel + e]; }
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+        const int64_t n = a.e2n[b * nel + e];
+        ux[b] = a.U[2 * n];
+        uz[b] = a.U[2 * n + 1];
+    }
+    const int nip = c_srule.nip;
+    for (int q = 0; q < nip; ++q) {
+        double dx[6], dz[6];
+        cartesian_gradients(th, r, c_srule.xi[q], c_srule.eta[q], dx, dz);
+        double uxx = 0, uzz = 0, uxz = 0, uzx = 0;
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+            uxx += dx[b] * ux[b];
+            uzz += dz[b] * uz[b];
+            uxz += dz[b] * ux[b];
+            uzx += dx[b] * uz[b];
+        }
+        // L = dt (grad U)^T = dt [[dUx/dx, dUx/dz], [dUz/dx, dUz/dz]]  (:304-326)
+        const double L00 = a.dt * uxx, L01 = a.dt * uxz, L10 = a.dt * uzx, L11 = a.dt * uzz;
+        double *Fp = a.F + 4 * (q * nel + e);
+        const double f00 = Fp[0], f10 = Fp[1], f01 = Fp[2], f11 = Fp[3];
+        // classic RK4 on F' = L F (:328-338)
+        auto mul = [&](double a00, double a10, double a01, double a11, double &b00, double &b10, double &b01,
+                       double &b11) {
+            b00 = L00 * a00 + L01 * a10;
+            b10 = L10 * a00 + L11 * a10;
+            b01 = L00 * a01 + L01 * a11;
+            b11 = L10 * a01 + L11 * a11;
+        };
+        double k1[4], k2[4], k3[4], k4[4];
+        mul(f00, f10, f01, f11, k1[0], k1[1], k1[2], k1[3]);
+        mul(0.5 * k1[0] + f00, 0.5 * k1[1] + f10, 0.5 * k1[2] + f01, 0.5 * k1[3] + f11, k2[0], k2[1], k2[2], k2[3]);
+        mul(0.5 * k2[0] + f00, 0.5 * k2[1] + f10, 0.5 * k2[2] + f01, 0.5 * k2[3] + f11, k3[0], k3[1], k3[2], k3[3]);
+        mul(k3[0] + f00, k3[1] + f10, k3[2] + f01, k3[3] + f11, k4[0], k4[1], k4[2], k4[3]);
+        const double f0[4] = {f00, f10, f01, f11};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Fp[i] = f0[i] + (k1[i] + 2.0 * (k2[i] + k3[i]) + k4[i]) * (1.0 / 6.0);
+    }
+}
+
+}  // namespace
+
+// Natural-order copies of the element connectivity and vertex coordinates (once per mesh).
+void stress_prepare(Context &c) {
+    if (c.d_e2n_nat) return;
+    const int64_t nel = c.nel;
+    std::vector<int32_t> e2n(6 * nel);
+    std::vector<double> th(3 * nel), r(3 * nel);
+    for (int64_t e = 0; e < nel; ++e) {
+        for (int a = 0; a < 6; ++a) e2n[a * nel + e] = c.e2n[e * 6 + a];
+        for (int j = 0; j < 3; ++j) {
+            th[j * nel + e] = c.el_th[e * 3 + j];
+            r[j * nel + e] = c.el_r[e * 3 + j];
+        }
+    }
+    c.d_e2n_nat.upload(e2n, c.stream);
+    c.d_th_nat.upload(th, c.stream);
+    c.d_r_nat.upload(r, c.stream);
+    STK_HIP(hipStreamSynchronize(c.stream));
+}
+
+void stress_set_rule(Context &c, int nip) {
+    if (nip != 6 && nip != 7) throw Error(STK_E_SHAPE, "the stress rule has 6 or 7 points");
+    const StressRule q = make_rule(nip);
+    STK_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_srule), &q, sizeof(q), 0, hipMemcpyHostToDevice, c.stream));
+    STK_HIP(hipStreamSynchronize(c.stream));
+    c.s_nip = nip;
+}
+
+void launch_stress(Context &c, const double *U, double *out) {
+    StressArgs a{};
+    a.nel = c.nel;
+    a.e2n = c.d_e2n_nat.p; a.th = c.d_th_nat.p; a.r = c.d_r_nat.p;
+    a.U = U;
+    a.eta = c.d_seta.p;
+    a.C = c.s_aniso ? c.d_sC.p : nullptr;
+    a.out = out;
+    k_stress<<<dim3((unsigned)((c.nel + 255) / 256)), dim3(256), 0, c.stream>>>(a);
+    STK_HIP(hipGetLastError());
+}
+
+void launch_deformation_gradient(Context &c, const double *U, double dt, double *F) {
+    DefGradArgs a{};
+    a.nel = c.nel;
+    a.e2n = c.d_e2n_nat.p; a.th = c.d_th_nat.p; a.r = c.d_r_nat.p;
+    a.U = U;
+    a.dt = dt;
+    a.F = F;
+    k_deformation_gradient<<<dim3((unsigned)((c.nel + 255) / 256)), dim3(256), 0, c.stream>>>(a);
+    STK_HIP(hipGetLastError());
+}
+
+}  // namespace stk

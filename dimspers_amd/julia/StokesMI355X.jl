@@ -122,4 +122,39 @@ function solveStokes(U, P, T, gr, Ucartesian, Upolar, g, ρ, η, 𝓒, εII, coo
     return Ucartesian, Upolar, U, Ucart, P, to
 end
 
+"""GPU `stress!` (src/Stress/StressCalculation.jl:244-249): fills ε, εII, τ, τII (nel × nip,
+nip = size(ε.xx, 2), 7 in MAIN.jl:94,164) from the Cartesian velocity `U` (MAIN passes Ucart).
+`anisotropic` selects the `Val{Anisotropic}` method (:129-242), else `Val{Isotropic}` (:23-127)."""
+function stress!(ε, εII, τ, τII, η, 𝓒, U, nel, DoF_U, coordinates, SF_Stress; gr, UBC, anisotropic=true)
+    ctx = context(gr, coordinates, UBC)
+    nip = size(ε.xx, 2)
+    ip = Matrix{Float64}(η.ip)
+    C = [Matrix{Float64}(getfield(𝓒, f)) for f in (:η11, :η33, :η55, :η13, :η15, :η35)]
+    ptrs = [pointer(c) for c in C]
+    GC.@preserve ip C ptrs begin
+        check(ctx, ccall((:stk_set_stress_fields, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int32, Ptr{Ptr{Float64}}),
+                         ctx.h, ip, nip, anisotropic ? ptrs : C_NULL))
+    end
+    outs = (ε.xx, ε.zz, ε.xz, εII, τ.xx, τ.zz, τ.xz, τII)
+    optr = [pointer(o) for o in outs]
+    Uc = Vector{Float64}(U)
+    GC.@preserve outs optr Uc begin
+        check(ctx, ccall((:stk_stress, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Ptr{Float64}}), ctx.h, Uc, optr))
+    end
+    return nothing
+end
+
+"""GPU `deformation_gradient!` (StressCalculation.jl:342-349): one RK4 step of every F[iel, ip]
+(a `Matrix{SMatrix{2,2,Float64,4}}`, 4 doubles per point).  Uses the stress rule set by the
+preceding `stress!` call, as MAIN.jl:241-254 orders them."""
+function deformation_gradient!(F, U, nel, DoF_U, coordinates, SF_Stress, Δt; gr, UBC)
+    ctx = context(gr, coordinates, UBC)
+    Uc = Vector{Float64}(U)
+    GC.@preserve F Uc begin
+        check(ctx, ccall((:stk_deformation_gradient, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Float64, Ptr{Float64}),
+                         ctx.h, Uc, Δt, Ptr{Float64}(pointer(F))))
+    end
+    return F
+end
+
 end # module

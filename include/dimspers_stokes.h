@@ -177,8 +177,31 @@ int stk_get_solution(stk_ctx *ctx, double *U, double *P);
 
 /* Net-rotation removal and Cartesian velocity.  Replaces updatevelocity2
  * (StokesSolver.jl:170-184) with volume_integral (src/Stress/FSE.jl:412-464).
- * Operates on the device solution; U (polar, nU) and Ucart (nU) may be NULL. */
+ * Operates on the device solution and keeps the Cartesian velocity on the device
+ * (the default input of stk_stress); U (polar, nU) and Ucart (nU) may be NULL. */
 int stk_updatevelocity(stk_ctx *ctx, double *U, double *Ucart, double *omega);
+
+/* Fields of the stress post-processor: eta.ip and, for the anisotropic law, the six
+ * StiffnessTensor arrays (order of stk_set_fields), each nel x ncol column-major with
+ * ncol >= nip; the first nip columns are the points of the nip-point rule (6 or 7;
+ * MAIN.jl:164 builds the stress shape functions with 7).  C = NULL selects the isotropic
+ * method of _stress! (src/Stress/StressCalculation.jl:23-127), else the anisotropic one
+ * (:129-242). */
+int stk_set_stress_fields(stk_ctx *ctx, const double *eta_ip, int32_t nip, const double *const C[6]);
+
+/* stress! (src/Stress/StressCalculation.jl:244-249): strain rate, its second invariant,
+ * deviatoric stress and its second invariant at every point of the stress rule, from the
+ * Cartesian velocity Ucart (x, z per node, nU; MAIN.jl:241-243 passes Ucart).  Ucart = NULL
+ * uses the device velocity of the last stk_updatevelocity.  out[8] receive eps.xx, eps.zz,
+ * eps.xz, epsII, tau.xx, tau.zz, tau.xz, tauII, each nel x nip column-major; any may be NULL. */
+int stk_stress(stk_ctx *ctx, const double *Ucart, double *const out[8]);
+
+/* deformation_gradient! (src/Stress/StressCalculation.jl:251-349): one RK4 step of
+ * F' = dt (grad U)^T F at every point of the stress rule.  F (inout) is the Julia
+ * Matrix{SMatrix{2,2}} nel x nip: 4 doubles (column-major 2x2) per point, points column-
+ * major.  U = NULL uses the device solution in the node frames, which is what MAIN.jl:254
+ * passes (the velocity after stk_updatevelocity's rotation removal). */
+int stk_deformation_gradient(stk_ctx *ctx, const double *U, double dt, double *F);
 
 /* ---------------------------------------------------------------------------
  * Learned local-Schur preconditioner (no counterpart in the reference).  A
@@ -250,7 +273,9 @@ enum {
     STK_KERNEL_CHEB_KH16 = 5,   /* Chebyshev step on the FP16 scaled copy of K (smoother) */
     STK_KERNEL_COPY = 6,        /* plain copy of 256 MiB (FP32): traffic-counter reference */
     STK_KERNEL_VCYCLE32 = 7,    /* one FP32 multigrid V-cycle (the inner preconditioner)  */
-    STK_KERNEL_INNER_IT32 = 8   /* one FP32 inner CG iteration (V-cycle, K p, updates)    */
+    STK_KERNEL_INNER_IT32 = 8,  /* one FP32 inner CG iteration (V-cycle, K p, updates)    */
+    STK_KERNEL_STRESS = 9,      /* stress! on the stress fields (synthetic velocity)      */
+    STK_KERNEL_DEFGRAD = 10     /* deformation_gradient! RK4 step (synthetic velocity)    */
 };
 
 /* Measurement hook with no counterpart in the reference: runs one of the hot

@@ -1,0 +1,102 @@
+"""FP64 CPU oracle of the stress post-processor -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/`` may import this module; the product path never calls it.
+
+Restates, vectorised over elements:
+
+* ``stress``               -- ``stress!`` (src/Stress/StressCalculation.jl:244-249) with
+  the isotropic ``_stress!`` (:23-127, law :117-120) and the anisotropic one
+  (:129-242, law :223-235); strain rate from the Cartesian velocity (MAIN.jl:241-243
+  passes Ucart), second invariants sqrt(0.5 (xx^2 + zz^2) + xz^2).
+* ``deformation_gradient`` -- ``deformation_gradient!`` (:251-349): one RK4 step of
+  F' = dt (grad U)^T F at every point (:328-338).
+
+Both use the stress shape functions of ``stress_shape_functions(nip=7)``
+(src/src/Stokes/AssemblerStokes.jl:303-311, MAIN.jl:164): P2 derivatives and P1 N3 at
+the points of ``ip_triangle7`` (src/Algebra/Quadrature.jl:109-138), mapped by the polar
+"double Jacobian" (StressCalculation.jl:145-205).
+
+Pinning (tests/test_stress_oracle.py): a velocity linear in (theta, r) is reproduced
+exactly by P2, so its Cartesian gradient has a closed form at every point; the RK4 step of
+F' = L F with constant L is the fourth-order Taylor polynomial of exp(L); the isotropic law
+is tau = 2 eta dev(eps) (three-dimensional deviator); the anisotropic law is checked against
+a separate transcription of :223-235 as a 3x3 coefficient matrix.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from dimspers_amd.quadrature import ip_triangle6, ip_triangle7, sf_dN_tri6, sf_N_tri3
+
+STRESS_FIELDS = ("exx", "ezz", "exz", "eII", "txx", "tzz", "txz", "tII")
+
+
+def _rule(nip):
+    return (ip_triangle7() if nip == 7 else ip_triangle6())[0]
+
+
+def cartesian_gradients(mesh, nip=7):
+    """dN/dx, dN/dz of the six P2 functions at every point: two (nel, nip, 6) arrays, and
+    the points' (theta, r) (StressCalculation.jl:150-205)."""
+    th, r = mesh.el_theta.T, mesh.el_r.T                      # (nel, 3)
+    R21, R31 = r[:, 1] - r[:, 0], r[:, 2] - r[:, 0]
+    Th31, Th21 = th[:, 2] - th[:, 0], th[:, 1] - th[:, 0]
+    invdet = 1.0 / (Th21 * R31 - R21 * Th31)
+    x = _rule(nip)
+    dN = sf_dN_tri6(x[:, 0], x[:, 1])                         # (nip, 2, 6)
+    N3 = sf_N_tri3(x[:, 0], x[:, 1])                          # (nip, 3)
+    thq = th @ N3.T                                           # (nel, nip)
+    rq = r @ N3.T
+    sn, cs = np.sin(thq), np.cos(thq)
+    cr, sr = cs / rq, sn / rq
+    j00 = R31[:, None] * cr - Th31[:, None] * sn
+    j01 = -R21[:, None] * cr + Th21[:, None] * sn
+    j10 = -R31[:, None] * sr - Th31[:, None] * cs
+    j11 = R21[:, None] * sr + Th21[:, None] * cs
+    dx = (j00[..., None] * dN[None, :, 0, :] + j01[..., None] * dN[None, :, 1, :]) * invdet[:, None, None]
+    dz = (j10[..., None] * dN[None, :, 0, :] + j11[..., None] * dN[None, :, 1, :]) * invdet[:, None, None]
+    return dx, dz, thq, rq
+
+
+def velocity_gradient(mesh, U, nip=7):
+    """(dUx/dx, dUz/dz, dUx/dz, dUz/dx) at every point, each (nel, nip); U holds (x, z) per node."""
+    dx, dz, _, _ = cartesian_gradients(mesh, nip)
+    ux = U[0::2][mesh.e2n.T]                                  # (nel, 6)
+    uz = U[1::2][mesh.e2n.T]
+    return (np.einsum("eqb,eb->eq", dx, ux), np.einsum("eqb,eb->eq", dz, uz),
+            np.einsum("eqb,eb->eq", dz, ux), np.einsum("eqb,eb->eq", dx, uz))
+
+
+def stress(mesh, Ucart, eta_ip, C=None, nip=7):
+    """stress!: dict of nel x nip arrays (STRESS_FIELDS)."""
+    exx, ezz, duxdz, duzdx = velocity_gradient(mesh, Ucart, nip)
+    exz = 0.5 * (duxdz + duzdx)
+    eta = np.asarray(eta_ip)[:, :nip]
+    if C is None:
+        txx = eta * (4.0 / 3.0 * exx - 2.0 / 3.0 * ezz)
+        tzz = eta * (-2.0 / 3.0 * exx + 4.0 / 3.0 * ezz)
+        txz = eta * 2.0 * exz
+    else:
+        c11, c33, c55, c13, c15, c35 = (np.asarray(c)[:, :nip] for c in C)
+        h11, h22, h33 = c11 * eta, c33 * eta, c55 * eta
+        h12, h13, h23 = c13 * eta, c15 * eta, c35 * eta
+        u = -exx / 3.0 + 2.0 * ezz / 3.0
+        v = 2.0 * exx / 3.0 - ezz / 3.0
+        txx = h12 * u + h11 * v + h13 * exz
+        tzz = h22 * u + h12 * v + h23 * 2.0 * exz
+        txz = h23 * u + h13 * v + h33 * exz
+    inv = lambda a, b, c: np.sqrt(0.5 * (a * a + b * b) + c * c)
+    return dict(exx=exx, ezz=ezz, exz=exz, eII=inv(exx, ezz, exz),
+                txx=txx, tzz=tzz, txz=txz, tII=inv(txx, tzz, txz))
+
+
+def deformation_gradient(mesh, U, F, dt, nip=7):
+    """deformation_gradient!: F (nel, nip, 2, 2) after one RK4 step (returns a new array)."""
+    uxx, uzz, uxz, uzx = velocity_gradient(mesh, U, nip)
+    L = dt * np.stack([np.stack([uxx, uxz], -1), np.stack([uzx, uzz], -1)], -2)   # (nel, nip, 2, 2)
+    F0 = np.asarray(F, dtype=np.float64)
+    k1 = L @ F0
+    k2 = L @ (0.5 * k1 + F0)
+    k3 = L @ (0.5 * k2 + F0)
+    k4 = L @ (k3 + F0)
+    return F0 + (k1 + 2.0 * (k2 + k3) + k4) / 6.0
