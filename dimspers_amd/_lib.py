@@ -27,7 +27,7 @@ EXPORTED = [
     "stk_assemble", "stk_export_csc", "stk_export_rhs", "stk_spmv", "stk_solve", "stk_get_solution",
     "stk_updatevelocity", "stk_time_kernel", "stk_default_train_opts", "stk_set_fabric", "stk_mlp_train",
     "stk_mlp_params", "stk_mlp_forward", "stk_mlp_gradient", "stk_local_schur", "stk_learned_setup",
-    "stk_set_stress_fields", "stk_stress", "stk_deformation_gradient",
+    "stk_set_stress_fields", "stk_stress", "stk_deformation_gradient", "stk_stats",
 ]
 N_PARAM = 9094
 KERNEL_SPMV_K32, KERNEL_SPMV_K64, KERNEL_CHEB_K32, KERNEL_CHEB_K64, KERNEL_ASSEMBLY, KERNEL_CHEB_KH16 = 0, 1, 2, 3, 4, 5
@@ -126,6 +126,7 @@ def load(path: Path | str | None = None):
     lib.stk_set_stress_fields.argtypes = [vp, dp, C.c_int32, C.POINTER(dp)]
     lib.stk_stress.argtypes = [vp, dp, C.POINTER(dp)]
     lib.stk_deformation_gradient.argtypes = [vp, dp, C.c_double, dp]
+    lib.stk_stats.argtypes = [vp, dp, dp, dp]
     for name in EXPORTED:
         if name not in ("stk_default_opts", "stk_destroy", "stk_last_error", "stk_default_train_opts"):
             getattr(lib, name).restype = C.c_int
@@ -347,6 +348,16 @@ class Context:
         u = None if Ucart is None else np.ascontiguousarray(Ucart, dtype=np.float64)
         self._check(self.lib.stk_stress(self.h, None if u is None else _d(u), arr))
         return outs
+
+    def stats(self, T, U=None):
+        """write_stats: (Urms, <T>, Nusselt top, Nusselt bottom)."""
+        t = np.ascontiguousarray(T, dtype=np.float64)
+        if t.shape != (self.nnod,):
+            raise ValueError("T must hold one value per node")
+        u = None if U is None else np.ascontiguousarray(U, dtype=np.float64)
+        out = np.zeros(4)
+        self._check(self.lib.stk_stats(self.h, None if u is None else _d(u), _d(t), _d(out)))
+        return tuple(out)
 
     def deformation_gradient(self, F, dt, U=None):
         """One RK4 step of F (nel x nip x 2 x 2 as float64[nel, nip, 2, 2] with F[e, q] the 2x2 matrix);

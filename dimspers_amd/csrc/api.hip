@@ -637,6 +637,23 @@ int stk_stress(stk_ctx *ctx, const double *Ucart, double *const out[8]) {
     });
 }
 
+int stk_stats(stk_ctx *ctx, const double *U, const double *T, double out[4]) {
+    if (!ctx || !T || !out) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        DBuf<double> u, t;
+        const double *Ud = c.U.p;
+        if (U) {
+            u.upload(U, c.nU, c.stream);
+            Ud = u.p;
+        }
+        STK_CHECK(Ud, STK_E_STATE, "no device solution: solve first or pass U");
+        t.upload(T, c.nnod, c.stream);
+        statistics(c, Ud, t.p, out);
+        return STK_OK;
+    });
+}
+
 int stk_deformation_gradient(stk_ctx *ctx, const double *U, double dt, double *F) {
     if (!ctx || !F) return STK_E_ARG;
     return guarded(ctx, [&]() {

@@ -209,6 +209,9 @@ struct Context {
     DBuf<int32_t> d_e2n_nat;                 // 6 x nel
     DBuf<double> d_th_nat, d_r_nat;          // 3 x nel
     DBuf<double> d_Ucart;                    // nU
+    // statistics: nodes one radius level inside each wall, spacings, wall radii
+    DBuf<int32_t> d_itop, d_ibot;
+    double nu_dr = 0, nu_dth = 0, nu_rin = 0, nu_rout = 0;
 
     // solution and work
     DBuf<double> U, P, work64;

@@ -11,5 +11,7 @@ void stress_set_rule(Context &c, int nip);
 void launch_stress(Context &c, const double *U, double *out);
 // F: 4 x (nel x nip), column-major 2x2 per point, advanced in place by one RK4 step
 void launch_deformation_gradient(Context &c, const double *U, double dt, double *F);
+// write_stats: out = (Urms, <T>, Nusselt top, Nusselt bottom); U, T device arrays
+void statistics(Context &c, const double *U, const double *T, double out[4]);
 
 }  // namespace stk

@@ -14,6 +14,7 @@
 // neighbours and mostly hit in cache).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -197,6 +198,46 @@ __global__ __launch_bounds__(256) void k_deformation_gradient(DefGradArgs a) {
     }
 }
 
+// volume_integral_cartesian (src/Stress/FSE.jl:466-514) of two nodal fields at once: per
+// element detJ (straight Cartesian triangle of the three vertices) times sum over the 6-point
+// rule of w_q * sum_a N_a(q) V_a, folded into the integrated weights cw[a] = sum_q w_q N_a(q).
+struct VolIntArgs {
+    int64_t nel;
+    const int32_t *e2n;             // 6 x nel natural order
+    const double *cs, *nr;          // node cos/sin (2 per node), node radius
+    const double *U;                // velocity (2 per node): S = Ux^2 + Uz^2 at the nodes
+    const double *T;                // temperature per node
+    double cw[6];
+};
+
+struct VolInt {
+    VolIntArgs a;
+    __device__ Dots<2> operator()(int64_t e) const {
+        const int64_t nel = a.nel;
+        double x[3], z[3];
+        for (int j = 0; j < 3; ++j) {
+            const int64_t n = a.e2n[j * nel + e];
+            x[j] = a.nr[n] * a.cs[2 * n + 1];   // x = r sin(theta)
+            z[j] = a.nr[n] * a.cs[2 * n];       // z = r cos(theta)
+        }
+        const double det = (x[1] - x[0]) * (z[2] - z[0]) - (z[1] - z[0]) * (x[2] - x[0]);
+        double s = 0, t = 0;
+        for (int b = 0; b < 6; ++b) {
+            const int64_t n = a.e2n[b * nel + e];
+            const double ux = a.U[2 * n], uz = a.U[2 * n + 1];
+            s += a.cw[b] * (ux * ux + uz * uz);
+            t += a.cw[b] * a.T[n];
+        }
+        return Dots<2>{{det * s, det * t}};
+    }
+};
+
+struct ListSum {
+    const int32_t *idx;
+    const double *T;
+    __device__ double operator()(int64_t i) const { return T[idx[i]]; }
+};
+
 }  // namespace
 
 // Natural-order copies of the element connectivity and vertex coordinates (once per mesh).
@@ -236,6 +277,78 @@ void launch_stress(Context &c, const double *U, double *out) {
     a.out = out;
     k_stress<<<dim3((unsigned)((c.nel + 255) / 256)), dim3(256), 0, c.stream>>>(a);
     STK_HIP(hipGetLastError());
+}
+
+// write_stats (src/IO/IOmanager.jl:277-322) without the file: Urms, <T>, Nusselt top/bottom.
+void statistics(Context &c, const double *U, const double *T, double out[4]) {
+    stress_prepare(c);
+    if (!c.d_itop) {
+        // ScratchNusselt (IOmanager.jl:338-354): nodes one radius level inside each wall
+        std::vector<double> ru(c.node_r);
+        std::sort(ru.begin(), ru.end());
+        ru.erase(std::unique(ru.begin(), ru.end()), ru.end());
+        if (ru.size() < 3) throw Error(STK_E_SHAPE, "statistics need at least three radius levels");
+        std::vector<int32_t> top, bot;
+        for (int64_t n = 0; n < c.nnod; ++n) {
+            if (c.node_r[n] == ru[ru.size() - 2]) top.push_back((int32_t)n);
+            if (c.node_r[n] == ru[1]) bot.push_back((int32_t)n);
+        }
+        if (top.size() < 2) throw Error(STK_E_SHAPE, "statistics need two nodes below the outer wall");
+        c.nu_dr = ru[1] - ru[0];
+        c.nu_dth = c.node_th[top[0]] - c.node_th[top[1]];
+        c.nu_rin = ru.front();
+        c.nu_rout = ru.back();
+        c.d_itop.upload(top, c.stream);
+        c.d_ibot.upload(bot, c.stream);
+        STK_HIP(hipStreamSynchronize(c.stream));
+    }
+    // integrated P2 weights of the 6-point rule (ip_triangle6, Quadrature.jl:80-107)
+    VolIntArgs a{};
+    {
+        const double s = std::sqrt(38.0 - 44.0 * std::sqrt(2.0 / 5.0));
+        const double g1 = (8.0 - std::sqrt(10.0) + s) / 18.0, g2 = (8.0 - std::sqrt(10.0) - s) / 18.0;
+        const double q = std::sqrt(213125.0 - 53320.0 * std::sqrt(10.0));
+        const double w1 = 0.5 * (620.0 + q) / 3720.0, w2 = 0.5 * (620.0 - q) / 3720.0;
+        const double xs[6] = {1.0 - 2.0 * g1, g1, g1, 1.0 - 2.0 * g2, g2, g2};
+        const double ys[6] = {g1, 1.0 - 2.0 * g1, g1, g2, 1.0 - 2.0 * g2, g2};
+        const double ws[6] = {w1, w1, w1, w2, w2, w2};
+        for (int b = 0; b < 6; ++b) a.cw[b] = 0.0;
+        for (int k = 0; k < 6; ++k) {
+            const double r = xs[k], t = ys[k], u = 1.0 - r - t;
+            const double N[6] = {u * (2 * u - 1), r * (2 * r - 1), t * (2 * t - 1), 4 * r * u, 4 * r * t, 4 * t * u};
+            for (int b = 0; b < 6; ++b) a.cw[b] += ws[k] * N[b];
+        }
+    }
+    a.nel = c.nel;
+    a.e2n = c.d_e2n_nat.p;
+    a.cs = c.d_nodecs.p;
+    a.nr = c.d_noder.p;
+    a.U = U;
+    a.T = T;
+    double *part = c.red64.p, *sc = c.d_sc.p;
+    const int nb = (int)std::min<int64_t>(kReduceBlocks, std::max<int64_t>(1, (c.nel + 255) / 256));
+    if ((int64_t)c.red64.n < 2 * (int64_t)nb) throw Error(STK_E_STATE, "reduction buffer too small");
+    k_reduce_map<256, 2, VolInt><<<dim3(nb), dim3(256), 0, c.stream>>>(c.nel, VolInt{a}, part);
+    k_reduce_final<256><<<dim3(1), dim3(256), 0, c.stream>>>(nb, 2, part, sc + 0);
+    ListSum lt{c.d_itop.p, T}, lb{c.d_ibot.p, T};
+    const int64_t ntop = (int64_t)c.d_itop.n, nbot = (int64_t)c.d_ibot.n;
+    const int nbt = (int)std::min<int64_t>(kReduceBlocks, std::max<int64_t>(1, (ntop + 255) / 256));
+    k_reduce_map<256, 1, ListSum><<<dim3(nbt), dim3(256), 0, c.stream>>>(ntop, lt, part);
+    k_reduce_final<256><<<dim3(1), dim3(256), 0, c.stream>>>(nbt, 1, part, sc + 2);
+    const int nbb = (int)std::min<int64_t>(kReduceBlocks, std::max<int64_t>(1, (nbot + 255) / 256));
+    k_reduce_map<256, 1, ListSum><<<dim3(nbb), dim3(256), 0, c.stream>>>(nbot, lb, part);
+    k_reduce_final<256><<<dim3(1), dim3(256), 0, c.stream>>>(nbb, 1, part, sc + 3);
+    STK_HIP(hipGetLastError());
+    double h[4];
+    STK_HIP(hipMemcpyAsync(h, sc, 4 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    STK_HIP(hipStreamSynchronize(c.stream));
+    const double area = M_PI * (c.nu_rout * c.nu_rout - c.nu_rin * c.nu_rin);
+    const double f = 1.22 / 2.22;                       // radii fixed in ScratchNusselt (:341)
+    const double A = std::log(f) / (2.0 * M_PI * 2.22 * (1.0 - f));
+    out[0] = std::sqrt(h[0] / area);
+    out[1] = h[1] / area;
+    out[2] = A * c.nu_dth * (-h[2] / c.nu_dr) * c.nu_rout;
+    out[3] = A * c.nu_dth * ((h[3] - (double)nbot) / c.nu_dr) * c.nu_rin;
 }
 
 void launch_deformation_gradient(Context &c, const double *U, double dt, double *F) {

@@ -157,4 +157,22 @@ function deformation_gradient!(F, U, nel, DoF_U, coordinates, SF_Stress, Δt; gr
     return F
 end
 
+"""GPU statistics of `write_stats` (src/IO/IOmanager.jl:277-322); appends the same line to
+`stats_file` and returns (Urms, ⟨T⟩, Nu_top, Nu_bot).  `ScratchNu` is not needed: the node
+lists are built once in the context."""
+function write_stats(U, T, np, gr, Time, ScratchNu, stats_file; coordinates, UBC)
+    ctx = context(gr, coordinates, UBC)
+    out = zeros(4)
+    Uc = Vector{Float64}(U); Tc = Vector{Float64}(T)
+    GC.@preserve Uc Tc out begin
+        check(ctx, ccall((:stk_stats, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                         ctx.h, Uc, Tc, out))
+    end
+    open(stats_file, "a") do io
+        println(io, Float32(Time), " ", Float32(out[1]), " ", Float32(out[2]), " ", Float32(out[3]), " ",
+                Float32(out[4]), " ", np)
+    end
+    return Tuple(out)
+end
+
 end # module

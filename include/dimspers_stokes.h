@@ -203,6 +203,14 @@ int stk_stress(stk_ctx *ctx, const double *Ucart, double *const out[8]);
  * passes (the velocity after stk_updatevelocity's rotation removal). */
 int stk_deformation_gradient(stk_ctx *ctx, const double *U, double dt, double *F);
 
+/* write_stats (src/IO/IOmanager.jl:277-322) without the file output: out[0] = Urms =
+ * sqrt(int |U|^2 / area), out[1] = <T> = int T / area (volume_integral_cartesian,
+ * src/Stress/FSE.jl:466-514: straight triangles, 6-point rule, P2 interpolation of the nodal
+ * values), out[2], out[3] = Nusselt number at the top and bottom walls from the nodes one
+ * radius level inside them (ScratchNusselt, IOmanager.jl:338-354).  U: velocity, 2 per node
+ * (NULL = the device solution); T: temperature at every node (nnod, host). */
+int stk_stats(stk_ctx *ctx, const double *U, const double *T, double out[4]);
+
 /* ---------------------------------------------------------------------------
  * Learned local-Schur preconditioner (no counterpart in the reference).  A
  * per-element MLP (5 -> 64 -> 64 -> 64 -> 6, GELU, BF16 weights, FP32

@@ -1,0 +1,65 @@
+"""FP64 CPU oracle of the run statistics -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/`` may import this module; the product path never calls it.
+
+Restates ``write_stats`` (src/IO/IOmanager.jl:277-322) without the file output:
+
+* ``volume_integral_cartesian`` (src/Stress/FSE.jl:466-514): per element the determinant of
+  the straight Cartesian triangle of its three vertices (x = r sin(theta), z = r cos(theta),
+  src/Grid/mesher.jl:36) times the 6-point rule of the P2 interpolation of nodal values;
+* Urms = sqrt(int (Ux^2 + Uz^2) / area) and <T> = int T / area, area = pi (r_out^2 - r_in^2)
+  from the extreme node radii (:279-283);
+* ``ScratchNusselt`` (:338-354): the nodes one radius level inside each wall, their spacing,
+  the theta difference of the first two nodes below the outer wall and
+  A = log(f) / (2 pi r_out (1 - f)) with the radii 1.22 / 2.22 written there;
+  Nusselt top / bottom of :286-290.
+
+Pinned (tests/test_stats_oracle.py): the integral of 1 equals the closed-form area of the
+inscribed polygonal annulus; a P2 vertex indicator integrates to 0 and a midpoint indicator
+to a third of the areas of its triangles; the Nusselt numbers of a linear temperature profile
+follow from the node counts in closed form.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from dimspers_amd.quadrature import ip_triangle6, sf_N_tri6
+
+
+def volume_integral_cartesian(mesh, V):
+    x = mesh.r * np.sin(mesh.theta)
+    z = mesh.r * np.cos(mesh.theta)
+    e2n = mesh.e2n                                   # (6, nel)
+    xe, ze = x[e2n[:3]], z[e2n[:3]]                  # (3, nel)
+    det = (xe[1] - xe[0]) * (ze[2] - ze[0]) - (ze[1] - ze[0]) * (xe[2] - xe[0])
+    pts, w = ip_triangle6()
+    N = sf_N_tri6(pts[:, 0], pts[:, 1])              # (6 ip, 6 nodes)
+    Ve = np.asarray(V)[e2n]                          # (6, nel)
+    total = np.zeros(e2n.shape[1])
+    for q in range(len(w)):
+        total += (N[q] @ Ve) * (det * w[q])
+    return total.sum()
+
+
+def scratch_nusselt(mesh):
+    ru = np.unique(mesh.r)
+    top = np.flatnonzero(mesh.r == ru[-2])
+    bot = np.flatnonzero(mesh.r == ru[1])
+    dr = ru[1] - ru[0]
+    dth = mesh.theta[top[0]] - mesh.theta[top[1]]
+    f = 1.22 / 2.22
+    A = np.log(f) / (2 * np.pi * 2.22 * (1 - f))
+    return A, dr, dth, top, bot
+
+
+def write_stats(mesh, U, T):
+    """(Urms, <T>, Nusselt top, Nusselt bottom)."""
+    r_in, r_out = mesh.r.min(), mesh.r.max()
+    area = np.pi * (r_out ** 2 - r_in ** 2)
+    S = U[0::2] ** 2 + U[1::2] ** 2
+    urms = np.sqrt(volume_integral_cartesian(mesh, S) / area)
+    tav = volume_integral_cartesian(mesh, T) / area
+    A, dr, dth, top, bot = scratch_nusselt(mesh)
+    nu_top = A * dth * np.sum((0.0 - T[top]) / dr) * r_out
+    nu_bot = A * dth * np.sum((T[bot] - 1.0) / dr) * r_in
+    return urms, tav, nu_top, nu_bot

@@ -125,3 +125,34 @@ def test_stress_error_paths():
         ctx.set_stress_fields(np.ones((inp.mesh.nel, 7), order="F"), nip=5)
     assert e.value.code == _lib.STK_E_SHAPE
     ctx.close()
+
+
+@pytest.mark.parametrize("mesh", [(8, 64, 3), (32, 256, 4)])
+def test_stats_match_oracle(mesh):
+    from oracle import stats_oracle
+    nr, nt, w = mesh
+    inp = make_inputs(scaled_workload(w, nr, nt))
+    rng = np.random.default_rng(9)
+    U = rng.normal(size=inp.mesh.nU)
+    T = rng.uniform(size=inp.mesh.theta.size)
+    ctx = _lib.Context.from_mesh(inp.mesh)
+    print(f"[case] stats {nr}x{nt}", flush=True)
+    got = ctx.stats(T, U)
+    want = stats_oracle.write_stats(inp.mesh, U, T)
+    for g, wv in zip(got, want):
+        assert abs(g - wv) <= 1e-12 * abs(wv), (got, want)
+    ctx.close()
+
+
+def test_stats_closed_forms_at_full_size():
+    """256 x 2048: a uniform unit velocity and T = 1 give Urms^2 = <T> = polygon area / annulus area."""
+    inp = make_inputs(scaled_workload(4, 256, 2048))
+    m = inp.mesh
+    ctx = _lib.Context.from_mesh(m)
+    print("[case] stats closed form 256x2048", flush=True)
+    U = np.tile([1.0, 0.0], m.theta.size)
+    urms, tav, _, _ = ctx.stats(np.ones(m.theta.size), U)
+    ratio = 0.5 * m.ncell_t * np.sin(2 * np.pi / m.ncell_t) / np.pi
+    assert abs(tav - ratio) < 1e-12
+    assert abs(urms ** 2 - ratio) < 1e-12
+    ctx.close()
