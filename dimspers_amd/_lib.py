@@ -16,7 +16,7 @@ STK_E_HIP, STK_E_SHAPE, STK_E_STATE, STK_E_NOTCONV, STK_E_NAN, STK_E_ARG, STK_E_
 
 ASM_FP64, ASM_FP32, ASM_CARTESIAN, ASM_NO_BC, ASM_ISOTROPIC, ASM_NO_MG = 1, 2, 4, 8, 16, 32
 ASM_FP16_SMOOTHER = 64
-MAT_K, MAT_G, MAT_GT, MAT_M, MAT_S = 0, 1, 2, 3, 4
+MAT_K, MAT_G, MAT_GT, MAT_M, MAT_S, MAT_KT, MAT_MT = 0, 1, 2, 3, 4, 5, 6
 SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED = 0, 1, 2
 INNER_MG, INNER_JACOBI = 0, 1
 STOP_SCHUR, STOP_FULL = 0, 1
@@ -27,7 +27,7 @@ EXPORTED = [
     "stk_assemble", "stk_export_csc", "stk_export_rhs", "stk_spmv", "stk_solve", "stk_get_solution",
     "stk_updatevelocity", "stk_time_kernel", "stk_default_train_opts", "stk_set_fabric", "stk_mlp_train",
     "stk_mlp_params", "stk_mlp_forward", "stk_mlp_gradient", "stk_local_schur", "stk_learned_setup",
-    "stk_set_stress_fields", "stk_stress", "stk_deformation_gradient", "stk_stats",
+    "stk_set_stress_fields", "stk_stress", "stk_deformation_gradient", "stk_stats", "stk_thermal_solve",
 ]
 N_PARAM = 9094
 KERNEL_SPMV_K32, KERNEL_SPMV_K64, KERNEL_CHEB_K32, KERNEL_CHEB_K64, KERNEL_ASSEMBLY, KERNEL_CHEB_KH16 = 0, 1, 2, 3, 4, 5
@@ -42,6 +42,7 @@ class StkOpts(C.Structure):
         ("null_mode", C.c_int32), ("pc_sweeps", C.c_int32),
         ("rtol", C.c_double), ("inner_rtol", C.c_double), ("mixed_inner_rtol", C.c_double),
         ("mixed_schur_rtol", C.c_double), ("stall_ratio", C.c_double),
+        ("inner_warm", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -127,6 +128,8 @@ def load(path: Path | str | None = None):
     lib.stk_stress.argtypes = [vp, dp, C.POINTER(dp)]
     lib.stk_deformation_gradient.argtypes = [vp, dp, C.c_double, dp]
     lib.stk_stats.argtypes = [vp, dp, dp, dp]
+    lib.stk_thermal_solve.argtypes = [vp, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, i64p, dp,
+                                      C.c_int64, C.c_double, C.POINTER(StkReport)]
     for name in EXPORTED:
         if name not in ("stk_default_opts", "stk_destroy", "stk_last_error", "stk_default_train_opts"):
             getattr(lib, name).restype = C.c_int
@@ -348,6 +351,20 @@ class Context:
         u = None if Ucart is None else np.ascontiguousarray(Ucart, dtype=np.float64)
         self._check(self.lib.stk_stress(self.h, None if u is None else _d(u), arr))
         return outs
+
+    def thermal_solve(self, T0, rho, K, Cp, dt, fixed_nodes_1based, fixed_values, dQdT=0.0, rtol=0.0):
+        """One thermal diffusion step; returns (T, report)."""
+        T = np.array(T0, dtype=np.float64, copy=True)
+        r = np.ascontiguousarray(rho, dtype=np.float64)
+        if T.shape != (self.nnod,) or r.shape != (self.nnod,):
+            raise ValueError("T0 and rho must hold one value per node")
+        fx = np.ascontiguousarray(fixed_nodes_1based, dtype=np.int64)
+        fv = np.ascontiguousarray(fixed_values, dtype=np.float64)
+        rep = StkReport()
+        rc = self.lib.stk_thermal_solve(self.h, _d(T), _d(r), float(K), float(Cp), float(dQdT), float(dt), _i(fx),
+                                        _d(fv), fx.size, float(rtol), C.byref(rep))
+        self._check(rc)
+        return T, rep
 
     def stats(self, T, U=None):
         """write_stats: (Urms, <T>, Nusselt top, Nusselt bottom)."""

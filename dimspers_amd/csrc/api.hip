@@ -8,6 +8,7 @@
 #include "mlp.h"
 #include "solver.h"
 #include "stress.h"
+#include "thermal.h"
 
 namespace stk {
 template <typename S, typename D> void convert(Context &c, const S *src, D *dst, int64_t n);
@@ -295,7 +296,8 @@ int stk_export_csc(stk_ctx *ctx, int32_t which, int32_t lower_only, int64_t *nro
     if (!ctx) return STK_E_ARG;
     return guarded(ctx, [&]() {
         Context &c = ctx->c;
-        STK_CHECK(c.assembled, STK_E_STATE, "nothing assembled");
+        const bool thermal = which == STK_MAT_KT || which == STK_MAT_MT;
+        STK_CHECK(thermal ? c.have_thermal : c.assembled, STK_E_STATE, "nothing assembled");
         const Pattern *p = nullptr;
         int br = 1, bc = 1;
         int64_t nr = 0, nc = 0;
@@ -307,6 +309,8 @@ int stk_export_csc(stk_ctx *ctx, int32_t which, int32_t lower_only, int64_t *nro
             case STK_MAT_GT: p = &c.GTN; bc = 2; nr = c.nP; nc = c.nU; v64 = c.GT64.p; v32 = c.GT32.p; break;
             case STK_MAT_M: p = &c.MN; nr = nc = c.nP; v64 = c.M64.p; v32 = c.M32.p; break;
             case STK_MAT_S: p = &c.MN; nr = nc = c.nP; v64 = c.S64.p; v32 = c.S32.p; break;
+            case STK_MAT_KT: p = &c.KN; nr = nc = c.nnod; v64 = c.KT.p; break;
+            case STK_MAT_MT: p = &c.KN; nr = nc = c.nnod; v64 = c.MT.p; break;
             default: throw Error(STK_E_ARG, "unknown operator");
         }
         STK_CHECK(v64 || v32, STK_E_STATE, "operator not available");
@@ -634,6 +638,33 @@ int stk_stress(stk_ctx *ctx, const double *Ucart, double *const out[8]) {
                                                    hipMemcpyDeviceToHost, c.stream));
         STK_HIP(hipStreamSynchronize(c.stream));
         return STK_OK;
+    });
+}
+
+int stk_thermal_solve(stk_ctx *ctx, double *T, const double *rho, double K, double Cp, double dQdT, double dt,
+                      const int64_t *fixed_nodes, const double *fixed_values, int64_t nfixed, double rtol,
+                      stk_report *report) {
+    if (!ctx || !T || !rho || (nfixed > 0 && (!fixed_nodes || !fixed_values)) || nfixed < 0) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        std::vector<int64_t> fx(nfixed);
+        for (int64_t k = 0; k < nfixed; ++k) {
+            STK_CHECK(fixed_nodes[k] >= 1 && fixed_nodes[k] <= c.nnod, STK_E_SHAPE, "fixed node out of range");
+            fx[k] = fixed_nodes[k] - 1;
+        }
+        DBuf<double> t, r, vf;
+        DBuf<int64_t> f;
+        t.upload(T, c.nnod, c.stream);
+        r.upload(rho, c.nnod, c.stream);
+        f.upload(fx, c.stream);
+        vf.upload(fixed_values, nfixed, c.stream);
+        ThermalStep s{K, Cp, dQdT, dt, rtol > 0 ? rtol : 1.4901161193847656e-8, 20000};
+        stk_report rep;
+        thermal_solve(c, t.p, r.p, s, f.p, vf.p, nfixed, rep);
+        t.download(T, c.nnod, c.stream);
+        STK_HIP(hipStreamSynchronize(c.stream));
+        if (report) *report = rep;
+        return rep.converged ? (int)STK_OK : (int)STK_E_NOTCONV;
     });
 }
 

@@ -116,6 +116,12 @@ struct EpAxpyOut {                     // y = beta*y + alpha*A x  (alpha, beta s
     __device__ double operator()(int64_t i, T ax) const { y[i] = beta * y[i] + alpha * ax; return 0.0; }
 };
 template <typename T>
+struct EpStorePlus {                   // y = A x + a
+    static constexpr int kDots = 1;
+    T *y; const T *a;
+    __device__ double operator()(int64_t i, T ax) const { y[i] = ax + a[i]; return 0.0; }
+};
+template <typename T>
 struct EpStoreDot2 {                   // y = A x, dots (p . A x, w . A x)
     static constexpr int kDots = 2;
     T *y; const T *p, *w;
@@ -264,6 +270,39 @@ __global__ __launch_bounds__(NT) void k_spmv_sell22(int64_t nrows, const int64_t
         add_dots(dsum, ep(2 * row + 1, a1));
     }
     if (partial) {   // partial[d * nblocks + block]
+        block_sum_n<NT, ND>(dsum);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) partial[d * nblocks + lb] = dsum[d];
+    }
+}
+
+// Scalar SpMV on the sliced layout of a node pattern (one value per node pair, e.g. the
+// P2 thermal operators): one lane per row, fused epilogue.
+template <int NT, typename T, typename Ep>
+__global__ __launch_bounds__(NT) void k_spmv_sell11(int64_t nrows, const int64_t *__restrict__ sptr,
+                                                     const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                     const T *__restrict__ x, Ep ep, double *__restrict__ partial,
+                                                     int64_t nblocks) {
+    constexpr int C = 64;
+    constexpr int ND = Ep::kDots;
+    const int64_t lb = blockIdx.x;
+    const int64_t slice = lb * (NT / C) + threadIdx.x / C;
+    const int lane = threadIdx.x % C;
+    const int64_t row = slice * C + lane;
+    double dsum[ND] = {};
+    if (row < nrows) {
+        const int64_t base = sptr[slice];
+        const int len = (int)((sptr[slice + 1] - base) / C);
+        const T *__restrict__ vp = val + base + lane;
+        const int32_t *__restrict__ cp = col + base + lane;
+        T a = T(0);
+#pragma unroll 4
+        for (int k = 0; k < len; ++k)
+            a += __builtin_nontemporal_load(&vp[(int64_t)k * C]) * x[__builtin_nontemporal_load(&cp[(int64_t)k * C])];
+        add_dots(dsum, ep(row, a));
+    }
+    if (partial) {
         block_sum_n<NT, ND>(dsum);
         if (threadIdx.x == 0)
 #pragma unroll

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <functional>
 #include <memory>
 #include <type_traits>
@@ -28,6 +29,7 @@
 #include "device_ops.h"
 #include "solver.h"
 #include "stress.h"
+#include "thermal.h"
 
 namespace stk {
 
@@ -131,6 +133,19 @@ int64_t spmv_sell(Context &c, const Pattern &p, const VT *val, const T *x, Ep ep
     const int64_t nb = (nslice + slices_per_block - 1) / slices_per_block;
     if (partial && (int64_t)c.red64.n < nb * Ep::kDots) throw Error(STK_E_STATE, "reduction buffer too small");
     k_spmv_sell22<NT, VT, T, Ep><<<dim3(nb), dim3(NT), 0, c.stream>>>(p.nrows, p.dsptr.p, p.dscol.p, val, x, ep,
+                                                                   partial, nb);
+    STK_HIP(hipGetLastError());
+    return nb;
+}
+
+// Scalar SpMV on a sliced node pattern.
+template <typename T, typename Ep>
+int64_t spmv_sell11(Context &c, const Pattern &p, const T *val, const T *x, Ep ep, double *partial) {
+    const int64_t slices_per_block = NT / Pattern::kSlice;
+    const int64_t nslice = (p.nrows + Pattern::kSlice - 1) / Pattern::kSlice;
+    const int64_t nb = (nslice + slices_per_block - 1) / slices_per_block;
+    if (partial && (int64_t)c.red64.n < nb * Ep::kDots) throw Error(STK_E_STATE, "reduction buffer too small");
+    k_spmv_sell11<NT, T, Ep><<<dim3(nb), dim3(NT), 0, c.stream>>>(p.nrows, p.dsptr.p, p.dscol.p, val, x, ep,
                                                                    partial, nb);
     STK_HIP(hipGetLastError());
     return nb;
@@ -361,7 +376,7 @@ T *vcycle(Context &c, MGView<T> &v, size_t l, const T *b, int deg, const T *dot_
 // norms every `check_every` iterations.
 template <typename T>
 int64_t inner_cg(Context &c, MGView<T> &v, bool use_mg, T *y, T *z, double rtol, int itmax, int deg,
-                 bool *converged, bool deflate = false) {
+                 bool *converged, bool deflate = false, bool warm = false) {
     Reducer red(c);
     const int64_t n = c.nU;
     const bool f64 = sizeof(T) == 8;
@@ -372,15 +387,22 @@ int64_t inner_cg(Context &c, MGView<T> &v, bool use_mg, T *y, T *z, double rtol,
     double *sc = c.d_sc.p;
     Op22<T> K{&c.KN, (const T *)pick<T>(c.K64, c.K32).p};
 
-    // project the near-null component out of y; r = y; z = 0; |r|^2
+    // project the near-null component out of y; r = y; z = 0 (or the given z); |y|^2
     red.map_reduce(n, [=] __device__(int64_t i) { return (double)rot[i] * (double)y[i]; }, SC_VY);
     red.map_reduce(n, [=] __device__(int64_t i) {
         const T v = y[i] - (T)sc[SC_VY] * rot[i];
         y[i] = v;
         r[i] = v;
-        z[i] = T(0);
+        if (!warm) z[i] = T(0);
         return (double)v * (double)v;
     }, SC_BB);
+    if (warm) {
+        // start from the given z (orthogonal to v): r = P (y - K z)
+        EpResidual<T> er{r, y};
+        apply22<8>(c, K, z, er);
+        red.map_reduce(n, [=] __device__(int64_t i) { return (double)rot[i] * (double)r[i]; }, SC_TMP);
+        launch_map(c, n, [=] __device__(int64_t i) { r[i] -= (T)sc[SC_TMP] * rot[i]; });
+    }
     const double lam = c.null_lambda;
     auto finish_null = [&]() {
         // z <- z - (v.z) v  (+ (v.y / lambda) v when deflating)
@@ -858,6 +880,8 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
     apply_pc(w.r, w.d);
     launch_map(c, nP, [=] __device__(int64_t i) { w.q[i] = w.d[i]; });
     int it = 0;
+    double beta_prev = 0.0;
+    const bool warm = o.inner_warm != 0 && !deflate;
     if (stalled) *stalled = false;
     for (it = 1; it <= itmax; ++it) {
         vmean_remove<T>(c, w.q, nP);
@@ -869,7 +893,14 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
         // y = G q ; z = K^-1 y ; Sq = G^T z
         EpStore<T> ey{w.y};
         spmv<2, 1, 8, T>(c, c.GN, G, w.q, ey, nullptr);
-        *inner_its += inner_cg<T>(c, v, use_mg, w.y, w.z, inner_rtol, o.inner_itmax, deg, &ic, deflate);
+        // K^-1 G q_k = K^-1 G d_k + beta K^-1 G q_k-1: beta z_k-1 is a good first guess
+        const bool start = warm && it > 1;
+        if (start) {
+            const T bp = (T)beta_prev;
+            T *zz = w.z;
+            launch_map(c, nU, [=] __device__(int64_t i) { zz[i] *= bp; });
+        }
+        *inner_its += inner_cg<T>(c, v, use_mg, w.y, w.z, inner_rtol, o.inner_itmax, deg, &ic, deflate, start);
         EpStore<T> es{w.Sq};
         spmv<1, 2, 8, T>(c, c.GTN, GT, w.z, es, nullptr);
         const double qSq = vdot<T>(c, w.q, w.Sq, nP);
@@ -901,6 +932,7 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
             return ((double)w.r[i] - (double)w.rl[i]) * (double)w.d[i];
         }, SC_TMP);
         const double beta = red.fetch(SC_TMP) / rd;
+        beta_prev = beta;
         const T be = (T)beta;
         launch_map(c, nP, [=] __device__(int64_t i) { w.q[i] = w.d[i] + be * w.q[i]; });
     }
@@ -1301,6 +1333,135 @@ double update_velocity(Context &c, double *Ucart_dev) {
     });
     STK_HIP(hipStreamSynchronize(c.stream));
     return omega;
+}
+
+}  // namespace stk
+
+// ------------------------------------------------------------------ thermal step
+// solveDiffusion_threaded (src/ThermalDiffusion/AssemblerThermal.jl:1-62): assembly, then
+// _prepare_matrices! (ThermalSolver.jl:54-63, alpha = 1): Rhs = M T0 + F, A = M + dt K;
+// _apply_bcs! (:38-41): T[fixed] = vfix, Rhs -= A[:, fixed] vfix; the free block solved by
+// Jacobi-preconditioned CG (A is SPD) to |r| <= rtol |Rhs_free| (the reference calls
+// Krylov.gmres, unpreconditioned, whose default stops at about 1.5e-8 of |Rhs|).
+namespace stk {
+
+void thermal_solve(Context &c, double *T, const double *rho, const ThermalStep &s, const int64_t *fixed,
+                   const double *vfix, int64_t nfix, stk_report &rep) {
+    std::memset(&rep, 0, sizeof(rep));
+    const int64_t nn = c.nnod, ns = c.KN.nstore();
+    hipEvent_t e0, e1;
+    STK_HIP(hipEventCreate(&e0));
+    STK_HIP(hipEventCreate(&e1));
+    STK_HIP(hipEventRecord(e0, c.stream));
+    c.KT.alloc(ns); c.MT.alloc(ns); c.AT.alloc(ns); c.FT.alloc(nn);
+    c.KT.zero(c.stream); c.MT.zero(c.stream); c.FT.zero(c.stream);
+    thermal_assemble(c, rho, s.K, s.Cp, s.dQdT, s.dt, c.KT.p, c.MT.p, c.FT.p);
+    c.have_thermal = true;
+    double *A = c.AT.p;
+    {
+        const double *Kt = c.KT.p, *Mt = c.MT.p;
+        const double dt = s.dt;
+        launch_map(c, ns, [=] __device__(int64_t q) { A[q] = Mt[q] + dt * Kt[q]; });
+    }
+    DBuf<double> b(nn), xf(nn), t(nn), r(nn), z(nn), p(nn), Ap(nn), dinv(nn);
+    DBuf<uint8_t> flag(nn);
+    xf.zero(c.stream);
+    flag.zero(c.stream);
+    {
+        EpStorePlus<double> ep{b.p, c.FT.p};
+        spmv_sell11<double>(c, c.KN, c.MT.p, T, ep, nullptr);
+        double *xv = xf.p;
+        uint8_t *fl = flag.p;
+        launch_map(c, nfix, [=] __device__(int64_t k) { xv[fixed[k]] = vfix[k]; fl[fixed[k]] = 1; });
+        EpStore<double> es{t.p};
+        spmv_sell11<double>(c, c.KN, A, xv, es, nullptr);
+        double *bv = b.p;
+        const double *tv = t.p;
+        launch_map(c, nn, [=] __device__(int64_t i) { bv[i] -= tv[i]; });
+    }
+    // rows and columns of fixed nodes become identity (structural entries only)
+    {
+        const int64_t *sptr = c.KN.dsptr.p;
+        const int32_t *scol = c.KN.dscol.p, *ptr = c.KN.dptr.p;
+        const uint8_t *fl = flag.p;
+        double *dv = dinv.p;
+        launch_map(c, nn, [=] __device__(int64_t i) {
+            const int64_t base = sptr[i / Pattern::kSlice] + i % Pattern::kSlice;
+            const int32_t len = ptr[i + 1] - ptr[i];
+            double diag = 0.0;
+            for (int32_t k = 0; k < len; ++k) {
+                const int64_t q = base + (int64_t)k * Pattern::kSlice;
+                const int64_t j = scol[q];
+                if (fl[i] || fl[j]) A[q] = (i == j && fl[i]) ? 1.0 : 0.0;
+                if (j == i) diag = A[q];
+            }
+            dv[i] = diag != 0.0 ? 1.0 / diag : 0.0;
+        });
+        double *bv = b.p;
+        launch_map(c, nfix, [=] __device__(int64_t k) { bv[fixed[k]] = vfix[k]; T[fixed[k]] = vfix[k]; });
+    }
+    // Jacobi PCG from T0
+    Reducer red(c);
+    double *sc = c.d_sc.p;
+    const uint8_t *fl = flag.p;
+    const double *bv = b.p, *dv = dinv.p;
+    double *rv = r.p, *zv = z.p, *pv = p.p, *Apv = Ap.p;
+    red.map_reduce(nn, [=] __device__(int64_t i) { return fl[i] ? 0.0 : bv[i] * bv[i]; }, SC_BB);
+    {
+        EpResidual<double> er{rv, bv};
+        const int64_t nb = spmv_sell11<double>(c, c.KN, A, T, er, c.red64.p);
+        red.finish(nb, SC_HIST);
+    }
+    red.map_reduce(nn, [=] __device__(int64_t i) {
+        const double zz = dv[i] * rv[i];
+        zv[i] = zz;
+        pv[i] = zz;
+        return rv[i] * zz;
+    }, SC_RZV);
+    double h[2];
+    red.fetch_many(SC_BB, 1, h);
+    const double bb = h[0];
+    if (!std::isfinite(bb)) throw Error(STK_E_NAN, "non-finite thermal right-hand side");
+    const double tol2 = s.rtol * s.rtol * bb;
+    double rr = red.fetch(SC_HIST);
+    int it = 0, par = 0;
+    bool conv = !(rr > tol2) && std::isfinite(rr);
+    while (!conv && it < s.itmax) {
+        const int rz_cur = SC_RZV + par, rz_new = SC_RZV + (1 - par);
+        EpStoreDot<double> ed{Apv, pv};
+        const int64_t nb = spmv_sell11<double>(c, c.KN, A, pv, ed, c.red64.p);
+        red.finish(nb, SC_PAP2);
+        red.map_reduce(nn, [=] __device__(int64_t i) {
+            const double al = sc[rz_cur] / sc[SC_PAP2];
+            T[i] += al * pv[i];
+            const double rn = rv[i] - al * Apv[i];
+            rv[i] = rn;
+            return rn * rn;
+        }, SC_HIST);
+        red.map_reduce(nn, [=] __device__(int64_t i) {
+            const double zz = dv[i] * rv[i];
+            zv[i] = zz;
+            return rv[i] * zz;
+        }, rz_new);
+        launch_map(c, nn, [=] __device__(int64_t i) { pv[i] = zv[i] + (sc[rz_new] / sc[rz_cur]) * pv[i]; });
+        ++it;
+        par = 1 - par;
+        rr = red.fetch(SC_HIST);
+        if (!std::isfinite(rr)) throw Error(STK_E_NAN, "non-finite residual in the thermal solve");
+        conv = rr <= tol2;
+    }
+    STK_HIP(hipEventRecord(e1, c.stream));
+    STK_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    STK_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    rep.converged = conv ? 1 : 0;
+    rep.outer_iterations = it;
+    rep.inner_iterations = it;
+    rep.schur_residual = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    rep.full_residual = rep.schur_residual;
+    rep.solve_ms = ms;
 }
 
 }  // namespace stk

@@ -213,6 +213,10 @@ struct Context {
     DBuf<int32_t> d_itop, d_ibot;
     double nu_dr = 0, nu_dth = 0, nu_rin = 0, nu_rout = 0;
 
+    // thermal diffusion operators (sliced P2 node pattern, one value per stored entry)
+    bool have_thermal = false;
+    DBuf<double> KT, MT, AT, FT;
+
     // solution and work
     DBuf<double> U, P, work64;
     DBuf<double> red64;                       // reduction partials

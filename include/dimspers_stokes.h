@@ -52,7 +52,9 @@ enum {
     STK_MAT_G = 1,    /* gradient, nU x nP                       */
     STK_MAT_GT = 2,   /* divergence G^T, nP x nU                 */
     STK_MAT_M = 3,    /* 1/eta pressure mass, nP x nP            */
-    STK_MAT_S = 4     /* learned/exact local-Schur sum, nP x nP  */
+    STK_MAT_S = 4,    /* learned/exact local-Schur sum, nP x nP  */
+    STK_MAT_KT = 5,   /* thermal diffusion matrix, nnod x nnod   */
+    STK_MAT_MT = 6    /* thermal mass matrix rho Cp, nnod x nnod */
 };
 
 /* Solver methods (stk_opts.method). */
@@ -104,6 +106,9 @@ typedef struct stk_opts {
     double mixed_schur_rtol;/* full-residual reduction asked of one FP32 correction */
     double stall_ratio;     /* learned: fall back when a sweep's Schur PCG misses its */
                             /* target and the full residual falls by less than this */
+    int32_t inner_warm;     /* 1: each inner solve of the Schur PCG starts from      */
+                            /* beta * z of the previous one (q_k = d_k + beta q_k-1) */
+    int32_t reserved;       /* zero                                                */
 } stk_opts;
 
 typedef struct stk_report {
@@ -210,6 +215,20 @@ int stk_deformation_gradient(stk_ctx *ctx, const double *U, double dt, double *F
  * radius level inside them (ScratchNusselt, IOmanager.jl:338-354).  U: velocity, 2 per node
  * (NULL = the device solution); T: temperature at every node (nnod, host). */
 int stk_stats(stk_ctx *ctx, const double *U, const double *T, double out[4]);
+
+/* One thermal diffusion step: solveDiffusion_threaded (src/ThermalDiffusion/
+ * AssemblerThermal.jl:1-62) with assemble_element! (:117-209; 7-point rule,
+ * MAIN.jl:167-169), _prepare_matrices! (ThermalSolver.jl:54-63: Rhs = M T0 + F,
+ * A = M + dt K) and _apply_bcs! (:38-41).  T (nnod) holds T0 on entry and the new
+ * temperature on return; rho: nnod (the vertex values are used); K, Cp, dQdT: the
+ * ThermalParameters fields (src/Setup/Setup.jl:1-12); fixed_nodes: 1-based node ids of
+ * TBC.Omega with fixed_values (heated_bottom, boundary_conditions.jl:77-93).  The free
+ * block is solved to |r| <= rtol |Rhs_free| (rtol <= 0: 1.5e-8, the default stop of the
+ * reference's Krylov.gmres); report: converged, iterations (outer_iterations), relative
+ * residual (schur_residual), solve_ms.  K_T and M_T stay available to stk_export_csc. */
+int stk_thermal_solve(stk_ctx *ctx, double *T, const double *rho, double K, double Cp, double dQdT, double dt,
+                      const int64_t *fixed_nodes, const double *fixed_values, int64_t nfixed, double rtol,
+                      stk_report *report);
 
 /* ---------------------------------------------------------------------------
  * Learned local-Schur preconditioner (no counterpart in the reference).  A

@@ -30,7 +30,7 @@ def test_binding_covers_the_header():
 
 def test_struct_layouts():
     # int32 fields first, doubles 8-byte aligned
-    assert ctypes.sizeof(_lib.StkOpts) == 10 * 4 + 5 * 8
+    assert ctypes.sizeof(_lib.StkOpts) == 10 * 4 + 5 * 8 + 2 * 4
     assert ctypes.sizeof(_lib.StkReport) == 8 + 8 + 8 + 3 * 8
     o = _lib.default_opts()
     assert o.itmax == 200 and o.itmin == 10 and o.rtol == 1e-8

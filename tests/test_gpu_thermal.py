@@ -1,0 +1,91 @@
+"""GPU parity of the thermal diffusion step (C ABI) against the FP64 oracle, and the steady
+conduction profile at the full 256 x 2048 size."""
+import numpy as np
+import pytest
+
+from dimspers_amd import _lib
+from dimspers_amd.mesh import build_annulus
+from oracle import thermal_oracle as to
+
+pytestmark = pytest.mark.gpu
+
+
+def _walls(m):
+    r_in, r_out = m.r.min(), m.r.max()
+    top = np.flatnonzero(m.r == r_out)
+    bot = np.flatnonzero(m.r == r_in)
+    # heated_bottom (boundary_conditions.jl:77-93): outer nodes then inner nodes
+    return np.concatenate([top, bot]), np.concatenate([np.zeros(top.size), np.ones(bot.size)])
+
+
+def _ctx(m):
+    print(f"[case] thermal mesh {m.ncell_r}x{m.ncell_t}", flush=True)
+    return _lib.Context.from_mesh(m)
+
+
+@pytest.mark.parametrize("size", [(8, 64), (32, 256)])
+def test_thermal_matrices_and_step_match_oracle(size):
+    m = build_annulus(*size)
+    nn = m.theta.size
+    rng = np.random.default_rng(21)
+    rho = 3300.0 * (1.0 + 0.1 * rng.uniform(size=nn))
+    T0 = rng.uniform(size=nn)
+    K, Cp, dt = 3.3, 1.2e3, 2.0e2
+    fixed, vfix = _walls(m)
+    ctx = _ctx(m)
+    T, rep = ctx.thermal_solve(T0, rho, K, Cp, dt, fixed + 1, vfix, rtol=1e-14)
+    assert rep.converged
+    KT, MT, _ = to.thermal_matrices(m, rho, K, Cp)
+    for which, want in ((_lib.MAT_KT, KT), (_lib.MAT_MT, MT)):
+        got = ctx.export_csc(which)
+        err = abs(got - want).max() / abs(want).max()
+        assert err < 1e-12, (which, err)
+        assert got.nnz == want.nnz
+    Tw = to.thermal_step(m, T0, rho, K, Cp, dt, fixed, vfix)
+    err = np.abs(T - Tw).max() / np.abs(Tw).max()
+    print(f"[result] thermal {size}: {rep.outer_iterations} CG its, T err {err:.2e}")
+    assert err < 1e-11
+    # the reference's default stop (about 1.5e-8 of |Rhs|) is close to it as well
+    T8, rep8 = ctx.thermal_solve(T0, rho, K, Cp, dt, fixed + 1, vfix)
+    assert rep8.converged and rep8.schur_residual <= 1.5e-8
+    assert np.abs(T8 - Tw).max() / np.abs(Tw).max() < 1e-6
+    ctx.close()
+
+
+def test_thermal_source_matches_oracle():
+    m = build_annulus(8, 64)
+    nn = m.theta.size
+    rng = np.random.default_rng(3)
+    rho = 1.0 + rng.uniform(size=nn)
+    T0 = rng.uniform(size=nn)
+    fixed, vfix = _walls(m)
+    ctx = _ctx(m)
+    T, rep = ctx.thermal_solve(T0, rho, 1.0, 1.0, 0.05, fixed + 1, vfix, dQdT=0.8, rtol=1e-14)
+    Tw = to.thermal_step(m, T0, rho, 1.0, 1.0, 0.05, fixed, vfix, dQdT=0.8)
+    assert np.abs(T - Tw).max() / np.abs(Tw).max() < 1e-11
+    ctx.close()
+
+
+def test_steady_conduction_at_full_size():
+    """256 x 2048, dt -> infinity: T = ln(r / r_out) / ln(r_in / r_out) to the solver tolerance
+    (the P2 discretisation error is ~1e-12 at this size)."""
+    m = build_annulus(256, 2048)
+    nn = m.theta.size
+    fixed, vfix = _walls(m)
+    ctx = _ctx(m)
+    T, rep = ctx.thermal_solve(np.zeros(nn), np.ones(nn), 1.0, 1.0, 1e12, fixed + 1, vfix, rtol=1e-12)
+    exact = np.log(m.r / m.r.max()) / np.log(m.r.min() / m.r.max())
+    err = np.abs(T - exact).max()
+    print(f"[result] steady conduction 256x2048: {rep.outer_iterations} CG its, {rep.solve_ms:.1f} ms, err {err:.2e}")
+    assert rep.converged and err < 1e-8
+    ctx.close()
+
+
+def test_thermal_error_paths():
+    m = build_annulus(4, 16)
+    ctx = _ctx(m)
+    nn = m.theta.size
+    with pytest.raises(_lib.StkError) as e:
+        ctx.thermal_solve(np.zeros(nn), np.ones(nn), 1.0, 1.0, 1.0, np.array([nn + 1]), np.array([0.0]))
+    assert e.value.code == _lib.STK_E_SHAPE
+    ctx.close()
