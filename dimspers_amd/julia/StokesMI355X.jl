@@ -159,6 +159,25 @@ function deformation_gradient!(F, U, nel, DoF_U, coordinates, SF_Stress, Δt; gr
     return F
 end
 
+"""GPU `solveDiffusion_threaded` (src/ThermalDiffusion/AssemblerThermal.jl:1-62): one
+diffusion step with the element matrices of `assemble_element!`, `_prepare_matrices!` and
+`_apply_bcs!`; the free block is solved on the device (stopping like Krylov.gmres' default)."""
+function solveDiffusion_threaded(color_list, CMidx, KT, MT, FT, DoF_T, coordinates, VarT, ScratchDifussion,
+                                 SF_Diffusion, valA, ρ, Δt, T, T0, TBC, to; gr, UBC)
+    ctx = context(gr, coordinates, UBC)
+    copyto!(T0, T)
+    fixed = Vector{Int64}(TBC.Ω); vfix = Vector{Float64}(TBC.vfix); rho = Vector{Float64}(ρ)
+    rep = Report()
+    GC.@preserve T fixed vfix rho begin
+        check(ctx, ccall((:stk_thermal_solve, LIB), Cint,
+                         (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Float64, Float64, Float64, Float64, Ptr{Int64},
+                          Ptr{Float64}, Int64, Float64, Ref{Report}),
+                         ctx.h, T, rho, VarT.K, VarT.Cp, VarT.dQdT, Δt, fixed, vfix, length(fixed), 0.0, rep))
+    end
+    ΔT = T .- T0
+    return T, T0, ΔT, to
+end
+
 """GPU statistics of `write_stats` (src/IO/IOmanager.jl:277-322); appends the same line to
 `stats_file` and returns (Urms, ⟨T⟩, Nu_top, Nu_bot).  `ScratchNu` is not needed: the node
 lists are built once in the context."""
