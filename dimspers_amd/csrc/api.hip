@@ -4,6 +4,8 @@
 #include <cstring>
 #include <numeric>
 
+#include <rocblas/rocblas.h>
+
 #include "device_ops.h"
 #include "mlp.h"
 #include "solver.h"
@@ -19,6 +21,7 @@ double update_velocity(Context &c, double *Ucart_dev);
 
 Context::~Context() {
     delete mlp;
+    if (blas) (void)rocblas_destroy_handle((rocblas_handle)blas);
     if (h_pinned) (void)hipHostFree(h_pinned);
     if (stream) (void)hipStreamDestroy(stream);
 }

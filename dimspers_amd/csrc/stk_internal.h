@@ -134,6 +134,7 @@ struct MGLevel {
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
+    void *blas = nullptr;                     // rocblas_handle (dense coarsest inverse), lazily
     std::string err;
 
     int64_t nel = 0, nnod = 0, nvert = 0, nU = 0, nP = 0;
