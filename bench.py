@@ -171,6 +171,15 @@ def main():
     log(f"stress!: {st_ms * 1e3:.1f} us ({st_bytes / st_ms / 1e6:.0f} GB/s compulsory); "
         f"deformation_gradient!: {fg_ms * 1e3:.1f} us ({fg_bytes / fg_ms / 1e6:.0f} GB/s)")
 
+    # one thermal diffusion step on the same mesh (heated bottom, non-dimensional K = Cp = rho = 1)
+    nn = m.theta.size
+    top, bot = np.flatnonzero(m.r == m.r.max()), np.flatnonzero(m.r == m.r.min())
+    tfix = np.concatenate([top, bot]) + 1
+    tval = np.concatenate([np.zeros(top.size), np.ones(bot.size)])
+    ctx.thermal_solve(inp.T, np.ones(nn), 1.0, 1.0, 1e-4, tfix, tval)
+    _, trep = ctx.thermal_solve(inp.T, np.ones(nn), 1.0, 1.0, 1e-4, tfix, tval)
+    log(f"thermal step (assembly + BCs + Jacobi-CG to 1.5e-8): {trep.solve_ms:.2f} ms, {trep.outer_iterations} its")
+
     kms, kbytes = ctx.time_kernel(_lib.KERNEL_CHEB_K32, reps=50)
     achieved = kbytes / (kms * 1e-3) / 1e9
     sms, sbytes = ctx.time_kernel(_lib.KERNEL_SPMV_K32, reps=50)
@@ -231,6 +240,8 @@ def main():
                      "kernel": "k_spmv_sell22<256,float,float,EpCheb<float>> (fused Chebyshev SpMV on K, sliced layout)",
                      "avg_us": round(kms * 1e3, 2), "bytes_per_launch": kbytes},
         "cpu_baseline": cpu,
+        "thermal_step": {"ms": round(trep.solve_ms, 2), "cg_iterations": trep.outer_iterations,
+                         "dt": 1e-4, "relative_residual": trep.schur_residual},
         "stress_postprocessor": {
             "stress_us": round(st_ms * 1e3, 1), "stress_GB_per_s": round(st_bytes / st_ms / 1e6, 1),
             "deformation_gradient_us": round(fg_ms * 1e3, 1),

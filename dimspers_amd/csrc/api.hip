@@ -199,8 +199,8 @@ template <typename T>
 void assemble_prec(Context &c, int flags, bool mg, DBuf<T> &K, DBuf<T> &G, DBuf<T> &GT, DBuf<T> &M, DBuf<T> &F,
                    bool f64) {
     K.alloc(4 * c.KN.nstore()); K.zero(c.stream);
-    G.alloc(2 * c.GN.nnz()); G.zero(c.stream);
-    GT.alloc(2 * c.GTN.nnz()); GT.zero(c.stream);
+    G.alloc(2 * c.GN.nstore()); G.zero(c.stream);
+    GT.alloc(2 * c.GTN.nstore()); GT.zero(c.stream);
     M.alloc(c.MN.nnz()); M.zero(c.stream);
     F.alloc(c.nU); F.zero(c.stream);
     T *A1 = nullptr, *E1 = nullptr;

@@ -82,6 +82,7 @@ struct AsmArgs {
     const int32_t *Kptr, *Gptr, *GTptr, *Mptr;
     const int64_t *Ksptr;               // sliced layout of K (null: CSR order)
     const int64_t *A1sptr;              // sliced layout of the P1 level operator (null: CSR)
+    const int64_t *Gsptr, *GTsptr;      // sliced layouts of G and G^T (null: CSR)
     T *K, *G, *GT, *M, *F;
     // finest multigrid level (vertex space)
     T *A1, *E1;                         // A1: BSR 2x2 over the M pattern; E1: 36 x nel by element id
@@ -289,10 +290,13 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(AsmArgs<T> a) {
             const int k = (aa * 3 + i) * 2;
             const T g0 = fx[2 * aa] ? T(0) : sG[k];
             const T g1 = fx[2 * aa + 1] ? T(0) : sG[k + 1];
-            const int64_t bg = (int64_t)a.Gptr[n[aa]] + a.sG[(aa * 3 + i) * nel + p];
+            const int64_t sg = a.sG[(aa * 3 + i) * nel + p], st = a.sGT[(i * 6 + aa) * nel + p];
+            const int64_t bg = a.Gsptr ? a.Gsptr[n[aa] / Pattern::kSlice] + sg * Pattern::kSlice + n[aa] % Pattern::kSlice
+                                       : (int64_t)a.Gptr[n[aa]] + sg;
             a.G[2 * bg] += g0;
             a.G[2 * bg + 1] += g1;
-            const int64_t bt = (int64_t)a.GTptr[n[i]] + a.sGT[(i * 6 + aa) * nel + p];
+            const int64_t bt = a.GTsptr ? a.GTsptr[n[i] / Pattern::kSlice] + st * Pattern::kSlice + n[i] % Pattern::kSlice
+                                        : (int64_t)a.GTptr[n[i]] + st;
             a.GT[2 * bt] += g0;
             a.GT[2 * bt + 1] += g1;
         }
@@ -832,6 +836,8 @@ void assemble_impl(Context &c, int flags, T *K, T *G, T *GT, T *M, T *F, T *A1, 
     a.Kptr = c.KN.dptr.p; a.Gptr = c.GN.dptr.p; a.GTptr = c.GTN.dptr.p; a.Mptr = c.MN.dptr.p;
     a.Ksptr = c.KN.sell ? c.KN.dsptr.p : nullptr;
     a.A1sptr = (A1 && c.mg[0].pat.sell) ? c.mg[0].pat.dsptr.p : nullptr;
+    a.Gsptr = c.GN.sell ? c.GN.dsptr.p : nullptr;
+    a.GTsptr = c.GTN.sell ? c.GTN.dsptr.p : nullptr;
     a.K = K; a.G = G; a.GT = GT; a.M = M; a.F = F; a.A1 = A1; a.E1 = E1;
     a.cart = (flags & STK_ASM_CARTESIAN) ? 1 : 0;
     a.bc = (flags & STK_ASM_NO_BC) ? 0 : 1;

@@ -277,6 +277,55 @@ __global__ __launch_bounds__(NT) void k_spmv_sell22(int64_t nrows, const int64_t
     }
 }
 
+// 2x1 / 1x2 block SpMV on the sliced layout (G: velocity node rows, pressure columns;
+// G^T: pressure rows, velocity node columns): one lane per row, two values per block.
+template <int BR, int BC, int NT, typename T, typename Ep>
+__global__ __launch_bounds__(NT) void k_spmv_sell_rc(int64_t nrows, const int64_t *__restrict__ sptr,
+                                                      const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                      const T *__restrict__ x, Ep ep, double *__restrict__ partial,
+                                                      int64_t nblocks) {
+    static_assert(BR * BC == 2, "2x1 or 1x2 blocks");
+    typedef typename Vec2<T>::type V2;
+    constexpr int C = 64;
+    constexpr int ND = Ep::kDots;
+    const int64_t lb = blockIdx.x;
+    const int64_t slice = lb * (NT / C) + threadIdx.x / C;
+    const int lane = threadIdx.x % C;
+    const int64_t row = slice * C + lane;
+    double dsum[ND] = {};
+    if (row < nrows) {
+        const int64_t base = sptr[slice];
+        const int len = (int)((sptr[slice + 1] - base) / C);
+        const V2 *__restrict__ vp = reinterpret_cast<const V2 *>(val) + base + lane;
+        const int32_t *__restrict__ cp = col + base + lane;
+        T a0 = T(0), a1 = T(0);
+#pragma unroll 4
+        for (int k = 0; k < len; ++k) {
+            const V2 v = __builtin_nontemporal_load(&vp[(int64_t)k * C]);
+            const int64_t cl = __builtin_nontemporal_load(&cp[(int64_t)k * C]);
+            if constexpr (BR == 2) {          // 2x1: both rows times one x value
+                const T xv = x[cl];
+                a0 += v.x * xv;
+                a1 += v.y * xv;
+            } else {                          // 1x2: one row times an x pair
+                a0 += v.x * x[2 * cl] + v.y * x[2 * cl + 1];
+            }
+        }
+        if constexpr (BR == 2) {
+            add_dots(dsum, ep(2 * row, a0));
+            add_dots(dsum, ep(2 * row + 1, a1));
+        } else {
+            add_dots(dsum, ep(row, a0));
+        }
+    }
+    if (partial) {
+        block_sum_n<NT, ND>(dsum);
+        if (threadIdx.x == 0)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) partial[d * nblocks + lb] = dsum[d];
+    }
+}
+
 // Scalar SpMV on the sliced layout of a node pattern (one value per node pair, e.g. the
 // P2 thermal operators): one lane per row, fused epilogue.
 template <int NT, typename T, typename Ep>
@@ -335,16 +384,28 @@ __global__ __launch_bounds__(NT) void k_reduce_final(int64_t npart, int nd, cons
     }
 }
 
-// Coarsest-level solve x = Ainv b: one wavefront per row, coalesced over the row.
+// Coarsest-level solve x = Ainv b: one wavefront per row, coalesced over the row, in
+// 16-byte vectors when the row length allows it (the matrix is streamed once: non-temporal).
 template <typename T>
 __global__ __launch_bounds__(256) void k_dense_matvec(int64_t n, const T *__restrict__ Ainv,
                                                       const T *__restrict__ b, T *__restrict__ x) {
+    typedef typename Vec4<T>::type V4;
     const int64_t row = (int64_t)blockIdx.x * 4 + threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     if (row >= n) return;
     const T *a = Ainv + row * n;
     T s = T(0);
-    for (int64_t k = lane; k < n; k += kWave) s += a[k] * b[k];
+    if (n % 4 == 0) {
+        const V4 *a4 = reinterpret_cast<const V4 *>(a);
+        const V4 *b4 = reinterpret_cast<const V4 *>(b);
+        for (int64_t k = lane; k < n / 4; k += kWave) {
+            const V4 av = __builtin_nontemporal_load(&a4[k]);
+            const V4 bv = b4[k];
+            s += av.x * bv.x + av.y * bv.y + av.z * bv.z + av.w * bv.w;
+        }
+    } else {
+        for (int64_t k = lane; k < n; k += kWave) s += a[k] * b[k];
+    }
     s = group_sum<kWave>(s);
     if (lane == 0) x[row] = s;
 }

@@ -112,6 +112,8 @@ void build_patterns(Context &c) {
         p->dcol.upload(p->col, c.stream);
     }
     make_sliced(c.KN, c.stream);
+    make_sliced(c.GN, c.stream);
+    make_sliced(c.GTN, c.stream);
 }
 
 void make_sliced(Pattern &p, hipStream_t s) {

@@ -141,6 +141,20 @@ int64_t spmv_sell(Context &c, const Pattern &p, const VT *val, const T *x, Ep ep
     return nb;
 }
 
+// G (2x1) / G^T (1x2) product: sliced kernel when the pattern is sliced, else CSR.
+template <int BR, int BC, typename T, typename Ep>
+int64_t apply_rc(Context &c, const Pattern &p, const T *val, const T *x, Ep ep, double *partial = nullptr) {
+    if (!p.sell) return spmv<BR, BC, 8, T, Ep>(c, p, val, x, ep, partial);
+    const int64_t slices_per_block = NT / Pattern::kSlice;
+    const int64_t nslice = (p.nrows + Pattern::kSlice - 1) / Pattern::kSlice;
+    const int64_t nb = (nslice + slices_per_block - 1) / slices_per_block;
+    if (partial && (int64_t)c.red64.n < nb * Ep::kDots) throw Error(STK_E_STATE, "reduction buffer too small");
+    k_spmv_sell_rc<BR, BC, NT, T, Ep><<<dim3(nb), dim3(NT), 0, c.stream>>>(p.nrows, p.dsptr.p, p.dscol.p, val, x,
+                                                                            ep, partial, nb);
+    STK_HIP(hipGetLastError());
+    return nb;
+}
+
 // Scalar SpMV on a sliced node pattern.
 template <typename T, typename Ep>
 int64_t spmv_sell11(Context &c, const Pattern &p, const T *val, const T *x, Ep ep, double *partial) {
@@ -876,7 +890,7 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
     // r = h - G^T du ; mean removed
     {
         EpStore<T> es{w.r};
-        spmv<1, 2, 8, T>(c, c.GTN, GT, w.du, es, nullptr);
+        apply_rc<1, 2, T>(c, c.GTN, GT, w.du, es, nullptr);
         T *r = w.r;
         launch_map(c, nP, [=] __device__(int64_t i) { r[i] = (h ? h[i] : T(0)) - r[i]; });
     }
@@ -924,7 +938,7 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
         }
         // y = G q ; z = K^-1 y ; Sq = G^T z
         EpStore<T> ey{w.y};
-        spmv<2, 1, 8, T>(c, c.GN, G, w.q, ey, nullptr);
+        apply_rc<2, 1, T>(c, c.GN, G, w.q, ey, nullptr);
         // K^-1 G q_k = K^-1 G d_k + beta K^-1 G q_k-1: beta z_k-1 is a good first guess
         const bool start = warm && it > 1;
         if (start) {
@@ -934,7 +948,7 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
         }
         *inner_its += inner_cg<T>(c, v, use_mg, w.y, w.z, inner_rtol, o.inner_itmax, deg, &ic, deflate, start);
         EpStore<T> es{w.Sq};
-        spmv<1, 2, 8, T>(c, c.GTN, GT, w.z, es, nullptr);
+        apply_rc<1, 2, T>(c, c.GTN, GT, w.z, es, nullptr);
         const double qSq = vdot<T>(c, w.q, w.Sq, nP);
         const double alpha = rd / qSq;
         if (!std::isfinite(alpha)) throw Error(STK_E_NAN, "non-finite step in the Schur PCG");
@@ -985,7 +999,7 @@ double full_residual(Context &c, const double *U, const double *P, double *ru, d
     // t = G P ; ru = F + t - ru  (masked)
     double *t = c.work64.p;
     EpStore<double> eg{t};
-    spmv<2, 1, 8, double>(c, c.GN, c.G64.p, P, eg, nullptr);
+    apply_rc<2, 1, double>(c, c.GN, c.G64.p, P, eg, nullptr);
     launch_map(c, nU, [=] __device__(int64_t i) { ru[i] = fx[i] ? 0.0 : F[i] + t[i] - ru[i]; });
     if (project_null) {
         // the near-null component is outside the solved space (STK_NULL_PROJECT)
@@ -997,7 +1011,7 @@ double full_residual(Context &c, const double *U, const double *P, double *ru, d
     // hc = -G^T U with its mean removed: the Schur iteration works modulo constants
     // (rm0space!, src/IterativeSolvers.jl:131,153), so the continuity residual does too.
     EpStore<double> ec{hc};
-    spmv<1, 2, 8, double>(c, c.GTN, c.GT64.p, U, ec, nullptr);
+    apply_rc<1, 2, double>(c, c.GTN, c.GT64.p, U, ec, nullptr);
     red.map_reduce(nP, [=] __device__(int64_t i) { return hc[i]; }, SC_TMP);
     launch_map(c, nP, [=] __device__(int64_t i) { hc[i] = sc[SC_TMP] / (double)nP - hc[i]; });
     red.map_reduce(nP, [=] __device__(int64_t i) { return hc[i] * hc[i]; }, SC_RC);
@@ -1265,8 +1279,8 @@ void host_spmv(Context &c, int which, int prec, const double *xh, double *yh) {
         EpStore<T> es{y};
         switch (which) {
             case STK_MAT_K: apply22<8>(c, Op22<T>{p, val}, x, es); break;
-            case STK_MAT_G: spmv<2, 1, 8, T>(c, *p, val, x, es, nullptr); break;
-            case STK_MAT_GT: spmv<1, 2, 8, T>(c, *p, val, x, es, nullptr); break;
+            case STK_MAT_G: apply_rc<2, 1, T>(c, *p, val, x, es, nullptr); break;
+            case STK_MAT_GT: apply_rc<1, 2, T>(c, *p, val, x, es, nullptr); break;
             default: spmv<1, 1, 8, T>(c, *p, val, x, es, nullptr); break;
         }
     };
