@@ -134,23 +134,43 @@ struct EpStoreDot2 {                   // y = A x, dots (p . A x, w . A x)
 template <typename T>
 struct EpChebDot2 {
     static constexpr int kDots = 2;
-    const T *b, *dinv, *x; T *d, *xo; T c1, c2; const T *w;
+    const T *b, *dinv, *x; const T *d; T *xo; T c1, c2; const T *w;
+    bool rd = true;   // false: c1 = 0 and d is not read (first step)
     __device__ Dots<2> operator()(int64_t i, T ax) const {
-        const T dn = c1 * d[i] + c2 * dinv[i] * (b[i] - ax);
-        d[i] = dn;
+        // the last step of a smoothing: its direction is not needed afterwards
+        const T dn = (rd ? c1 * d[i] : T(0)) + c2 * dinv[i] * (b[i] - ax);
         const T v = x[i] + dn;
         xo[i] = v;
         return Dots<2>{{(double)b[i] * (double)v, (double)w[i] * (double)v}};
     }
 };
 // Chebyshev step with point-Jacobi: d = c1 d + c2 Dinv (b - A x); xo = x + d.
+// rd = false: first step (c1 = 0), d is not read; wd = false: last step, d is not stored.
 template <typename T>
 struct EpCheb {
     const T *b, *dinv, *x; T *d, *xo; T c1, c2;
+    bool rd = true, wd = true;
     static constexpr int kDots = 1;
     __device__ double operator()(int64_t i, T ax) const {
-        const T dn = c1 * d[i] + c2 * dinv[i] * (b[i] - ax);
-        d[i] = dn; xo[i] = x[i] + dn; return 0.0;
+        const T dn = (rd ? c1 * d[i] : T(0)) + c2 * dinv[i] * (b[i] - ax);
+        if (wd) d[i] = dn;
+        xo[i] = x[i] + dn; return 0.0;
+    }
+};
+
+// First Chebyshev step from a zero start, whose iterate x0 = d0 = s D^-1 b is formed here
+// (and in the gather) rather than read: d = c1 x0 + c2 D^-1 (b - A x0); xo = x0 + d.
+template <typename T>
+struct EpChebFirst {
+    const T *b, *dinv; T *d, *xo; T s, c1, c2;
+    bool wd = true;   // false: last step, d is not stored
+    static constexpr int kDots = 1;
+    __device__ double operator()(int64_t i, T ax) const {
+        const T di = dinv[i], bi = b[i];
+        const T x0 = s * di * bi;
+        const T dn = c1 * x0 + c2 * di * (bi - ax);
+        if (wd) d[i] = dn;
+        xo[i] = x0 + dn; return 0.0;
     }
 };
 
@@ -235,39 +255,68 @@ template <typename T> struct Vec2;
 template <> struct Vec2<float> { typedef float type __attribute__((ext_vector_type(2))); };
 template <> struct Vec2<double> { typedef double type __attribute__((ext_vector_type(2))); };
 
-// VT is the storage type of the values (T, or _Float16 for the scaled smoother copy).
-template <int NT, typename VT, typename T, typename Ep>
+// Gather of the x pair of a column node: plainly, or formed on the fly as s * dinv * b
+// (the first Chebyshev step from a zero start, whose iterate is never stored).
+template <typename T>
+struct XPlain {
+    const T *x;
+    __device__ __forceinline__ typename Vec2<T>::type operator()(int64_t c) const {
+        return reinterpret_cast<const typename Vec2<T>::type *>(x)[c];
+    }
+};
+template <typename T>
+struct XJacobi {
+    const T *b, *dinv; T s;
+    __device__ __forceinline__ typename Vec2<T>::type operator()(int64_t c) const {
+        typedef typename Vec2<T>::type V2;
+        const V2 bv = reinterpret_cast<const V2 *>(b)[c], dv = reinterpret_cast<const V2 *>(dinv)[c];
+        return V2{s * dv.x * bv.x, s * dv.y * bv.y};
+    }
+};
+
+// Slice of the wavefront: the same for its 64 lanes, so it is made a scalar (the slice
+// pointer loads become scalar loads and the block loop is uniform).  Lanes past the last
+// row run the loop on the zero padding of the last slice and skip the epilogue.
+template <int NT>
+__device__ __forceinline__ int64_t wave_slice() {
+    return (int64_t)blockIdx.x * (NT / kWave) + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+}
+
+// VT is the storage type of the values (T, or _Float16 for the scaled smoother copy); XL
+// gathers the x pair of a column node.
+template <int NT, typename VT, typename T, typename Ep, typename XL = XPlain<T>>
 __global__ __launch_bounds__(NT) void k_spmv_sell22(int64_t nrows, const int64_t *__restrict__ sptr,
                                                      const int32_t *__restrict__ col, const VT *__restrict__ val,
-                                                     const T *__restrict__ x, Ep ep, double *__restrict__ partial,
-                                                     int64_t nblocks) {
+                                                     XL xl, Ep ep, double *__restrict__ partial, int64_t nblocks) {
     typedef typename Vec4<VT>::type V4;
     typedef typename Vec2<T>::type V2;
     constexpr int C = 64;
+    static_assert(C == kWave, "one slice per wavefront");
     constexpr int ND = Ep::kDots;
     const int64_t lb = blockIdx.x;
-    const int64_t slice = lb * (NT / C) + threadIdx.x / C;
+    const int64_t slice = wave_slice<NT>();
     const int lane = threadIdx.x % C;
     const int64_t row = slice * C + lane;
     double dsum[ND] = {};
-    if (row < nrows) {
+    if (slice * C < nrows) {
         const int64_t base = sptr[slice];
         const int len = (int)((sptr[slice + 1] - base) / C);
         const V4 *__restrict__ vp = reinterpret_cast<const V4 *>(val) + base + lane;
         const int32_t *__restrict__ cp = col + base + lane;
-        const V2 *__restrict__ xp = reinterpret_cast<const V2 *>(x);
         T a0 = T(0), a1 = T(0);
         // values and columns are streamed once: non-temporal loads keep them from
         // evicting the gathered x lines (measured: 117 -> 102 us on K at 256 x 2048)
 #pragma unroll 4
         for (int k = 0; k < len; ++k) {
             const V4 v = __builtin_nontemporal_load(&vp[(int64_t)k * C]);
-            const V2 xv = xp[__builtin_nontemporal_load(&cp[(int64_t)k * C])];
+            const V2 xv = xl(__builtin_nontemporal_load(&cp[(int64_t)k * C]));
             a0 += (T)v.x * xv.x + (T)v.y * xv.y;
             a1 += (T)v.z * xv.x + (T)v.w * xv.y;
         }
-        add_dots(dsum, ep(2 * row, a0));
-        add_dots(dsum, ep(2 * row + 1, a1));
+        if (row < nrows) {
+            add_dots(dsum, ep(2 * row, a0));
+            add_dots(dsum, ep(2 * row + 1, a1));
+        }
     }
     if (partial) {   // partial[d * nblocks + block]
         block_sum_n<NT, ND>(dsum);
@@ -289,11 +338,11 @@ __global__ __launch_bounds__(NT) void k_spmv_sell_rc(int64_t nrows, const int64_
     constexpr int C = 64;
     constexpr int ND = Ep::kDots;
     const int64_t lb = blockIdx.x;
-    const int64_t slice = lb * (NT / C) + threadIdx.x / C;
+    const int64_t slice = wave_slice<NT>();
     const int lane = threadIdx.x % C;
     const int64_t row = slice * C + lane;
     double dsum[ND] = {};
-    if (row < nrows) {
+    if (slice * C < nrows) {
         const int64_t base = sptr[slice];
         const int len = (int)((sptr[slice + 1] - base) / C);
         const V2 *__restrict__ vp = reinterpret_cast<const V2 *>(val) + base + lane;
@@ -311,11 +360,13 @@ __global__ __launch_bounds__(NT) void k_spmv_sell_rc(int64_t nrows, const int64_
                 a0 += v.x * x[2 * cl] + v.y * x[2 * cl + 1];
             }
         }
-        if constexpr (BR == 2) {
-            add_dots(dsum, ep(2 * row, a0));
-            add_dots(dsum, ep(2 * row + 1, a1));
-        } else {
-            add_dots(dsum, ep(row, a0));
+        if (row < nrows) {
+            if constexpr (BR == 2) {
+                add_dots(dsum, ep(2 * row, a0));
+                add_dots(dsum, ep(2 * row + 1, a1));
+            } else {
+                add_dots(dsum, ep(row, a0));
+            }
         }
     }
     if (partial) {
@@ -336,11 +387,11 @@ __global__ __launch_bounds__(NT) void k_spmv_sell11(int64_t nrows, const int64_t
     constexpr int C = 64;
     constexpr int ND = Ep::kDots;
     const int64_t lb = blockIdx.x;
-    const int64_t slice = lb * (NT / C) + threadIdx.x / C;
+    const int64_t slice = wave_slice<NT>();
     const int lane = threadIdx.x % C;
     const int64_t row = slice * C + lane;
     double dsum[ND] = {};
-    if (row < nrows) {
+    if (slice * C < nrows) {
         const int64_t base = sptr[slice];
         const int len = (int)((sptr[slice + 1] - base) / C);
         const T *__restrict__ vp = val + base + lane;
@@ -349,7 +400,7 @@ __global__ __launch_bounds__(NT) void k_spmv_sell11(int64_t nrows, const int64_t
 #pragma unroll 4
         for (int k = 0; k < len; ++k)
             a += __builtin_nontemporal_load(&vp[(int64_t)k * C]) * x[__builtin_nontemporal_load(&cp[(int64_t)k * C])];
-        add_dots(dsum, ep(row, a));
+        if (row < nrows) add_dots(dsum, ep(row, a));
     }
     if (partial) {
         block_sum_n<NT, ND>(dsum);
@@ -372,16 +423,31 @@ __global__ __launch_bounds__(NT) void k_reduce_map(int64_t n, F f, double *__res
         for (int d = 0; d < ND; ++d) partial[d * gridDim.x + blockIdx.x] = s[d];
 }
 
-// Sum nd columns of npart partials (partial[d * npart + i]) in a fixed order: out[d].
-template <int NT>
-__global__ __launch_bounds__(NT) void k_reduce_final(int64_t npart, int nd, const double *__restrict__ partial,
+// Sum ND columns of npart partials (partial[d * npart + i]) in a fixed order: out[d].
+// The partials were just written by another kernel, so the loop is latency-bound: each
+// thread keeps four independent loads per column in flight before it adds.
+template <int NT, int ND>
+__global__ __launch_bounds__(NT) void k_reduce_final(int64_t npart, const double *__restrict__ partial,
                                                      double *__restrict__ out) {
-    for (int d = 0; d < nd; ++d) {
-        double s = 0.0;
-        for (int64_t i = threadIdx.x; i < npart; i += NT) s += partial[d * npart + i];
-        s = block_sum<NT>(s);
-        if (threadIdx.x == 0) out[d] = s;
+    double s[ND];
+    for (int d = 0; d < ND; ++d) s[d] = 0.0;
+    int64_t i = threadIdx.x;
+    for (; i + 3 * NT < npart; i += 4 * NT) {
+        double v[ND][4];
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[d][u] = partial[d * npart + i + u * NT];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) s[d] += (v[d][0] + v[d][1]) + (v[d][2] + v[d][3]);
     }
+    for (; i < npart; i += NT)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) s[d] += partial[d * npart + i];
+    block_sum_n<NT, ND>(s);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) out[d] = s[d];
 }
 
 // Coarsest-level solve x = Ainv b: one wavefront per row, coalesced over the row, in

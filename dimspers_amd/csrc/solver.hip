@@ -86,7 +86,10 @@ void Reducer::fetch_many(int slot0, int n, double *out) {
 }
 
 void Reducer::finish(int64_t npart, int slot, int nd) {
-    k_reduce_final<NT><<<dim3(1), dim3(NT), 0, c.stream>>>(npart, nd, c.red64.p, c.d_sc.p + slot);
+    constexpr int NF = 1024;
+    if (nd == 1) k_reduce_final<NF, 1><<<dim3(1), dim3(NF), 0, c.stream>>>(npart, c.red64.p, c.d_sc.p + slot);
+    else if (nd == 2) k_reduce_final<NF, 2><<<dim3(1), dim3(NF), 0, c.stream>>>(npart, c.red64.p, c.d_sc.p + slot);
+    else throw Error(STK_E_STATE, "reduction of more than two columns");
     STK_HIP(hipGetLastError());
 }
 
@@ -129,16 +132,22 @@ struct Op22 {   // BSR 2x2 operator
     const T *val;
 };
 
-template <typename VT, typename T, typename Ep>
-int64_t spmv_sell(Context &c, const Pattern &p, const VT *val, const T *x, Ep ep, double *partial) {
+// Sliced 2x2 SpMV; xl gathers the x pair of a column node (XPlain: stored x).
+template <typename VT, typename T, typename Ep, typename XL>
+int64_t spmv_sell_x(Context &c, const Pattern &p, const VT *val, XL xl, Ep ep, double *partial) {
     const int64_t slices_per_block = NT / Pattern::kSlice;
     const int64_t nslice = (p.nrows + Pattern::kSlice - 1) / Pattern::kSlice;
     const int64_t nb = (nslice + slices_per_block - 1) / slices_per_block;
     if (partial && (int64_t)c.red64.n < nb * Ep::kDots) throw Error(STK_E_STATE, "reduction buffer too small");
-    k_spmv_sell22<NT, VT, T, Ep><<<dim3(nb), dim3(NT), 0, c.stream>>>(p.nrows, p.dsptr.p, p.dscol.p, val, x, ep,
-                                                                   partial, nb);
+    k_spmv_sell22<NT, VT, T, Ep, XL><<<dim3(nb), dim3(NT), 0, c.stream>>>(p.nrows, p.dsptr.p, p.dscol.p, val, xl, ep,
+                                                                       partial, nb);
     STK_HIP(hipGetLastError());
     return nb;
+}
+
+template <typename VT, typename T, typename Ep>
+int64_t spmv_sell(Context &c, const Pattern &p, const VT *val, const T *x, Ep ep, double *partial) {
+    return spmv_sell_x<VT, T, Ep>(c, p, val, XPlain<T>{x}, ep, partial);
 }
 
 // G (2x1) / G^T (1x2) product: sliced kernel when the pattern is sliced, else CSR.
@@ -254,6 +263,25 @@ void chebyshev(Context &c, const typename MGView<T>::Lv &l, const T *b, T *x, T 
     double rho = 1.0 / sigma;
     T *cur = x, *nxt = xa;
     const T *dinv = l.dinv;
+    if (zero_start && deg > 1 && l.A.pat->sell) {
+        // x0 = d0 = D^-1 b / theta is never stored: the first step gathers it from b and
+        // D^-1 and forms it again in the epilogue (one pass and one launch fewer)
+        const T it = (T)(1.0 / theta);
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        EpChebFirst<T> ep{b, dinv, d, nxt, it, (T)(rn * rho), (T)(2.0 * rn / delta), deg > 2};
+        spmv_sell_x<T, T>(c, *l.A.pat, l.A.val, XJacobi<T>{b, dinv, it}, ep, nullptr);
+        std::swap(cur, nxt);
+        rho = rn;
+        for (int k = 2; k < deg; ++k) {
+            const double rk = 1.0 / (2.0 * sigma - rho);
+            EpCheb<T> ek{b, dinv, cur, d, nxt, (T)(rk * rho), (T)(2.0 * rk / delta), true, k < deg - 1};
+            apply22_auto<T>(c, l.A, cur, ek);
+            std::swap(cur, nxt);
+            rho = rk;
+        }
+        *xout = cur;
+        return;
+    }
     if (zero_start) {
         const T it = (T)(1.0 / theta);
         launch_map(c, l.n, [=] __device__(int64_t i) {
@@ -262,21 +290,22 @@ void chebyshev(Context &c, const typename MGView<T>::Lv &l, const T *b, T *x, T 
             cur[i] = v;
         });
     }
-    auto step = [&](double c1, double c2, bool last) {
+    // the first step of a non-zero start does not read d (c1 = 0), the last does not store it
+    auto step = [&](double c1, double c2, bool first, bool last) {
         if (last && dot_w) {
-            EpChebDot2<T> ep{b, dinv, cur, d, nxt, (T)c1, (T)c2, dot_w};
+            EpChebDot2<T> ep{b, dinv, cur, d, nxt, (T)c1, (T)c2, dot_w, !first};
             *dot_nb = apply22_auto<T>(c, l.A, cur, ep, c.red64.p);
         } else {
-            EpCheb<T> ep{b, dinv, cur, d, nxt, (T)c1, (T)c2};
+            EpCheb<T> ep{b, dinv, cur, d, nxt, (T)c1, (T)c2, !first, !last};
             apply22_auto<T>(c, l.A, cur, ep);
         }
         std::swap(cur, nxt);
     };
     // d = D^-1 (b - A x) / theta ; x += d
-    if (!zero_start) step(0.0, 1.0 / theta, deg == 1);
+    if (!zero_start) step(0.0, 1.0 / theta, true, deg == 1);
     for (int k = 1; k < deg; ++k) {
         const double rn = 1.0 / (2.0 * sigma - rho);
-        step(rn * rho, 2.0 * rn / delta, k == deg - 1);
+        step(rn * rho, 2.0 * rn / delta, false, k == deg - 1);
         rho = rn;
     }
     *xout = cur;
@@ -1520,8 +1549,10 @@ double time_kernel(Context &c, int kernel, int reps, double *bytes) {
     hipEvent_t e0, e1;
     STK_HIP(hipEventCreate(&e0));
     STK_HIP(hipEventCreate(&e1));
-    const int64_t nU = c.nU, nb = c.KN.nnz(), nn = c.nnod;   // compulsory: structural blocks only
-    auto spmv_bytes = [&](int sv) { return (double)nb * (4 * sv + 4) + (double)(nn + 1) * 4 + 2.0 * nU * sv; };
+    // compulsory bytes: structural blocks only (four values and a 32-bit column each), the
+    // slice pointers (8 bytes per 64 rows), x read once, y written once
+    const int64_t nU = c.nU, nb = c.KN.nnz(), nsl = (c.KN.nrows + Pattern::kSlice - 1) / Pattern::kSlice;
+    auto spmv_bytes = [&](int sv) { return (double)nb * (4 * sv + 4) + (double)(nsl + 1) * 8 + 2.0 * nU * sv; };
     std::function<void()> launch;
     DBuf<double> x64, y64, b64, d64, o64;
     DBuf<float> x32, y32, b32, d32, o32;
@@ -1538,11 +1569,13 @@ double time_kernel(Context &c, int kernel, int reps, double *bytes) {
             } else {
                 float *o = o32.p;
                 const float *dinv = c.Kdinv32.p;
+                // the first post-smoothing step of the V-cycle: d = c2 Dinv (b - K x) is
+                // stored, the previous d is not read; b and Dinv read, d written besides x, x'
                 launch = [&c, K, x, b, d, o, dinv]() {
-                    EpCheb<float> ep{b, dinv, x, d, o, 0.5f, 0.25f};
+                    EpCheb<float> ep{b, dinv, x, d, o, 0.0f, 0.25f, false, true};
                     apply22<8>(c, K, x, ep);
                 };
-                *bytes = spmv_bytes(4) + 5.0 * nU * 4;
+                *bytes = spmv_bytes(4) + 3.0 * nU * 4;
             }
             break;
         }
@@ -1641,10 +1674,10 @@ double time_kernel(Context &c, int kernel, int reps, double *bytes) {
                 double *o = o64.p;
                 const double *dinv = c.Kdinv64.p;
                 launch = [&c, K, x, b, d, o, dinv]() {
-                    EpCheb<double> ep{b, dinv, x, d, o, 0.5, 0.25};
+                    EpCheb<double> ep{b, dinv, x, d, o, 0.0, 0.25, false, true};
                     apply22<8>(c, K, x, ep);
                 };
-                *bytes = spmv_bytes(8) + 5.0 * nU * 8;
+                *bytes = spmv_bytes(8) + 3.0 * nU * 8;
             }
             break;
         }

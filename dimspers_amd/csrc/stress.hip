@@ -329,15 +329,15 @@ void statistics(Context &c, const double *U, const double *T, double out[4]) {
     const int nb = (int)std::min<int64_t>(kReduceBlocks, std::max<int64_t>(1, (c.nel + 255) / 256));
     if ((int64_t)c.red64.n < 2 * (int64_t)nb) throw Error(STK_E_STATE, "reduction buffer too small");
     k_reduce_map<256, 2, VolInt><<<dim3(nb), dim3(256), 0, c.stream>>>(c.nel, VolInt{a}, part);
-    k_reduce_final<256><<<dim3(1), dim3(256), 0, c.stream>>>(nb, 2, part, sc + 0);
+    k_reduce_final<256, 2><<<dim3(1), dim3(256), 0, c.stream>>>(nb, part, sc + 0);
     ListSum lt{c.d_itop.p, T}, lb{c.d_ibot.p, T};
     const int64_t ntop = (int64_t)c.d_itop.n, nbot = (int64_t)c.d_ibot.n;
     const int nbt = (int)std::min<int64_t>(kReduceBlocks, std::max<int64_t>(1, (ntop + 255) / 256));
     k_reduce_map<256, 1, ListSum><<<dim3(nbt), dim3(256), 0, c.stream>>>(ntop, lt, part);
-    k_reduce_final<256><<<dim3(1), dim3(256), 0, c.stream>>>(nbt, 1, part, sc + 2);
+    k_reduce_final<256, 1><<<dim3(1), dim3(256), 0, c.stream>>>(nbt, part, sc + 2);
     const int nbb = (int)std::min<int64_t>(kReduceBlocks, std::max<int64_t>(1, (nbot + 255) / 256));
     k_reduce_map<256, 1, ListSum><<<dim3(nbb), dim3(256), 0, c.stream>>>(nbot, lb, part);
-    k_reduce_final<256><<<dim3(1), dim3(256), 0, c.stream>>>(nbb, 1, part, sc + 3);
+    k_reduce_final<256, 1><<<dim3(1), dim3(256), 0, c.stream>>>(nbb, part, sc + 3);
     STK_HIP(hipGetLastError());
     double h[4];
     STK_HIP(hipMemcpyAsync(h, sc, 4 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
