@@ -16,7 +16,7 @@ STK_E_HIP, STK_E_SHAPE, STK_E_STATE, STK_E_NOTCONV, STK_E_NAN, STK_E_ARG, STK_E_
 
 ASM_FP64, ASM_FP32, ASM_CARTESIAN, ASM_NO_BC, ASM_ISOTROPIC, ASM_NO_MG = 1, 2, 4, 8, 16, 32
 ASM_FP16_SMOOTHER = 64
-MAT_K, MAT_G, MAT_GT, MAT_M, MAT_S, MAT_KT, MAT_MT = 0, 1, 2, 3, 4, 5, 6
+MAT_K, MAT_G, MAT_GT, MAT_M, MAT_S, MAT_KT, MAT_MT, MAT_R, MAT_RK = 0, 1, 2, 3, 4, 5, 6, 7, 8
 SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED = 0, 1, 2
 INNER_MG, INNER_JACOBI = 0, 1
 STOP_SCHUR, STOP_FULL = 0, 1
@@ -250,7 +250,8 @@ class Context:
 
     def spmv(self, which, x, precision=64):
         x = np.ascontiguousarray(x, dtype=np.float64)
-        ny = {MAT_K: self.nU, MAT_G: self.nU, MAT_GT: self.nP, MAT_M: self.nP, MAT_S: self.nP}[which]
+        ny = {MAT_K: self.nU, MAT_G: self.nU, MAT_GT: self.nP, MAT_M: self.nP, MAT_S: self.nP,
+              MAT_R: 2 * self.nP, MAT_RK: 2 * self.nP}[which]
         y = np.zeros(ny)
         self._check(self.lib.stk_spmv(self.h, which, precision, _d(x), _d(y)))
         return y

@@ -197,7 +197,7 @@ namespace {
 
 template <typename T>
 void assemble_prec(Context &c, int flags, bool mg, DBuf<T> &K, DBuf<T> &G, DBuf<T> &GT, DBuf<T> &M, DBuf<T> &F,
-                   bool f64) {
+                   DBuf<T> &RK, bool f64) {
     K.alloc(4 * c.KN.nstore()); K.zero(c.stream);
     G.alloc(2 * c.GN.nstore()); G.zero(c.stream);
     GT.alloc(2 * c.GTN.nstore()); GT.zero(c.stream);
@@ -218,8 +218,12 @@ void assemble_prec(Context &c, int flags, bool mg, DBuf<T> &K, DBuf<T> &G, DBuf<
         }
         A1 = Abuf(c.mg[0]).p;
         E1 = Ebuf(c.mg[0]).p;
+        RK.alloc(4 * c.GTN.nstore());
+        RK.zero(c.stream);
+    } else {
+        RK.release();
     }
-    assemble_impl<T>(c, flags, K.p, G.p, GT.p, M.p, F.p, A1, E1);
+    assemble_impl<T>(c, flags, K.p, G.p, GT.p, M.p, F.p, A1, E1, mg ? RK.p : nullptr);
     if (mg)
         for (size_t l = 0; l + 1 < c.mg.size(); ++l)
             galerkin_impl<T>(c, l, Ebuf(c.mg[l]).p, Ebuf(c.mg[l + 1]).p, Abuf(c.mg[l + 1]).p);
@@ -248,9 +252,10 @@ int stk_assemble(stk_ctx *ctx, int32_t flags) {
         c.have_mg64 = c.have_mg32 = false;
         // stale products of an earlier assembly
         c.K32.release(); c.G32.release(); c.GT32.release(); c.M32.release(); c.F32.release(); c.pc32.release();
+        c.RK32.release();
         c.S64.release(); c.S32.release(); c.Sdinv64.release(); c.Sdinv32.release();
         if (fp64) {
-            assemble_prec<double>(c, flags, mg, c.K64, c.G64, c.GT64, c.M64, c.F64, true);
+            assemble_prec<double>(c, flags, mg, c.K64, c.G64, c.GT64, c.M64, c.F64, c.RK64, true);
             c.pc64.alloc(c.nP);
             lumped_inverse<double>(c, c.MN, c.M64.p, c.pc64.p);
             if (fp32) {
@@ -260,15 +265,18 @@ int stk_assemble(stk_ctx *ctx, int32_t flags) {
                 c.M32.alloc(c.M64.n); convert<double, float>(c, c.M64.p, c.M32.p, c.M64.n);
                 c.F32.alloc(c.F64.n); convert<double, float>(c, c.F64.p, c.F32.p, c.F64.n);
                 c.pc32.alloc(c.nP); convert<double, float>(c, c.pc64.p, c.pc32.p, c.nP);
-                if (mg)
+                if (mg) {
                     for (MGLevel &L : c.mg) {
                         L.A32.alloc(L.A64.n);
                         convert<double, float>(c, L.A64.p, L.A32.p, L.A64.n);
                     }
+                    c.RK32.alloc(c.RK64.n);
+                    convert<double, float>(c, c.RK64.p, c.RK32.p, c.RK64.n);
+                }
             }
         } else {
-            c.K64.release(); c.G64.release(); c.GT64.release(); c.M64.release(); c.pc64.release();
-            assemble_prec<float>(c, flags, mg, c.K32, c.G32, c.GT32, c.M32, c.F32, false);
+            c.K64.release(); c.G64.release(); c.GT64.release(); c.M64.release(); c.pc64.release(); c.RK64.release();
+            assemble_prec<float>(c, flags, mg, c.K32, c.G32, c.GT32, c.M32, c.F32, c.RK32, false);
             c.pc32.alloc(c.nP);
             lumped_inverse<float>(c, c.MN, c.M32.p, c.pc32.p);
             c.F64.alloc(c.nU);
@@ -314,6 +322,15 @@ int stk_export_csc(stk_ctx *ctx, int32_t which, int32_t lower_only, int64_t *nro
             case STK_MAT_S: p = &c.MN; nr = nc = c.nP; v64 = c.S64.p; v32 = c.S32.p; break;
             case STK_MAT_KT: p = &c.KN; nr = nc = c.nnod; v64 = c.KT.p; break;
             case STK_MAT_MT: p = &c.KN; nr = nc = c.nnod; v64 = c.MT.p; break;
+            case STK_MAT_R:
+                STK_CHECK(c.mg_built, STK_E_STATE, "no multigrid hierarchy");
+                p = &c.mg[0].R_pat; br = bc = 2; nr = 2 * c.mg[0].nvert; nc = c.nU;
+                v64 = c.mg[0].R64.p; v32 = c.mg[0].R32.p;
+                break;
+            case STK_MAT_RK:
+                STK_CHECK(c.mg_built, STK_E_STATE, "no multigrid hierarchy");
+                p = &c.GTN; br = bc = 2; nr = 2 * c.nP; nc = c.nU; v64 = c.RK64.p; v32 = c.RK32.p;
+                break;
             default: throw Error(STK_E_ARG, "unknown operator");
         }
         STK_CHECK(v64 || v32, STK_E_STATE, "operator not available");
@@ -431,7 +448,8 @@ int stk_time_kernel(stk_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms, 
             STK_HIP(hipEventRecord(e0, c.stream));
             for (int k = 0; k < reps; ++k)
                 assemble_impl<double>(c, flags, c.K64.p, c.G64.p, c.GT64.p, c.M64.p, c.F64.p,
-                                      mg ? c.mg[0].A64.p : nullptr, mg ? c.mg[0].E64.p : nullptr);
+                                      mg ? c.mg[0].A64.p : nullptr, mg ? c.mg[0].E64.p : nullptr,
+                                      mg ? c.RK64.p : nullptr);
             STK_HIP(hipEventRecord(e1, c.stream));
             STK_HIP(hipEventSynchronize(e1));
             float ms = 0;

@@ -86,6 +86,7 @@ struct AsmArgs {
     T *K, *G, *GT, *M, *F;
     // finest multigrid level (vertex space)
     T *A1, *E1;                         // A1: BSR 2x2 over the M pattern; E1: 36 x nel by element id
+    T *RK;                              // R K: BSR 2x2 over the G^T pattern (null: not built)
     int cart, bc, iso;
 };
 
@@ -383,6 +384,23 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(AsmArgs<T> a) {
                 t1[0][2 * v + 1] += kb[0] * P[1] + kb[1] * P[3];
                 t1[1][2 * v + 0] += kb[2] * P[0] + kb[3] * P[2];
                 t1[1][2 * v + 1] += kb[2] * P[1] + kb[3] * P[3];
+            }
+        }
+        if (a.RK) {
+            // (Ph^T K_c)(vertex u, node f) = T1(f, u)^T with K_c the constrained K: its
+            // columns of fixed dofs are zero (the unit diagonal of a fixed dof meets a
+            // zero column of the masked Ph^T)
+#pragma unroll
+            for (int u = 0; u < 3; ++u) {
+                const int64_t st = a.sGT[(u * 6 + f) * nel + p];
+                const int64_t bt = a.GTsptr ? a.GTsptr[n[u] / Pattern::kSlice] + st * Pattern::kSlice +
+                                                  n[u] % Pattern::kSlice
+                                            : (int64_t)a.GTptr[n[u]] + st;
+#pragma unroll
+                for (int cu = 0; cu < 2; ++cu)
+#pragma unroll
+                    for (int rc = 0; rc < 2; ++rc)
+                        if (!fxa[2 * f + rc]) a.RK[4 * bt + cu * 2 + rc] += t1[rc][2 * u + cu];
             }
         }
 #pragma unroll
@@ -826,7 +844,7 @@ void upload_quadrature(hipStream_t s) {
 }
 
 template <typename T>
-void assemble_impl(Context &c, int flags, T *K, T *G, T *GT, T *M, T *F, T *A1, T *E1) {
+void assemble_impl(Context &c, int flags, T *K, T *G, T *GT, T *M, T *F, T *A1, T *E1, T *RK) {
     AsmArgs<T> a{};
     a.nel = c.nel;
     a.perm = c.d_perm.p; a.e2n = c.d_e2n.p; a.th = c.d_elth.p; a.r = c.d_elr.p;
@@ -838,7 +856,7 @@ void assemble_impl(Context &c, int flags, T *K, T *G, T *GT, T *M, T *F, T *A1, 
     a.A1sptr = (A1 && c.mg[0].pat.sell) ? c.mg[0].pat.dsptr.p : nullptr;
     a.Gsptr = c.GN.sell ? c.GN.dsptr.p : nullptr;
     a.GTsptr = c.GTN.sell ? c.GTN.dsptr.p : nullptr;
-    a.K = K; a.G = G; a.GT = GT; a.M = M; a.F = F; a.A1 = A1; a.E1 = E1;
+    a.K = K; a.G = G; a.GT = GT; a.M = M; a.F = F; a.A1 = A1; a.E1 = E1; a.RK = A1 ? RK : nullptr;
     a.cart = (flags & STK_ASM_CARTESIAN) ? 1 : 0;
     a.bc = (flags & STK_ASM_NO_BC) ? 0 : 1;
     a.iso = ((flags & STK_ASM_ISOTROPIC) || !c.have_C) ? 1 : 0;
@@ -878,8 +896,10 @@ void fix_diag(Context &c, const Pattern &p, int64_t nfix, const int64_t *dofs, T
     STK_HIP(hipGetLastError());
 }
 
-template void assemble_impl<double>(Context &, int, double *, double *, double *, double *, double *, double *, double *);
-template void assemble_impl<float>(Context &, int, float *, float *, float *, float *, float *, float *, float *);
+template void assemble_impl<double>(Context &, int, double *, double *, double *, double *, double *, double *, double *,
+                                    double *);
+template void assemble_impl<float>(Context &, int, float *, float *, float *, float *, float *, float *, float *,
+                                   float *);
 template void galerkin_impl<double>(Context &, size_t, double *, double *, double *);
 template void galerkin_impl<float>(Context &, size_t, float *, float *, float *);
 template void fix_diag<double>(Context &, const Pattern &, int64_t, const int64_t *, double *);

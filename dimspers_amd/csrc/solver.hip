@@ -202,6 +202,7 @@ struct MGView {
         int64_t n;            // dofs
         T *b, *x, *xa, *r, *d;
         Op22<T> P, R;         // transfer to / from the next coarser level (P: this <- coarser)
+        Op22<T> RK;           // R A (level 0 only; pattern null when not built)
     };
     std::vector<Lv> lv;
     const T *coarse_inv = nullptr;
@@ -227,6 +228,8 @@ MGView<T> make_view(Context &c) {
     if (!c.mg.empty()) {
         l0.P = {&c.mg[0].P_pat, f64 ? (const T *)c.mg[0].P64.p : (const T *)c.mg[0].P32.p};
         l0.R = {&c.mg[0].R_pat, f64 ? (const T *)c.mg[0].R64.p : (const T *)c.mg[0].R32.p};
+        const T *rk = (const T *)pick<T>(c.RK64, c.RK32).p;
+        if (rk) l0.RK = {&c.GTN, rk};
     }
     v.lv.push_back(l0);
     for (size_t i = 0; i < c.mg.size(); ++i) {
@@ -394,13 +397,20 @@ T *vcycle(Context &c, MGView<T> &v, size_t l, const T *b, int deg, const T *dot_
     }
     T *x;
     chebyshev<T>(c, L, b, L.x, L.xa, L.d, deg, true, &x);
-    // r = b - A x
-    EpResidual<T> er{L.r, b};
-    apply22_auto<T>(c, L.A, x, er);
-    // b_c = R r
     auto &C = v.lv[l + 1];
-    EpStore<T> es{C.b};
-    apply22_auto<T>(c, L.R, L.r, es);
+    if (L.RK.pat) {
+        // b_c = R b - (R A) x: R A has the coarse level's rows, a quarter of A's
+        EpStore<T> es{C.b};
+        apply22_auto<T>(c, L.R, b, es);
+        EpAxpyOut<T> ea{C.b, T(-1), T(1)};
+        apply22_auto<T>(c, L.RK, x, ea);
+    } else {
+        // r = b - A x; b_c = R r
+        EpResidual<T> er{L.r, b};
+        apply22_auto<T>(c, L.A, x, er);
+        EpStore<T> es{C.b};
+        apply22_auto<T>(c, L.R, L.r, es);
+    }
     T *xc = vcycle<T>(c, v, l + 1, C.b, deg, nullptr, nullptr);
     // x += P xc
     EpAxpyOut<T> ea{x, T(1), T(1)};
@@ -1299,6 +1309,14 @@ void host_spmv(Context &c, int which, int prec, const double *xh, double *yh) {
         case STK_MAT_G: p = &c.GN; nx = c.nP; ny = c.nU; break;
         case STK_MAT_GT: p = &c.GTN; nx = c.nU; ny = c.nP; break;
         case STK_MAT_M: case STK_MAT_S: p = &c.MN; nx = ny = c.nP; break;
+        case STK_MAT_R:
+            STK_CHECK(c.mg_built, STK_E_STATE, "no multigrid hierarchy");
+            p = &c.mg[0].R_pat; nx = c.nU; ny = 2 * c.mg[0].nvert;
+            break;
+        case STK_MAT_RK:
+            STK_CHECK(c.mg_built, STK_E_STATE, "no multigrid hierarchy");
+            p = &c.GTN; nx = c.nU; ny = 2 * c.nP;
+            break;
         default: throw Error(STK_E_ARG, "unknown operator");
     }
     DBuf<double> x64(nx), y64(ny);
@@ -1308,6 +1326,7 @@ void host_spmv(Context &c, int which, int prec, const double *xh, double *yh) {
         EpStore<T> es{y};
         switch (which) {
             case STK_MAT_K: apply22<8>(c, Op22<T>{p, val}, x, es); break;
+            case STK_MAT_R: case STK_MAT_RK: apply22_auto<T>(c, Op22<T>{p, val}, x, es); break;
             case STK_MAT_G: apply_rc<2, 1, T>(c, *p, val, x, es, nullptr); break;
             case STK_MAT_GT: apply_rc<1, 2, T>(c, *p, val, x, es, nullptr); break;
             default: spmv<1, 1, 8, T>(c, *p, val, x, es, nullptr); break;
@@ -1319,6 +1338,8 @@ void host_spmv(Context &c, int which, int prec, const double *xh, double *yh) {
             case STK_MAT_G: return c.G64.p;
             case STK_MAT_GT: return c.GT64.p;
             case STK_MAT_M: return c.M64.p;
+            case STK_MAT_R: return c.mg[0].R64.p;
+            case STK_MAT_RK: return c.RK64.p;
             default: return c.S64.p;
         }
     };
@@ -1328,6 +1349,8 @@ void host_spmv(Context &c, int which, int prec, const double *xh, double *yh) {
             case STK_MAT_G: return c.G32.p;
             case STK_MAT_GT: return c.GT32.p;
             case STK_MAT_M: return c.M32.p;
+            case STK_MAT_R: return c.mg[0].R32.p;
+            case STK_MAT_RK: return c.RK32.p;
             default: return c.S32.p;
         }
     };

@@ -177,6 +177,10 @@ struct Context {
     bool assembled = false, loaded = false, have_mg64 = false, have_mg32 = false;
     DBuf<double> K64, G64, GT64, M64, F64, pc64;
     DBuf<float> K32, G32, GT32, M32, F32, pc32;
+    // R K: the P2 -> P1 restriction times the constrained K, 2x2 blocks on the G^T pattern
+    // (vertex rows, node columns); the V-cycle forms the coarse right-hand side
+    // R (b - K x) as R b - (R K) x, a product with a quarter of K's rows
+    DBuf<double> RK64; DBuf<float> RK32;
     DBuf<double> S64; DBuf<float> S32;        // learned Schur sum (pattern MN)
     DBuf<double> Sdinv64; DBuf<float> Sdinv32;
     DBuf<double> Kdinv64; DBuf<float> Kdinv32; // inverse diagonal of K (per dof)
@@ -255,7 +259,7 @@ struct SchurSampleHost {
 void launch_schur_samples(Context &c, const SchurSampleHost &h);
 void launch_learned_scatter(Context &c, const float *Y, const double *scale, double *S64);
 template <typename T>
-void assemble_impl(Context &c, int flags, T *K, T *G, T *GT, T *M, T *F, T *A1, T *E1);
+void assemble_impl(Context &c, int flags, T *K, T *G, T *GT, T *M, T *F, T *A1, T *E1, T *RK = nullptr);
 template <typename T>
 void galerkin_impl(Context &c, size_t l, T *Ef, T *Ec, T *A);
 template <typename T>

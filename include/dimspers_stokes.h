@@ -54,7 +54,11 @@ enum {
     STK_MAT_M = 3,    /* 1/eta pressure mass, nP x nP            */
     STK_MAT_S = 4,    /* learned/exact local-Schur sum, nP x nP  */
     STK_MAT_KT = 5,   /* thermal diffusion matrix, nnod x nnod   */
-    STK_MAT_MT = 6    /* thermal mass matrix rho Cp, nnod x nnod */
+    STK_MAT_MT = 6,   /* thermal mass matrix rho Cp, nnod x nnod */
+    STK_MAT_R = 7,    /* multigrid restriction P2 -> P1 (rotated */
+                      /* frames, fixed dofs masked), 2 nP x nU   */
+    STK_MAT_RK = 8    /* R K (K with boundary conditions), the   */
+                      /* coarse residual operator, 2 nP x nU     */
 };
 
 /* Solver methods (stk_opts.method). */

@@ -97,6 +97,48 @@ def test_spmv_matches_scipy(medium):
     ctx.close()
 
 
+@pytest.mark.parametrize("mesh", [(8, 64, 1), (64, 512, 3)])
+def test_restriction_times_k_is_the_product(mesh):
+    """R K, assembled element by element (rotated P2 -> P1 interpolation blocks, constrained
+    K), equals the product of the exported restriction and K to rounding; R itself holds the
+    identity at vertices and half rotations (row norm 1/2) at edge midpoints, with the dofs of
+    fixed nodes masked.  The V-cycle forms its coarse right-hand side as R b - (R K) x."""
+    nr, nt, w = mesh
+    inp = make_inputs(scaled_workload(w, nr, nt))
+    m = inp.mesh
+    ctx = _ctx(inp, _lib.ASM_FP64 | _lib.ASM_FP32)
+    K = ctx.export_csc(_lib.MAT_K)
+    R = ctx.export_csc(_lib.MAT_R)
+    RK = ctx.export_csc(_lib.MAT_RK)
+    assert R.shape == RK.shape == (2 * m.nvert, m.nU)
+    fixed, _ = m.free_slip()
+    Rd = R.toarray() if m.nU < 20000 else None
+    if Rd is not None:
+        free = np.ones(m.nU, dtype=bool)
+        free[fixed] = False
+        assert np.abs(Rd[:, ~free]).max() == 0.0              # fixed fine dofs: zero columns
+        assert np.abs(Rd[~free[:2 * m.nvert], :]).max() == 0.0  # fixed vertex dofs: zero rows
+        v = np.flatnonzero(free[:2 * m.nvert])
+        assert np.array_equal(Rd[v, v], np.ones(v.size))      # vertex rows: identity block
+        mid = Rd[:, 2 * m.nvert:].reshape(m.nvert, 2, -1, 2)   # (vertex, c, midpoint, c')
+        norms = np.sqrt((mid ** 2).sum(axis=3))
+        full = (np.abs(mid).sum(axis=(1, 3)) > 0) & np.all(free[:2 * m.nvert].reshape(-1, 2), axis=1)[:, None] \
+            & np.all(free[2 * m.nvert:].reshape(-1, 2), axis=1)[None, :]
+        assert full.sum() > 0
+        assert np.allclose(norms.transpose(0, 2, 1)[full], 0.5, atol=1e-15)
+    ref = (R @ K).tocsc()
+    err = abs(RK - ref).max() / abs(ref).max()
+    print(f"[result] R K vs R @ K {nr}x{nt}: {err:.2e}")
+    assert err < 1e-13
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal(m.nU)
+    want = R @ (K @ x)
+    assert np.abs(ctx.spmv(_lib.MAT_RK, x, 64) - want).max() < 1e-12 * np.abs(want).max()
+    assert np.abs(ctx.spmv(_lib.MAT_RK, x, 32) - want).max() < 1e-5 * np.abs(want).max()
+    assert np.abs(ctx.spmv(_lib.MAT_R, x, 64) - R @ x).max() < 1e-13 * np.abs(R @ x).max()
+    ctx.close()
+
+
 def test_assembly_is_bitwise_deterministic(medium):
     ctx = _ctx(medium, _lib.ASM_FP64)
     a = ctx.export_csc(_lib.MAT_K).data.copy()
