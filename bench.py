@@ -237,7 +237,7 @@ def main():
         "speedup_vs_plain": round(plain["ms"] / ms, 2) if plain else None,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "kernel": "k_spmv_sell22<256,float,float,EpCheb<float>> (fused Chebyshev SpMV on K, sliced layout)",
+                     "kernel": "k_spmv_sell22<256,float,float,EpCheb<float>,XPlain<float>> (fused Chebyshev step on K, sliced layout)",
                      "avg_us": round(kms * 1e3, 2), "bytes_per_launch": kbytes},
         "cpu_baseline": cpu,
         "thermal_step": {"ms": round(trep.solve_ms, 2), "cg_iterations": trep.outer_iterations,

@@ -16,6 +16,7 @@ STK_E_HIP, STK_E_SHAPE, STK_E_STATE, STK_E_NOTCONV, STK_E_NAN, STK_E_ARG, STK_E_
 
 ASM_FP64, ASM_FP32, ASM_CARTESIAN, ASM_NO_BC, ASM_ISOTROPIC, ASM_NO_MG = 1, 2, 4, 8, 16, 32
 ASM_FP16_SMOOTHER = 64
+SMOOTH_GS, SMOOTH_CHEB = 0, 1
 MAT_K, MAT_G, MAT_GT, MAT_M, MAT_S, MAT_KT, MAT_MT, MAT_R, MAT_RK = 0, 1, 2, 3, 4, 5, 6, 7, 8
 SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED = 0, 1, 2
 INNER_MG, INNER_JACOBI = 0, 1
@@ -31,7 +32,7 @@ EXPORTED = [
 ]
 N_PARAM = 9094
 KERNEL_SPMV_K32, KERNEL_SPMV_K64, KERNEL_CHEB_K32, KERNEL_CHEB_K64, KERNEL_ASSEMBLY, KERNEL_CHEB_KH16 = 0, 1, 2, 3, 4, 5
-KERNEL_COPY, KERNEL_VCYCLE32, KERNEL_INNER_IT32, KERNEL_STRESS, KERNEL_DEFGRAD = 6, 7, 8, 9, 10
+KERNEL_COPY, KERNEL_VCYCLE32, KERNEL_INNER_IT32, KERNEL_STRESS, KERNEL_DEFGRAD, KERNEL_KP32 = 6, 7, 8, 9, 10, 11
 STRESS_FIELDS = ("exx", "ezz", "exz", "eII", "txx", "tzz", "txz", "tII")
 
 
@@ -42,7 +43,7 @@ class StkOpts(C.Structure):
         ("null_mode", C.c_int32), ("pc_sweeps", C.c_int32),
         ("rtol", C.c_double), ("inner_rtol", C.c_double), ("mixed_inner_rtol", C.c_double),
         ("mixed_schur_rtol", C.c_double), ("stall_ratio", C.c_double),
-        ("inner_warm", C.c_int32), ("reserved", C.c_int32),
+        ("inner_warm", C.c_int32), ("fine_smoother", C.c_int32),
     ]
 
 

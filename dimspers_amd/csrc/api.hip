@@ -72,6 +72,7 @@ void stk_default_opts(stk_opts *o) {
     o->stall_ratio = 0.5;
     o->null_mode = STK_NULL_PROJECT;
     o->pc_sweeps = 3;
+    o->fine_smoother = STK_SMOOTH_GS;
 }
 
 const char *stk_last_error(const stk_ctx *ctx) { return ctx ? ctx->c.err.c_str() : g_err.c_str(); }
@@ -123,6 +124,10 @@ int stk_create(stk_ctx **out, int64_t nel, int64_t nnod, int64_t nvert, const in
         build_slot_maps(c);
         detect_structured(c);
         build_mg_structure(c);
+        if (!c.mg.empty()) {
+            build_gs_layout(c);
+            c.gs.on = true;
+        }
         if (!c.mg.empty()) {
             STK_CHECK(c.mg[0].pat.ptr == c.MN.ptr && c.mg[0].pat.col == c.MN.col, STK_E_SHAPE,
                       "vertex pattern mismatch between mesh and multigrid level 0");
@@ -413,6 +418,8 @@ int stk_solve(stk_ctx *ctx, const stk_opts *opts, stk_report *report) {
         STK_CHECK(o.method >= STK_SOLVE_PLAIN && o.method <= STK_SOLVE_LEARNED, STK_E_ARG, "bad method");
         STK_CHECK(o.inner == STK_INNER_MG || o.inner == STK_INNER_JACOBI, STK_E_ARG, "bad inner solver");
         STK_CHECK(o.itmax > 0 && o.inner_itmax > 0 && o.rtol > 0 && o.cheb_degree >= 1, STK_E_ARG, "bad options");
+        STK_CHECK(o.fine_smoother == STK_SMOOTH_GS || o.fine_smoother == STK_SMOOTH_CHEB, STK_E_ARG, "bad smoother");
+        c.fine_smoother = o.fine_smoother;
         if (o.method == STK_SOLVE_PLAIN) STK_CHECK(c.K64, STK_E_STATE, "plain solve needs FP64 operators");
         const int rc = run_solve(c, o, rep);
         if (rc == STK_E_NOTCONV) c.err = "stopped at the iteration cap before the tolerance";

@@ -410,6 +410,73 @@ __global__ __launch_bounds__(NT) void k_spmv_sell11(int64_t nrows, const int64_t
     }
 }
 
+// One colour of a multicolour block Gauss-Seidel sweep on the coloured sliced layout:
+// x_n += D_n^-1 (b_n - (K x)_n) for the node rows n of slices [s0, s1).  No two nodes of a
+// colour are coupled, so the in-place update of x is race free.  ZERO: forward sweep from
+// x = 0, where only the row's lower-colour blocks (the first nlow) meet non-zero x and the
+// row's own x_n is not read.  ND = 2: also (b . x, w . x) of the updated rows into
+// partial[d * pstride + block].
+template <int NT, bool ZERO, int ND>
+__global__ __launch_bounds__(NT) void k_gs_colour(int64_t s0, int64_t s1, const int64_t *__restrict__ sptr,
+                                                   const int32_t *__restrict__ col, const float *__restrict__ val,
+                                                   const int32_t *__restrict__ rowid, const int32_t *__restrict__ slow,
+                                                   const uint8_t *__restrict__ nlow, const float *__restrict__ dinv,
+                                                   const float *__restrict__ b, float *__restrict__ x,
+                                                   const float *__restrict__ w, double *__restrict__ partial,
+                                                   int64_t pstride) {
+    typedef typename Vec4<float>::type V4;
+    typedef typename Vec2<float>::type V2;
+    constexpr int C = 64;
+    static_assert(C == kWave, "one slice per wavefront");
+    const int64_t slice = s0 + wave_slice<NT>();
+    const int lane = threadIdx.x % C;
+    double dsum[ND > 0 ? ND : 1] = {};
+    if (slice < s1) {
+        const int64_t base = sptr[slice];
+        const int len = ZERO ? slow[slice] : (int)((sptr[slice + 1] - base) / C);
+        const int64_t q = slice * C + lane;
+        const int32_t r = rowid[q];
+        const int nl = ZERO ? (int)nlow[q] : len;
+        const V4 *__restrict__ vp = reinterpret_cast<const V4 *>(val) + base + lane;
+        const int32_t *__restrict__ cp = col + base + lane;
+        V2 *__restrict__ x2 = reinterpret_cast<V2 *>(x);
+        float a0 = 0.0f, a1 = 0.0f;
+#pragma unroll 4
+        for (int k = 0; k < len; ++k) {
+            if (!ZERO || k < nl) {
+                const V4 v = __builtin_nontemporal_load(&vp[(int64_t)k * C]);
+                const V2 xv = x2[__builtin_nontemporal_load(&cp[(int64_t)k * C])];
+                a0 += v.x * xv.x + v.y * xv.y;
+                a1 += v.z * xv.x + v.w * xv.y;
+            }
+        }
+        if (r >= 0) {
+            const V2 bv = reinterpret_cast<const V2 *>(b)[r];
+            const V4 di = reinterpret_cast<const V4 *>(dinv)[r];
+            const float r0 = bv.x - a0, r1 = bv.y - a1;
+            V2 xn{di.x * r0 + di.y * r1, di.z * r0 + di.w * r1};
+            if (!ZERO) {
+                const V2 xo = x2[r];
+                xn.x += xo.x;
+                xn.y += xo.y;
+            }
+            x2[r] = xn;
+            if constexpr (ND == 2) {
+                const V2 wv = reinterpret_cast<const V2 *>(w)[r];
+                dsum[0] = (double)bv.x * (double)xn.x + (double)bv.y * (double)xn.y;
+                dsum[1] = (double)wv.x * (double)xn.x + (double)wv.y * (double)xn.y;
+            }
+        }
+    }
+    if constexpr (ND == 2) {
+        block_sum_n<NT, 2>(dsum);
+        if (threadIdx.x == 0) {
+            partial[blockIdx.x] = dsum[0];
+            partial[pstride + blockIdx.x] = dsum[1];
+        }
+    }
+}
+
 // ------------------------------------------------------------------ reductions
 // f(i) returns a double (ND = 1) or Dots<ND>; it may also write (fused update + dot).
 template <int NT, int ND, typename F>

@@ -150,6 +150,99 @@ void make_sliced(Pattern &p, hipStream_t s) {
 
 void build_colouring(Context &c) { colour_elements(c.nel, 6, c.e2n, c.nnod, c.perm, c.col_off); }
 
+void build_gs_layout(Context &c) {
+    const Pattern &K = c.KN;
+    const int64_t n = K.nrows, C = Pattern::kSlice;
+    Context::GSSmoother &g = c.gs;
+    // greedy node colouring in natural order (P2 nodes: 7 colours on the annulus meshes)
+    std::vector<int8_t> colour(n, -1);
+    int nc = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        uint64_t used = 0;
+        for (int32_t k = K.ptr[i]; k < K.ptr[i + 1]; ++k)
+            if (colour[K.col[k]] >= 0) used |= 1ull << colour[K.col[k]];
+        int ci = 0;
+        while (used >> ci & 1ull) ++ci;
+        if (ci >= 64) throw Error(STK_E_SHAPE, "node colouring needs more than 64 colours");
+        colour[i] = (int8_t)ci;
+        nc = std::max(nc, ci + 1);
+    }
+    // rows of each colour, longest first (homogeneous slices), then by node
+    std::vector<std::vector<int32_t>> rows(nc);
+    for (int64_t i = 0; i < n; ++i) rows[colour[i]].push_back((int32_t)i);
+    auto len = [&](int32_t r) { return K.ptr[r + 1] - K.ptr[r]; };
+    for (auto &v : rows)
+        std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) { return len(a) > len(b); });
+    g.ncolour = nc;
+    g.slice_off.assign(nc + 1, 0);
+    std::vector<int32_t> rowid;
+    for (int k = 0; k < nc; ++k) {
+        const int64_t ns = ((int64_t)rows[k].size() + C - 1) / C;
+        g.slice_off[k + 1] = g.slice_off[k] + ns;
+        for (int64_t q = 0; q < ns * C; ++q) rowid.push_back(q < (int64_t)rows[k].size() ? rows[k][q] : -1);
+    }
+    const int64_t nsl = g.slice_off[nc];
+    std::vector<int64_t> sptr(nsl + 1, 0);
+    std::vector<int32_t> slow(nsl, 0);
+    std::vector<uint8_t> nlow(nsl * C, 0);
+    // per row: block order (indices into the CSR row), lower-colour columns first
+    std::vector<int64_t> rstart(n + 1, 0);
+    std::vector<int32_t> order(K.col.size());
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t q = K.ptr[i];
+        int nl = 0;
+        for (int32_t k = K.ptr[i]; k < K.ptr[i + 1]; ++k)
+            if (colour[K.col[k]] < colour[i]) { order[q++] = k - K.ptr[i]; ++nl; }
+        for (int32_t k = K.ptr[i]; k < K.ptr[i + 1]; ++k)
+            if (colour[K.col[k]] >= colour[i]) order[q++] = k - K.ptr[i];
+        if (nl > 255) throw Error(STK_E_SHAPE, "row with more than 255 lower-colour blocks");
+        rstart[i] = nl;   // reused below as the row's nlow
+    }
+    for (int64_t s = 0; s < nsl; ++s) {
+        int64_t L = 0, lo = 0;
+        for (int64_t l = 0; l < C; ++l) {
+            const int32_t r = rowid[s * C + l];
+            if (r < 0) continue;
+            L = std::max<int64_t>(L, len(r));
+            lo = std::max<int64_t>(lo, rstart[r]);
+            nlow[s * C + l] = (uint8_t)rstart[r];
+        }
+        sptr[s + 1] = sptr[s] + L * C;
+        slow[s] = (int32_t)lo;
+    }
+    g.nstore = sptr[nsl];
+    std::vector<int32_t> col(g.nstore), src(g.nstore);
+    for (int64_t s = 0; s < nsl; ++s) {
+        const int64_t L = (sptr[s + 1] - sptr[s]) / C;
+        for (int64_t l = 0; l < C; ++l) {
+            const int32_t r = rowid[s * C + l];
+            for (int64_t k = 0; k < L; ++k) {
+                const int64_t q = sptr[s] + k * C + l;
+                if (r >= 0 && k < len(r)) {
+                    const int32_t j = order[K.ptr[r] + k];
+                    col[q] = K.col[K.ptr[r] + j];
+                    src[q] = (int32_t)K.pos(r, j);
+                } else {   // padding: zero block on a valid column
+                    col[q] = r >= 0 ? K.col[K.ptr[r + 1] - 1] : 0;
+                    src[q] = -1;
+                }
+            }
+        }
+    }
+    std::vector<int64_t> diagpos(n);
+    for (int64_t i = 0; i < n; ++i) diagpos[i] = K.pos(i, find_offset(K, (int32_t)i, (int32_t)i));
+    for (int32_t x : col)
+        if (x < 0 || x >= K.ncols) throw Error(STK_E_STATE, "coloured pattern column out of range");
+    g.sptr.upload(sptr, c.stream);
+    g.col.upload(col, c.stream);
+    g.src.upload(src, c.stream);
+    g.rowid.upload(rowid, c.stream);
+    g.slow.upload(slow, c.stream);
+    g.nlow.upload(nlow, c.stream);
+    g.diagpos.upload(diagpos, c.stream);
+    STK_HIP(hipStreamSynchronize(c.stream));
+}
+
 void build_slot_maps(Context &c) {
     const int64_t nel = c.nel;
     std::vector<uint8_t> sK(36 * nel), sG(18 * nel), sGT(18 * nel), sM(9 * nel);

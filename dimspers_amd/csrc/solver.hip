@@ -208,6 +208,7 @@ struct MGView {
     const T *coarse_inv = nullptr;
     int64_t coarse_n = 0;
     bool half0 = false;       // level 0 smooths with the FP16 scaled copy of K (FP32 only)
+    bool gs0 = false;         // level 0 smooths with multicolour block Gauss-Seidel (FP32 only)
 };
 
 template <typename T>
@@ -250,6 +251,7 @@ MGView<T> make_view(Context &c) {
     v.coarse_inv = (const T *)pick<T>(c.coarse_inv64, c.coarse_inv32).p;
     v.coarse_n = c.coarse_n;
     v.half0 = !f64 && c.ksm.on && v.lv.size() > 1;
+    v.gs0 = !f64 && !v.half0 && c.gs.on && c.gs.filled && c.fine_smoother == STK_SMOOTH_GS && v.lv.size() > 1;
     return v;
 }
 
@@ -377,6 +379,72 @@ float *vcycle0_half(Context &c, MGView<float> &v, const float *b, int deg) {
     return xu;
 }
 
+// One multicolour block Gauss-Seidel sweep on K (FP32): forward from x = 0 (colours in
+// order, lower-colour blocks only), or backward from the given x (colours in reverse
+// order, all blocks).  With dot_w the backward sweep accumulates (b . x, dot_w . x) of the
+// final x, colour by colour, into block partials; *dot_nb receives their count.
+void gs_sweep(Context &c, const float *b, float *x, bool forward_zero, const float *dot_w, int64_t *dot_nb) {
+    const Context::GSSmoother &g = c.gs;
+    constexpr int NTG = 256;
+    const int64_t spb = NTG / Pattern::kSlice;
+    auto nblk = [&](int k) { return (g.slice_off[k + 1] - g.slice_off[k] + spb - 1) / spb; };
+    if (forward_zero) {
+        for (int k = 0; k < g.ncolour; ++k) {
+            const int64_t nb = nblk(k);
+            if (nb == 0) continue;
+            k_gs_colour<NTG, true, 0><<<dim3(nb), dim3(NTG), 0, c.stream>>>(
+                g.slice_off[k], g.slice_off[k + 1], g.sptr.p, g.col.p, g.val.p, g.rowid.p, g.slow.p, g.nlow.p,
+                g.dinv.p, b, x, nullptr, nullptr, 0);
+            STK_HIP(hipGetLastError());
+        }
+        return;
+    }
+    int64_t ntot = 0;
+    for (int k = 0; k < g.ncolour; ++k) ntot += nblk(k);
+    if (dot_w && (int64_t)c.red64.n < 2 * ntot) throw Error(STK_E_STATE, "reduction buffer too small");
+    int64_t off = 0;
+    for (int k = g.ncolour - 1; k >= 0; --k) {
+        const int64_t nb = nblk(k);
+        if (nb == 0) continue;
+        if (dot_w)
+            k_gs_colour<NTG, false, 2><<<dim3(nb), dim3(NTG), 0, c.stream>>>(
+                g.slice_off[k], g.slice_off[k + 1], g.sptr.p, g.col.p, g.val.p, g.rowid.p, g.slow.p, g.nlow.p,
+                g.dinv.p, b, x, dot_w, c.red64.p + off, ntot);
+        else
+            k_gs_colour<NTG, false, 0><<<dim3(nb), dim3(NTG), 0, c.stream>>>(
+                g.slice_off[k], g.slice_off[k + 1], g.sptr.p, g.col.p, g.val.p, g.rowid.p, g.slow.p, g.nlow.p,
+                g.dinv.p, b, x, nullptr, nullptr, 0);
+        STK_HIP(hipGetLastError());
+        off += nb;
+    }
+    if (dot_w) *dot_nb = ntot;
+}
+
+// Level 0 of an FP32 V-cycle with the Gauss-Seidel smoother: forward sweep from zero,
+// coarse correction, backward sweep (the pair is symmetric, as CG needs).
+float *vcycle0_gs(Context &c, MGView<float> &v, const float *b, int deg, const float *dot_w, int64_t *dot_nb) {
+    auto &L = v.lv[0];
+    float *x = L.x;
+    gs_sweep(c, b, x, true, nullptr, nullptr);
+    auto &C = v.lv[1];
+    if (L.RK.pat) {
+        EpStore<float> es{C.b};
+        apply22_auto<float>(c, L.R, b, es);
+        EpAxpyOut<float> ea{C.b, -1.0f, 1.0f};
+        apply22_auto<float>(c, L.RK, x, ea);
+    } else {
+        EpResidual<float> er{L.r, b};
+        apply22_auto<float>(c, L.A, x, er);
+        EpStore<float> es{C.b};
+        apply22_auto<float>(c, L.R, L.r, es);
+    }
+    float *xc = vcycle<float>(c, v, 1, C.b, deg, nullptr, nullptr);
+    EpAxpyOut<float> ea{x, 1.0f, 1.0f};
+    apply22_auto<float>(c, L.P, xc, ea);
+    gs_sweep(c, b, x, false, dot_w, dot_nb);
+    return x;
+}
+
 // x_l = V(b_l); result pointer returned (lives in lv[l].x or lv[l].xa).
 // With dot_w, the level-0 result x also gets its dots (b . x, dot_w . x) accumulated
 // into block partials by the last smoothing step; *dot_nb = their count (0: not done).
@@ -385,6 +453,7 @@ T *vcycle(Context &c, MGView<T> &v, size_t l, const T *b, int deg, const T *dot_
     if (dot_nb) *dot_nb = 0;
     if constexpr (sizeof(T) == 4) {
         if (l == 0 && v.half0) return vcycle0_half(c, v, b, deg);
+        if (l == 0 && v.gs0) return vcycle0_gs(c, v, b, deg, dot_w, dot_nb);
     }
     auto &L = v.lv[l];
     if (l + 1 == v.lv.size()) {
@@ -785,6 +854,39 @@ void setup_solver_data(Context &c) {
     kd.alloc(c.nU);
     const T *K = (const T *)pick<T>(c.K64, c.K32).p;
     diag_inverse<T>(c, c.KN, K, kd.p);
+    if constexpr (sizeof(T) == 4) {
+        // Gauss-Seidel smoother data: K's values in the coloured layout, inverse diagonal
+        // blocks (from the FP64 K when there is one)
+        Context::GSSmoother &g = c.gs;
+        g.filled = false;
+        if (g.on && !c.mg.empty()) {
+            g.val.alloc(4 * g.nstore);
+            g.dinv.alloc(4 * c.nnod);
+            const float *K32 = c.K32.p;
+            const double *K64 = c.K64.p;
+            const int32_t *src = g.src.p;
+            float *val = g.val.p;
+            launch_map(c, g.nstore, [=] __device__(int64_t q) {
+                const int64_t s = src[q];
+                for (int j = 0; j < 4; ++j) val[4 * q + j] = s >= 0 ? K32[4 * s + j] : 0.0f;
+            });
+            const int64_t *dp = g.diagpos.p;
+            float *di = g.dinv.p;
+            launch_map(c, c.nnod, [=] __device__(int64_t i) {
+                const int64_t p = dp[i];
+                const double a = K64 ? K64[4 * p] : (double)K32[4 * p];
+                const double bb = K64 ? K64[4 * p + 1] : (double)K32[4 * p + 1];
+                const double cc = K64 ? K64[4 * p + 2] : (double)K32[4 * p + 2];
+                const double d = K64 ? K64[4 * p + 3] : (double)K32[4 * p + 3];
+                const double det = a * d - bb * cc;
+                di[4 * i + 0] = (float)(d / det);
+                di[4 * i + 1] = (float)(-bb / det);
+                di[4 * i + 2] = (float)(-cc / det);
+                di[4 * i + 3] = (float)(a / det);
+            });
+            g.filled = true;
+        }
+    }
     // rigid rotation in the node frames: u_theta = r, u_r = 0 (zero on fixed dofs), unit norm
     DBuf<T> &rot = pick<T>(c.rot64, c.rot32);
     rot.alloc(c.nU);
@@ -1580,6 +1682,23 @@ double time_kernel(Context &c, int kernel, int reps, double *bytes) {
     DBuf<double> x64, y64, b64, d64, o64;
     DBuf<float> x32, y32, b32, d32, o32;
     switch (kernel) {
+        case STK_KERNEL_KP32: {
+            STK_CHECK(c.K32 && c.rot32, STK_E_STATE, "FP32 operators not assembled");
+            x32.alloc(nU); y32.alloc(nU);
+            float *x = x32.p;
+            launch_map(c, nU, [=] __device__(int64_t i) { x[i] = 1.0f + 1e-3f * (float)(i % 17); });
+            Op22<float> K{&c.KN, c.K32.p};
+            float *y = y32.p;
+            const float *rot = c.rot32.p;
+            // as in the inner CG: Ap = K p with (p . Ap, v . Ap) block partials; p, Ap and v
+            // once each besides the blocks (the partials are 16 B per 256 rows)
+            launch = [&c, K, x, y, rot]() {
+                EpStoreDot2<float> ed{y, x, rot};
+                apply22<8>(c, K, x, ed, c.red64.p);
+            };
+            *bytes = spmv_bytes(4) + 1.0 * nU * 4;
+            break;
+        }
         case STK_KERNEL_SPMV_K32: case STK_KERNEL_CHEB_K32: {
             STK_CHECK(c.K32 && c.Kdinv32, STK_E_STATE, "FP32 operators not assembled");
             x32.alloc(nU); y32.alloc(nU); b32.alloc(nU); d32.alloc(nU); o32.alloc(nU);

@@ -191,6 +191,25 @@ struct Context {
         DBuf<uint16_t> A;
         DBuf<float> s, sinv, bh, xh, xha, xu;
     } ksm;
+    // Multicolour block Gauss-Seidel smoother of the fine level (FP32): K's node rows grouped
+    // by colour (no two nodes of a colour share an element), each colour in slices of 64
+    // rows of about equal length, its blocks ordered lower-colour columns first.  A forward
+    // sweep from zero reads only those (nlow per row, slow per slice); a full sweep reads
+    // every block.  dinv: inverse 2x2 diagonal block per node.
+    struct GSSmoother {
+        bool on = false;
+        int ncolour = 0;
+        std::vector<int64_t> slice_off;        // first slice of each colour (ncolour + 1)
+        DBuf<int64_t> sptr;                    // slice block offsets (nslices + 1)
+        DBuf<int32_t> col, src;                // per stored block: node column, position in K
+        DBuf<int32_t> rowid, slow;             // per coloured row: node (-1: padding); per slice
+        DBuf<uint8_t> nlow;                    // per coloured row
+        DBuf<int64_t> diagpos;                 // per node: position of its diagonal block in K
+        DBuf<float> val, dinv;                 // 4 per stored block; 4 per node
+        int64_t nstore = 0;
+        bool filled = false;                   // values and dinv of the current FP32 K
+    } gs;
+    int fine_smoother = 0;                     // STK_SMOOTH_* of the last stk_solve
     DBuf<double> rot64; DBuf<float> rot32;     // unit near-null vector of K (discrete rigid rotation)
     double null_lambda = 0;                    // its Rayleigh quotient v^T K v
     DBuf<int64_t> d_fixlist;
@@ -245,6 +264,7 @@ constexpr int64_t kSellMinRows = 16384;
 constexpr int64_t kDenseMaxVerts = 2300;
 void build_colouring(Context &c);
 void build_slot_maps(Context &c);
+void build_gs_layout(Context &c);
 void detect_structured(Context &c);
 void build_mg_structure(Context &c);
 

@@ -20,7 +20,7 @@ mutable struct Opts
     method::Int32; inner::Int32; stop::Int32; itmax::Int32; itmin::Int32; inner_itmax::Int32
     refine_max::Int32; cheb_degree::Int32; null_mode::Int32; pc_sweeps::Int32
     rtol::Float64; inner_rtol::Float64; mixed_inner_rtol::Float64; mixed_schur_rtol::Float64; stall_ratio::Float64
-    inner_warm::Int32; reserved::Int32
+    inner_warm::Int32; fine_smoother::Int32
 end
 Opts() = (o = Opts(zeros(Int32, 10)..., zeros(5)..., Int32(0), Int32(0));
           ccall((:stk_default_opts, LIB), Cvoid, (Ref{Opts},), o); o)

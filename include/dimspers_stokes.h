@@ -61,6 +61,13 @@ enum {
                       /* coarse residual operator, 2 nP x nU     */
 };
 
+/* Fine-level smoother of the FP32 multigrid V-cycle (stk_opts.fine_smoother). */
+enum {
+    STK_SMOOTH_GS = 0,      /* multicolour block Gauss-Seidel: forward sweep before the coarse */
+                            /* correction, backward sweep after (default)                     */
+    STK_SMOOTH_CHEB = 1     /* Chebyshev of degree cheb_degree with point Jacobi               */
+};
+
 /* Solver methods (stk_opts.method). */
 enum {
     STK_SOLVE_PLAIN = 0,    /* FP64 Schur PCG, lumped 1/eta-mass preconditioner (reference) */
@@ -112,7 +119,8 @@ typedef struct stk_opts {
                             /* target and the full residual falls by less than this */
     int32_t inner_warm;     /* 1: each inner solve of the Schur PCG starts from      */
                             /* beta * z of the previous one (q_k = d_k + beta q_k-1) */
-    int32_t reserved;       /* zero                                                */
+    int32_t fine_smoother;  /* STK_SMOOTH_*: smoother of the fine multigrid level  */
+                            /* in the FP32 inner solves                            */
 } stk_opts;
 
 typedef struct stk_report {
@@ -306,7 +314,9 @@ enum {
     STK_KERNEL_VCYCLE32 = 7,    /* one FP32 multigrid V-cycle (the inner preconditioner)  */
     STK_KERNEL_INNER_IT32 = 8,  /* one FP32 inner CG iteration (V-cycle, K p, updates)    */
     STK_KERNEL_STRESS = 9,      /* stress! on the stress fields (synthetic velocity)      */
-    STK_KERNEL_DEFGRAD = 10     /* deformation_gradient! RK4 step (synthetic velocity)    */
+    STK_KERNEL_DEFGRAD = 10,    /* deformation_gradient! RK4 step (synthetic velocity)    */
+    STK_KERNEL_KP32 = 11        /* the inner CG product Ap = K p, FP32, with (p . Ap,     */
+                                /* v . Ap) block partials                                 */
 };
 
 /* Measurement hook with no counterpart in the reference: runs one of the hot

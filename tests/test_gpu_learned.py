@@ -128,15 +128,17 @@ def test_poisoned_model_falls_back_to_the_plain_preconditioner():
 
 def test_solver_options_give_the_same_solution():
     """inner_warm (warm-started inner solves), pc_sweeps (more Jacobi sweeps on the learned
-    operator) and the FP16 smoother change the work, not the answer: all reach the full
-    residual and agree with the plain FP64 solve to 1e-7."""
+    operator) and the fine-level smoother (Gauss-Seidel or Chebyshev) change the work, not the
+    answer: all reach the full residual and agree with the plain FP64 solve to 1e-7."""
     inp = make_inputs(scaled_workload(3, 32, 256))
     ctx = _ctx(inp)
     ctx.mlp_train(_lib.default_train_opts(snapshots=200, per_snapshot=256, epochs=2))
     ctx.learned_setup()
     ctx.solve(_lib.default_opts(method=_lib.SOLVE_PLAIN, stop=_lib.STOP_FULL, rtol=1e-10, itmax=400))
     U0, P0 = ctx.solution()
-    for kw in (dict(inner_warm=1), dict(pc_sweeps=8), dict(inner_warm=1, pc_sweeps=1)):
+    its = {}
+    for kw in (dict(inner_warm=1), dict(pc_sweeps=8), dict(inner_warm=1, pc_sweeps=1),
+               dict(fine_smoother=_lib.SMOOTH_CHEB), dict(fine_smoother=_lib.SMOOTH_GS)):
         rep = ctx.solve(_lib.default_opts(method=_lib.SOLVE_LEARNED, stop=_lib.STOP_FULL, rtol=1e-9, **kw))
         assert rep.converged and rep.full_residual < 1e-9, kw
         U, P = ctx.solution()
@@ -144,4 +146,9 @@ def test_solver_options_give_the_same_solution():
         ep = np.linalg.norm((P - P.mean()) - (P0 - P0.mean())) / np.linalg.norm(P0 - P0.mean())
         print(f"[result] {kw}: outer {rep.outer_iterations} inner {rep.inner_iterations} errU {eu:.1e} errP {ep:.1e}")
         assert eu < 1e-7 and ep < 1e-7, kw
+        its[str(kw)] = rep.inner_iterations
+    # one forward and one backward Gauss-Seidel sweep smooth as well as Chebyshev(2) (three
+    # products): about as many inner iterations, not more (a wrong sweep would still converge)
+    gs, ch = its[str(dict(fine_smoother=_lib.SMOOTH_GS))], its[str(dict(fine_smoother=_lib.SMOOTH_CHEB))]
+    assert gs <= 1.15 * ch + 2, (gs, ch)
     ctx.close()
