@@ -411,13 +411,15 @@ __global__ __launch_bounds__(NT) void k_spmv_sell11(int64_t nrows, const int64_t
 }
 
 // One colour of a multicolour block Gauss-Seidel sweep on the coloured sliced layout:
-// x_n += D_n^-1 (b_n - (K x)_n) for the node rows n of slices [s0, s1).  No two nodes of a
+// x_n += D_n^-1 (b_n - (K x)_n) for the node rows n of slices [s0, s1), one lane per row
+// (row row0 + (s - s0) * 64 + lane, block k at sptr[s] + 64 k + lane).  No two nodes of a
 // colour are coupled, so the in-place update of x is race free.  ZERO: forward sweep from
 // x = 0, where only the row's lower-colour blocks (the first nlow) meet non-zero x and the
-// row's own x_n is not read.  ND = 2: also (b . x, w . x) of the updated rows into
+// row's own x_n is not read.  dinv holds the inverse diagonal blocks in coloured row order
+// (coalesced).  ND = 2: also (b . x, w . x) of the updated rows into
 // partial[d * pstride + block].
 template <int NT, bool ZERO, int ND>
-__global__ __launch_bounds__(NT) void k_gs_colour(int64_t s0, int64_t s1, const int64_t *__restrict__ sptr,
+__global__ __launch_bounds__(NT) void k_gs_colour(int64_t s0, int64_t s1, int64_t row0, const int64_t *__restrict__ sptr,
                                                    const int32_t *__restrict__ col, const float *__restrict__ val,
                                                    const int32_t *__restrict__ rowid, const int32_t *__restrict__ slow,
                                                    const uint8_t *__restrict__ nlow, const float *__restrict__ dinv,
@@ -426,15 +428,14 @@ __global__ __launch_bounds__(NT) void k_gs_colour(int64_t s0, int64_t s1, const 
                                                    int64_t pstride) {
     typedef typename Vec4<float>::type V4;
     typedef typename Vec2<float>::type V2;
-    constexpr int C = 64;
-    static_assert(C == kWave, "one slice per wavefront");
+    constexpr int C = kWave;
     const int64_t slice = s0 + wave_slice<NT>();
     const int lane = threadIdx.x % C;
     double dsum[ND > 0 ? ND : 1] = {};
     if (slice < s1) {
         const int64_t base = sptr[slice];
         const int len = ZERO ? slow[slice] : (int)((sptr[slice + 1] - base) / C);
-        const int64_t q = slice * C + lane;
+        const int64_t q = row0 + (slice - s0) * C + lane;
         const int32_t r = rowid[q];
         const int nl = ZERO ? (int)nlow[q] : len;
         const V4 *__restrict__ vp = reinterpret_cast<const V4 *>(val) + base + lane;
@@ -452,7 +453,7 @@ __global__ __launch_bounds__(NT) void k_gs_colour(int64_t s0, int64_t s1, const 
         }
         if (r >= 0) {
             const V2 bv = reinterpret_cast<const V2 *>(b)[r];
-            const V4 di = reinterpret_cast<const V4 *>(dinv)[r];
+            const V4 di = reinterpret_cast<const V4 *>(dinv)[q];
             const float r0 = bv.x - a0, r1 = bv.y - a1;
             V2 xn{di.x * r0 + di.y * r1, di.z * r0 + di.w * r1};
             if (!ZERO) {

@@ -175,18 +175,23 @@ void build_gs_layout(Context &c) {
         std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) { return len(a) > len(b); });
     g.ncolour = nc;
     g.slice_off.assign(nc + 1, 0);
+    g.row_off.assign(nc + 1, 0);
+    // one lane per row (two or four lanes per row, with shorter per-lane load chains, were
+    // measured no faster at 256 x 2048)
     std::vector<int32_t> rowid;
     for (int k = 0; k < nc; ++k) {
-        const int64_t ns = ((int64_t)rows[k].size() + C - 1) / C;
+        const int64_t S = C;
+        const int64_t ns = ((int64_t)rows[k].size() + S - 1) / S;
         g.slice_off[k + 1] = g.slice_off[k] + ns;
-        for (int64_t q = 0; q < ns * C; ++q) rowid.push_back(q < (int64_t)rows[k].size() ? rows[k][q] : -1);
+        g.row_off[k + 1] = g.row_off[k] + ns * S;
+        for (int64_t q = 0; q < ns * S; ++q) rowid.push_back(q < (int64_t)rows[k].size() ? rows[k][q] : -1);
     }
     const int64_t nsl = g.slice_off[nc];
     std::vector<int64_t> sptr(nsl + 1, 0);
     std::vector<int32_t> slow(nsl, 0);
-    std::vector<uint8_t> nlow(nsl * C, 0);
+    std::vector<uint8_t> nlow(rowid.size(), 0);
     // per row: block order (indices into the CSR row), lower-colour columns first
-    std::vector<int64_t> rstart(n + 1, 0);
+    std::vector<int64_t> rlow(n, 0);
     std::vector<int32_t> order(K.col.size());
     for (int64_t i = 0; i < n; ++i) {
         int64_t q = K.ptr[i];
@@ -196,35 +201,45 @@ void build_gs_layout(Context &c) {
         for (int32_t k = K.ptr[i]; k < K.ptr[i + 1]; ++k)
             if (colour[K.col[k]] >= colour[i]) order[q++] = k - K.ptr[i];
         if (nl > 255) throw Error(STK_E_SHAPE, "row with more than 255 lower-colour blocks");
-        rstart[i] = nl;   // reused below as the row's nlow
+        rlow[i] = nl;
     }
-    for (int64_t s = 0; s < nsl; ++s) {
-        int64_t L = 0, lo = 0;
-        for (int64_t l = 0; l < C; ++l) {
-            const int32_t r = rowid[s * C + l];
-            if (r < 0) continue;
-            L = std::max<int64_t>(L, len(r));
-            lo = std::max<int64_t>(lo, rstart[r]);
-            nlow[s * C + l] = (uint8_t)rstart[r];
+    // slice s of colour k holds rows row_off[k] + (s - slice_off[k]) * 64 + j, j < 64;
+    // block b of its row j at sptr[s] + b * 64 + j
+    for (int k = 0; k < nc; ++k) {
+        const int64_t S = C;
+        for (int64_t s = g.slice_off[k]; s < g.slice_off[k + 1]; ++s) {
+            const int64_t r0 = g.row_off[k] + (s - g.slice_off[k]) * S;
+            int64_t Lb = 0, lo = 0;
+            for (int64_t j = 0; j < S; ++j) {
+                const int32_t r = rowid[r0 + j];
+                if (r < 0) continue;
+                Lb = std::max<int64_t>(Lb, len(r));
+                lo = std::max<int64_t>(lo, rlow[r]);
+                nlow[r0 + j] = (uint8_t)rlow[r];
+            }
+            sptr[s + 1] = sptr[s] + Lb * S;
+            slow[s] = (int32_t)lo;
         }
-        sptr[s + 1] = sptr[s] + L * C;
-        slow[s] = (int32_t)lo;
     }
     g.nstore = sptr[nsl];
     std::vector<int32_t> col(g.nstore), src(g.nstore);
-    for (int64_t s = 0; s < nsl; ++s) {
-        const int64_t L = (sptr[s + 1] - sptr[s]) / C;
-        for (int64_t l = 0; l < C; ++l) {
-            const int32_t r = rowid[s * C + l];
-            for (int64_t k = 0; k < L; ++k) {
-                const int64_t q = sptr[s] + k * C + l;
-                if (r >= 0 && k < len(r)) {
-                    const int32_t j = order[K.ptr[r] + k];
-                    col[q] = K.col[K.ptr[r] + j];
-                    src[q] = (int32_t)K.pos(r, j);
-                } else {   // padding: zero block on a valid column
-                    col[q] = r >= 0 ? K.col[K.ptr[r + 1] - 1] : 0;
-                    src[q] = -1;
+    for (int k = 0; k < nc; ++k) {
+        const int64_t S = C;
+        for (int64_t s = g.slice_off[k]; s < g.slice_off[k + 1]; ++s) {
+            const int64_t r0 = g.row_off[k] + (s - g.slice_off[k]) * S;
+            const int64_t Lb = (sptr[s + 1] - sptr[s]) / S;
+            for (int64_t j = 0; j < S; ++j) {
+                const int32_t r = rowid[r0 + j];
+                for (int64_t b = 0; b < Lb; ++b) {
+                    const int64_t q = sptr[s] + b * S + j;
+                    if (r >= 0 && b < len(r)) {
+                        const int32_t jj = order[K.ptr[r] + b];
+                        col[q] = K.col[K.ptr[r] + jj];
+                        src[q] = (int32_t)K.pos(r, jj);
+                    } else {   // padding: zero block on a valid column
+                        col[q] = r >= 0 ? K.col[K.ptr[r + 1] - 1] : 0;
+                        src[q] = -1;
+                    }
                 }
             }
         }

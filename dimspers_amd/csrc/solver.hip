@@ -386,17 +386,23 @@ float *vcycle0_half(Context &c, MGView<float> &v, const float *b, int deg) {
 void gs_sweep(Context &c, const float *b, float *x, bool forward_zero, const float *dot_w, int64_t *dot_nb) {
     const Context::GSSmoother &g = c.gs;
     constexpr int NTG = 256;
-    const int64_t spb = NTG / Pattern::kSlice;
+    const int64_t spb = NTG / kWave;
     auto nblk = [&](int k) { return (g.slice_off[k + 1] - g.slice_off[k] + spb - 1) / spb; };
+    // one colour: forward from zero, or backward with or without the dots
+    auto launch = [&](int k, bool zero, double *part, int64_t pstride) {
+        const int64_t nb = nblk(k);
+        if (nb == 0) return;
+        const dim3 grid((unsigned)nb), block(NTG);
+        const int64_t s0 = g.slice_off[k], s1 = g.slice_off[k + 1], r0 = g.row_off[k];
+#define STK_GS_ARGS s0, s1, r0, g.sptr.p, g.col.p, g.val.p, g.rowid.p, g.slow.p, g.nlow.p, g.dinv.p, b, x, dot_w, part, pstride
+        if (zero) k_gs_colour<NTG, true, 0><<<grid, block, 0, c.stream>>>(STK_GS_ARGS);
+        else if (part) k_gs_colour<NTG, false, 2><<<grid, block, 0, c.stream>>>(STK_GS_ARGS);
+        else k_gs_colour<NTG, false, 0><<<grid, block, 0, c.stream>>>(STK_GS_ARGS);
+#undef STK_GS_ARGS
+        STK_HIP(hipGetLastError());
+    };
     if (forward_zero) {
-        for (int k = 0; k < g.ncolour; ++k) {
-            const int64_t nb = nblk(k);
-            if (nb == 0) continue;
-            k_gs_colour<NTG, true, 0><<<dim3(nb), dim3(NTG), 0, c.stream>>>(
-                g.slice_off[k], g.slice_off[k + 1], g.sptr.p, g.col.p, g.val.p, g.rowid.p, g.slow.p, g.nlow.p,
-                g.dinv.p, b, x, nullptr, nullptr, 0);
-            STK_HIP(hipGetLastError());
-        }
+        for (int k = 0; k < g.ncolour; ++k) launch(k, true, nullptr, 0);
         return;
     }
     int64_t ntot = 0;
@@ -404,18 +410,8 @@ void gs_sweep(Context &c, const float *b, float *x, bool forward_zero, const flo
     if (dot_w && (int64_t)c.red64.n < 2 * ntot) throw Error(STK_E_STATE, "reduction buffer too small");
     int64_t off = 0;
     for (int k = g.ncolour - 1; k >= 0; --k) {
-        const int64_t nb = nblk(k);
-        if (nb == 0) continue;
-        if (dot_w)
-            k_gs_colour<NTG, false, 2><<<dim3(nb), dim3(NTG), 0, c.stream>>>(
-                g.slice_off[k], g.slice_off[k + 1], g.sptr.p, g.col.p, g.val.p, g.rowid.p, g.slow.p, g.nlow.p,
-                g.dinv.p, b, x, dot_w, c.red64.p + off, ntot);
-        else
-            k_gs_colour<NTG, false, 0><<<dim3(nb), dim3(NTG), 0, c.stream>>>(
-                g.slice_off[k], g.slice_off[k + 1], g.sptr.p, g.col.p, g.val.p, g.rowid.p, g.slow.p, g.nlow.p,
-                g.dinv.p, b, x, nullptr, nullptr, 0);
-        STK_HIP(hipGetLastError());
-        off += nb;
+        launch(k, false, dot_w ? c.red64.p + off : nullptr, ntot);
+        off += nblk(k);
     }
     if (dot_w) *dot_nb = ntot;
 }
@@ -861,7 +857,8 @@ void setup_solver_data(Context &c) {
         g.filled = false;
         if (g.on && !c.mg.empty()) {
             g.val.alloc(4 * g.nstore);
-            g.dinv.alloc(4 * c.nnod);
+            const int64_t nrc = g.row_off.back();
+            g.dinv.alloc(4 * nrc);
             const float *K32 = c.K32.p;
             const double *K64 = c.K64.p;
             const int32_t *src = g.src.p;
@@ -871,18 +868,24 @@ void setup_solver_data(Context &c) {
                 for (int j = 0; j < 4; ++j) val[4 * q + j] = s >= 0 ? K32[4 * s + j] : 0.0f;
             });
             const int64_t *dp = g.diagpos.p;
+            const int32_t *rid = g.rowid.p;
             float *di = g.dinv.p;
-            launch_map(c, c.nnod, [=] __device__(int64_t i) {
+            launch_map(c, nrc, [=] __device__(int64_t q) {
+                const int32_t i = rid[q];
+                if (i < 0) {
+                    for (int j = 0; j < 4; ++j) di[4 * q + j] = 0.0f;
+                    return;
+                }
                 const int64_t p = dp[i];
                 const double a = K64 ? K64[4 * p] : (double)K32[4 * p];
                 const double bb = K64 ? K64[4 * p + 1] : (double)K32[4 * p + 1];
                 const double cc = K64 ? K64[4 * p + 2] : (double)K32[4 * p + 2];
                 const double d = K64 ? K64[4 * p + 3] : (double)K32[4 * p + 3];
                 const double det = a * d - bb * cc;
-                di[4 * i + 0] = (float)(d / det);
-                di[4 * i + 1] = (float)(-bb / det);
-                di[4 * i + 2] = (float)(-cc / det);
-                di[4 * i + 3] = (float)(a / det);
+                di[4 * q + 0] = (float)(d / det);
+                di[4 * q + 1] = (float)(-bb / det);
+                di[4 * q + 2] = (float)(-cc / det);
+                di[4 * q + 3] = (float)(a / det);
             });
             g.filled = true;
         }

@@ -193,19 +193,20 @@ struct Context {
     } ksm;
     // Multicolour block Gauss-Seidel smoother of the fine level (FP32): K's node rows grouped
     // by colour (no two nodes of a colour share an element), each colour in slices of 64
-    // rows of about equal length, its blocks ordered lower-colour columns first.  A forward
+    // rows of about equal length, blocks ordered lower-colour columns first.  A forward
     // sweep from zero reads only those (nlow per row, slow per slice); a full sweep reads
-    // every block.  dinv: inverse 2x2 diagonal block per node.
+    // every block.  dinv: inverse 2x2 diagonal block per coloured row.
     struct GSSmoother {
         bool on = false;
         int ncolour = 0;
         std::vector<int64_t> slice_off;        // first slice of each colour (ncolour + 1)
+        std::vector<int64_t> row_off;          // first coloured row of each colour (ncolour + 1)
         DBuf<int64_t> sptr;                    // slice block offsets (nslices + 1)
         DBuf<int32_t> col, src;                // per stored block: node column, position in K
         DBuf<int32_t> rowid, slow;             // per coloured row: node (-1: padding); per slice
         DBuf<uint8_t> nlow;                    // per coloured row
         DBuf<int64_t> diagpos;                 // per node: position of its diagonal block in K
-        DBuf<float> val, dinv;                 // 4 per stored block; 4 per node
+        DBuf<float> val, dinv;                 // 4 per stored block; 4 per coloured row
         int64_t nstore = 0;
         bool filled = false;                   // values and dinv of the current FP32 K
     } gs;
