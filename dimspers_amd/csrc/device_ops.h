@@ -411,20 +411,24 @@ __global__ __launch_bounds__(NT) void k_spmv_sell11(int64_t nrows, const int64_t
 }
 
 // One colour of a multicolour block Gauss-Seidel sweep on the coloured sliced layout:
-// x_n += D_n^-1 (b_n - (K x)_n) for the node rows n of slices [s0, s1), one lane per row
-// (row row0 + (s - s0) * 64 + lane, block k at sptr[s] + 64 k + lane).  No two nodes of a
-// colour are coupled, so the in-place update of x is race free.  ZERO: forward sweep from
-// x = 0, where only the row's lower-colour blocks (the first nlow) meet non-zero x and the
-// row's own x_n is not read.  dinv holds the inverse diagonal blocks in coloured row order
-// (coalesced).  ND = 2: also (b . x, w . x) of the updated rows into
-// partial[d * pstride + block].
+// x_q += D_q^-1 (b_q - (K x)_q) for the coloured rows q of slices [s0, s1), one lane per
+// row (q = row0 + (s - s0) * 64 + lane, block k at sptr[s] + 64 k + lane, columns in
+// coloured row order).  The iterate xq, right-hand side bq, near-null vector wq and the
+// inverse diagonal blocks are stored in that order, so every per-row access is coalesced;
+// rowid maps q to its node n (-1: padding row).  No two nodes of a colour are coupled, so
+// the in-place update is race free.
+// ZERO: forward sweep from x = 0: only the row's first nlow (lower-colour) blocks meet a
+// non-zero x; b is read in node order and copied to bq.
+// Otherwise: full sweep; the result is also written in node order to z.  ND = 2: also
+// (b . x, w . x) of the updated rows into partial[d * pstride + block].
 template <int NT, bool ZERO, int ND>
 __global__ __launch_bounds__(NT) void k_gs_colour(int64_t s0, int64_t s1, int64_t row0, const int64_t *__restrict__ sptr,
                                                    const int32_t *__restrict__ col, const float *__restrict__ val,
                                                    const int32_t *__restrict__ rowid, const int32_t *__restrict__ slow,
                                                    const uint8_t *__restrict__ nlow, const float *__restrict__ dinv,
-                                                   const float *__restrict__ b, float *__restrict__ x,
-                                                   const float *__restrict__ w, double *__restrict__ partial,
+                                                   const float *__restrict__ bn, float *__restrict__ bq,
+                                                   float *__restrict__ xq, float *__restrict__ z,
+                                                   const float *__restrict__ wq, double *__restrict__ partial,
                                                    int64_t pstride) {
     typedef typename Vec4<float>::type V4;
     typedef typename Vec2<float>::type V2;
@@ -440,7 +444,7 @@ __global__ __launch_bounds__(NT) void k_gs_colour(int64_t s0, int64_t s1, int64_
         const int nl = ZERO ? (int)nlow[q] : len;
         const V4 *__restrict__ vp = reinterpret_cast<const V4 *>(val) + base + lane;
         const int32_t *__restrict__ cp = col + base + lane;
-        V2 *__restrict__ x2 = reinterpret_cast<V2 *>(x);
+        V2 *__restrict__ x2 = reinterpret_cast<V2 *>(xq);
         float a0 = 0.0f, a1 = 0.0f;
 #pragma unroll 4
         for (int k = 0; k < len; ++k) {
@@ -452,18 +456,25 @@ __global__ __launch_bounds__(NT) void k_gs_colour(int64_t s0, int64_t s1, int64_
             }
         }
         if (r >= 0) {
-            const V2 bv = reinterpret_cast<const V2 *>(b)[r];
+            V2 bv;
+            if (ZERO) {
+                bv = reinterpret_cast<const V2 *>(bn)[r];
+                reinterpret_cast<V2 *>(bq)[q] = bv;
+            } else {
+                bv = reinterpret_cast<const V2 *>(bq)[q];
+            }
             const V4 di = reinterpret_cast<const V4 *>(dinv)[q];
             const float r0 = bv.x - a0, r1 = bv.y - a1;
             V2 xn{di.x * r0 + di.y * r1, di.z * r0 + di.w * r1};
             if (!ZERO) {
-                const V2 xo = x2[r];
+                const V2 xo = x2[q];
                 xn.x += xo.x;
                 xn.y += xo.y;
+                reinterpret_cast<V2 *>(z)[r] = xn;
             }
-            x2[r] = xn;
+            x2[q] = xn;
             if constexpr (ND == 2) {
-                const V2 wv = reinterpret_cast<const V2 *>(w)[r];
+                const V2 wv = reinterpret_cast<const V2 *>(wq)[q];
                 dsum[0] = (double)bv.x * (double)xn.x + (double)bv.y * (double)xn.y;
                 dsum[1] = (double)wv.x * (double)xn.x + (double)wv.y * (double)xn.y;
             }

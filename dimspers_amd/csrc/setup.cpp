@@ -248,6 +248,63 @@ void build_gs_layout(Context &c) {
     for (int64_t i = 0; i < n; ++i) diagpos[i] = K.pos(i, find_offset(K, (int32_t)i, (int32_t)i));
     for (int32_t x : col)
         if (x < 0 || x >= K.ncols) throw Error(STK_E_STATE, "coloured pattern column out of range");
+    // the smoother's iterate lives in coloured row order (q): columns of the coloured K, of
+    // R K and the rows of the prolongation are renumbered to q
+    const int64_t nq = g.row_off[nc];
+    std::vector<int32_t> qpos(n, -1);
+    for (int64_t q = 0; q < nq; ++q)
+        if (rowid[q] >= 0) qpos[rowid[q]] = (int32_t)q;
+    for (int64_t i = 0; i < n; ++i)
+        if (qpos[i] < 0) throw Error(STK_E_STATE, "node missing from the colouring");
+    for (int32_t &x : col) x = qpos[x];
+    {
+        const Pattern &GT = c.GTN;
+        if (!GT.sell) throw Error(STK_E_STATE, "G^T pattern is not sliced");
+        Pattern &Q = g.rkq;
+        Q.nrows = GT.nrows; Q.ncols = nq; Q.sell = true;
+        Q.sptr = GT.sptr;
+        Q.scol.resize(GT.scol.size());
+        for (size_t k = 0; k < GT.scol.size(); ++k) Q.scol[k] = qpos[GT.scol[k]];
+        Q.dsptr.upload(Q.sptr, c.stream);
+        Q.dscol.upload(Q.scol, c.stream);
+    }
+    {
+        // prolongation rows in q order: row q is P's row of node rowid[q] (empty for padding)
+        const Pattern &P = c.mg[0].P_pat;
+        Pattern &Q = g.pq;
+        Q.nrows = nq; Q.ncols = P.ncols; Q.sell = true;
+        const int64_t ns = nq / C;
+        Q.sptr.assign(ns + 1, 0);
+        auto plen = [&](int32_t r) { return r < 0 ? 0 : P.ptr[r + 1] - P.ptr[r]; };
+        for (int64_t s = 0; s < ns; ++s) {
+            int64_t Lb = 0;
+            for (int64_t j = 0; j < C; ++j) Lb = std::max<int64_t>(Lb, plen(rowid[s * C + j]));
+            Q.sptr[s + 1] = Q.sptr[s] + Lb * C;
+        }
+        Q.scol.assign(Q.sptr[ns], 0);
+        std::vector<int32_t> psrc(Q.sptr[ns], -1);
+        for (int64_t s = 0; s < ns; ++s) {
+            const int64_t Lb = (Q.sptr[s + 1] - Q.sptr[s]) / C;
+            for (int64_t j = 0; j < C; ++j) {
+                const int32_t r = rowid[s * C + j];
+                const int64_t lr = plen(r);
+                for (int64_t b = 0; b < Lb; ++b) {
+                    const int64_t qq = Q.sptr[s] + b * C + j;
+                    if (b < lr) {
+                        Q.scol[qq] = P.col[P.ptr[r] + b];
+                        psrc[qq] = (int32_t)P.pos(r, b);
+                    } else {
+                        Q.scol[qq] = lr > 0 ? P.col[P.ptr[r + 1] - 1] : 0;
+                    }
+                }
+            }
+        }
+        for (int32_t x : Q.scol)
+            if (x < 0 || x >= Q.ncols) throw Error(STK_E_STATE, "coloured prolongation column out of range");
+        Q.dsptr.upload(Q.sptr, c.stream);
+        Q.dscol.upload(Q.scol, c.stream);
+        g.pq_src.upload(psrc, c.stream);
+    }
     g.sptr.upload(sptr, c.stream);
     g.col.upload(col, c.stream);
     g.src.upload(src, c.stream);

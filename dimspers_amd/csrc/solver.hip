@@ -251,7 +251,8 @@ MGView<T> make_view(Context &c) {
     v.coarse_inv = (const T *)pick<T>(c.coarse_inv64, c.coarse_inv32).p;
     v.coarse_n = c.coarse_n;
     v.half0 = !f64 && c.ksm.on && v.lv.size() > 1;
-    v.gs0 = !f64 && !v.half0 && c.gs.on && c.gs.filled && c.fine_smoother == STK_SMOOTH_GS && v.lv.size() > 1;
+    v.gs0 = !f64 && !v.half0 && c.gs.on && c.gs.filled && c.fine_smoother == STK_SMOOTH_GS && v.lv.size() > 1 &&
+            v.lv[0].RK.pat != nullptr;
     return v;
 }
 
@@ -379,22 +380,24 @@ float *vcycle0_half(Context &c, MGView<float> &v, const float *b, int deg) {
     return xu;
 }
 
-// One multicolour block Gauss-Seidel sweep on K (FP32): forward from x = 0 (colours in
-// order, lower-colour blocks only), or backward from the given x (colours in reverse
-// order, all blocks).  With dot_w the backward sweep accumulates (b . x, dot_w . x) of the
-// final x, colour by colour, into block partials; *dot_nb receives their count.
-void gs_sweep(Context &c, const float *b, float *x, bool forward_zero, const float *dot_w, int64_t *dot_nb) {
-    const Context::GSSmoother &g = c.gs;
+// One multicolour block Gauss-Seidel sweep on K (FP32) with the iterate in coloured row
+// order (c.gs.xq): forward from x = 0 (colours in order, lower-colour blocks only; b in
+// node order, copied to c.gs.bq), or backward (colours in reverse order, all blocks; the
+// result also written in node order to z).  With dots the backward sweep accumulates
+// (b . x, v . x), v the near-null vector, colour by colour into block partials; *dot_nb
+// receives their count.
+void gs_sweep(Context &c, const float *b, float *z, bool forward_zero, bool dots, int64_t *dot_nb) {
+    Context::GSSmoother &g = c.gs;
     constexpr int NTG = 256;
     const int64_t spb = NTG / kWave;
     auto nblk = [&](int k) { return (g.slice_off[k + 1] - g.slice_off[k] + spb - 1) / spb; };
-    // one colour: forward from zero, or backward with or without the dots
     auto launch = [&](int k, bool zero, double *part, int64_t pstride) {
         const int64_t nb = nblk(k);
         if (nb == 0) return;
         const dim3 grid((unsigned)nb), block(NTG);
         const int64_t s0 = g.slice_off[k], s1 = g.slice_off[k + 1], r0 = g.row_off[k];
-#define STK_GS_ARGS s0, s1, r0, g.sptr.p, g.col.p, g.val.p, g.rowid.p, g.slow.p, g.nlow.p, g.dinv.p, b, x, dot_w, part, pstride
+#define STK_GS_ARGS s0, s1, r0, g.sptr.p, g.col.p, g.val.p, g.rowid.p, g.slow.p, g.nlow.p, g.dinv.p, b, g.bq.p, \
+                    g.xq.p, z, g.wq.p, part, pstride
         if (zero) k_gs_colour<NTG, true, 0><<<grid, block, 0, c.stream>>>(STK_GS_ARGS);
         else if (part) k_gs_colour<NTG, false, 2><<<grid, block, 0, c.stream>>>(STK_GS_ARGS);
         else k_gs_colour<NTG, false, 0><<<grid, block, 0, c.stream>>>(STK_GS_ARGS);
@@ -407,38 +410,35 @@ void gs_sweep(Context &c, const float *b, float *x, bool forward_zero, const flo
     }
     int64_t ntot = 0;
     for (int k = 0; k < g.ncolour; ++k) ntot += nblk(k);
-    if (dot_w && (int64_t)c.red64.n < 2 * ntot) throw Error(STK_E_STATE, "reduction buffer too small");
+    if (dots && (int64_t)c.red64.n < 2 * ntot) throw Error(STK_E_STATE, "reduction buffer too small");
     int64_t off = 0;
     for (int k = g.ncolour - 1; k >= 0; --k) {
-        launch(k, false, dot_w ? c.red64.p + off : nullptr, ntot);
+        launch(k, false, dots ? c.red64.p + off : nullptr, ntot);
         off += nblk(k);
     }
-    if (dot_w) *dot_nb = ntot;
+    if (dots) *dot_nb = ntot;
 }
 
 // Level 0 of an FP32 V-cycle with the Gauss-Seidel smoother: forward sweep from zero,
-// coarse correction, backward sweep (the pair is symmetric, as CG needs).
+// coarse right-hand side R b - (R K) x, coarse correction x += P x_c, backward sweep (the
+// pair is symmetric, as CG needs).  x stays in coloured row order throughout; R K and P
+// address it through their renumbered columns / rows.  Returns x in node order (lv[0].x).
 float *vcycle0_gs(Context &c, MGView<float> &v, const float *b, int deg, const float *dot_w, int64_t *dot_nb) {
     auto &L = v.lv[0];
-    float *x = L.x;
-    gs_sweep(c, b, x, true, nullptr, nullptr);
+    Context::GSSmoother &g = c.gs;
+    if (dot_w && dot_w != c.rot32.p) throw Error(STK_E_STATE, "V-cycle dots are taken with the near-null vector");
+    float *z = L.x;
+    gs_sweep(c, b, nullptr, true, false, nullptr);
     auto &C = v.lv[1];
-    if (L.RK.pat) {
-        EpStore<float> es{C.b};
-        apply22_auto<float>(c, L.R, b, es);
-        EpAxpyOut<float> ea{C.b, -1.0f, 1.0f};
-        apply22_auto<float>(c, L.RK, x, ea);
-    } else {
-        EpResidual<float> er{L.r, b};
-        apply22_auto<float>(c, L.A, x, er);
-        EpStore<float> es{C.b};
-        apply22_auto<float>(c, L.R, L.r, es);
-    }
+    EpStore<float> es{C.b};
+    apply22_auto<float>(c, L.R, b, es);
+    EpAxpyOut<float> er{C.b, -1.0f, 1.0f};
+    apply22_auto<float>(c, Op22<float>{&g.rkq, L.RK.val}, g.xq.p, er);
     float *xc = vcycle<float>(c, v, 1, C.b, deg, nullptr, nullptr);
-    EpAxpyOut<float> ea{x, 1.0f, 1.0f};
-    apply22_auto<float>(c, L.P, xc, ea);
-    gs_sweep(c, b, x, false, dot_w, dot_nb);
-    return x;
+    EpAxpyOut<float> ea{g.xq.p, 1.0f, 1.0f};
+    apply22_auto<float>(c, Op22<float>{&g.pq, g.pq_val.p}, xc, ea);
+    gs_sweep(c, nullptr, z, false, dot_w != nullptr, dot_nb);
+    return z;
 }
 
 // x_l = V(b_l); result pointer returned (lives in lv[l].x or lv[l].xa).
@@ -867,6 +867,27 @@ void setup_solver_data(Context &c) {
                 const int64_t s = src[q];
                 for (int j = 0; j < 4; ++j) val[4 * q + j] = s >= 0 ? K32[4 * s + j] : 0.0f;
             });
+            // prolongation values in coloured row order, near-null vector in that order,
+            // zeroed work vectors (padding rows stay zero)
+            g.pq_val.alloc(4 * g.pq.nstore());
+            {
+                const float *P32 = c.mg[0].P32.p;
+                const int32_t *ps = g.pq_src.p;
+                float *pv = g.pq_val.p;
+                launch_map(c, g.pq.nstore(), [=] __device__(int64_t q) {
+                    const int64_t s = ps[q];
+                    for (int j = 0; j < 4; ++j) pv[4 * q + j] = s >= 0 ? P32[4 * s + j] : 0.0f;
+                });
+            }
+            {
+                const int64_t nq = g.row_off.back();
+                g.xq.alloc(2 * nq); g.bq.alloc(2 * nq); g.wq.alloc(2 * nq);
+                float *xq = g.xq.p, *bq = g.bq.p;
+                launch_map(c, 2 * nq, [=] __device__(int64_t i) {
+                    xq[i] = 0.0f;
+                    bq[i] = 0.0f;
+                });
+            }
             const int64_t *dp = g.diagpos.p;
             const int32_t *rid = g.rowid.p;
             float *di = g.dinv.p;
@@ -887,7 +908,7 @@ void setup_solver_data(Context &c) {
                 di[4 * q + 2] = (float)(-cc / det);
                 di[4 * q + 3] = (float)(a / det);
             });
-            g.filled = true;
+            // g.filled (the smoother in use) once the near-null vector is final, below
         }
     }
     // rigid rotation in the node frames: u_theta = r, u_r = 0 (zero on fixed dofs), unit norm
@@ -985,6 +1006,21 @@ void setup_solver_data(Context &c) {
                          (int)conv, inv, c.null_lambda, f64 ? c.Klmax64 : c.Klmax32);
     }
     pt.mark("near-null vector");
+    if constexpr (sizeof(T) == 4) {
+        // the Gauss-Seidel smoother's copy of the near-null vector in coloured row order
+        Context::GSSmoother &g = c.gs;
+        if (g.on && !c.mg.empty()) {
+            const int64_t nq = g.row_off.back();
+            const int32_t *rid = g.rowid.p;
+            const float *rp = c.rot32.p;
+            float *wq = g.wq.p;
+            launch_map(c, 2 * nq, [=] __device__(int64_t i) {
+                const int32_t n = rid[i / 2];
+                wq[i] = n >= 0 ? rp[2 * (int64_t)n + i % 2] : 0.0f;
+            });
+            g.filled = true;
+        }
+    }
     STK_HIP(hipStreamSynchronize(c.stream));
 }
 

@@ -195,7 +195,8 @@ struct Context {
     // by colour (no two nodes of a colour share an element), each colour in slices of 64
     // rows of about equal length, blocks ordered lower-colour columns first.  A forward
     // sweep from zero reads only those (nlow per row, slow per slice); a full sweep reads
-    // every block.  dinv: inverse 2x2 diagonal block per coloured row.
+    // every block.  dinv: inverse 2x2 diagonal block per coloured row.  col holds coloured
+    // row positions q: the smoother's iterate is kept in coloured row order.
     struct GSSmoother {
         bool on = false;
         int ncolour = 0;
@@ -208,6 +209,12 @@ struct Context {
         DBuf<int64_t> diagpos;                 // per node: position of its diagonal block in K
         DBuf<float> val, dinv;                 // 4 per stored block; 4 per coloured row
         int64_t nstore = 0;
+        // iterate and right-hand side in coloured row order (2 per coloured row), the
+        // near-null vector in that order, R K with q columns, P with q rows (+ its values)
+        DBuf<float> xq, bq, wq;
+        Pattern rkq, pq;
+        DBuf<int32_t> pq_src;
+        DBuf<float> pq_val;
         bool filled = false;                   // values and dinv of the current FP32 K
     } gs;
     int fine_smoother = 0;                     // STK_SMOOTH_* of the last stk_solve
