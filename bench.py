@@ -91,7 +91,8 @@ def main():
 
     if args.kernels or args.diagnostics:
         res = {}
-        kernels = ((("cheb_k32", _lib.KERNEL_CHEB_K32, 50), ("copy_256MiB", _lib.KERNEL_COPY, 20)) if args.kernels else
+        kernels = ((("gs_backward_sweep32_per_launch", _lib.KERNEL_GS_BWD32, 50), ("kp32", _lib.KERNEL_KP32, 50),
+                    ("copy_256MiB", _lib.KERNEL_COPY, 20)) if args.kernels else
                    (("spmv_k32", _lib.KERNEL_SPMV_K32, 50), ("vcycle32", _lib.KERNEL_VCYCLE32, 20),
                     ("inner_cg_iteration32", _lib.KERNEL_INNER_IT32, 5)))
         for name, kid, reps in kernels:
@@ -180,10 +181,14 @@ def main():
     _, trep = ctx.thermal_solve(inp.T, np.ones(nn), 1.0, 1.0, 1e-4, tfix, tval)
     log(f"thermal step (assembly + BCs + Jacobi-CG to 1.5e-8): {trep.solve_ms:.2f} ms, {trep.outer_iterations} its")
 
-    kms, kbytes = ctx.time_kernel(_lib.KERNEL_CHEB_K32, reps=50)
+    # dominant kernel of the solve by total time: the backward Gauss-Seidel colour kernel
+    # (seven launches per V-cycle, timed over whole sweeps, reported per launch)
+    kms, kbytes = ctx.time_kernel(_lib.KERNEL_GS_BWD32, reps=50)
     achieved = kbytes / (kms * 1e-3) / 1e9
+    pms, pbytes = ctx.time_kernel(_lib.KERNEL_KP32, reps=50)
     sms, sbytes = ctx.time_kernel(_lib.KERNEL_SPMV_K32, reps=50)
-    log(f"dominant kernel (fused Chebyshev SpMV, FP32): {kms * 1e3:.1f} us, {achieved:.0f} GB/s; "
+    log(f"dominant kernel (backward Gauss-Seidel colour launch, FP32): {kms * 1e3:.1f} us, {achieved:.0f} GB/s; "
+        f"CG product K p {pms * 1e3:.1f} us, {pbytes / pms / 1e6:.0f} GB/s; "
         f"plain FP32 SpMV {sms * 1e3:.1f} us, {sbytes / sms / 1e6:.0f} GB/s")
 
     cpu = None
@@ -197,7 +202,7 @@ def main():
                          f"(a lower bound: the LU factorisation grows faster than linearly)"}
 
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_cheb_spmv_k32_256x2048.json")
+    tfile = os.path.join(ROOT, "profiles", "traffic_gs_backward_k32_256x2048.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")
@@ -237,8 +242,12 @@ def main():
         "speedup_vs_plain": round(plain["ms"] / ms, 2) if plain else None,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "kernel": "k_spmv_sell22<256,float,float,EpCheb<float>,XPlain<float>> (fused Chebyshev step on K, sliced layout)",
+                     "kernel": "k_gs_colour<256,false,2> (backward multicolour block Gauss-Seidel sweep on K, FP32; "
+                               "seven colour launches per sweep, figures per launch)",
                      "avg_us": round(kms * 1e3, 2), "bytes_per_launch": kbytes},
+        "cg_product_kp": {"kernel": "k_spmv_sell22<256,float,float,EpStoreDot2<float>,XPlain<float>>",
+                          "avg_us": round(pms * 1e3, 2), "bytes_per_launch": pbytes,
+                          "GB_per_s": round(pbytes / pms / 1e6, 1)},
         "cpu_baseline": cpu,
         "thermal_step": {"ms": round(trep.solve_ms, 2), "cg_iterations": trep.outer_iterations,
                          "dt": 1e-4, "relative_residual": trep.schur_residual},

@@ -1787,6 +1787,30 @@ double time_kernel(Context &c, int kernel, int reps, double *bytes) {
             *bytes = 0.0;
             break;
         }
+        case STK_KERNEL_GS_BWD32: {
+            // the backward Gauss-Seidel sweep of the V-cycle with its dots, per colour launch
+            STK_CHECK(c.gs.on && c.gs.filled, STK_E_STATE, "Gauss-Seidel smoother not set up");
+            x32.alloc(nU);
+            float *z = x32.p, *xq = c.gs.xq.p, *bq = c.gs.bq.p;
+            const int32_t *rid = c.gs.rowid.p;
+            const int64_t nq = c.gs.row_off.back();
+            launch_map(c, 2 * nq, [=] __device__(int64_t i) {
+                const bool real = rid[i / 2] >= 0;
+                xq[i] = real ? 1.0f + 1e-3f * (float)(i % 17) : 0.0f;
+                bq[i] = real ? 1.0f : 0.0f;
+            });
+            launch = [&c, z]() {
+                int64_t nb = 0;
+                gs_sweep(c, nullptr, z, false, true, &nb);
+            };
+            // structural blocks (four values and a column each), per real row: b, x read,
+            // x written twice (coloured and node order), D^-1 (4 values), w, row id; slice
+            // pointers; all over the colour launches
+            const int64_t nrow = c.nnod, nsl = c.gs.slice_off.back();
+            *bytes = ((double)c.KN.nnz() * 20 + (double)nrow * (8 + 8 + 16 + 16 + 8 + 4) + (double)(nsl + 1) * 8) /
+                     c.gs.ncolour;
+            break;
+        }
         case STK_KERNEL_INNER_IT32: {
             // a fixed number of inner CG iterations (tolerance unreachable), per iteration
             STK_CHECK(c.K32 && c.have_mg32, STK_E_STATE, "FP32 multigrid not set up");
@@ -1874,7 +1898,9 @@ double time_kernel(Context &c, int kernel, int reps, double *bytes) {
     STK_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    return kernel == STK_KERNEL_INNER_IT32 ? ms / reps / 20 : ms / reps;
+    if (kernel == STK_KERNEL_INNER_IT32) return ms / reps / 20;
+    if (kernel == STK_KERNEL_GS_BWD32) return ms / reps / c.gs.ncolour;
+    return ms / reps;
 }
 
 }  // namespace stk

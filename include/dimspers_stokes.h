@@ -315,8 +315,10 @@ enum {
     STK_KERNEL_INNER_IT32 = 8,  /* one FP32 inner CG iteration (V-cycle, K p, updates)    */
     STK_KERNEL_STRESS = 9,      /* stress! on the stress fields (synthetic velocity)      */
     STK_KERNEL_DEFGRAD = 10,    /* deformation_gradient! RK4 step (synthetic velocity)    */
-    STK_KERNEL_KP32 = 11        /* the inner CG product Ap = K p, FP32, with (p . Ap,     */
+    STK_KERNEL_KP32 = 11,       /* the inner CG product Ap = K p, FP32, with (p . Ap,     */
                                 /* v . Ap) block partials                                 */
+    STK_KERNEL_GS_BWD32 = 12    /* backward Gauss-Seidel sweep of the V-cycle with its    */
+                                /* dots; time and bytes per colour launch                 */
 };
 
 /* Measurement hook with no counterpart in the reference: runs one of the hot
