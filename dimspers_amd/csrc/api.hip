@@ -258,7 +258,7 @@ int stk_assemble(stk_ctx *ctx, int32_t flags) {
         // stale products of an earlier assembly
         c.K32.release(); c.G32.release(); c.GT32.release(); c.M32.release(); c.F32.release(); c.pc32.release();
         c.RK32.release();
-        c.S64.release(); c.S32.release(); c.Sdinv64.release(); c.Sdinv32.release();
+        c.S64.release(); c.S32.release(); c.S32s.release(); c.Sdinv64.release(); c.Sdinv32.release();
         if (fp64) {
             assemble_prec<double>(c, flags, mg, c.K64, c.G64, c.GT64, c.M64, c.F64, c.RK64, true);
             c.pc64.alloc(c.nP);

@@ -116,6 +116,12 @@ struct EpAxpyOut {                     // y = beta*y + alpha*A x  (alpha, beta s
     __device__ double operator()(int64_t i, T ax) const { y[i] = beta * y[i] + alpha * ax; return 0.0; }
 };
 template <typename T>
+struct EpJacobi {                      // damped Jacobi: dout = din + w Dinv (r - A din)
+    static constexpr int kDots = 1;
+    const T *r, *dinv, *din; T *dout; T w;
+    __device__ double operator()(int64_t i, T ax) const { dout[i] = din[i] + w * dinv[i] * (r[i] - ax); return 0.0; }
+};
+template <typename T>
 struct EpStorePlus {                   // y = A x + a
     static constexpr int kDots = 1;
     T *y; const T *a;

@@ -16,6 +16,10 @@ enum : int {
     SC_VZ = 17,      // v . z of the inner solution
     SC_PAP2 = 20,    // (p . K p, v . K p) of the inner CG (20, 21)
     SC_RZV = 22,     // (r . z, v . z) of the inner CG, ping-pong pairs (22, 23) and (24, 25)
+    SC_SRD = 26,     // r . d of the current Schur iteration
+    SC_SQSQ = 27,    // q . S q
+    SC_SRN = 28,     // ||r||^2 after the update; then (29, 30): r . d and (r - r_last) . d of the
+                     // next direction, fetched together (29 is the next iteration's r . d)
     SC_COUNT = 32
 };
 

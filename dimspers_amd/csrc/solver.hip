@@ -1080,13 +1080,30 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
     launch_map(c, nP, [=] __device__(int64_t i) { w.dp[i] = T(0); });
     const double r0 = std::sqrt(vdot<T>(c, w.r, w.r, nP));
     const double tol = std::max(rtol * r0, abs_tol);
+    // learned: damped Jacobi sweeps d <- d + w Dinv (r - S d) on the assembled learned Schur
+    // operator, from d = Dinv r; on the sliced vertex layout each sweep is one product with
+    // the update in its epilogue, alternating between dout and w.Sq so the last lands in dout
+    const bool sliced_s = learned && !f64 && c.S32s && !c.mg.empty() && c.mg[0].pat.sell;
     auto apply_pc = [&](const T *rin, T *dout) {
         if (learned) {
-            // damped Jacobi sweeps on the assembled learned Schur operator
-            const T *S = (const T *)pick<T>(c.S64, c.S32).p;
             const T *sd = pcvec;   // inverse diagonal of S
-            T *tmp = w.Sq;
             const int nsweep = o.pc_sweeps > 0 ? o.pc_sweeps : 3;
+            if constexpr (sizeof(T) == 4) {
+                if (sliced_s) {
+                    T *buf[2] = {dout, w.Sq};
+                    int cur = nsweep % 2;   // the last sweep writes buf[0] = dout
+                    T *d0 = buf[cur];
+                    launch_map(c, nP, [=] __device__(int64_t i) { d0[i] = sd[i] * rin[i]; });
+                    for (int s = 0; s < nsweep; ++s) {
+                        EpJacobi<T> ej{rin, sd, buf[cur], buf[1 - cur], T(0.7)};
+                        spmv_sell11<T>(c, c.mg[0].pat, c.S32s.p, buf[cur], ej, nullptr);
+                        cur = 1 - cur;
+                    }
+                    return;
+                }
+            }
+            const T *S = (const T *)pick<T>(c.S64, c.S32).p;
+            T *tmp = w.Sq;
             launch_map(c, nP, [=] __device__(int64_t i) { dout[i] = sd[i] * rin[i]; });
             for (int s = 0; s < nsweep; ++s) {
                 EpStore<T> es{tmp};
@@ -1105,15 +1122,21 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
         *schur_res = 0.0;
         return 0;
     }
+    // The scalars of the iteration stay on the device (slots of d_sc): alpha = (r . d) /
+    // (q . S q) is read by the update kernels; the host fetches once per iteration the
+    // residual norm and the next (r . d) and beta numerator.
+    Reducer red(c);
+    double *sc = c.d_sc.p;
     apply_pc(w.r, w.d);
     launch_map(c, nP, [=] __device__(int64_t i) { w.q[i] = w.d[i]; });
+    red.map_reduce(nP, [=] __device__(int64_t i) { return (double)w.r[i] * (double)w.d[i]; }, SC_SRD);
+    double rd = red.fetch(SC_SRD);
     int it = 0;
     double beta_prev = 0.0;
     const bool warm = o.inner_warm != 0 && !deflate;
     if (stalled) *stalled = false;
     for (it = 1; it <= itmax; ++it) {
         vmean_remove<T>(c, w.q, nP);
-        const double rd = vdot<T>(c, w.r, w.d, nP);
         if (stalled && !(rd > 0)) {   // the learned preconditioner is not positive definite here
             *stalled = true;
             return it - 1;
@@ -1131,21 +1154,31 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
         *inner_its += inner_cg<T>(c, v, use_mg, w.y, w.z, inner_rtol, o.inner_itmax, deg, &ic, deflate, start);
         EpStore<T> es{w.Sq};
         apply_rc<1, 2, T>(c, c.GTN, GT, w.z, es, nullptr);
-        const double qSq = vdot<T>(c, w.q, w.Sq, nP);
-        const double alpha = rd / qSq;
-        if (!std::isfinite(alpha)) throw Error(STK_E_NAN, "non-finite step in the Schur PCG");
-        const T al = (T)alpha;
+        red.map_reduce(nP, [=] __device__(int64_t i) { return (double)w.q[i] * (double)w.Sq[i]; }, SC_SQSQ);
+        // r . d of this direction: from before the loop, then from the previous iteration's fetch
+        const int rds = it == 1 ? SC_SRD : SC_SRN + 1;
         launch_map(c, nP, [=] __device__(int64_t i) {
+            const T al = (T)(sc[rds] / sc[SC_SQSQ]);
             w.rl[i] = w.r[i];
             w.dp[i] += al * w.q[i];
             w.r[i] -= al * w.Sq[i];
         });
-        launch_map(c, nU, [=] __device__(int64_t i) { w.du[i] += al * w.z[i]; });
+        launch_map(c, nU, [=] __device__(int64_t i) { w.du[i] += (T)(sc[rds] / sc[SC_SQSQ]) * w.z[i]; });
         vmean_remove<T>(c, w.r, nP);
-        const double rn = std::sqrt(vdot<T>(c, w.r, w.r, nP));
+        red.map_reduce(nP, [=] __device__(int64_t i) { return (double)w.r[i] * (double)w.r[i]; }, SC_SRN);
+        // the next direction's preconditioned residual and its dots (unused after the last
+        // iteration: one preconditioner application more than needed)
+        apply_pc(w.r, w.d);
+        red.map_reduce<2>(nP, [=] __device__(int64_t i) {
+            const double d = (double)w.d[i];
+            return Dots<2>{{(double)w.r[i] * d, ((double)w.r[i] - (double)w.rl[i]) * d}};
+        }, SC_SRN + 1);
+        double h[3];
+        red.fetch_many(SC_SRN, 3, h);
+        const double rn = std::sqrt(h[0]);
         *schur_res = rn / r0;
         if (trace_on())
-            std::fprintf(stderr, "[stk] schur it %d  |r|/|r0| %.3e  alpha %.3e  inner its %lld\n", it, rn / r0, alpha,
+            std::fprintf(stderr, "[stk] schur it %d  |r|/|r0| %.3e  inner its %lld\n", it, rn / r0,
                          (long long)*inner_its);
         if (!std::isfinite(rn)) throw Error(STK_E_NAN, "non-finite residual in the Schur PCG");
         if (rn < tol && it >= itmin) { *conv = true; break; }
@@ -1154,12 +1187,8 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
             *stalled = true;
             break;
         }
-        apply_pc(w.r, w.d);
-        Reducer red(c);
-        red.map_reduce(nP, [=] __device__(int64_t i) {
-            return ((double)w.r[i] - (double)w.rl[i]) * (double)w.d[i];
-        }, SC_TMP);
-        const double beta = red.fetch(SC_TMP) / rd;
+        const double beta = h[2] / rd;
+        rd = h[1];
         beta_prev = beta;
         const T be = (T)beta;
         launch_map(c, nP, [=] __device__(int64_t i) { w.q[i] = w.d[i] + be * w.q[i]; });
@@ -1441,6 +1470,22 @@ void finish_learned_operator(Context &c) {
     convert<double, float>(c, c.S64.p, c.S32.p, nnz);
     csr_diag_inverse<double>(c, c.MN, c.S64.p, c.Sdinv64.p);
     convert<double, float>(c, c.Sdinv64.p, c.Sdinv32.p, c.nP);
+    // the same values in the sliced layout of the vertex pattern (the multigrid's first level
+    // has M's pattern), for the sliced product in the preconditioner's sweeps
+    c.S32s.release();
+    if (!c.mg.empty() && c.mg[0].pat.sell) {
+        const Pattern &P = c.mg[0].pat;
+        c.S32s.alloc(P.nstore());
+        c.S32s.zero(c.stream);
+        const int32_t *ptr = c.MN.dptr.p;
+        const int64_t *sptr = P.dsptr.p;
+        const float *S = c.S32.p;
+        float *Ss = c.S32s.p;
+        launch_map(c, c.nP, [=] __device__(int64_t r) {
+            const int64_t base = sptr[r / Pattern::kSlice] + r % Pattern::kSlice;
+            for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) Ss[base + (int64_t)(k - ptr[r]) * Pattern::kSlice] = S[k];
+        });
+    }
 }
 
 // y = A x for an assembled operator, host FP64 vectors.

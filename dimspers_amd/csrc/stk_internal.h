@@ -182,6 +182,7 @@ struct Context {
     // R (b - K x) as R b - (R K) x, a product with a quarter of K's rows
     DBuf<double> RK64; DBuf<float> RK32;
     DBuf<double> S64; DBuf<float> S32;        // learned Schur sum (pattern MN)
+    DBuf<float> S32s;                          // its FP32 values in the sliced layout of mg[0].pat
     DBuf<double> Sdinv64; DBuf<float> Sdinv32;
     DBuf<double> Kdinv64; DBuf<float> Kdinv32; // inverse diagonal of K (per dof)
     // FP16 copy of the scaled K^ = S^-1 K S^-1 (S^2 = diag K, layout of K) for the
