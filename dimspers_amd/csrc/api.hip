@@ -67,7 +67,7 @@ void stk_default_opts(stk_opts *o) {
     o->cheb_degree = 2;
     o->rtol = 1e-8;
     o->inner_rtol = 1e-11;
-    o->mixed_inner_rtol = 0.1;
+    o->mixed_inner_rtol = 0.2;
     o->mixed_schur_rtol = 0.1;
     o->stall_ratio = 0.5;
     o->null_mode = STK_NULL_PROJECT;
