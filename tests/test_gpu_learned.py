@@ -152,3 +152,43 @@ def test_solver_options_give_the_same_solution():
     gs, ch = its[str(dict(fine_smoother=_lib.SMOOTH_GS))], its[str(dict(fine_smoother=_lib.SMOOTH_CHEB))]
     assert gs <= 1.15 * ch + 2, (gs, ch)
     ctx.close()
+
+
+def test_reassembly_gives_the_fresh_context_result():
+    """A time step re-assembles on the same mesh with new fields: every per-assembly product
+    (FP32 copies, R K, the coloured smoother values, the coarse inverse, the near-null vector,
+    the learned operator) is rebuilt, so a re-assembled context solves bit for bit like a fresh
+    one assembled with the new fields."""
+    inp = make_inputs(scaled_workload(3, 32, 256))
+    rng = np.random.default_rng(21)
+    eta_b = np.asfortranarray(inp.eta_ip * np.exp(rng.normal(scale=0.5, size=inp.eta_ip.shape)))
+    params = None
+    results = []
+    for fresh in (False, True):
+        ctx = _lib.Context.from_mesh(inp.mesh)
+        if not fresh:
+            ctx.set_fields(inp.g, inp.rho, inp.eta_ip, inp.C6)
+            ctx.set_fabric(inp.phi, inp.delta)
+            ctx.assemble(_lib.ASM_FP64 | _lib.ASM_FP32)
+            ctx.mlp_train(_lib.default_train_opts(snapshots=200, per_snapshot=256, epochs=2))
+            params = ctx.mlp_params()
+            ctx.learned_setup()
+            ctx.solve(_lib.default_opts(method=_lib.SOLVE_LEARNED, stop=_lib.STOP_FULL, rtol=1e-8))
+        else:
+            ctx.set_fabric(inp.phi, inp.delta)
+            ctx.mlp_params(*params)
+        ctx.set_fields(inp.g, inp.rho, eta_b, inp.C6)
+        ctx.assemble(_lib.ASM_FP64 | _lib.ASM_FP32)
+        ctx.learned_setup()
+        out = []
+        for m in (_lib.SOLVE_LEARNED, _lib.SOLVE_MIXED):
+            rep = ctx.solve(_lib.default_opts(method=m, stop=_lib.STOP_FULL, rtol=1e-8))
+            assert rep.converged and rep.full_residual < 1e-8
+            U, P = ctx.solution()
+            out.append((rep.outer_iterations, rep.inner_iterations, U, P))
+        results.append(out)
+        ctx.close()
+    for (o1, i1, U1, P1), (o2, i2, U2, P2) in zip(*results):
+        print(f"[result] re-assembled {o1}/{i1}, fresh {o2}/{i2}")
+        assert (o1, i1) == (o2, i2)
+        assert np.array_equal(U1, U2) and np.array_equal(P1, P2)
