@@ -857,7 +857,7 @@ void setup_solver_data(Context &c) {
         // blocks (from the FP64 K when there is one)
         Context::GSSmoother &g = c.gs;
         g.filled = false;
-        if (g.on && !c.mg.empty()) {
+        if (g.on && c.mg_built) {
             g.val.alloc(4 * g.nstore);
             const int64_t nrc = g.row_off.back();
             g.dinv.alloc(4 * nrc);
@@ -927,7 +927,9 @@ void setup_solver_data(Context &c) {
             rp[i] = (i % 2 == 0 && !fx[i]) ? (T)(nr[i / 2] * inv) : T(0);
         });
     }
-    if (!c.mg.empty()) {
+    // the hierarchy's operators exist only when this assembly built them (not with
+    // STK_ASM_NO_MG, though the mesh's multigrid structure is there)
+    if (c.mg_built) {
         for (MGLevel &L : c.mg) {
             alloc_bufs(level_bufs<T>(L), 2 * L.nvert);
             DBuf<T> &di = pick<T>(L.dinv64, L.dinv32);
@@ -1011,7 +1013,7 @@ void setup_solver_data(Context &c) {
     if constexpr (sizeof(T) == 4) {
         // the Gauss-Seidel smoother's copy of the near-null vector in coloured row order
         Context::GSSmoother &g = c.gs;
-        if (g.on && !c.mg.empty()) {
+        if (g.on && c.mg_built) {
             const int64_t nq = g.row_off.back();
             const int32_t *rid = g.rowid.p;
             const float *rp = c.rot32.p;

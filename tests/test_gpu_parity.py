@@ -187,6 +187,33 @@ def test_solve_matches_oracle_projected(mesh, method):
     ctx.close()
 
 
+def test_inner_solvers_without_multigrid_match_oracle():
+    """The plain method with the Jacobi-preconditioned inner CG (STK_INNER_JACOBI), both when the
+    multigrid is not built (STK_ASM_NO_MG) and when it is built but not used, gives the oracle's
+    projected solution; a multigrid solve asked of an assembly without the hierarchy is refused
+    (STK_E_NOMESH).  (The FP32 methods are built around the multigrid inner solve: with Jacobi
+    inner solves at their loose inner tolerance their sweeps stall.)"""
+    inp = make_inputs(scaled_workload(3, 4, 16))
+    Uo, Po, _, _ = _oracle_projected(inp)
+    method, m = "plain", _lib.SOLVE_PLAIN
+    for flags in (_lib.ASM_FP64 | _lib.ASM_FP32 | _lib.ASM_NO_MG, _lib.ASM_FP64 | _lib.ASM_FP32):
+        ctx = _ctx(inp, flags)
+        inner = _lib.INNER_JACOBI
+        if flags & _lib.ASM_NO_MG:
+            with pytest.raises(_lib.StkError) as e:
+                ctx.solve(_lib.default_opts(method=m, inner=_lib.INNER_MG, stop=_lib.STOP_FULL, rtol=1e-10))
+            assert e.value.code == _lib.STK_E_NOMESH
+        rep = ctx.solve(_lib.default_opts(method=m, inner=inner, stop=_lib.STOP_FULL, rtol=1e-10, itmax=400,
+                                          inner_itmax=20000))
+        assert rep.converged and rep.full_residual < 1e-10
+        U, P = ctx.solution()
+        eu, ep = _errs(U, P, Uo, Po)
+        print(f"[result] {method} flags {flags} inner {inner}: outer {rep.outer_iterations} "
+              f"inner {rep.inner_iterations} errU {eu:.2e} errP {ep:.2e}")
+        assert eu < 1e-7 and ep < 1e-7
+        ctx.close()
+
+
 def test_fp16_smoother_matches_fp32_smoother():
     """The FP16 scaled copy of K only changes the fine-level smoother: the mixed solve
     reaches the same full residual, the same solution (1e-7 of the plain FP64 one) and
