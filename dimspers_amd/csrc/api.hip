@@ -419,6 +419,10 @@ int stk_solve(stk_ctx *ctx, const stk_opts *opts, stk_report *report) {
         STK_CHECK(o.inner == STK_INNER_MG || o.inner == STK_INNER_JACOBI, STK_E_ARG, "bad inner solver");
         STK_CHECK(o.itmax > 0 && o.inner_itmax > 0 && o.rtol > 0 && o.cheb_degree >= 1, STK_E_ARG, "bad options");
         STK_CHECK(o.fine_smoother == STK_SMOOTH_GS || o.fine_smoother == STK_SMOOTH_CHEB, STK_E_ARG, "bad smoother");
+        // the FP32 methods stop their inner solves at a loose tolerance that only the multigrid
+        // preconditioner turns into progress; with point-Jacobi inner solves their sweeps stall
+        STK_CHECK(o.method == STK_SOLVE_PLAIN || o.inner == STK_INNER_MG, STK_E_ARG,
+                  "the mixed and learned methods need the multigrid inner solver");
         c.fine_smoother = o.fine_smoother;
         if (o.method == STK_SOLVE_PLAIN) STK_CHECK(c.K64, STK_E_STATE, "plain solve needs FP64 operators");
         const int rc = run_solve(c, o, rep);

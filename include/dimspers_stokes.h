@@ -78,7 +78,9 @@ enum {
 /* Inner velocity solver (stk_opts.inner). */
 enum {
     STK_INNER_MG = 0,       /* CG + geometric multigrid V-cycle   */
-    STK_INNER_JACOBI = 1    /* CG + point Jacobi                  */
+    STK_INNER_JACOBI = 1    /* CG + point Jacobi (plain method only: */
+                            /* the mixed and learned methods need */
+                            /* the multigrid, STK_E_ARG otherwise)  */
 };
 
 /* Treatment of the discrete rigid rotation (stk_opts.null_mode).  With free slip

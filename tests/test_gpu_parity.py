@@ -191,8 +191,8 @@ def test_inner_solvers_without_multigrid_match_oracle():
     """The plain method with the Jacobi-preconditioned inner CG (STK_INNER_JACOBI), both when the
     multigrid is not built (STK_ASM_NO_MG) and when it is built but not used, gives the oracle's
     projected solution; a multigrid solve asked of an assembly without the hierarchy is refused
-    (STK_E_NOMESH).  (The FP32 methods are built around the multigrid inner solve: with Jacobi
-    inner solves at their loose inner tolerance their sweeps stall.)"""
+    (STK_E_NOMESH), and so are the FP32 methods with Jacobi inner solves (STK_E_ARG: at their
+    loose inner tolerance only the multigrid preconditioner makes progress)."""
     inp = make_inputs(scaled_workload(3, 4, 16))
     Uo, Po, _, _ = _oracle_projected(inp)
     method, m = "plain", _lib.SOLVE_PLAIN
@@ -203,6 +203,9 @@ def test_inner_solvers_without_multigrid_match_oracle():
             with pytest.raises(_lib.StkError) as e:
                 ctx.solve(_lib.default_opts(method=m, inner=_lib.INNER_MG, stop=_lib.STOP_FULL, rtol=1e-10))
             assert e.value.code == _lib.STK_E_NOMESH
+        with pytest.raises(_lib.StkError) as e:
+            ctx.solve(_lib.default_opts(method=_lib.SOLVE_MIXED, inner=inner, stop=_lib.STOP_FULL, rtol=1e-8))
+        assert e.value.code == _lib.STK_E_ARG
         rep = ctx.solve(_lib.default_opts(method=m, inner=inner, stop=_lib.STOP_FULL, rtol=1e-10, itmax=400,
                                           inner_itmax=20000))
         assert rep.converged and rep.full_residual < 1e-10
