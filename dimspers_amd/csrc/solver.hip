@@ -1329,6 +1329,7 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
     } else {
         // FP64 refinement around FP32 Schur PCG corrections
         if (!c.K32) throw Error(STK_E_STATE, "mixed solve needs the FP32 operators (STK_ASM_FP32)");
+        if (!c.K64) throw Error(STK_E_STATE, "mixed solve needs the FP64 operators of its refinement (STK_ASM_FP64)");
         const bool learned = (o.method == STK_SOLVE_LEARNED);
         if (learned && !c.S32) throw Error(STK_E_STATE, "learned solve needs the learned Schur operator");
         SchurWork<float> w = schur_work<float>(c);
