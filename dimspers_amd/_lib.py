@@ -29,12 +29,15 @@ EXPORTED = [
     "stk_updatevelocity", "stk_time_kernel", "stk_default_train_opts", "stk_set_fabric", "stk_mlp_train",
     "stk_mlp_params", "stk_mlp_forward", "stk_mlp_gradient", "stk_local_schur", "stk_learned_setup",
     "stk_set_stress_fields", "stk_stress", "stk_deformation_gradient", "stk_stats", "stk_thermal_solve",
+    "stk_fse", "stk_set_dem", "stk_anisotropic_tensor",
 ]
 N_PARAM = 9094
 KERNEL_SPMV_K32, KERNEL_SPMV_K64, KERNEL_CHEB_K32, KERNEL_CHEB_K64, KERNEL_ASSEMBLY, KERNEL_CHEB_KH16 = 0, 1, 2, 3, 4, 5
 KERNEL_COPY, KERNEL_VCYCLE32, KERNEL_INNER_IT32, KERNEL_STRESS, KERNEL_DEFGRAD, KERNEL_KP32 = 6, 7, 8, 9, 10, 11
 KERNEL_GS_BWD32 = 12
 STRESS_FIELDS = ("exx", "ezz", "exz", "eII", "txx", "tzz", "txz", "tII")
+FSE_NO_HEALING, FSE_FABRIC_DESTRUCTION, FSE_ANNEALING_DESTRUCTION = 0, 1, 2
+INSTALL_ASSEMBLY, INSTALL_STRESS = 1, 2
 
 
 class StkOpts(C.Structure):
@@ -132,6 +135,9 @@ def load(path: Path | str | None = None):
     lib.stk_stats.argtypes = [vp, dp, dp, dp]
     lib.stk_thermal_solve.argtypes = [vp, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, i64p, dp,
                                       C.c_int64, C.c_double, C.POINTER(StkReport)]
+    lib.stk_fse.argtypes = [vp, dp, C.c_int32, C.c_int32, C.c_double, dp]
+    lib.stk_set_dem.argtypes = [vp, dp, dp, C.c_int64, C.c_double]
+    lib.stk_anisotropic_tensor.argtypes = [vp, dp, C.c_int32, dp, dp, dp, C.c_int64, C.c_int32, C.POINTER(dp)]
     for name in EXPORTED:
         if name not in ("stk_default_opts", "stk_destroy", "stk_last_error", "stk_default_train_opts"):
             getattr(lib, name).restype = C.c_int
@@ -390,3 +396,56 @@ class Context:
         u = None if U is None else np.ascontiguousarray(U, dtype=np.float64)
         self._check(self.lib.stk_deformation_gradient(self.h, None if u is None else _d(u), float(dt), _d(buf)))
         return np.transpose(buf.reshape(self.s_nip, self.nel, 2, 2), (1, 0, 3, 2)).copy()
+
+    def fse(self, F, mode=FSE_FABRIC_DESTRUCTION, eps=1e3):
+        """getFSE on F (float64[nel, nip, 2, 2], nip 6 or 7).  Returns (ellipses float64[nel, nip, 6]
+        with the fields x1, x2, y1, y2, a1, a2, F after the healing rule)."""
+        F = np.asarray(F, dtype=np.float64)
+        if F.ndim != 4 or F.shape[0] != self.nel or F.shape[2:] != (2, 2) or F.shape[1] not in (6, 7):
+            raise ValueError("F must be nel x nip x 2 x 2 with nip 6 or 7")
+        nip = F.shape[1]
+        buf = np.ascontiguousarray(np.transpose(F, (1, 0, 3, 2))).reshape(-1)
+        out = np.zeros(6 * self.nel * nip)
+        self._check(self.lib.stk_fse(self.h, _d(buf), nip, int(mode), float(eps), _d(out)))
+        self.fse_nip = nip
+        ell = np.transpose(out.reshape(nip, self.nel, 6), (1, 0, 2)).copy()
+        return ell, np.transpose(buf.reshape(nip, self.nel, 2, 2), (1, 0, 3, 2)).copy()
+
+    def set_dem(self, tensor, axes, w):
+        """getDEM from the DEMloader arrays: tensor n x 9, axes n x 3; w the viscosity cut-off."""
+        t = np.asfortranarray(tensor, dtype=np.float64)
+        a = np.asfortranarray(axes, dtype=np.float64)
+        if t.ndim != 2 or t.shape[1] != 9 or a.shape != (t.shape[0], 3):
+            raise ValueError("tensor must be n x 9 and axes n x 3")
+        self._check(self.lib.stk_set_dem(self.h, _d(t), _d(a), t.shape[0], float(w)))
+
+    def anisotropic_tensor(self, fse=None, nip=None, shape_table=None, shape_points=None, install=0):
+        """anisotropic_tensor: the six nel x 7 arrays (eta11, eta33, eta55, eta13, eta15, eta35).
+
+        fse: float64[nel, nip, 6] or None for the device ellipses of the last ``fse`` call.
+        shape_table = (a1 nodes, r1, r2) or shape_points = (r1, r2), each float64[nel, nip]."""
+        e = None
+        if fse is not None:
+            fse = np.asarray(fse, dtype=np.float64)
+            if fse.ndim != 3 or fse.shape[0] != self.nel or fse.shape[2] != 6:
+                raise ValueError("fse must be nel x nip x 6")
+            nip = fse.shape[1]
+            e = np.ascontiguousarray(np.transpose(fse, (1, 0, 2))).reshape(-1)
+        elif nip is None:
+            nip = getattr(self, "fse_nip", 0)
+        keep = []
+        if shape_table is not None:
+            nodes, r1, r2 = (np.ascontiguousarray(v, dtype=np.float64) for v in shape_table)
+            keep += [nodes, r1, r2]
+            args = (_d(nodes), _d(r1), _d(r2), nodes.size)
+        elif shape_points is not None:
+            r1, r2 = (np.asfortranarray(v, dtype=np.float64).ravel(order="F") for v in shape_points)
+            keep += [r1, r2]
+            args = (None, _d(r1), _d(r2), r1.size)
+        else:
+            raise ValueError("give shape_table or shape_points")
+        outs = [np.zeros((self.nel, 7), order="F") for _ in range(6)]
+        arr = (dp * 6)(*[_d(o) for o in outs])
+        self._check(self.lib.stk_anisotropic_tensor(self.h, None if e is None else _d(e), int(nip), *args,
+                                                    int(install), arr))
+        return outs

@@ -7,6 +7,7 @@
 #include <rocblas/rocblas.h>
 
 #include "device_ops.h"
+#include "fabric.h"
 #include "mlp.h"
 #include "solver.h"
 #include "stress.h"
@@ -735,6 +736,107 @@ int stk_deformation_gradient(stk_ctx *ctx, const double *U, double dt, double *F
         f.upload(F, n, c.stream);
         launch_deformation_gradient(c, Ud, dt, f.p);
         f.download(F, n, c.stream);
+        STK_HIP(hipStreamSynchronize(c.stream));
+        return STK_OK;
+    });
+}
+
+int stk_fse(stk_ctx *ctx, double *F, int32_t nip, int32_t mode, double eps, double *fse) {
+    if (!ctx || !F) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK(nip == 6 || nip == 7, STK_E_SHAPE, "F needs 6 or 7 points per element");
+        STK_CHECK(mode >= STK_FSE_NO_HEALING && mode <= STK_FSE_ANNEALING_DESTRUCTION, STK_E_ARG,
+                  "unknown healing mode");
+        STK_CHECK(mode == STK_FSE_NO_HEALING || eps > 0, STK_E_ARG, "the destruction ratio eps must be positive");
+        const int64_t npt = c.nel * (int64_t)nip;
+        DBuf<double> f;
+        f.upload(F, 4 * npt, c.stream);
+        c.fse_nip = 0;
+        c.d_fse.alloc(6 * npt);
+        launch_fse(c, f.p, npt, mode, eps, c.d_fse.p);
+        if (mode != STK_FSE_NO_HEALING) f.download(F, 4 * npt, c.stream);
+        if (fse) c.d_fse.download(fse, 6 * npt, c.stream);
+        STK_HIP(hipStreamSynchronize(c.stream));
+        c.fse_nip = nip;
+        return STK_OK;
+    });
+}
+
+int stk_set_dem(stk_ctx *ctx, const double *tensor, const double *axes, int64_t n, double w) {
+    if (!ctx || !tensor || !axes) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        c.dem.set = false;
+        dem_prepare(c, tensor, axes, n, w);
+        return STK_OK;
+    });
+}
+
+int stk_anisotropic_tensor(stk_ctx *ctx, const double *fse, int32_t nip, const double *shape_a1,
+                           const double *shape_r1, const double *shape_r2, int64_t nshape, int32_t install,
+                           double *const out[6]) {
+    if (!ctx) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK(c.dem.set, STK_E_STATE, "no DEM table: call stk_set_dem first");
+        STK_CHECK(nip == 6 || nip == 7, STK_E_SHAPE, "the ellipses need 6 or 7 points per element");
+        STK_CHECK(shape_r1 && shape_r2, STK_E_ARG, "NULL fabric shape");
+        STK_CHECK((install & ~(STK_INSTALL_ASSEMBLY | STK_INSTALL_STRESS)) == 0, STK_E_ARG, "unknown install flags");
+        STK_CHECK(!(install & STK_INSTALL_ASSEMBLY) || c.have_fields, STK_E_STATE,
+                  "install into the assembly needs its fields (stk_set_fields) first");
+        STK_CHECK(!(install & STK_INSTALL_STRESS) || c.have_sfields, STK_E_STATE,
+                  "install into the stress fields needs stk_set_stress_fields first");
+        const int64_t npt = c.nel * (int64_t)nip;
+        DBuf<double> f, sn, s1, s2;
+        const double *dfse = c.d_fse.p;
+        if (fse) {
+            f.upload(fse, 6 * npt, c.stream);
+            dfse = f.p;
+        } else {
+            STK_CHECK(dfse && c.fse_nip == nip, STK_E_STATE,
+                      "no device ellipses with this nip: call stk_fse first or pass fse");
+        }
+        ShapeIn shape;
+        if (shape_a1) {
+            STK_CHECK(nshape >= 2, STK_E_SHAPE, "the fabric-shape table needs at least two nodes");
+            for (int64_t k = 0; k + 1 < nshape; ++k)
+                STK_CHECK(shape_a1[k] < shape_a1[k + 1], STK_E_ARG, "the fabric-shape nodes must increase strictly");
+            sn.upload(shape_a1, nshape, c.stream);
+            s1.upload(shape_r1, nshape, c.stream);
+            s2.upload(shape_r2, nshape, c.stream);
+            shape.nodes = sn.p;
+            shape.nshape = nshape;
+        } else {
+            STK_CHECK(nshape == npt, STK_E_SHAPE, "per-point fabric shapes need nel x nip values");
+            s1.upload(shape_r1, npt, c.stream);
+            s2.upload(shape_r2, npt, c.stream);
+        }
+        shape.r1 = s1.p;
+        shape.r2 = s2.p;
+        const int64_t plane7 = 7 * c.nel;
+        c.d_aniso.alloc(6 * plane7);
+        launch_anisotropic_tensor(c, dfse, nip, shape, c.d_aniso.p);
+        if (install & STK_INSTALL_ASSEMBLY) {
+            // assembly tensor: 6 components x (nel x 6), the first six columns of each plane
+            c.d_C.alloc(36 * c.nel);
+            for (int k = 0; k < 6; ++k)
+                STK_HIP(hipMemcpyAsync(c.d_C.p + k * 6 * c.nel, c.d_aniso.p + k * plane7, 6 * c.nel * sizeof(double),
+                                       hipMemcpyDeviceToDevice, c.stream));
+            c.have_C = true;
+        }
+        if (install & STK_INSTALL_STRESS) {
+            const int64_t plane = c.nel * (int64_t)c.s_nip;
+            c.d_sC.alloc(6 * plane);
+            for (int k = 0; k < 6; ++k)
+                STK_HIP(hipMemcpyAsync(c.d_sC.p + k * plane, c.d_aniso.p + k * plane7, plane * sizeof(double),
+                                       hipMemcpyDeviceToDevice, c.stream));
+            c.s_aniso = true;
+        }
+        if (out)
+            for (int k = 0; k < 6; ++k)
+                if (out[k]) STK_HIP(hipMemcpyAsync(out[k], c.d_aniso.p + k * plane7, plane7 * sizeof(double),
+                                                   hipMemcpyDeviceToHost, c.stream));
         STK_HIP(hipStreamSynchronize(c.stream));
         return STK_OK;
     });

@@ -246,6 +246,20 @@ struct Context {
     DBuf<int32_t> d_itop, d_ibot;
     double nu_dr = 0, nu_dth = 0, nu_rin = 0, nu_rout = 0;
 
+    // fabric: finite strain ellipses of the last stk_fse, the DEM table, the viscous tensor of
+    // the last stk_anisotropic_tensor
+    DBuf<double> d_fse;                      // 6 per point (x1, x2, y1, y2, a1, a2), nel x fse_nip points
+    int fse_nip = 0;
+    struct DemTable {
+        bool set = false;
+        int32_t sblk = 0, nblk = 0;          // entries per block of equal a1, number of blocks
+        double w = 0;                        // viscosity cut-off
+        DBuf<double> a1a2, a2a3;             // nblk block values; sblk sorted values
+        DBuf<int32_t> perm;                  // sorted position -> entry within a block
+        DBuf<double> tab;                    // 4 per entry: C11, C33, C55, C13
+    } dem;
+    DBuf<double> d_aniso;                    // 6 components x nel x 7
+
     // thermal diffusion operators (sliced P2 node pattern, one value per stored entry)
     bool have_thermal = false;
     DBuf<double> KT, MT, AT, FT;

@@ -245,6 +245,54 @@ int stk_thermal_solve(stk_ctx *ctx, double *T, const double *rho, double K, doub
                       stk_report *report);
 
 /* ---------------------------------------------------------------------------
+ * Fabric: finite strain ellipses and the DEM viscous tensor.
+ * ------------------------------------------------------------------------- */
+
+/* Healing type of FiniteStrain (src/Stress/FSE.jl:39-67) that getFSE dispatches on. */
+enum {
+    STK_FSE_NO_HEALING = 0,            /* getFSE{NoHealing} / _FSE (:131-161)                    */
+    STK_FSE_FABRIC_DESTRUCTION = 1,    /* getFSE{FabricDestructionOnly} / healing_FSE! (:191-231) */
+    STK_FSE_ANNEALING_DESTRUCTION = 2  /* getFSE{AnnealingFabricDestruction} (:297-355)          */
+};
+
+/* getFSE (src/Stress/FSE.jl:131-231, 297-355): the finite strain ellipse of F F^T at every
+ * point of F.  F (inout) has the layout of stk_deformation_gradient (nel x nip points, 4
+ * doubles per point, column-major 2x2), nip = 6 or 7 (MAIN.jl:114 uses 7).  With the
+ * destruction modes, a point whose a1/a2 >= eps (MAIN.jl:117: 1e3) gets F = I and the unit
+ * ellipse; STK_FSE_ANNEALING_DESTRUCTION first divides every F by max |F11| when that exceeds
+ * 1e30 (normalize_F!).  fse (may be NULL) receives the FiniteStrainEllipsoid fields (x1, x2,
+ * y1, y2, a1, a2) of each point, 6 doubles per point in F's point order.  The ellipses also
+ * stay on the device for stk_anisotropic_tensor. */
+int stk_fse(stk_ctx *ctx, double *F, int32_t nip, int32_t mode, double eps, double *fse);
+
+/* getDEM (src/Rheology/anisotropy.jl:60-93) from the arrays DEMloader (:48-54) reads out of
+ * the HDF5 file: tensor = dataset "C" (n x 9: eta11 eta22 eta33 eta44 eta55 eta66 eta12 eta13
+ * eta23), axes = dataset "ax" (n x 3), both column-major.  The table is n / sblk blocks of sblk
+ * entries with equal ax[:,1]; w is the viscosity cut-off weakening(dη, φ) of Fabrics.jl, an
+ * external package whose value the caller supplies. */
+int stk_set_dem(stk_ctx *ctx, const double *tensor, const double *axes, int64_t n, double w);
+
+/* Where stk_anisotropic_tensor also writes its result on the device. */
+enum {
+    STK_INSTALL_ASSEMBLY = 1,  /* the tensor of stk_set_fields (columns 1-6); assemble again after */
+    STK_INSTALL_STRESS = 2     /* the tensor of stk_set_stress_fields (its nip columns)            */
+};
+
+/* anisotropic_tensor (src/Rheology/anisotropy.jl:140-222) with get_tensor_and_rotate!
+ * (:244-285): per point the DEM entry nearest to the fabric shape, its tensor rotated into the
+ * ellipse axis (x1, y1), eta55 clamped to w; the 7th column is the mean of the first six.
+ * fse: 6 doubles per point of nel x nip points (nip 6 or 7), or NULL for the device ellipses of
+ * the last stk_fse (same nip).  The fabric shape (r1, r2) = fabric_shape(a1, 1,
+ * parameterization) of Fabrics.jl is an input: with shape_a1 non-NULL, r1 and r2 sampled at
+ * nshape >= 2 strictly increasing a1 nodes, linear between them and constant beyond; with
+ * shape_a1 NULL, r1 and r2 at every point (nshape = nel x nip).  out[6] (any may be NULL)
+ * receive eta11, eta33, eta55, eta13, eta15, eta35, each nel x 7 column-major (the
+ * StiffnessTensor the reference returns).  install: STK_INSTALL_* flags. */
+int stk_anisotropic_tensor(stk_ctx *ctx, const double *fse, int32_t nip, const double *shape_a1,
+                           const double *shape_r1, const double *shape_r2, int64_t nshape, int32_t install,
+                           double *const out[6]);
+
+/* ---------------------------------------------------------------------------
  * Learned local-Schur preconditioner (no counterpart in the reference).  A
  * per-element MLP (5 -> 64 -> 64 -> 64 -> 6, GELU, BF16 weights, FP32
  * accumulation on MFMA) maps (mean log eta, cos 2(phi - theta_c),
