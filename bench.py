@@ -172,6 +172,21 @@ def main():
     log(f"stress!: {st_ms * 1e3:.1f} us ({st_bytes / st_ms / 1e6:.0f} GB/s compulsory); "
         f"deformation_gradient!: {fg_ms * 1e3:.1f} us ({fg_bytes / fg_ms / 1e6:.0f} GB/s)")
 
+    # fabric chain on the same mesh: getFSE on 7 points per element, then anisotropic_tensor on the
+    # DEM table DEM_1e-3_vol30 (the reference's newDEM data, kept as a test fixture)
+    fse_ms, fse_bytes = ctx.time_kernel(_lib.KERNEL_FSE, reps=20)
+    dem = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "dem_tables.npz"))
+    ctx.set_dem(dem["DEM_1em3_vol30_C"], dem["DEM_1em3_vol30_ax"], 0.03)
+    ang = np.linspace(0.0, np.pi, 7 * m.nel).reshape(m.nel, 7)
+    Fsyn = np.zeros((m.nel, 7, 2, 2))
+    Fsyn[..., 0, 0], Fsyn[..., 1, 0] = 3.0 * np.cos(ang), 3.0 * np.sin(ang)
+    Fsyn[..., 0, 1], Fsyn[..., 1, 1] = -np.sin(ang) / 3.0, np.cos(ang) / 3.0
+    ctx.fse(Fsyn)
+    del Fsyn, ang
+    an_ms, an_bytes = ctx.time_kernel(_lib.KERNEL_ANISO, reps=20)
+    log(f"getFSE: {fse_ms * 1e3:.1f} us ({fse_bytes / fse_ms / 1e6:.0f} GB/s); "
+        f"anisotropic_tensor: {an_ms * 1e3:.1f} us ({an_bytes / an_ms / 1e6:.0f} GB/s)")
+
     # one thermal diffusion step on the same mesh (heated bottom, non-dimensional K = Cp = rho = 1)
     nn = m.theta.size
     top, bot = np.flatnonzero(m.r == m.r.max()), np.flatnonzero(m.r == m.r.min())
@@ -255,6 +270,10 @@ def main():
             "stress_us": round(st_ms * 1e3, 1), "stress_GB_per_s": round(st_bytes / st_ms / 1e6, 1),
             "deformation_gradient_us": round(fg_ms * 1e3, 1),
             "deformation_gradient_GB_per_s": round(fg_bytes / fg_ms / 1e6, 1)},
+        "fabric": {
+            "fse_us": round(fse_ms * 1e3, 1), "fse_GB_per_s": round(fse_bytes / fse_ms / 1e6, 1),
+            "anisotropic_tensor_us": round(an_ms * 1e3, 1),
+            "anisotropic_tensor_GB_per_s": round(an_bytes / an_ms / 1e6, 1), "points": 7 * m.nel},
     }
     ctx.close()
     print(json.dumps(out), flush=True)

@@ -448,7 +448,8 @@ int stk_time_kernel(stk_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms, 
     if (!ctx || !avg_ms || !bytes) return STK_E_ARG;
     return guarded(ctx, [&]() {
         Context &c = ctx->c;
-        STK_CHECK(c.assembled, STK_E_STATE, "nothing assembled");
+        const bool standalone = kernel == STK_KERNEL_FSE || kernel == STK_KERNEL_ANISO || kernel == STK_KERNEL_COPY;
+        STK_CHECK(standalone || c.assembled, STK_E_STATE, "nothing assembled");
         if (kernel == STK_KERNEL_ASSEMBLY) {
             STK_CHECK(c.have_fields && c.K64, STK_E_STATE, "FP64 assembly needs fields");
             const int flags = c.asm_flags;
@@ -756,7 +757,12 @@ int stk_fse(stk_ctx *ctx, double *F, int32_t nip, int32_t mode, double eps, doub
         c.d_fse.alloc(6 * npt);
         launch_fse(c, f.p, npt, mode, eps, c.d_fse.p);
         if (mode != STK_FSE_NO_HEALING) f.download(F, 4 * npt, c.stream);
-        if (fse) c.d_fse.download(fse, 6 * npt, c.stream);
+        DBuf<double> pts;
+        if (fse) {
+            pts.alloc(6 * npt);
+            fse_layout(c, c.d_fse.p, pts.p, npt, true);
+            pts.download(fse, 6 * npt, c.stream);
+        }
         STK_HIP(hipStreamSynchronize(c.stream));
         c.fse_nip = nip;
         return STK_OK;
@@ -788,10 +794,12 @@ int stk_anisotropic_tensor(stk_ctx *ctx, const double *fse, int32_t nip, const d
         STK_CHECK(!(install & STK_INSTALL_STRESS) || c.have_sfields, STK_E_STATE,
                   "install into the stress fields needs stk_set_stress_fields first");
         const int64_t npt = c.nel * (int64_t)nip;
-        DBuf<double> f, sn, s1, s2;
+        DBuf<double> f, fp, sn, s1, s2;
         const double *dfse = c.d_fse.p;
         if (fse) {
-            f.upload(fse, 6 * npt, c.stream);
+            fp.upload(fse, 6 * npt, c.stream);
+            f.alloc(6 * npt);
+            fse_layout(c, fp.p, f.p, npt, false);
             dfse = f.p;
         } else {
             STK_CHECK(dfse && c.fse_nip == nip, STK_E_STATE,

@@ -14,10 +14,11 @@
 // C13 are rotated into the ellipse axis (rotmatrix :287-292, directRotation2D :503-536); eta55 is
 // clamped to w again and the 7th column is the mean of the first six.
 //
-// One thread per point (ellipses) or per element (tensor).  F and the ellipses are 4 / 6 doubles
-// per point, so a wave reads and writes one contiguous run; the tensor planes are nel x 7
-// column-major (element fastest), written coalesced.  The DEM table (a few hundred doubles) is
-// read through the caches.
+// One thread per point (ellipses) or per element (tensor).  F is 4 doubles per point (the
+// reference's layout), so a wave reads and writes one contiguous run; the device keeps the
+// ellipses as six planes (field k of point p at k * npt + p), so the tensor reads only the
+// three fields it uses, coalesced; the tensor planes are nel x 7 column-major (element fastest),
+// written coalesced.  The DEM table (a few hundred doubles) is read through the caches.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -136,13 +137,25 @@ __global__ void __launch_bounds__(kBlock) k_fse(double *__restrict__ F, int64_t 
         }
         for (int k = 0; k < 4; ++k) F[4 * p + k] = f[k];
     }
-    for (int k = 0; k < 6; ++k) fse[6 * p + k] = o[k];
+    for (int k = 0; k < 6; ++k) fse[k * npt + p] = o[k];
+}
+
+// ellipses between the device planes (field k of point p at k * npt + p) and the host layout
+// (6 consecutive fields per point)
+__global__ void __launch_bounds__(kBlock) k_fse_layout(const double *__restrict__ src, double *__restrict__ dst,
+                                                       int64_t npt, int to_points) {
+    const int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (p >= npt) return;
+    for (int k = 0; k < 6; ++k) {
+        if (to_points) dst[6 * p + k] = src[k * npt + p];
+        else dst[k * npt + p] = src[6 * p + k];
+    }
 }
 
 struct AnisoArgs {
     int64_t nel;
     int nip;
-    const double *fse;
+    const double *fse;                // 6 planes of nel x nip points
     const double *nodes, *sr1, *sr2;  // shape table (nodes != null) or per-point r1, r2
     int64_t nshape;
     const double *a1a2, *a2a3, *tab;
@@ -153,7 +166,7 @@ struct AnisoArgs {
 };
 
 // argminsorted: first i with v[i] < v[i+1], else the last index (0-based)
-__device__ __forceinline__ int argminsorted(const double *blk, int m, double r) {
+__device__ __forceinline__ int argminsorted(const double *__restrict__ blk, int m, double r) {
     double prev = fabs(r - blk[0]);
     for (int i = 0; i + 1 < m; ++i) {
         const double next = fabs(r - blk[i + 1]);
@@ -163,25 +176,22 @@ __device__ __forceinline__ int argminsorted(const double *blk, int m, double r) 
     return m - 1;
 }
 
-__device__ __forceinline__ void shape_at(const AnisoArgs &a, int64_t p, double a1, double &r1, double &r2) {
+// fabric shape from the table: r at a1, linear between nodes, the end values beyond
+__device__ __forceinline__ void shape_from_table(const double *__restrict__ nodes, const double *__restrict__ t1,
+                                                 const double *__restrict__ t2, int64_t K, double a1, double &r1,
+                                                 double &r2) {
 #pragma clang fp contract(off)
-    if (!a.nodes) {
-        r1 = a.sr1[p];
-        r2 = a.sr2[p];
-        return;
-    }
-    const int64_t K = a.nshape;
-    if (a1 <= a.nodes[0]) { r1 = a.sr1[0]; r2 = a.sr2[0]; return; }
-    if (a1 >= a.nodes[K - 1]) { r1 = a.sr1[K - 1]; r2 = a.sr2[K - 1]; return; }
+    if (a1 <= nodes[0]) { r1 = t1[0]; r2 = t2[0]; return; }
+    if (a1 >= nodes[K - 1]) { r1 = t1[K - 1]; r2 = t2[K - 1]; return; }
     // largest k with nodes[k] <= a1 (NaN: none, k = 0 and t = NaN)
     int64_t lo = 0, hi = K - 1;
     while (hi - lo > 1) {
         const int64_t mid = (lo + hi) / 2;
-        if (a.nodes[mid] <= a1) lo = mid; else hi = mid;
+        if (nodes[mid] <= a1) lo = mid; else hi = mid;
     }
-    const double t = (a1 - a.nodes[lo]) / (a.nodes[lo + 1] - a.nodes[lo]);
-    r1 = a.sr1[lo] + t * (a.sr1[lo + 1] - a.sr1[lo]);
-    r2 = a.sr2[lo] + t * (a.sr2[lo + 1] - a.sr2[lo]);
+    const double t = (a1 - nodes[lo]) / (nodes[lo + 1] - nodes[lo]);
+    r1 = t1[lo] + t * (t1[lo + 1] - t1[lo]);
+    r2 = t2[lo] + t * (t2[lo + 1] - t2[lo]);
 }
 
 __device__ __forceinline__ void rotate(double x, double y, double a, double c, double e, double h, double n[6]) {
@@ -201,27 +211,40 @@ __global__ void __launch_bounds__(kBlock) k_anisotropic_tensor(AnisoArgs a) {
 #pragma clang fp contract(off)
     const int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if (e >= a.nel) return;
-    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const int64_t nel = a.nel;
+    const double *__restrict__ fse = a.fse;
+    double *__restrict__ out = a.out;
     // the 7th point of a 7-point ellipse field is computed by the reference and then replaced
-    // by the mean, so only the first six are evaluated
+    // by the mean, so only the first six are evaluated; all loads of the element first
+    const int64_t npt = nel * a.nip;
+    double x[6], y[6], r1[6], r2[6];
     for (int q = 0; q < 6; ++q) {
-        const int64_t p = e + q * a.nel;
-        const double *f = a.fse + 6 * p;
-        const double x = f[0], y = f[2], a1 = f[4];
-        double r1, r2;
-        shape_at(a, p, a1, r1, r2);
-        const int i1 = argminsorted(a.a1a2, a.nblk, r1);
-        const int i2 = a.perm[argminsorted(a.a2a3, a.sblk, r2)];
-        const double *t = a.tab + 4 * ((int64_t)a.sblk * i1 + i2);
+        x[q] = fse[e + q * nel];
+        y[q] = fse[2 * npt + e + q * nel];
+        r1[q] = fse[4 * npt + e + q * nel];
+    }
+    if (a.nodes) {
+        for (int q = 0; q < 6; ++q) shape_from_table(a.nodes, a.sr1, a.sr2, a.nshape, r1[q], r1[q], r2[q]);
+    } else {
+        for (int q = 0; q < 6; ++q) {
+            r1[q] = a.sr1[e + q * nel];
+            r2[q] = a.sr2[e + q * nel];
+        }
+    }
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < 6; ++q) {
+        const int i1 = argminsorted(a.a1a2, a.nblk, r1[q]);
+        const int i2 = a.perm[argminsorted(a.a2a3, a.sblk, r2[q])];
+        const double *__restrict__ t = a.tab + 4 * ((int64_t)a.sblk * i1 + i2);
         double n[6];
-        rotate(x, y, t[0], t[1], nan_max(t[2], a.w), t[3], n);
+        rotate(x[q], y[q], t[0], t[1], nan_max(t[2], a.w), t[3], n);
         n[2] = nan_max(n[2], a.w);
         for (int k = 0; k < 6; ++k) {
-            a.out[(int64_t)k * 7 * a.nel + q * a.nel + e] = n[k];
+            out[(int64_t)k * 7 * nel + q * nel + e] = n[k];
             acc[k] += n[k];
         }
     }
-    for (int k = 0; k < 6; ++k) a.out[(int64_t)k * 7 * a.nel + 6 * a.nel + e] = acc[k] * (1.0 / 6.0);
+    for (int k = 0; k < 6; ++k) out[(int64_t)k * 7 * nel + 6 * nel + e] = acc[k] * (1.0 / 6.0);
 }
 
 }  // namespace
@@ -245,6 +268,13 @@ void launch_fse(Context &c, double *F, int64_t npt, int mode, double eps, double
     }
     k_fse<<<dim3((unsigned)((npt + kBlock - 1) / kBlock)), dim3(kBlock), 0, c.stream>>>(F, npt, mode, eps, nullptr,
                                                                                         fse);
+    STK_HIP(hipGetLastError());
+}
+
+void fse_layout(Context &c, const double *src, double *dst, int64_t npt, bool to_points) {
+    if (npt <= 0) return;
+    k_fse_layout<<<dim3((unsigned)((npt + kBlock - 1) / kBlock)), dim3(kBlock), 0, c.stream>>>(src, dst, npt,
+                                                                                               to_points ? 1 : 0);
     STK_HIP(hipGetLastError());
 }
 

@@ -31,6 +31,7 @@
 
 #include "device_ops.h"
 #include "solver.h"
+#include "fabric.h"
 #include "stress.h"
 #include "thermal.h"
 
@@ -1900,6 +1901,47 @@ double time_kernel(Context &c, int kernel, int reps, double *bytes) {
                 launch = [&c, u, F]() { launch_deformation_gradient(c, u, 1e-3, F); };
                 *bytes = (double)nel * (6 * 4 + 6 * 8) + (double)nU * 8 + 2.0 * 4 * plane * 8;
             }
+            break;
+        }
+        case STK_KERNEL_FSE: {
+            // nel x 7 points of F = R(a) diag(s, 1/s) with a1/a2 = s^2 below the destruction ratio
+            const int64_t npt = 7 * c.nel;
+            o64.alloc(4 * npt);
+            x64.alloc(6 * npt);
+            double *F = o64.p, *fse = x64.p;
+            launch_map(c, npt, [=] __device__(int64_t p) {
+                const double a = 0.001 * (double)(p % 6283), s = 1.0 + 0.01 * (double)(p % 97);
+                F[4 * p] = cos(a) * s; F[4 * p + 1] = sin(a) * s;
+                F[4 * p + 2] = -sin(a) / s; F[4 * p + 3] = cos(a) / s;
+            });
+            launch = [&c, F, npt, fse]() { launch_fse(c, F, npt, FSE_FABRIC_DESTRUCTION, 1e3, fse); };
+            // F read and written back, the ellipse written
+            *bytes = (double)npt * (32 + 32 + 48);
+            break;
+        }
+        case STK_KERNEL_ANISO: {
+            STK_CHECK(c.dem.set, STK_E_STATE, "no DEM table (stk_set_dem)");
+            STK_CHECK(c.d_fse && c.fse_nip > 0, STK_E_STATE, "no device ellipses (stk_fse)");
+            // fabric-shape table of 16 nodes over a1 in [1, 61]
+            constexpr int K = 16;
+            x64.alloc(3 * K);
+            double *tb = x64.p;
+            launch_map(c, K, [=] __device__(int64_t k) {
+                tb[k] = 1.0 + 4.0 * (double)k;
+                tb[K + k] = 1.7 * (double)k / (K - 1);
+                tb[2 * K + k] = 2.0 - 2.0 * (double)k / (K - 1);
+            });
+            o64.alloc(42 * c.nel);
+            double *o = o64.p;
+            const double *fse = c.d_fse.p;
+            const int nip = c.fse_nip;
+            launch = [&c, tb, o, fse, nip]() {
+                ShapeIn s;
+                s.nodes = tb; s.r1 = tb + K; s.r2 = tb + 2 * K; s.nshape = K;
+                launch_anisotropic_tensor(c, fse, nip, s, o);
+            };
+            // per element: x1, y1, a1 of six points, the six nel x 7 planes written
+            *bytes = (double)c.nel * (6 * 3 * 8 + 42 * 8);
             break;
         }
         case STK_KERNEL_COPY: {

@@ -248,7 +248,7 @@ struct Context {
 
     // fabric: finite strain ellipses of the last stk_fse, the DEM table, the viscous tensor of
     // the last stk_anisotropic_tensor
-    DBuf<double> d_fse;                      // 6 per point (x1, x2, y1, y2, a1, a2), nel x fse_nip points
+    DBuf<double> d_fse;                      // 6 planes (x1, x2, y1, y2, a1, a2) of nel x fse_nip points
     int fse_nip = 0;
     struct DemTable {
         bool set = false;

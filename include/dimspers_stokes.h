@@ -367,8 +367,11 @@ enum {
     STK_KERNEL_DEFGRAD = 10,    /* deformation_gradient! RK4 step (synthetic velocity)    */
     STK_KERNEL_KP32 = 11,       /* the inner CG product Ap = K p, FP32, with (p . Ap,     */
                                 /* v . Ap) block partials                                 */
-    STK_KERNEL_GS_BWD32 = 12    /* backward Gauss-Seidel sweep of the V-cycle with its    */
+    STK_KERNEL_GS_BWD32 = 12,   /* backward Gauss-Seidel sweep of the V-cycle with its    */
                                 /* dots; time and bytes per colour launch                 */
+    STK_KERNEL_FSE = 13,        /* getFSE (fabric destruction) on nel x 7 synthetic F     */
+    STK_KERNEL_ANISO = 14       /* anisotropic_tensor on the device ellipses of the last  */
+                                /* stk_fse and the DEM table (16-node fabric-shape table) */
 };
 
 /* Measurement hook with no counterpart in the reference: runs one of the hot
