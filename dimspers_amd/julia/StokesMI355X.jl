@@ -35,6 +35,9 @@ const SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED = Int32(0), Int32(1), Int32(2)
 const STOP_SCHUR, STOP_FULL = Int32(0), Int32(1)
 const ASM_FP64, ASM_FP32, ASM_CARTESIAN, ASM_NO_BC = Int32(1), Int32(2), Int32(4), Int32(8)
 const MAT_K, MAT_G, MAT_GT, MAT_M = Int32(0), Int32(1), Int32(2), Int32(3)
+const FSE_MODE = Dict(:NoHealing => Int32(0), :FabricDestructionOnly => Int32(1),
+                      :AnnealingFabricDestruction => Int32(2))
+const INSTALL_ASSEMBLY, INSTALL_STRESS = Int32(1), Int32(2)
 
 mutable struct Context
     h::Ptr{Cvoid}
@@ -194,6 +197,50 @@ function write_stats(U, T, np, gr, Time, ScratchNu, stats_file; coordinates, UBC
                 Float32(out[4]), " ", np)
     end
     return Tuple(out)
+end
+
+"""GPU `getFSE(F, FSE)` (src/Stress/FSE.jl:131-355): fills `FSE.fse` (a `Matrix{FiniteStrainEllipsoid}`,
+6 doubles per point) from `F` (a `Matrix{SMatrix{2,2}}`) and applies the healing rule of FSE's type
+parameter to F in place.  The ellipses also stay on the device for `anisotropic_tensor`."""
+function getFSE(F, FSE; gr, coordinates, UBC)
+    ctx = context(gr, coordinates, UBC)
+    mode = get(FSE_MODE, nameof(typeof(FSE).parameters[1])) do
+        error("getFSE: healing type ", typeof(FSE).parameters[1], " has no device version")
+    end
+    GC.@preserve F FSE begin
+        check(ctx, ccall((:stk_fse, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int32, Int32, Float64, Ptr{Float64}),
+                         ctx.h, Ptr{Float64}(pointer(F)), size(F, 2), mode, FSE.destruction.ϵ,
+                         Ptr{Float64}(pointer(FSE.fse))))
+    end
+    return FSE, F
+end
+
+"""GPU `anisotropic_tensor(FSE.fse, D, Val(η))` (src/Rheology/anisotropy.jl:140-285): returns the six
+nel × 7 matrices of the `StiffnessTensor` (wrap them with `StiffnessTensor(C...)`).  `fabric_shape` is
+Fabrics.jl's function: evaluated at every point (exact), or at `nodes` (increasing a1 values) and
+interpolated linearly on the device.  `install` (INSTALL_*) also writes 𝓒 into the device fields of the
+next `solveStokes` / `stress!` of this grid."""
+function anisotropic_tensor(fse, D, ::Val; gr, coordinates, UBC, fabric_shape, nodes=nothing, install=Int32(0))
+    ctx = context(gr, coordinates, UBC)
+    nel, nip = size(fse)
+    tensor = Matrix{Float64}(D.𝓒); axes = Matrix{Float64}(D.a)
+    GC.@preserve tensor axes begin
+        check(ctx, ccall((:stk_set_dem, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}, Int64, Float64),
+                         ctx.h, tensor, axes, size(tensor, 1), D.w))
+    end
+    at = nodes === nothing ? [f.a1 for f in fse] : Vector{Float64}(nodes)
+    r = [fabric_shape(a, 1.0, D.parameterization) for a in at]
+    r1 = Float64[x[1] for x in r]; r2 = Float64[x[2] for x in r]
+    a1p = nodes === nothing ? Ptr{Float64}(C_NULL) : pointer(at)
+    C = [Matrix{Float64}(undef, nel, 7) for _ in 1:6]
+    ptrs = [pointer(c) for c in C]
+    GC.@preserve fse at r1 r2 C ptrs begin
+        check(ctx, ccall((:stk_anisotropic_tensor, LIB), Cint,
+                         (Ptr{Cvoid}, Ptr{Float64}, Int32, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Int64, Int32,
+                          Ptr{Ptr{Float64}}),
+                         ctx.h, Ptr{Float64}(pointer(fse)), nip, a1p, r1, r2, length(r1), install, ptrs))
+    end
+    return C
 end
 
 end # module
