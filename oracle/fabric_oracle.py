@@ -28,7 +28,8 @@ their closed form and against LAPACK's ``eigh``; the healing rules on constructe
 table lookup against a brute-force nearest search on the raw table; the rotation against the
 rotation of the fourth-order tensor; the DEM tables of the reference (tests/golden) through their
 block structure.  The eigenvector direction of an exactly isotropic F (equal eigenvalues: every
-direction is one) follows StaticArrays' diagonal branch and is not pinned by mathematics.
+direction is one) follows StaticArrays' diagonal branch, which the mathematics does not fix and no
+Julia run checked: parity unpinned for that case.
 """
 from __future__ import annotations
 

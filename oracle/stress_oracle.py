@@ -19,8 +19,11 @@ the points of ``ip_triangle7`` (src/Algebra/Quadrature.jl:109-138), mapped by th
 Pinning (tests/test_stress_oracle.py): a velocity linear in (theta, r) is reproduced
 exactly by P2, so its Cartesian gradient has a closed form at every point; the RK4 step of
 F' = L F with constant L is the fourth-order Taylor polynomial of exp(L); the isotropic law
-is tau = 2 eta dev(eps) (three-dimensional deviator); the anisotropic law is checked against
-a separate transcription of :223-235 as a 3x3 coefficient matrix.
+is tau = 2 eta dev(eps) (three-dimensional deviator); with the initial isotropic tensor
+(anisotropy.jl:13-22) the anisotropic law differs from it by exactly -(2/9) eta tr(eps) on the
+normal stresses, which pins its eta11, eta33, eta13 and eta55 terms.  Its eta15 and eta35 terms
+are checked only against a separate transcription of :223-235 as a 3x3 coefficient matrix:
+parity partially pinned for those two terms.
 """
 from __future__ import annotations
 

@@ -113,3 +113,23 @@ def test_anisotropic_law_matches_matrix_transcription(mesh):
     scale = np.abs(tau).max()
     for i, k in enumerate(("txx", "tzz", "txz")):
         assert np.abs(out[k] - tau[..., i]).max() < 1e-13 * scale
+
+
+def test_anisotropic_law_with_the_initial_isotropic_tensor(mesh):
+    """With 𝓒init's tensor (anisotropy.jl:13-22: eta11 = eta33 = 4/3, eta55 = 2, eta13 = -2/3,
+    eta15 = eta35 = 0) the anisotropic law (:223-235) differs from the isotropic one (:117-120)
+    by exactly -(2/9) eta tr(eps) on the normal stresses and not at all on the shear, for any
+    velocity: the two agree for incompressible flow.  Pins the eta11, eta33, eta13 and eta55 terms
+    of the anisotropic law independently of its transcription."""
+    rng = np.random.default_rng(6)
+    U = rng.normal(size=2 * mesh.theta.size)
+    eta = np.exp(rng.normal(size=(mesh.nel, 7)))
+    init = [np.full((mesh.nel, 7), v) for v in (4 / 3, 4 / 3, 2.0, -2 / 3, 0.0, 0.0)]
+    an = so.stress(mesh, U, eta, init)
+    iso = so.stress(mesh, U, eta)
+    shift = -2 / 9 * eta * (iso["exx"] + iso["ezz"])
+    scale = np.abs(iso["txx"]).max()
+    assert np.abs(an["txx"] - iso["txx"] - shift).max() < 1e-13 * scale
+    assert np.abs(an["tzz"] - iso["tzz"] - shift).max() < 1e-13 * scale
+    assert np.abs(an["txz"] - iso["txz"]).max() < 1e-13 * scale
+    assert np.abs(shift).max() > 1e-3 * scale           # the velocity is compressible: the shift shows
