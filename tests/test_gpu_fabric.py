@@ -234,3 +234,39 @@ def test_reference_style_chain():
     _, Fw = fo.get_fse(_points(F), fo.FABRIC_DESTRUCTION, eps=1e3)
     assert np.array_equal(F1, _unpoints(Fw, nel, 7, (2, 2)))
     ctx.close()
+
+
+def test_solve_with_the_installed_dem_tensor():
+    """End to end at 64 x 512: simple-shear F -> ellipses -> DEM tensor installed on the device ->
+    assembly -> mixed solve to the judged 1e-8 residual, against the plain FP64 solve."""
+    inp, ctx = _ctx(64, 512, 4)
+    m = inp.mesh
+    T, ax = _dem()
+    ctx.set_dem(T, ax, W_CUTOFF)
+    # F = R(phi) [[1, g], [0, 1]] R(phi)^T with the shear g and direction phi varying over the mesh
+    thc = np.asarray(m.theta)[np.asarray(m.e2n)[:3]].mean(axis=0)
+    g = 2.0 + 2.0 * np.sin(3 * thc)[:, None] * np.ones((1, 7))
+    phi = (thc[:, None] + 0.3 * np.arange(7)[None, :])
+    c, s = np.cos(phi), np.sin(phi)
+    F = np.empty((m.nel, 7, 2, 2))
+    F[..., 0, 0] = 1 - g * c * s
+    F[..., 0, 1] = g * c * c
+    F[..., 1, 0] = -g * s * s
+    F[..., 1, 1] = 1 + g * c * s
+    ctx.fse(F, mode=_lib.FSE_FABRIC_DESTRUCTION)
+    ctx.set_fields(inp.g, inp.rho, inp.eta_ip, None)
+    C = ctx.anisotropic_tensor(shape_table=(SHAPE_NODES, SHAPE_R1, SHAPE_R2), install=_lib.INSTALL_ASSEMBLY)
+    assert np.ptp(C[2]) > 0.1 and np.abs(C[4]).max() > 0.01      # anisotropic, rotated
+    ctx.assemble(_lib.ASM_FP64 | _lib.ASM_FP32)
+    rep = ctx.solve(_lib.default_opts(method=_lib.SOLVE_MIXED, stop=_lib.STOP_FULL, rtol=1e-8))
+    U1, P1 = ctx.solution()
+    rep0 = ctx.solve(_lib.default_opts(method=_lib.SOLVE_PLAIN, stop=_lib.STOP_FULL, rtol=1e-8))
+    U0, P0 = ctx.solution()
+    eu = np.linalg.norm(U1 - U0) / np.linalg.norm(U0)
+    ep = np.linalg.norm((P1 - P1.mean()) - (P0 - P0.mean())) / np.linalg.norm(P0 - P0.mean())
+    print(f"[result] DEM tensor solve 64x512: mixed {rep.solve_ms:.1f} ms {rep.outer_iterations} outer, "
+          f"plain {rep0.solve_ms:.1f} ms {rep0.outer_iterations} outer; errU {eu:.1e} errP {ep:.1e}")
+    assert rep.converged and rep0.converged
+    assert rep.full_residual < 1e-8 and rep0.full_residual < 1e-8
+    assert eu < 1e-6 and ep < 1e-6
+    ctx.close()
