@@ -282,6 +282,8 @@ void dem_prepare(Context &c, const double *tensor, const double *axes, int64_t n
     STK_CHECK(n >= 2, STK_E_SHAPE, "the DEM table needs at least two entries");
     std::vector<double> a1a2(n), a2a3(n);
     for (int64_t i = 0; i < n; ++i) {
+        STK_CHECK(std::isfinite(axes[i]) && axes[i] > 0 && std::isfinite(axes[2 * n + i]) && axes[2 * n + i] > 0,
+                  STK_E_ARG, "the DEM semi-axes ax[:,1] and ax[:,3] must be positive and finite");
         a1a2[i] = std::log10(axes[i] / 1.0);
         a2a3[i] = std::log10(1.0 / axes[2 * n + i]);
     }

@@ -172,6 +172,11 @@ def test_fabric_error_paths():
     with pytest.raises(_lib.StkError) as e:
         ctx.set_dem(T[:10], ax[:10], W_CUTOFF)                           # one block of equal a1
     assert e.value.code == _lib.STK_E_SHAPE
+    bad = ax.copy()
+    bad[3, 2] = 0.0
+    with pytest.raises(_lib.StkError) as e:
+        ctx.set_dem(T, bad, W_CUTOFF)                                    # log10 of a zero semi-axis
+    assert e.value.code == _lib.STK_E_ARG
     # identity F everywhere: unit ellipses on StaticArrays' diagonal branch
     ell, _ = ctx.fse(np.tile(np.eye(2), (nel, 6, 1, 1)), mode=_lib.FSE_NO_HEALING)
     assert np.array_equal(ell.reshape(-1, 6), np.tile([0.0, 1.0, 1.0, 0.0, 1.0, 1.0], (nel * 6, 1)))
