@@ -318,6 +318,35 @@ def test_error_paths(small):
     ctx.close()
 
 
+def test_forced_failures_are_reported(small):
+    """The failure cases a caller can force: the singular system without boundary conditions is
+    refused; a NaN in the right-hand side (density) stops the solve with STK_E_NAN; a negative
+    viscosity (K negative definite) is refused by the multigrid set-up or the solve, never solved
+    silently (the coarsest multigrid factorisation reports it).  The reference runs all three to
+    its iteration cap without a flag."""
+    ctx = _lib.Context.from_mesh(small.mesh)
+    ctx.set_fields(small.g, small.rho, small.eta_ip, small.C6)
+    ctx.assemble(_lib.ASM_FP64 | _lib.ASM_NO_BC)
+    with pytest.raises(_lib.StkError) as e:
+        ctx.solve(_lib.default_opts())
+    assert e.value.code == _lib.STK_E_STATE
+    rho = small.rho.copy()
+    rho[7] = np.nan
+    ctx.set_fields(small.g, rho, small.eta_ip, small.C6)
+    ctx.assemble(_lib.ASM_FP64 | _lib.ASM_FP32)
+    for method in (_lib.SOLVE_PLAIN, _lib.SOLVE_MIXED):
+        with pytest.raises(_lib.StkError) as e:
+            ctx.solve(_lib.default_opts(method=method))
+        assert e.value.code == _lib.STK_E_NAN
+    ctx.set_fields(small.g, small.rho, -small.eta_ip, small.C6)
+    with pytest.raises(_lib.StkError) as e:
+        ctx.assemble(_lib.ASM_FP64)
+        ctx.solve(_lib.default_opts())
+    print(f"[result] negative viscosity refused with code {e.value.code}: {e.value}")
+    assert e.value.code == _lib.STK_E_SHAPE
+    ctx.close()
+
+
 def test_fp32_only_assembly(medium):
     """STK_ASM_FP32 alone assembles the operators in FP32 (the FP32 element kernels, Galerkin
     levels and set-up): K and G equal the FP64 assembly to FP32 arithmetic (measured 2.5e-6 of
