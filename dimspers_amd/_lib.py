@@ -29,7 +29,7 @@ EXPORTED = [
     "stk_updatevelocity", "stk_time_kernel", "stk_default_train_opts", "stk_set_fabric", "stk_mlp_train",
     "stk_mlp_params", "stk_mlp_forward", "stk_mlp_gradient", "stk_local_schur", "stk_learned_setup",
     "stk_set_stress_fields", "stk_stress", "stk_deformation_gradient", "stk_stats", "stk_thermal_solve",
-    "stk_fse", "stk_set_dem", "stk_anisotropic_tensor",
+    "stk_fse", "stk_set_dem", "stk_anisotropic_tensor", "stk_colouring",
 ]
 N_PARAM = 9094
 KERNEL_SPMV_K32, KERNEL_SPMV_K64, KERNEL_CHEB_K32, KERNEL_CHEB_K64, KERNEL_ASSEMBLY, KERNEL_CHEB_KH16 = 0, 1, 2, 3, 4, 5
@@ -38,6 +38,7 @@ KERNEL_GS_BWD32, KERNEL_FSE, KERNEL_ANISO = 12, 13, 14
 STRESS_FIELDS = ("exx", "ezz", "exz", "eII", "txx", "tzz", "txz", "tII")
 FSE_NO_HEALING, FSE_FABRIC_DESTRUCTION, FSE_ANNEALING_DESTRUCTION = 0, 1, 2
 INSTALL_ASSEMBLY, INSTALL_STRESS = 1, 2
+COLOUR_ELEMENTS, COLOUR_NODES = 0, 1
 
 
 class StkOpts(C.Structure):
@@ -136,6 +137,7 @@ def load(path: Path | str | None = None):
     lib.stk_thermal_solve.argtypes = [vp, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, i64p, dp,
                                       C.c_int64, C.c_double, C.POINTER(StkReport)]
     lib.stk_fse.argtypes = [vp, dp, C.c_int32, C.c_int32, C.c_double, dp]
+    lib.stk_colouring.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     lib.stk_set_dem.argtypes = [vp, dp, dp, C.c_int64, C.c_double]
     lib.stk_anisotropic_tensor.argtypes = [vp, dp, C.c_int32, dp, dp, dp, C.c_int64, C.c_int32, C.POINTER(dp)]
     for name in EXPORTED:
@@ -449,3 +451,12 @@ class Context:
         self._check(self.lib.stk_anisotropic_tensor(self.h, None if e is None else _d(e), int(nip), *args,
                                                     int(install), arr))
         return outs
+
+    def colouring(self, which=COLOUR_ELEMENTS):
+        """(colours, number of colours): per element (COLOUR_ELEMENTS) or per node (COLOUR_NODES)."""
+        n = self.nel if which == COLOUR_ELEMENTS else self.nnod
+        col = np.zeros(n, dtype=np.int32)
+        nc = C.c_int32()
+        self._check(self.lib.stk_colouring(self.h, int(which), col.ctypes.data_as(C.POINTER(C.c_int32)),
+                                           C.byref(nc)))
+        return col, nc.value

@@ -168,6 +168,26 @@ int stk_info(const stk_ctx *ctx, int64_t *o) {
     return STK_OK;
 }
 
+int stk_colouring(stk_ctx *ctx, int32_t which, int32_t *colour, int32_t *ncolour) {
+    if (!ctx || !colour || !ncolour) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        if (which == STK_COLOUR_ELEMENTS) {
+            const int nc = (int)c.col_off.size() - 1;
+            for (int k = 0; k < nc; ++k)
+                for (int64_t p = c.col_off[k]; p < c.col_off[k + 1]; ++p) colour[c.perm[p]] = k;
+            *ncolour = nc;
+        } else if (which == STK_COLOUR_NODES) {
+            STK_CHECK(c.gs.on, STK_E_STATE, "no Gauss-Seidel colouring (mesh without the multigrid)");
+            for (int64_t i = 0; i < c.nnod; ++i) colour[i] = c.gs.node_colour[i];
+            *ncolour = c.gs.ncolour;
+        } else {
+            throw Error(STK_E_ARG, "unknown colouring");
+        }
+        return STK_OK;
+    });
+}
+
 int stk_set_fields(stk_ctx *ctx, double g, const double *rho, const double *eta_ip, int64_t nip,
                    const double *const C[6]) {
     if (!ctx) return STK_E_ARG;

@@ -174,6 +174,7 @@ void build_gs_layout(Context &c) {
     for (auto &v : rows)
         std::stable_sort(v.begin(), v.end(), [&](int32_t a, int32_t b) { return len(a) > len(b); });
     g.ncolour = nc;
+    g.node_colour = colour;
     g.slice_off.assign(nc + 1, 0);
     g.row_off.assign(nc + 1, 0);
     // one lane per row (two or four lanes per row, with shorter per-lane load chains, were

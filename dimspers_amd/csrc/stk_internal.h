@@ -201,6 +201,7 @@ struct Context {
     struct GSSmoother {
         bool on = false;
         int ncolour = 0;
+        std::vector<int8_t> node_colour;       // colour of each node row
         std::vector<int64_t> slice_off;        // first slice of each colour (ncolour + 1)
         std::vector<int64_t> row_off;          // first coloured row of each colour (ncolour + 1)
         DBuf<int64_t> sptr;                    // slice block offsets (nslices + 1)

@@ -157,6 +157,19 @@ const char *stk_last_error(const stk_ctx *ctx);
 /* Sizes: out[0..6] = nel, nnod, nvert, nU, nP, colours, multigrid levels. */
 int stk_info(const stk_ctx *ctx, int64_t *out7);
 
+/* Which colouring stk_colouring returns. */
+enum {
+    STK_COLOUR_ELEMENTS = 0,  /* element colours of the assembly scatter (nel values)          */
+    STK_COLOUR_NODES = 1      /* node colours of the multicolour Gauss-Seidel smoother (nnod)  */
+};
+
+/* The colourings built by stk_create, for checking: no two elements of one colour share a
+ * node (the conflict-free scatter of color_mesh, src/Grid/coloring.jl:5-64, whose colouring
+ * check_colored_mesh :97-111 is the reference's counterpart), and no two coupled nodes share a
+ * colour (the smoother's independent sets).  colour receives 0-based colours, *ncolour their
+ * number. */
+int stk_colouring(stk_ctx *ctx, int32_t which, int32_t *colour, int32_t *ncolour);
+
 /* Upload the per-step fields assembly_stokes_cylindric reads
  * (StokesSolver.jl:247-249): g, rho.rho (nnod), eta.ip (nel x nip column-major,
  * nip >= 6, first 6 columns used, :608) and the six StiffnessTensor arrays

@@ -374,3 +374,29 @@ def test_bad_mesh_is_rejected():
     with pytest.raises(_lib.StkError) as e:
         _lib.Context(m.nel, m.nnod, m.nvert, e2n + 1, m.el_theta, m.el_r, m.theta, m.r, m.free_slip()[0] + 1)
     assert e.value.code == _lib.STK_E_SHAPE
+
+
+@pytest.mark.parametrize("size", [(4, 16), (32, 256)])
+def test_colourings_are_valid(size):
+    """check_colored_mesh (src/Grid/coloring.jl:97-111) for the assembly's element colouring: no
+    two elements of a colour share a node; and for the smoother's node colouring: no two nodes
+    coupled in K share a colour."""
+    m = make_inputs(scaled_workload(1, *size)).mesh
+    print(f"[case] colourings {size}", flush=True)
+    ctx = _lib.Context.from_mesh(m)
+    ec, nec = ctx.colouring(_lib.COLOUR_ELEMENTS)
+    assert ec.min() == 0 and ec.max() == nec - 1
+    for k in range(nec):
+        nodes = np.asarray(m.e2n)[:, ec == k].ravel()
+        assert np.unique(nodes).size == nodes.size, k
+    inp = make_inputs(scaled_workload(1, *size))
+    ctx.set_fields(inp.g, inp.rho, inp.eta_ip, inp.C6)
+    ctx.assemble(_lib.ASM_FP64 | _lib.ASM_NO_BC)
+    K = ctx.export_csc(_lib.MAT_K).tocoo()
+    nc, nnc = ctx.colouring(_lib.COLOUR_NODES)
+    i, j = K.row // 2, K.col // 2
+    off = i != j
+    assert np.all(nc[i[off]] != nc[j[off]])
+    assert nnc == nc.max() + 1
+    print(f"[result] {nec} element colours, {nnc} node colours, {off.sum()} coupled node pairs checked")
+    ctx.close()
