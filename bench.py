@@ -59,6 +59,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--kernels", action="store_true",
                     help="only time the dominant kernel (50 launches) and the copy reference (for counter runs)")
+    ap.add_argument("--rates", action="store_true",
+                    help="time every kernel family of the solve with its algorithmic bytes (GB/s, fraction of 8 TB/s)")
     ap.add_argument("--diagnostics", action="store_true",
                     help="only time the SpMV, one V-cycle and one inner CG iteration")
     args = ap.parse_args()
@@ -88,6 +90,24 @@ def main():
     log(f"assembly + solver set-up {time.perf_counter() - t0:.2f}s")
     asm_ms, asm_bytes = ctx.time_kernel(_lib.KERNEL_ASSEMBLY, reps=3)
     log(f"element assembly kernels {asm_ms:.2f} ms ({asm_bytes / asm_ms / 1e6:.0f} GB/s compulsory)")
+
+    if args.rates:
+        res = {"element_assembly_all_colours": {"avg_us": round(asm_ms * 1e3, 1), "bytes": asm_bytes,
+                                                "GB_per_s": round(asm_bytes / asm_ms / 1e6, 1)}}
+        for name, kid, reps in (("spmv_k32", _lib.KERNEL_SPMV_K32, 50), ("spmv_k64", _lib.KERNEL_SPMV_K64, 50),
+                                ("cheb_k32", _lib.KERNEL_CHEB_K32, 50), ("cheb_k64", _lib.KERNEL_CHEB_K64, 50),
+                                ("cg_product_kp32", _lib.KERNEL_KP32, 50),
+                                ("gs_backward_sweep32_per_launch", _lib.KERNEL_GS_BWD32, 50),
+                                ("g32", _lib.KERNEL_G32, 50), ("gt32", _lib.KERNEL_GT32, 50),
+                                ("coarsest_dense32", _lib.KERNEL_COARSE32, 50),
+                                ("cg_update_and_norm32", _lib.KERNEL_CGUPD32, 50), ("copy_256MiB", _lib.KERNEL_COPY, 20)):
+            kms, kb = ctx.time_kernel(kid, reps=reps)
+            res[name] = {"avg_us": round(kms * 1e3, 2), "bytes": kb, "GB_per_s": round(kb / kms / 1e6, 1),
+                         "frac_of_8TBps": round(kb / kms / 1e6 / 8000.0, 3)}
+            log(f"{name}: {kms * 1e3:.1f} us, {kb / kms / 1e6:.0f} GB/s")
+        ctx.close()
+        print(json.dumps({"rates": res}), flush=True)
+        return
 
     if args.kernels or args.diagnostics:
         res = {}

@@ -1903,6 +1903,62 @@ double time_kernel(Context &c, int kernel, int reps, double *bytes) {
             }
             break;
         }
+        case STK_KERNEL_G32: case STK_KERNEL_GT32: {
+            STK_CHECK(c.G32 && c.GT32, STK_E_STATE, "FP32 operators not assembled");
+            const bool g = kernel == STK_KERNEL_G32;
+            const Pattern &pat = g ? c.GN : c.GTN;
+            const int64_t nin = g ? c.nP : nU, nout = g ? nU : c.nP;
+            x32.alloc(nin); y32.alloc(nout);
+            float *x = x32.p, *y = y32.p;
+            launch_map(c, nin, [=] __device__(int64_t i) { x[i] = 1.0f + 1e-3f * (float)(i % 17); });
+            const float *val = g ? c.G32.p : c.GT32.p;
+            launch = [&c, &pat, val, x, y, g]() {
+                EpStore<float> es{y};
+                if (g) apply_rc<2, 1, float>(c, pat, val, x, es, nullptr);
+                else apply_rc<1, 2, float>(c, pat, val, x, es, nullptr);
+            };
+            // structural blocks (two FP32 values and a 32-bit column each), slice pointers, x and y once
+            const int64_t nsl = (pat.nrows + Pattern::kSlice - 1) / Pattern::kSlice;
+            *bytes = (double)pat.nnz() * (2 * 4 + 4) + (double)(nsl + 1) * 8 + (double)(nin + nout) * 4;
+            break;
+        }
+        case STK_KERNEL_COARSE32: {
+            STK_CHECK(c.coarse_inv32 && c.coarse_n > 0, STK_E_STATE, "no FP32 coarsest inverse");
+            const int64_t n = c.coarse_n;
+            x32.alloc(n); y32.alloc(n);
+            float *x = x32.p, *y = y32.p;
+            const float *A = c.coarse_inv32.p;
+            launch_map(c, n, [=] __device__(int64_t i) { x[i] = 1.0f + 1e-3f * (float)(i % 17); });
+            launch = [&c, A, x, y, n]() {
+                k_dense_matvec<float><<<dim3((unsigned)((n + 3) / 4)), dim3(256), 0, c.stream>>>(n, A, x, y);
+                STK_HIP(hipGetLastError());
+            };
+            *bytes = (double)n * n * 4 + 2.0 * n * 4;
+            break;
+        }
+        case STK_KERNEL_CGUPD32: {
+            STK_CHECK(c.rot32, STK_E_STATE, "FP32 operators not assembled");
+            x32.alloc(nU); y32.alloc(nU); b32.alloc(nU); d32.alloc(nU);
+            float *z = x32.p, *p = y32.p, *r = b32.p, *Ap = d32.p;
+            const float *rot = c.rot32.p;
+            launch_map(c, nU, [=] __device__(int64_t i) {
+                z[i] = 0.0f; p[i] = 1e-3f * (float)(i % 13); r[i] = 1.0f; Ap[i] = 0.5f;
+            });
+            Reducer red(c);
+            launch = [red, z, p, r, Ap, rot, nU]() mutable {
+                red.map_reduce(nU, [=] __device__(int64_t i) {
+                    const float al = 1e-6f;
+                    z[i] += al * p[i];
+                    const float rv = r[i] - al * (Ap[i] - 1e-7f * rot[i]);
+                    r[i] = rv;
+                    return (double)rv * (double)rv;
+                }, SC_TMP);
+            };
+            // z, p, r, Ap and the near-null vector read, z and r written (the block partials are
+            // 8 B per 256 rows)
+            *bytes = (double)nU * 7 * 4;
+            break;
+        }
         case STK_KERNEL_FSE: {
             // nel x 7 points of F = R(a) diag(s, 1/s) with a1/a2 = s^2 below the destruction ratio
             const int64_t npt = 7 * c.nel;

@@ -383,8 +383,13 @@ enum {
     STK_KERNEL_GS_BWD32 = 12,   /* backward Gauss-Seidel sweep of the V-cycle with its    */
                                 /* dots; time and bytes per colour launch                 */
     STK_KERNEL_FSE = 13,        /* getFSE (fabric destruction) on nel x 7 synthetic F     */
-    STK_KERNEL_ANISO = 14       /* anisotropic_tensor on the device ellipses of the last  */
+    STK_KERNEL_ANISO = 14,      /* anisotropic_tensor on the device ellipses of the last  */
                                 /* stk_fse and the DEM table (16-node fabric-shape table) */
+    STK_KERNEL_G32 = 15,        /* y = G p (pressure to velocity), FP32                   */
+    STK_KERNEL_GT32 = 16,       /* y = G^T u (velocity to pressure), FP32                 */
+    STK_KERNEL_COARSE32 = 17,   /* dense coarsest-level solve (inverse times b), FP32     */
+    STK_KERNEL_CGUPD32 = 18     /* inner CG update z += a p, r -= a P Ap with |r|^2 (the  */
+                                /* map pass and the final reduction), FP32                */
 };
 
 /* Measurement hook with no counterpart in the reference: runs one of the hot
