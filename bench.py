@@ -87,7 +87,10 @@ def main():
     ctx.set_fields(inp.g, inp.rho, inp.eta_ip, inp.C6)
     t0 = time.perf_counter()
     ctx.assemble(_lib.ASM_FP64 | _lib.ASM_FP32)
-    log(f"assembly + solver set-up {time.perf_counter() - t0:.2f}s")
+    # first assembly of the context: also builds the multigrid operators, the coarsest inverse and
+    # the near-null vector; a time step's re-assembly repeats the same work (timed below)
+    first_setup_ms = (time.perf_counter() - t0) * 1000.0
+    log(f"assembly + solver set-up {first_setup_ms / 1000.0:.2f}s")
     asm_ms, asm_bytes = ctx.time_kernel(_lib.KERNEL_ASSEMBLY, reps=3)
     log(f"element assembly kernels {asm_ms:.2f} ms ({asm_bytes / asm_ms / 1e6:.0f} GB/s compulsory)")
 
@@ -183,6 +186,13 @@ def main():
         log(f"plain FP64: {prep.solve_ms:.1f} ms, {prep.outer_iterations} outer, {prep.inner_iterations} inner, "
             f"residual {prep.full_residual:.2e}; fast-vs-plain errU {eu:.2e} errP {ep:.2e}")
 
+    # per time step the reference re-assembles and refactorises: time one re-assembly with the
+    # solver set-up it triggers (multigrid operators, coarsest inverse, near-null vector)
+    t0 = time.perf_counter()
+    ctx.assemble(_lib.ASM_FP64 | _lib.ASM_FP32)
+    resetup_ms = (time.perf_counter() - t0) * 1000.0
+    log(f"re-assembly with the solver set-up: {resetup_ms:.1f} ms")
+
     # stress post-processor on the same mesh (7-point rule, anisotropic law)
     eta7 = np.asfortranarray(np.repeat(np.asarray(inp.eta_ip)[:, :1], 7, axis=1))
     C7 = [np.asfortranarray(np.repeat(np.asarray(c)[:, :1], 7, axis=1)) for c in inp.C6]
@@ -267,6 +277,7 @@ def main():
         "full_residual": reps[-1].full_residual,
         "schur_residual_last_sweep": reps[-1].schur_residual,
         "assembly_ms": round(asm_ms, 3),
+        "assembly_and_solver_setup_ms": round(resetup_ms, 1),
         "assembly": {"ms": round(asm_ms, 3), "compulsory_GB_per_s": round(asm_bytes / asm_ms / 1e6, 1),
                      "GFLOP_per_s_fp64": round(ASM_FLOP_PER_ELEMENT * m.nel / asm_ms / 1e6, 1),
                      "flop_per_element": ASM_FLOP_PER_ELEMENT},
