@@ -152,6 +152,9 @@ int stk_create(stk_ctx **ctx, int64_t nel, int64_t nnod, int64_t nvert,
 
 void stk_destroy(stk_ctx *ctx);
 
+/* Message of the last failing call on ctx (NULL: of the last failing stk_create).  A call that
+ * succeeds clears it; the pointer stays valid until the next call on ctx, so copy the text
+ * before calling again. */
 const char *stk_last_error(const stk_ctx *ctx);
 
 /* Sizes: out[0..6] = nel, nnod, nvert, nU, nP, colours, multigrid levels. */
