@@ -1,6 +1,6 @@
 """FP64 CPU oracle of the Stokes hot path -- TEST INFRASTRUCTURE ONLY.
 
-This module is the checker for the GPU library.  Only ``tests/``, the smoke
+This module is the oracle of the GPU library.  Only ``tests/``, the smoke
 check and the CPU-baseline leg of ``bench.py`` may import it; the product
 path never calls it.
 

@@ -243,7 +243,7 @@ def test_reference_style_chain():
 
 def test_solve_with_the_installed_dem_tensor():
     """End to end at 64 x 512: simple-shear F -> ellipses -> DEM tensor installed on the device ->
-    assembly -> mixed solve to the judged 1e-8 residual, against the plain FP64 solve."""
+    assembly -> mixed solve to the 1e-8 full-system residual, against the plain FP64 solve."""
     inp, ctx = _ctx(64, 512, 4)
     m = inp.mesh
     T, ax = _dem()

@@ -1,5 +1,5 @@
 """Full-size (256 x 2048, 4.73M dofs) solves: the learned + mixed path that the benchmark times
-agrees with the plain FP64 Schur PCG, and both meet the judged 1e-8 residual.  The oracle cannot
+agrees with the plain FP64 Schur PCG, and both meet the 1e-8 full-system residual.  The oracle cannot
 run at this size; the two GPU methods share only the assembled operators, so their agreement to
 the parity bar (1e-7 relative L2) is the size-independent check."""
 import numpy as np
@@ -37,7 +37,7 @@ def test_learned_mixed_matches_plain_fp64_at_full_size():
 
 def test_plain_fp64_drop_in_at_128x1024():
     """128 x 1024 (1.18M dofs, contrast 1e4, ratio 10): the plain FP64 Schur PCG (the reference's
-    lumped-mass preconditioner) meets the judged 1e-8 residual and agrees with the mixed solve."""
+    lumped-mass preconditioner) meets the 1e-8 full-system residual and agrees with the mixed solve."""
     inp = make_inputs(2)
     m = inp.mesh
     print(f"[case] plain drop-in {m.ncell_r}x{m.ncell_t}", flush=True)
@@ -60,7 +60,7 @@ def test_plain_fp64_drop_in_at_128x1024():
 
 def test_learned_and_mixed_agree_at_the_largest_size():
     """512 x 4096 (18.9M dofs, viscosity contrast 1e6): learned + mixed, mixed without the model
-    and plain FP64 all meet the judged 1e-8 residual; the fast solves agree with plain FP64 to the
+    and plain FP64 all meet the 1e-8 full-system residual; the fast solves agree with plain FP64 to the
     parity bar."""
     inp = make_inputs(5)
     m = inp.mesh
