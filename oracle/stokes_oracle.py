@@ -28,8 +28,18 @@ Pinning (tests/test_oracle_*.py): quadrature exactness and partition of
 unity, exact annulus area, element-matrix symmetry / rank / translation
 null spaces, isotropic-tensor equivalence with the isotropic _fill_Kblk!
 (StokesSolver.jl:688-742), tensor rotation against an independent fourth-order
-tensor rotation, closed-form force sums, a dense LAPACK solve of the whole
-saddle-point system, and golden fixtures.  The reference ships no tests or
+tensor rotation, a dense LAPACK solve of the whole saddle-point system, golden
+fixtures, and closed forms for each piece a misreading could change:
+``rotation_matrix`` maps the node frames onto (theta_hat, r_hat) of
+x = r sin(theta), z = r cos(theta); the body force of a density linear in r
+has the exact half-annulus sums of g rho r_hat; 1^T M 1 = int 1/eta for
+constant and radially varying eta; u^T K_e u equals the viscous energy
+sum_q omega_q eta_q eps:C:eps of a fourth-order tensor for a velocity P2
+holds exactly (fixes where eta15 and eta35 sit in D), and u^T G_e p equals
+sum_q omega_q p div u; ``prepare_matrices`` gives a rigid rotation near-zero
+energy and the field (x, z) divergence 2; ``volume_integral`` integrates
+1, r, r^2 exactly.  The Polak-Ribiere beta is not pinned apart from the
+Fletcher-Reeves form: with exact inner solves the two are equal.  The reference ships no tests or
 vectors and Julia is absent, so nothing is compared against a run of the
 reference itself.
 """

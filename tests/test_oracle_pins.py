@@ -263,3 +263,233 @@ def test_reference_stopping_rule_and_itmin():
     assert info.iterations == 10          # itmin = 10 (IterativeSolvers.jl:97,156)
     _, _, info = so.stokes_pcg(K, MM, G, F, ifree, rtol=1e-30, itmax=7)
     assert info.iterations == 7 and not info.converged   # silent cap (:96)
+
+
+# ----------------------------------------------------------------------------- closed-form pins of the
+# frame, the body force, the 1/eta mass and the placement of the tensor in D.  Each one is fixed by
+# the mathematics (polar geometry, integrals of polynomials in r, the fourth-order tensor contraction),
+# not by a second transcription of the oracle's formulas.
+
+def _polar_to_cartesian(theta, r):
+    """polar2cartesian (src/Grid/mesher.jl:36): x = r sin(theta), z = r cos(theta)."""
+    return r * np.sin(theta), r * np.cos(theta)
+
+
+def test_rotation_matrix_maps_node_frames_to_the_polar_unit_vectors():
+    # rotation_matrix (StokesSolver.jl:203-245): U_cart = TT U with U = (u_theta, u_r) per node.
+    # With x = r sin(theta), z = r cos(theta): r_hat = (x, z) / r and theta_hat = d(x, z)/dtheta / r.
+    m = build_annulus(4, 16)
+    TT = so.rotation_matrix(m.theta)
+    x, z = _polar_to_cartesian(m.theta, m.r)
+    radial = np.zeros(m.nU)
+    radial[1::2] = 1.0
+    u = TT @ radial
+    assert np.allclose(u[0::2], x / m.r, atol=1e-15) and np.allclose(u[1::2], z / m.r, atol=1e-15)
+    tangential = np.zeros(m.nU)
+    tangential[0::2] = 1.0
+    u = TT @ tangential
+    assert np.allclose(u[0::2], z / m.r, atol=1e-15) and np.allclose(u[1::2], -x / m.r, atol=1e-15)
+    # a rigid rotation u = omega x r = (z, -x) has u_theta = r and u_r = 0 in the node frames
+    rot = np.zeros(m.nU)
+    rot[0::2], rot[1::2] = z, -x
+    back = TT.T @ rot
+    assert np.allclose(back[0::2], m.r, atol=1e-14) and np.allclose(back[1::2], 0.0, atol=1e-14)
+
+
+def _element_centroid_theta(m):
+    return np.mod(m.el_theta.mean(axis=0), 2 * np.pi)
+
+
+def test_body_force_is_radial_buoyancy_with_linear_density():
+    # _fill_Fbblk! + _element_density (StokesSolver.jl:838-845, :1017-1022): f = g rho r_hat with rho
+    # interpolated linearly from the vertex values.  rho = a + b r is held exactly by that
+    # interpolation (elements are affine in (theta, r)), so over the half-annulus x > 0
+    #   sum f_x = g int_0^pi int (a + b r) sin(theta) r dr dtheta = 2 g [a (ro^2 - ri^2)/2 + b (ro^3 - ri^3)/3]
+    # and the same for f_z over z > 0; the other half-plane sums are the negatives.  A force along
+    # theta_hat, or sin and cos exchanged, gives zero for these sums.
+    inp = _small_inputs(8, 32)
+    m = inp.mesh
+    a, b, g = 0.7, 1.9, 2.5
+    rho = a + b * m.r
+    inp = StokesInputs(m, inp.workload, inp.T, rho, inp.eta_ip, inp.C6, inp.phi, inp.delta, g=g)
+    _, _, _, fe = so.element_matrices(inp)
+    want = 2 * g * (a * (R_OUT ** 2 - R_IN ** 2) / 2 + b * (R_OUT ** 3 - R_IN ** 3) / 3)
+    th_c = _element_centroid_theta(m)
+    fx, fz = fe[:, 0::2].sum(1), fe[:, 1::2].sum(1)
+    right = th_c < np.pi                                   # x > 0
+    top = (th_c < np.pi / 2) | (th_c > 3 * np.pi / 2)      # z > 0
+    # the rule is exact for polynomials; sin and cos over a cell of width 2 pi / 32 leave ~1e-7
+    assert fx[right].sum() == pytest.approx(want, rel=1e-6)
+    assert fx[~right].sum() == pytest.approx(-want, rel=1e-6)
+    assert fz[top].sum() == pytest.approx(want, rel=1e-6)
+    assert fz[~top].sum() == pytest.approx(-want, rel=1e-6)
+    # in the node frames the force has a positive radial sum and no net tangential part (the 2-triangle
+    # split is not mirror-symmetric, so the tangential sum is small rather than zero: 1.6e-10 here)
+    KK, GG, MM, Fb = so.assembly_stokes_cylindric(inp)
+    F = so.rotation_matrix(m.theta).T @ Fb
+    total = g * 2 * np.pi * (a * (R_OUT ** 2 - R_IN ** 2) / 2 + b * (R_OUT ** 3 - R_IN ** 3) / 3)
+    assert F[1::2].sum() == pytest.approx(total, rel=2e-2)        # int g rho dOmega, up to cos(dtheta)
+    assert abs(F[0::2].sum()) < 1e-6 * np.abs(F).sum()
+
+
+@pytest.mark.parametrize("kind", ["constant", "radial"])
+def test_pressure_mass_scales_with_inverse_viscosity(kind):
+    # _fill_Mblk! (StokesSolver.jl:825-836): M_e = sum_q omega_q / eta_q NP NP^T, so 1^T M 1 = int 1/eta.
+    # constant eta = 3.7: area / 3.7.  eta_q = 1 / (1 + r_q): int (1 + r) r dr dtheta, a polynomial the
+    # 6-point rule integrates exactly (this also pins which point each eta.ip column belongs to).
+    inp = _small_inputs(8, 32)
+    m = inp.mesh
+    x_ip, _ = ip_triangle6()
+    if kind == "constant":
+        eta = np.full((m.nel, 6), 3.7, order="F")
+        want = np.pi * (R_OUT ** 2 - R_IN ** 2) / 3.7
+    else:
+        lam = np.stack([1 - x_ip[:, 0] - x_ip[:, 1], x_ip[:, 0], x_ip[:, 1]])   # barycentric (3, 6)
+        rq = (m.el_r.T @ lam)                                                    # (nel, 6)
+        eta = np.asfortranarray(1.0 / (1.0 + rq))
+        want = 2 * np.pi * ((R_OUT ** 2 - R_IN ** 2) / 2 + (R_OUT ** 3 - R_IN ** 3) / 3)
+    inp = StokesInputs(m, inp.workload, inp.T, inp.rho, eta, inp.C6, inp.phi, inp.delta)
+    _, _, Me, _ = so.element_matrices(inp)
+    assert Me.sum() == pytest.approx(want, rel=1e-13)
+
+
+def _fabric_fourth_order(a, c, e, h):
+    """Orthotropic 2-D viscous tensor in its own frame (indices 0 = x, 1 = z)."""
+    Cf = np.zeros((2, 2, 2, 2))
+    Cf[0, 0, 0, 0], Cf[1, 1, 1, 1] = a, c
+    Cf[0, 0, 1, 1] = Cf[1, 1, 0, 0] = h
+    for idx in [(0, 1, 0, 1), (0, 1, 1, 0), (1, 0, 0, 1), (1, 0, 1, 0)]:
+        Cf[idx] = e
+    return Cf
+
+
+def _quadratic_field(rng):
+    """u_x, u_z quadratic in (theta, r): held exactly by P2 on elements affine in (theta, r)."""
+    cx, cz = rng.standard_normal(6), rng.standard_normal(6)
+
+    def val(cf, t, r):
+        return cf[0] + cf[1] * t + cf[2] * r + cf[3] * t * r + cf[4] * t * t + cf[5] * r * r
+
+    def d_t(cf, t, r):
+        return cf[1] + cf[3] * r + 2 * cf[4] * t
+
+    def d_r(cf, t, r):
+        return cf[2] + cf[3] * t + 2 * cf[5] * r
+
+    return cx, cz, val, d_t, d_r
+
+
+def _element_nodes_polar(m):
+    """(theta, r) of the six element nodes from the unwrapped vertex angles, (6, nel)."""
+    th, r = m.el_theta, m.el_r
+    mid = [(0, 1), (1, 2), (2, 0)]
+    tn = np.concatenate([th, np.stack([0.5 * (th[i] + th[j]) for i, j in mid])])
+    rn = np.concatenate([r, np.stack([0.5 * (r[i] + r[j]) for i, j in mid])])
+    return tn, rn
+
+
+def _points_polar(m, x_ip):
+    lam = np.stack([1 - x_ip[:, 0] - x_ip[:, 1], x_ip[:, 0], x_ip[:, 1]])     # (3, nq)
+    return m.el_theta.T @ lam, m.el_r.T @ lam                                   # (nel, nq)
+
+
+def _cartesian_gradient(cf, t, r, d_t, d_r):
+    """Chain rule for x = r sin t, z = r cos t: d/dx = sin t d/dr + cos t / r d/dt,
+    d/dz = cos t d/dr - sin t / r d/dt."""
+    gt, gr = d_t(cf, t, r), d_r(cf, t, r)
+    return np.sin(t) * gr + np.cos(t) / r * gt, np.cos(t) * gr - np.sin(t) / r * gt
+
+
+def _point_weights(m, w_ip):
+    """omega_q = r_q |det d(theta, r)/d(xi, eta)| w_q: the polar area element r dr dtheta."""
+    th, r = m.el_theta, m.el_r
+    det = (th[1] - th[0]) * (r[2] - r[0]) - (th[2] - th[0]) * (r[1] - r[0])
+    return np.abs(det)[:, None] * w_ip[None, :]
+
+
+def test_velocity_block_is_the_viscous_energy_of_the_rotated_tensor():
+    # _fill_Kblk!{Anisotropic} (src/src/Stokes/StokesSolver.jl:744-808): u^T K_e u must equal
+    #   sum_q omega_q eta_q eps_ij C_ijkl eps_kl
+    # with C the fourth-order tensor whose Voigt entries (xx, zz, xz) are eta11 = C_xxxx,
+    # eta33 = C_zzzz, eta55 = C_xzxz, eta13 = C_xxzz, eta15 = C_xxxz, eta35 = C_zzxz (the naming that
+    # directRotation2D's outputs carry, pinned above).  The velocity is quadratic in (theta, r), so
+    # P2 holds it exactly and its Cartesian strain rate follows from the chain rule at each point.
+    # eta15 != eta35 for a rotated tensor with C11 != C33, so exchanging them in D fails here.
+    rng = np.random.default_rng(7)
+    inp = _small_inputs(4, 16)
+    m = inp.mesh
+    nel = m.nel
+    x_ip, w_ip = ip_triangle6()
+    Cf = _fabric_fourth_order(4 / 3, 0.6, 0.3, -0.5)
+    phi = rng.uniform(0, np.pi, nel)
+    R = np.stack([np.stack([np.cos(phi), -np.sin(phi)], -1), np.stack([np.sin(phi), np.cos(phi)], -1)], -2)
+    C4 = np.einsum("eia,ejb,ekc,eld,abcd->eijkl", R, R, R, R, Cf)
+    comps = [C4[:, 0, 0, 0, 0], C4[:, 1, 1, 1, 1], C4[:, 0, 1, 0, 1], C4[:, 0, 0, 1, 1], C4[:, 0, 0, 0, 1],
+             C4[:, 1, 1, 0, 1]]
+    assert np.abs(comps[4] - comps[5]).min() > 1e-3 or np.abs(comps[4] - comps[5]).max() > 0.1
+    C6 = tuple(np.asfortranarray(np.repeat(c[:, None], 6, axis=1)) for c in comps)
+    eta = np.asfortranarray(0.5 + rng.random((nel, 6)))
+    inp = StokesInputs(m, inp.workload, inp.T, inp.rho, eta, C6, phi, inp.delta)
+    Ke, Ge, _, _ = so.element_matrices(inp)
+
+    cx, cz, val, d_t, d_r = _quadratic_field(rng)
+    tn, rn = _element_nodes_polar(m)
+    ue = np.zeros((nel, 12))
+    ue[:, 0::2], ue[:, 1::2] = val(cx, tn, rn).T, val(cz, tn, rn).T
+    tq, rq = _points_polar(m, x_ip)
+    uxx, uxz = _cartesian_gradient(cx, tq, rq, d_t, d_r)
+    uzx, uzz = _cartesian_gradient(cz, tq, rq, d_t, d_r)
+    eps = np.empty((nel, 6, 2, 2))
+    eps[..., 0, 0], eps[..., 1, 1] = uxx, uzz
+    eps[..., 0, 1] = eps[..., 1, 0] = 0.5 * (uxz + uzx)
+    dens = np.einsum("eqij,eijkl,eqkl->eq", eps, C4, eps)
+    omega = rq * _point_weights(m, w_ip)
+    want = (omega * eta * dens).sum(1)
+    got = np.einsum("ea,eab,eb->e", ue, Ke, ue)
+    assert np.allclose(got, want, rtol=1e-11, atol=1e-12 * np.abs(want).max())
+
+    # _fill_Gblk! (:812-823): u^T G_e p = sum_q omega_q p_q div u, for p linear (P1) and u as above
+    pv = rng.standard_normal((3, nel))
+    lam = np.stack([1 - x_ip[:, 0] - x_ip[:, 1], x_ip[:, 0], x_ip[:, 1]])
+    pq = pv.T @ lam
+    want = (omega * pq * (uxx + uzz)).sum(1)
+    got = np.einsum("ea,eai,ie->e", ue, Ge, pv)
+    assert np.allclose(got, want, rtol=1e-11, atol=1e-12 * np.abs(want).max())
+
+
+def test_prepare_matrices_act_in_the_node_frames():
+    # _prepare_matrices (StokesSolver.jl:192-201) rotates K, G and F into the (u_theta, u_r) frames of
+    # the nodes.  In those frames:
+    #  * u = (r, 0) per node is the rigid rotation (z, -x): its energy u^T K u is at the level of K's
+    #    near-null rotation mode, while a field of the same size that is not rigid is not;
+    #  * u = (0, r) per node is the field (x, z), whose divergence is 2: sum_i (G^T u)_i = int div u = 2 area
+    #    (P2 holds (x, z) only approximately on the curved cells, hence the tolerance);
+    #  * the body force g rho r_hat of a uniform density has a positive radial sum ~ g rho area and a tangential
+    #    sum near zero.
+    inp = _small_inputs(8, 64)
+    m = inp.mesh
+    inp = StokesInputs(m, inp.workload, inp.T, np.ones_like(inp.rho), inp.eta_ip, inp.C6, inp.phi, inp.delta)
+    KK, GG, MM, Fb = so.assembly_stokes_cylindric(inp)
+    K, G, F = so.prepare_matrices(KK, GG, Fb, so.rotation_matrix(m.theta))
+    area = np.pi * (R_OUT ** 2 - R_IN ** 2)
+    rigid = np.zeros(m.nU)
+    rigid[0::2] = m.r
+    shear = np.zeros(m.nU)
+    shear[0::2] = m.r * np.cos(2 * m.theta)
+    e_rigid, e_shear = rigid @ (K @ rigid), shear @ (K @ shear)
+    assert e_shear > 0 and abs(e_rigid) < 1e-6 * e_shear
+    radial = np.zeros(m.nU)
+    radial[1::2] = m.r
+    assert (G.T @ radial).sum() == pytest.approx(2 * area, rel=1e-3)
+    assert F[1::2].sum() == pytest.approx(inp.g * area, rel=1e-2)
+    assert abs(F[0::2].sum()) < 1e-6 * np.abs(F).sum()
+
+
+def test_volume_integral_of_polynomials_in_r_is_exact():
+    # volume_integral (src/Stress/FSE.jl:412-464): int V dOmega with the polar weight r detJ w_q and P2
+    # interpolation of the nodal values.  P2 holds 1, r and r^2 exactly on elements affine in (theta, r),
+    # and the integrands r^k r are polynomials the 6-point rule integrates exactly.
+    m = build_annulus(8, 32)
+    for k in range(3):
+        want = 2 * np.pi * (R_OUT ** (k + 2) - R_IN ** (k + 2)) / (k + 2)
+        assert so.volume_integral(m.r ** k, m) == pytest.approx(want, rel=1e-13)
