@@ -206,8 +206,11 @@ class Context:
         self.nel, self.nnod, self.nvert, self.nU, self.nP, self.ncolours, self.mg_levels = (int(x) for x in info)
 
     @classmethod
-    def from_mesh(cls, mesh):
-        fixed, _ = mesh.free_slip()
+    def from_mesh(cls, mesh, fixed=None):
+        """Context of a mesh with the 0-based fixed dofs ``fixed`` (default: free slip)."""
+        if fixed is None:
+            fixed, _ = mesh.free_slip()
+        fixed = np.asarray(fixed, dtype=np.int64)
         return cls(mesh.nel, mesh.nnod, mesh.nvert, mesh.e2n + 1, mesh.el_theta, mesh.el_r, mesh.theta, mesh.r,
                    fixed + 1)
 

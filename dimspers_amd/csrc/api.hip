@@ -96,6 +96,8 @@ int stk_create(stk_ctx **out, int64_t nel, int64_t nnod, int64_t nvert, const in
         STK_CHECK(nel > 0 && nnod > 0 && nvert > 0 && nvert <= nnod, STK_E_SHAPE, "bad mesh sizes");
         STK_CHECK(nnod < (int64_t(1) << 30), STK_E_SHAPE, "mesh too large for 32-bit node ids");
         STK_CHECK(e2n && el_theta && el_r && node_theta && node_r, STK_E_ARG, "NULL mesh array");
+        STK_CHECK(nfixed >= 0, STK_E_ARG, "negative number of fixed dofs");
+        STK_CHECK(nfixed == 0 || fixed_dofs, STK_E_ARG, "fixed_dofs is NULL");
         STK_HIP(hipGetDevice(&c.device));
         STK_HIP(hipStreamCreate(&c.stream));
         c.nel = nel; c.nnod = nnod; c.nvert = nvert; c.nU = 2 * nnod; c.nP = nvert;
@@ -124,7 +126,20 @@ int stk_create(stk_ctx **out, int64_t nel, int64_t nnod, int64_t nvert, const in
         build_colouring(c);
         build_slot_maps(c);
         detect_structured(c);
-        build_mg_structure(c);
+        // The rigid rotation (u_theta = r, u_r = 0) satisfies the boundary conditions, and is then
+        // a near-null mode of K, exactly when no tangential dof is fixed.
+        c.rotation_free = true;
+        for (int64_t d : c.fixed_list)
+            if (d % 2 == 0) c.rotation_free = false;
+        // The multigrid's coarse constraints are those of free slip on both walls (the radial dof
+        // of every node on the smallest and largest radius, free_slip in
+        // src/Setup/boundary_conditions.jl:27-37); other fixed sets solve without it.
+        if (!c.structured)
+            c.mg_note = "the multigrid needs the structured 2-triangle annulus mesh";
+        else if (!fixed_set_is_free_slip(c))
+            c.mg_note = "the multigrid needs free slip on both walls (the fixed set differs)";
+        else
+            build_mg_structure(c);
         if (!c.mg.empty()) {
             build_gs_layout(c);
             c.gs.on = true;
@@ -147,7 +162,6 @@ int stk_create(stk_ctx **out, int64_t nel, int64_t nnod, int64_t nvert, const in
         c.d_sc.alloc(SC_COUNT);
         c.d_sc.zero(c.stream);
         STK_HIP(hipHostMalloc(&c.h_pinned, SC_COUNT * sizeof(double), hipHostMallocDefault));
-        upload_quadrature(c.stream);
         STK_HIP(hipStreamSynchronize(c.stream));
         return STK_OK;
     });
@@ -465,7 +479,7 @@ int stk_get_solution(stk_ctx *ctx, double *U, double *P) {
 }
 
 int stk_time_kernel(stk_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms, double *bytes) {
-    if (!ctx || !avg_ms || !bytes) return STK_E_ARG;
+    if (!ctx || !avg_ms || !bytes || reps <= 0) return STK_E_ARG;
     return guarded(ctx, [&]() {
         Context &c = ctx->c;
         const bool standalone = kernel == STK_KERNEL_FSE || kernel == STK_KERNEL_ANISO || kernel == STK_KERNEL_COPY;
@@ -473,9 +487,8 @@ int stk_time_kernel(stk_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms, 
         if (kernel == STK_KERNEL_ASSEMBLY) {
             STK_CHECK(c.have_fields && c.K64, STK_E_STATE, "FP64 assembly needs fields");
             const int flags = c.asm_flags;
-            hipEvent_t e0, e1;
-            STK_HIP(hipEventCreate(&e0));
-            STK_HIP(hipEventCreate(&e1));
+            EventPair ev_;
+            hipEvent_t e0 = ev_.a, e1 = ev_.b;
             // element kernels only (all colours), into the FP64 operators of the last assembly
             const bool mg = c.mg_built;
             STK_HIP(hipEventRecord(e0, c.stream));
@@ -487,8 +500,6 @@ int stk_time_kernel(stk_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms, 
             STK_HIP(hipEventSynchronize(e1));
             float ms = 0;
             STK_HIP(hipEventElapsedTime(&ms, e0, e1));
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
             *avg_ms = ms / reps;
             // compulsory bytes: per-element inputs, per-node inputs, one write of every operator
             const double in = (double)c.nel * (6 * 4 + 6 * 8 + 6 * 8 + 36 * 8 + 81) + (double)c.nnod * (8 + 16);

@@ -27,10 +27,10 @@
 
 namespace stk {
 
+// Integration rule of the element assembly, passed by value in every launch.
 struct QuadTables {
     double xi[6], eta[6], w[6];
 };
-__constant__ QuadTables c_quad;
 
 namespace {
 
@@ -88,6 +88,7 @@ struct AsmArgs {
     T *A1, *E1;                         // A1: BSR 2x2 over the M pattern; E1: 36 x nel by element id
     T *RK;                              // R K: BSR 2x2 over the G^T pattern (null: not built)
     int cart, bc, iso;
+    QuadTables quad;
 };
 
 // Integrates one element with the 6-point rule (StokesSolver.jl:543-845): the lower
@@ -96,7 +97,8 @@ struct AsmArgs {
 // eta_at(q) gives eta.ip at integration point q, tensor_at(q, d) the six viscous
 // tensor components (eta11, eta33, eta55, eta13, eta15, eta35) there.
 template <typename T, typename EtaF, typename TensF>
-__device__ __forceinline__ void integrate_element(const T th[3], const T rr[3], const T rhov[3], T g, bool iso,
+__device__ __forceinline__ void integrate_element(const QuadTables &qt, const T th[3], const T rr[3], const T rhov[3],
+                                                  T g, bool iso,
                                                   EtaF eta_at, TensF tensor_at, T (*sK)[kAsmThreads],
                                                   T (*sM)[kAsmThreads], int tid, T sG[36], T sF[12]) {
     // _fill_R_J!
@@ -111,7 +113,7 @@ __device__ __forceinline__ void integrate_element(const T th[3], const T rr[3], 
     for (int k = 0; k < 12; ++k) sF[k] = T(0);
 
     for (int q = 0; q < 6; ++q) {
-        const T xi = (T)c_quad.xi[q], et = (T)c_quad.eta[q];
+        const T xi = (T)qt.xi[q], et = (T)qt.eta[q];
         T N[6], dr[6], ds[6];
         p2_shape<T>(xi, et, N, dr, ds);
         const T NP[3] = {T(1) - xi - et, xi, et};
@@ -125,7 +127,7 @@ __device__ __forceinline__ void integrate_element(const T th[3], const T rr[3], 
         const T jx1 = (-R21 * cs + Th21 * sn * rq) * inv;
         const T jz0 = (-R31 * sn - Th31 * cs * rq) * inv;
         const T jz1 = (R21 * sn + Th21 * cs * rq) * inv;
-        const T om = rq * detJ * (T)c_quad.w[q];
+        const T om = rq * detJ * (T)qt.w[q];
         T dx[6], dz[6];
 #pragma unroll
         for (int b = 0; b < 6; ++b) {
@@ -216,7 +218,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble(AsmArgs<T> a) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) d[k] = (T)C[k * cstr];
     };
-    integrate_element<T>(th, rr, rhov, (T)a.g, a.iso != 0, eta_at, tensor_at, sK, sM, tid, sG, sF);
+    integrate_element<T>(a.quad, th, rr, rhov, (T)a.g, a.iso != 0, eta_at, tensor_at, sK, sM, tid, sG, sF);
 
     // rotate to the node frames: blk(a,b) <- Q_a blk Q_b^T, Q = [[c, -s], [s, c]]
     if (!a.cart) {
@@ -533,6 +535,7 @@ struct SchurSampleArgs {
     float *Tg;                   // 6 x nsamp log-Cholesky targets (mode 1, may be null)
     double *S;                   // 6 x nsamp canonical S_e / s_e (may be null)
     double *scale;               // s_e = area / eta_bar (may be null)
+    QuadTables quad;
 };
 
 // Features (and, for training, exact targets) of the learned local-Schur model:
@@ -574,7 +577,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_schur_samples(SchurSampleArgs a
     auto eta_at = [&](int q) { return a.eta[(int64_t)q * a.nel + e] * fscale; };
     auto tensor_at = [&](int, double d[6]) { for (int k = 0; k < 6; ++k) d[k] = tens[k]; };
     double sG[36], sF[12];
-    integrate_element<double>(th, rr, rhov, 0.0, false, eta_at, tensor_at, sK, sM, tid, sG, sF);
+    integrate_element<double>(a.quad, th, rr, rhov, 0.0, false, eta_at, tensor_at, sK, sM, tid, sG, sF);
     const double detJ = (th[1] - th[0]) * (rr[2] - rr[0]) - (th[2] - th[0]) * (rr[1] - rr[0]);
     const double area = detJ * (rr[0] + rr[1] + rr[2]) / 6.0;
     const double thc = (th[0] + th[1] + th[2]) / 3.0;
@@ -811,6 +814,7 @@ __global__ void k_fix_diag(int64_t nfix, const int64_t *__restrict__ dofs, const
 
 void launch_schur_samples(Context &c, const SchurSampleHost &h) {
     SchurSampleArgs a{};
+    a.quad = make_quad();
     a.nel = c.nel; a.nsamp = h.nsamp; a.per_snap = h.per_snap; a.first_snap = h.first_snap;
     a.mode = h.mode; a.ratio_kind = h.ratio_kind; a.mean_area = c.mean_area;
     a.iperm = c.d_iperm.p; a.th = c.d_elth.p; a.r = c.d_elr.p; a.eta = c.d_eta.p;
@@ -834,18 +838,10 @@ void launch_learned_scatter(Context &c, const float *Y, const double *scale, dou
     }
 }
 
-void upload_quadrature(hipStream_t s) {
-    static bool done = false;
-    if (done) return;
-    const QuadTables q = make_quad();
-    STK_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_quad), &q, sizeof(q), 0, hipMemcpyHostToDevice, s));
-    STK_HIP(hipStreamSynchronize(s));
-    done = true;
-}
-
 template <typename T>
 void assemble_impl(Context &c, int flags, T *K, T *G, T *GT, T *M, T *F, T *A1, T *E1, T *RK) {
     AsmArgs<T> a{};
+    a.quad = make_quad();
     a.nel = c.nel;
     a.perm = c.d_perm.p; a.e2n = c.d_e2n.p; a.th = c.d_elth.p; a.r = c.d_elr.p;
     a.nodecs = c.d_nodecs.p; a.fixed = c.d_fixed.p;

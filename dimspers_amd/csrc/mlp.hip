@@ -459,9 +459,8 @@ void mlp_train(Context &c, const stk_train_opts &o, stk_train_report &rep) {
     STK_CHECK(o.snapshots > 0 && o.per_snapshot > 0 && o.epochs > 0 && o.batch >= 32, STK_E_ARG, "bad training options");
     if (!c.mlp) c.mlp = new MLP();
     MLP &net = *c.mlp;
-    hipEvent_t e0, e1;
-    STK_HIP(hipEventCreate(&e0));
-    STK_HIP(hipEventCreate(&e1));
+    EventPair ev_;
+    hipEvent_t e0 = ev_.a, e1 = ev_.b;
     STK_HIP(hipEventRecord(e0, c.stream));
     net.init(o.seed, c.stream);
     const int64_t ns = (int64_t)o.snapshots * o.per_snapshot;
@@ -525,8 +524,6 @@ void mlp_train(Context &c, const stk_train_opts &o, stk_train_report &rep) {
     STK_HIP(hipEventSynchronize(e1));
     float ms = 0;
     STK_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     net.trained = true;
     rep.initial_loss = first;
     rep.final_loss = last;

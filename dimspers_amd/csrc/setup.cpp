@@ -393,6 +393,19 @@ void detect_structured(Context &c) {
     c.structured = true;
 }
 
+// True when the fixed dofs are exactly the radial dofs of the nodes on the smallest and the
+// largest node radius (point_ids, src/Grid/mesher.jl:419-441, compares radii for equality).
+bool fixed_set_is_free_slip(const Context &c) {
+    if (c.nnod == 0) return false;
+    const auto mm = std::minmax_element(c.node_r.begin(), c.node_r.end());
+    const double rmin = *mm.first, rmax = *mm.second;
+    for (int64_t n = 0; n < c.nnod; ++n) {
+        const bool wall = c.node_r[n] == rmin || c.node_r[n] == rmax;
+        if (c.fixed[2 * n] != 0 || (c.fixed[2 * n + 1] != 0) != wall) return false;
+    }
+    return true;
+}
+
 namespace {
 
 // Local vertex lattice coordinates (i, j) of the 3 vertices of element type t in cell (I, J).

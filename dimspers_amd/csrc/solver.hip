@@ -923,7 +923,9 @@ void setup_solver_data(Context &c) {
         T *rp = rot.p;
         double s2 = 0;
         for (double r : c.node_r) s2 += r * r;
-        const double inv = 1.0 / std::sqrt(s2);
+        // a fixed tangential dof pins the rotation: K has no near-null mode, the vector is zero
+        // and every projection on it is the identity
+        const double inv = c.rotation_free ? 1.0 / std::sqrt(s2) : 0.0;
         launch_map(c, c.nU, [=] __device__(int64_t i) {
             rp[i] = (i % 2 == 0 && !fx[i]) ? (T)(nr[i / 2] * inv) : T(0);
         });
@@ -963,7 +965,9 @@ void setup_solver_data(Context &c) {
     // Near-null vector: the nodal rigid rotation z0 corrected by one solve on its
     // complement, v = z0 - Pi K^-1 Pi K z0 (Pi = I - z0 z0^T), which is the lowest
     // eigenvector of K to rounding at the mesh sizes where it can be checked.
-    if (!f64 && c.rot64) {
+    if (!c.rotation_free) {
+        if (f64) c.null_lambda = 0.0;
+    } else if (!f64 && c.rot64) {
         convert_vec(c, c.rot64.p, rot.p, c.nU);
     } else {
         // Jacobi-Davidson steps v <- v - P (P K P)^-1 P K v, quadratically convergent
@@ -1257,16 +1261,17 @@ constexpr double kSweepFloor = 1e-5;
 int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
     std::memset(&rep, 0, sizeof(rep));
     const bool use_mg = (o.inner == STK_INNER_MG) && c.mg_built && !c.loaded;
-    if (o.inner == STK_INNER_MG && !use_mg) throw Error(STK_E_NOMESH, "multigrid not available for this system");
+    if (o.inner == STK_INNER_MG && !use_mg)
+        throw Error(STK_E_NOMESH, "multigrid not available for this system" + (c.mg_note.empty() ? std::string()
+                                                                                             : ": " + c.mg_note));
     const int64_t nU = c.nU, nP = c.nP;
     c.U.alloc(nU);
     c.P.alloc(nP);
     c.U.zero(c.stream);
     c.P.zero(c.stream);
     c.work64.alloc(std::max(nU, nP));
-    hipEvent_t e0, e1;
-    STK_HIP(hipEventCreate(&e0));
-    STK_HIP(hipEventCreate(&e1));
+    EventPair ev_;
+    hipEvent_t e0 = ev_.a, e1 = ev_.b;
     STK_HIP(hipEventRecord(e0, c.stream));
     // masked right-hand side
     DBuf<double> Fm(nU);
@@ -1282,7 +1287,8 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
     const bool deflate = (o.null_mode == STK_NULL_DEFLATE);
     const bool project = !deflate;
     if (deflate && (o.method != STK_SOLVE_PLAIN || !(c.null_lambda > 0)))
-        throw Error(STK_E_ARG, "null-space deflation needs the plain FP64 method");
+        throw Error(STK_E_ARG, "null-space deflation needs the plain FP64 method and a near-null rotation "
+                               "(no fixed tangential dof)");
     if (o.method == STK_SOLVE_PLAIN) {
         SchurWork<double> w = schur_work<double>(c);
         bool conv;
@@ -1423,8 +1429,6 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
     float ms = 0;
     STK_HIP(hipEventElapsedTime(&ms, e0, e1));
     rep.solve_ms = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (!rep.converged) status = STK_E_NOTCONV;
     return status;
 }
@@ -1639,9 +1643,8 @@ void thermal_solve(Context &c, double *T, const double *rho, const ThermalStep &
                    const double *vfix, int64_t nfix, stk_report &rep) {
     std::memset(&rep, 0, sizeof(rep));
     const int64_t nn = c.nnod, ns = c.KN.nstore();
-    hipEvent_t e0, e1;
-    STK_HIP(hipEventCreate(&e0));
-    STK_HIP(hipEventCreate(&e1));
+    EventPair ev_;
+    hipEvent_t e0 = ev_.a, e1 = ev_.b;
     STK_HIP(hipEventRecord(e0, c.stream));
     c.KT.alloc(ns); c.MT.alloc(ns); c.AT.alloc(ns); c.FT.alloc(nn);
     c.KT.zero(c.stream); c.MT.zero(c.stream); c.FT.zero(c.stream);
@@ -1744,8 +1747,6 @@ void thermal_solve(Context &c, double *T, const double *rho, const ThermalStep &
     STK_HIP(hipEventSynchronize(e1));
     float ms = 0;
     STK_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     rep.converged = conv ? 1 : 0;
     rep.outer_iterations = it;
     rep.inner_iterations = it;
@@ -1761,9 +1762,8 @@ namespace stk {
 
 double time_kernel(Context &c, int kernel, int reps, double *bytes) {
     STK_CHECK(reps > 0, STK_E_ARG, "reps must be positive");
-    hipEvent_t e0, e1;
-    STK_HIP(hipEventCreate(&e0));
-    STK_HIP(hipEventCreate(&e1));
+    EventPair ev_;
+    hipEvent_t e0 = ev_.a, e1 = ev_.b;
     // compulsory bytes: structural blocks only (four values and a 32-bit column each), the
     // slice pointers (8 bytes per 64 rows), x read once, y written once
     const int64_t nU = c.nU, nb = c.KN.nnz(), nsl = (c.KN.nrows + Pattern::kSlice - 1) / Pattern::kSlice;
@@ -2044,8 +2044,6 @@ double time_kernel(Context &c, int kernel, int reps, double *bytes) {
     STK_HIP(hipEventSynchronize(e1));
     float ms = 0;
     STK_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (kernel == STK_KERNEL_INNER_IT32) return ms / reps / 20;
     if (kernel == STK_KERNEL_GS_BWD32) return ms / reps / c.gs.ncolour;
     return ms / reps;

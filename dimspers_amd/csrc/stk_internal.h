@@ -31,6 +31,25 @@ struct Error : std::runtime_error {
         if (!(cond)) throw ::stk::Error((code), (msg));      \
     } while (0)
 
+// Two HIP events destroyed on scope exit, also when a call between them throws.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() {
+        STK_HIP(hipEventCreate(&a));
+        const hipError_t e = hipEventCreate(&b);
+        if (e != hipSuccess) {
+            (void)hipEventDestroy(a);
+            throw Error(STK_E_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
+        }
+    }
+    ~EventPair() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    EventPair(const EventPair &) = delete;
+    EventPair &operator=(const EventPair &) = delete;
+};
+
 // Owning device buffer.
 template <typename T>
 struct DBuf {
@@ -228,6 +247,8 @@ struct Context {
 
     // multigrid
     bool structured = false;
+    bool rotation_free = true;                // no tangential dof fixed: the rotation is admissible
+    std::string mg_note;                      // why the multigrid is not built (when it is not)
     int32_t ncell_r = 0, ncell_t = 0;
     std::vector<MGLevel> mg;                  // mg[0] = P1 on the original mesh
     bool mg_built = false;
@@ -290,10 +311,10 @@ void build_colouring(Context &c);
 void build_slot_maps(Context &c);
 void build_gs_layout(Context &c);
 void detect_structured(Context &c);
+bool fixed_set_is_free_slip(const Context &c);
 void build_mg_structure(Context &c);
 
 // ---------------------------------------------------------------- kernels (assembly.hip)
-void upload_quadrature(hipStream_t s);
 struct SchurSampleHost {
     int64_t nsamp = 0, per_snap = 1, first_snap = 0;
     int mode = 0, ratio_kind = 1;

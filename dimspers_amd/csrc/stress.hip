@@ -25,11 +25,12 @@ namespace stk {
 
 namespace {
 
+// Points of the stress rule, passed by value in every launch: the rule belongs to the
+// context (stk_set_stress_fields), not to the process or the device.
 struct StressRule {
     int nip;
     double xi[7], eta[7];
 };
-__constant__ StressRule c_srule;
 
 // Points of ip_triangle6 / ip_triangle7 (src/Algebra/Quadrature.jl:80-138), in their order.
 StressRule make_rule(int nip) {
@@ -83,6 +84,7 @@ struct StressArgs {
     const double *eta;              // nel x nip
     const double *C;                // 6 components x (nel x nip), or null (isotropic)
     double *out;                    // 8 x (nel x nip): exx, ezz, exz, eII, txx, tzz, txz, tII
+    StressRule rule;
 };
 
 __global__ __launch_bounds__(256) void k_stress(StressArgs a) {
@@ -98,11 +100,11 @@ __global__ __launch_bounds__(256) void k_stress(StressArgs a) {
         ux[b] = a.U[2 * n];
         uz[b] = a.U[2 * n + 1];
     }
-    const int nip = c_srule.nip;
+    const int nip = a.rule.nip;
     const int64_t plane = nel * nip;
     for (int q = 0; q < nip; ++q) {
         double dx[6], dz[6];
-        cartesian_gradients(th, r, c_srule.xi[q], c_srule.eta[q], dx, dz);
+        cartesian_gradients(th, r, a.rule.xi[q], a.rule.eta[q], dx, dz);
         double exx = 0, ezz = 0, duxdz = 0, duzdx = 0;
 #pragma unroll
         for (int b = 0; b < 6; ++b) {
@@ -148,6 +150,7 @@ struct DefGradArgs {
     const double *U;                // velocity as the caller passes it (x, z per node)
     double dt;
     double *F;                      // 4 x (nel x nip): column-major 2x2 per point
+    StressRule rule;
 };
 
 __global__ __launch_bounds__(256) void k_deformation_gradient(DefGradArgs a) {
@@ -163,10 +166,10 @@ __global__ __launch_bounds__(256) void k_deformation_gradient(DefGradArgs a) {
         ux[b] = a.U[2 * n];
         uz[b] = a.U[2 * n + 1];
     }
-    const int nip = c_srule.nip;
+    const int nip = a.rule.nip;
     for (int q = 0; q < nip; ++q) {
         double dx[6], dz[6];
-        cartesian_gradients(th, r, c_srule.xi[q], c_srule.eta[q], dx, dz);
+        cartesian_gradients(th, r, a.rule.xi[q], a.rule.eta[q], dx, dz);
         double uxx = 0, uzz = 0, uxz = 0, uzx = 0;
 #pragma unroll
         for (int b = 0; b < 6; ++b) {
@@ -261,9 +264,6 @@ void stress_prepare(Context &c) {
 
 void stress_set_rule(Context &c, int nip) {
     if (nip != 6 && nip != 7) throw Error(STK_E_SHAPE, "the stress rule has 6 or 7 points");
-    const StressRule q = make_rule(nip);
-    STK_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_srule), &q, sizeof(q), 0, hipMemcpyHostToDevice, c.stream));
-    STK_HIP(hipStreamSynchronize(c.stream));
     c.s_nip = nip;
 }
 
@@ -275,6 +275,7 @@ void launch_stress(Context &c, const double *U, double *out) {
     a.eta = c.d_seta.p;
     a.C = c.s_aniso ? c.d_sC.p : nullptr;
     a.out = out;
+    a.rule = make_rule(c.s_nip);
     k_stress<<<dim3((unsigned)((c.nel + 255) / 256)), dim3(256), 0, c.stream>>>(a);
     STK_HIP(hipGetLastError());
 }
@@ -358,6 +359,7 @@ void launch_deformation_gradient(Context &c, const double *U, double dt, double 
     a.U = U;
     a.dt = dt;
     a.F = F;
+    a.rule = make_rule(c.s_nip);
     k_deformation_gradient<<<dim3((unsigned)((c.nel + 255) / 256)), dim3(256), 0, c.stream>>>(a);
     STK_HIP(hipGetLastError());
 }

@@ -22,7 +22,6 @@ namespace {
 struct Rule7 {
     double xi[7], eta[7], w[7];
 };
-__constant__ Rule7 c_rule7;
 
 // ip_triangle7 (src/Algebra/Quadrature.jl:109-138), weights including the 1/2.
 Rule7 make_rule7() {
@@ -49,6 +48,7 @@ struct ThermalArgs {
     const uint8_t *sK;                  // node-pair slot offsets (36 x nel by position)
     const int64_t *sptr;                // sliced layout of the node pattern
     double *KT, *MT, *F;
+    Rule7 rule;                         // the 7-point rule, passed by value
 };
 
 __global__ __launch_bounds__(64) void k_thermal_assemble(ThermalArgs a) {
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(64) void k_thermal_assemble(ThermalArgs a) {
 #pragma unroll
     for (int b = 0; b < 6; ++b) fe[b] = 0.0;
     for (int q = 0; q < 7; ++q) {
-        const double xi = c_rule7.xi[q], et = c_rule7.eta[q], t = 1.0 - xi - et;
+        const double xi = a.rule.xi[q], et = a.rule.eta[q], t = 1.0 - xi - et;
         const double N[6] = {t * (2 * t - 1), xi * (2 * xi - 1), et * (2 * et - 1), 4 * xi * t, 4 * xi * et, 4 * et * t};
         const double dr[6] = {-(4 * t - 1), 4 * xi - 1, 0.0, 4 * (t - xi), 4 * et, -4 * et};
         const double ds[6] = {-(4 * t - 1), 0.0, 4 * et - 1, -4 * xi, 4 * xi, 4 * (t - et)};
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(64) void k_thermal_assemble(ThermalArgs a) {
             dx[b] = (j00 * dr[b] + j01 * ds[b]) / detJ;
             dz[b] = (j10 * dr[b] + j11 * ds[b]) / detJ;
         }
-        const double om = rq * detJ * c_rule7.w[q];
+        const double om = rq * detJ * a.rule.w[q];
         const double kw = a.K * om, mw = om * rhoq * a.Cp;
 #pragma unroll
         for (int u = 0; u < 6; ++u)
@@ -123,14 +123,8 @@ __global__ __launch_bounds__(64) void k_thermal_assemble(ThermalArgs a) {
 void thermal_assemble(Context &c, const double *rho, double K, double Cp, double dQdT, double dt, double *KT,
                       double *MT, double *F) {
     if (!c.KN.sell) throw Error(STK_E_STATE, "the thermal operators need the sliced node pattern");
-    static bool uploaded = false;
-    if (!uploaded) {
-        const Rule7 q = make_rule7();
-        STK_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_rule7), &q, sizeof(q), 0, hipMemcpyHostToDevice, c.stream));
-        STK_HIP(hipStreamSynchronize(c.stream));
-        uploaded = true;
-    }
     ThermalArgs a{};
+    a.rule = make_rule7();
     a.nel = c.nel;
     a.perm = c.d_perm.p; a.e2n = c.d_e2n.p; a.th = c.d_elth.p; a.r = c.d_elr.p;
     a.rho = rho;

@@ -98,10 +98,26 @@ class AnnulusMesh:
         """
         top, bot = self.boundary_nodes()
         nodes = np.concatenate([top, bot])
-        fixed = 2 * nodes + 1
+        return self._with_free(2 * nodes + 1)
+
+    def free_slip_pinned(self, nr: int):
+        """``free_slip(nr, ids)`` (boundary_conditions.jl:40-53): free slip plus the tangential
+        dof of node ``nr`` (1-based, so dof 2nr - 1), appended last."""
+        fixed, _ = self.free_slip()
+        return self._with_free(np.concatenate([fixed, [2 * nr - 2]]))
+
+    def no_slip(self):
+        """``no_slip`` (boundary_conditions.jl:55-66): both dofs of every boundary node, the
+        radial dofs of all of them first, then the tangential ones (nodes in id order)."""
+        top, bot = self.boundary_nodes()
+        nodes = np.sort(np.concatenate([top, bot]))
+        return self._with_free(np.concatenate([2 * nodes + 1, 2 * nodes]))
+
+    def _with_free(self, fixed):
+        fixed = np.asarray(fixed, dtype=np.int64)
         mask = np.ones(self.nU, dtype=bool)
         mask[fixed] = False
-        return fixed.astype(np.int64), np.flatnonzero(mask).astype(np.int64)
+        return fixed, np.flatnonzero(mask).astype(np.int64)
 
 
 def build_annulus(ncell_r: int, ncell_t: int) -> AnnulusMesh:

@@ -387,17 +387,18 @@ class OracleSolution:
     ifree: np.ndarray
 
 
-def solve_stokes(inp, rtol=1e-8, itmax=200, itmin=10, null_mode="literal"):
+def solve_stokes(inp, rtol=1e-8, itmax=200, itmin=10, null_mode="literal", bcs=None):
     """solveStokes (StokesSolver.jl:7-54) with the :suitesparse inner solve.
 
     null_mode "literal" is the reference algorithm (lu(K[ifree, ifree]));
     "project" solves on the complement of the near-null vector of K.
+    ``bcs`` = (fixed, ifree), 0-based (UBC.Omega, UBC.ifree); default free slip.
     """
     mesh = inp.mesh
     KK, GG, MM, Fb = assembly_stokes_cylindric(inp)
     TT = rotation_matrix(mesh.theta)
     K, G, F = prepare_matrices(KK, GG, Fb, TT)
-    fixed, ifree = mesh.free_slip()
+    fixed, ifree = mesh.free_slip() if bcs is None else bcs
     inner = None
     if null_mode == "project":
         v, _ = null_vector(K, ifree, mesh)

@@ -156,3 +156,28 @@ def test_stats_closed_forms_at_full_size():
     assert abs(tav - ratio) < 1e-12
     assert abs(urms ** 2 - ratio) < 1e-12
     ctx.close()
+
+
+def test_two_contexts_keep_their_own_stress_rules():
+    """Context A uses the 6-point rule, context B the 7-point rule; each computes stress! and
+    deformation_gradient! after the other's set-up, and both still match the oracle with their own
+    rule (the rule travels with each launch, it is not shared by the contexts of a process)."""
+    inp = make_inputs(scaled_workload(3, 8, 64))
+    rng = np.random.default_rng(21)
+    eta, C7 = _fields7(inp, rng)
+    U = rng.normal(size=inp.mesh.nU)
+    a, b = _ctx(inp), _ctx(inp)
+    a.set_stress_fields(eta, C7, nip=6)
+    b.set_stress_fields(eta, C7, nip=7)
+    for ctx, nip in ((a, 6), (b, 7), (a, 6)):
+        got = ctx.stress(U)
+        want = so.stress(inp.mesh, U, eta, C7, nip=nip)
+        assert got["exx"].shape == (inp.mesh.nel, nip)
+        for k in FIELDS:
+            assert _rel(got[k], want[k]) < 1e-12, (nip, k)
+        F0 = np.tile(np.eye(2), (inp.mesh.nel, nip, 1, 1)) + 0.01 * rng.normal(size=(inp.mesh.nel, nip, 2, 2))
+        gotF = ctx.deformation_gradient(F0, 0.02, U)
+        assert _rel(gotF, so.deformation_gradient(inp.mesh, U, F0, 0.02, nip=nip)) < 1e-13, nip
+    print("[result] two contexts with 6- and 7-point stress rules match the oracle", flush=True)
+    a.close()
+    b.close()
