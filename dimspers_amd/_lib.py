@@ -22,6 +22,7 @@ SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED = 0, 1, 2
 INNER_MG, INNER_JACOBI = 0, 1
 STOP_SCHUR, STOP_FULL = 0, 1
 NULL_PROJECT, NULL_DEFLATE = 0, 1
+PPC_LUMPED, PPC_IC = 0, 1
 
 EXPORTED = [
     "stk_default_opts", "stk_create", "stk_destroy", "stk_last_error", "stk_info", "stk_set_fields",
@@ -29,7 +30,7 @@ EXPORTED = [
     "stk_updatevelocity", "stk_time_kernel", "stk_default_train_opts", "stk_set_fabric", "stk_mlp_train",
     "stk_mlp_params", "stk_mlp_forward", "stk_mlp_gradient", "stk_local_schur", "stk_learned_setup",
     "stk_set_stress_fields", "stk_stress", "stk_deformation_gradient", "stk_stats", "stk_thermal_solve",
-    "stk_fse", "stk_set_dem", "stk_anisotropic_tensor", "stk_colouring",
+    "stk_fse", "stk_set_dem", "stk_anisotropic_tensor", "stk_colouring", "stk_set_solution", "stk_set_operators",
 ]
 N_PARAM = 9094
 KERNEL_SPMV_K32, KERNEL_SPMV_K64, KERNEL_CHEB_K32, KERNEL_CHEB_K64, KERNEL_ASSEMBLY, KERNEL_CHEB_KH16 = 0, 1, 2, 3, 4, 5
@@ -49,7 +50,8 @@ class StkOpts(C.Structure):
         ("null_mode", C.c_int32), ("pc_sweeps", C.c_int32),
         ("rtol", C.c_double), ("inner_rtol", C.c_double), ("mixed_inner_rtol", C.c_double),
         ("mixed_schur_rtol", C.c_double), ("stall_ratio", C.c_double),
-        ("inner_warm", C.c_int32), ("fine_smoother", C.c_int32),
+        ("inner_warm", C.c_int32), ("fine_smoother", C.c_int32), ("warm_start", C.c_int32),
+        ("pressure_pc", C.c_int32),
     ]
 
 
@@ -114,6 +116,9 @@ def load(path: Path | str | None = None):
     lib.stk_info.argtypes = [vp, i64p]
     lib.stk_set_fields.argtypes = [vp, C.c_double, dp, dp, C.c_int64, C.POINTER(dp)]
     lib.stk_assemble.argtypes = [vp, C.c_int32]
+    lib.stk_set_solution.argtypes = [vp, dp, dp]
+    lib.stk_set_operators.argtypes = [vp, C.c_int32, i64p, i64p, dp, C.c_int32, i64p, i64p, dp, i64p, i64p, dp, dp,
+                                      C.c_int32]
     lib.stk_export_csc.argtypes = [vp, C.c_int32, C.c_int32, i64p, i64p, i64p, i64p, i64p, dp]
     lib.stk_export_rhs.argtypes = [vp, dp]
     lib.stk_spmv.argtypes = [vp, C.c_int32, C.c_int32, dp, dp]
@@ -276,6 +281,28 @@ class Context:
         rc = self.lib.stk_solve(self.h, C.byref(o), C.byref(rep))
         self._check(rc, allow=(STK_E_NOTCONV,) if allow_notconv else ())
         return rep
+
+    def set_solution(self, U=None, P=None):
+        """The (U, P) a solve with warm_start = 1 starts from (None: zero)."""
+        u = None if U is None else np.ascontiguousarray(U, dtype=np.float64)
+        p = None if P is None else np.ascontiguousarray(P, dtype=np.float64)
+        self._check(self.lib.stk_set_solution(self.h, None if u is None else _d(u), None if p is None else _d(p)))
+
+    def set_operators(self, K, MM, GG, Rhs, flags=ASM_FP64, K_lower=False, M_lower=True):
+        """StokesPcCG's operators from scipy sparse matrices (0-based), sent as 1-based CSC."""
+        import scipy.sparse as sp
+
+        def csc(A):
+            A = sp.csc_matrix(A)
+            A.sort_indices()
+            return (np.ascontiguousarray(A.indptr + 1, dtype=np.int64),
+                    np.ascontiguousarray(A.indices + 1, dtype=np.int64),
+                    np.ascontiguousarray(A.data, dtype=np.float64))
+        k, m, g = csc(K), csc(MM), csc(GG)
+        f = np.ascontiguousarray(Rhs, dtype=np.float64)
+        self._check(self.lib.stk_set_operators(self.h, int(K_lower), _i(k[0]), _i(k[1]), _d(k[2]), int(M_lower),
+                                               _i(m[0]), _i(m[1]), _d(m[2]), _i(g[0]), _i(g[1]), _d(g[2]), _d(f),
+                                               flags))
 
     def solution(self):
         U = np.zeros(self.nU)

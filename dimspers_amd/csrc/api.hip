@@ -273,6 +273,20 @@ void assemble_prec(Context &c, int flags, bool mg, DBuf<T> &K, DBuf<T> &G, DBuf<
     (void)f64;
 }
 
+// The lumped preconditioner of the FP64 operators and, with fp32, their rounded copies.
+void derive_from_fp64(Context &c, bool fp32) {
+    c.pc64.alloc(c.nP);
+    lumped_inverse<double>(c, c.MN, c.M64.p, c.pc64.p);
+    if (fp32) {
+        c.K32.alloc(c.K64.n); convert<double, float>(c, c.K64.p, c.K32.p, c.K64.n);
+        c.G32.alloc(c.G64.n); convert<double, float>(c, c.G64.p, c.G32.p, c.G64.n);
+        c.GT32.alloc(c.GT64.n); convert<double, float>(c, c.GT64.p, c.GT32.p, c.GT64.n);
+        c.M32.alloc(c.M64.n); convert<double, float>(c, c.M64.p, c.M32.p, c.M64.n);
+        c.F32.alloc(c.F64.n); convert<double, float>(c, c.F64.p, c.F32.p, c.F64.n);
+        c.pc32.alloc(c.nP); convert<double, float>(c, c.pc64.p, c.pc32.p, c.nP);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -294,17 +308,11 @@ int stk_assemble(stk_ctx *ctx, int32_t flags) {
         c.K32.release(); c.G32.release(); c.GT32.release(); c.M32.release(); c.F32.release(); c.pc32.release();
         c.RK32.release();
         c.S64.release(); c.S32.release(); c.S32s.release(); c.Sdinv64.release(); c.Sdinv32.release();
+        c.pic.valid = false;
         if (fp64) {
             assemble_prec<double>(c, flags, mg, c.K64, c.G64, c.GT64, c.M64, c.F64, c.RK64, true);
-            c.pc64.alloc(c.nP);
-            lumped_inverse<double>(c, c.MN, c.M64.p, c.pc64.p);
+            derive_from_fp64(c, fp32);
             if (fp32) {
-                c.K32.alloc(c.K64.n); convert<double, float>(c, c.K64.p, c.K32.p, c.K64.n);
-                c.G32.alloc(c.G64.n); convert<double, float>(c, c.G64.p, c.G32.p, c.G64.n);
-                c.GT32.alloc(c.GT64.n); convert<double, float>(c, c.GT64.p, c.GT32.p, c.GT64.n);
-                c.M32.alloc(c.M64.n); convert<double, float>(c, c.M64.p, c.M32.p, c.M64.n);
-                c.F32.alloc(c.F64.n); convert<double, float>(c, c.F64.p, c.F32.p, c.F64.n);
-                c.pc32.alloc(c.nP); convert<double, float>(c, c.pc64.p, c.pc32.p, c.nP);
                 if (mg) {
                     for (MGLevel &L : c.mg) {
                         L.A32.alloc(L.A64.n);
@@ -454,6 +462,10 @@ int stk_solve(stk_ctx *ctx, const stk_opts *opts, stk_report *report) {
         STK_CHECK(o.inner == STK_INNER_MG || o.inner == STK_INNER_JACOBI, STK_E_ARG, "bad inner solver");
         STK_CHECK(o.itmax > 0 && o.inner_itmax > 0 && o.rtol > 0 && o.cheb_degree >= 1, STK_E_ARG, "bad options");
         STK_CHECK(o.fine_smoother == STK_SMOOTH_GS || o.fine_smoother == STK_SMOOTH_CHEB, STK_E_ARG, "bad smoother");
+        STK_CHECK(o.warm_start == 0 || o.warm_start == 1, STK_E_ARG, "bad warm_start");
+        STK_CHECK(o.pressure_pc == STK_PPC_LUMPED || o.pressure_pc == STK_PPC_IC, STK_E_ARG, "bad pressure_pc");
+        STK_CHECK(o.pressure_pc == STK_PPC_LUMPED || o.method == STK_SOLVE_PLAIN, STK_E_ARG,
+                  "the incomplete Cholesky pressure preconditioner belongs to the plain FP64 method");
         // the FP32 methods stop their inner solves at a loose tolerance that only the multigrid
         // preconditioner turns into progress; with point-Jacobi inner solves their sweeps stall
         STK_CHECK(o.method == STK_SOLVE_PLAIN || o.inner == STK_INNER_MG, STK_E_ARG,
@@ -463,6 +475,60 @@ int stk_solve(stk_ctx *ctx, const stk_opts *opts, stk_report *report) {
         const int rc = run_solve(c, o, rep);
         if (rc == STK_E_NOTCONV) c.err = "stopped at the iteration cap before the tolerance";
         return rc;
+    });
+}
+
+int stk_set_solution(stk_ctx *ctx, const double *U, const double *P) {
+    if (!ctx) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        c.U.alloc(c.nU);
+        c.P.alloc(c.nP);
+        if (U) c.U.upload(U, c.nU, c.stream); else c.U.zero(c.stream);
+        if (P) c.P.upload(P, c.nP, c.stream); else c.P.zero(c.stream);
+        STK_HIP(hipStreamSynchronize(c.stream));
+        return STK_OK;
+    });
+}
+
+int stk_set_operators(stk_ctx *ctx, int32_t K_lower, const int64_t *K_colptr, const int64_t *K_rowval,
+                      const double *K_nzval, int32_t M_lower, const int64_t *M_colptr, const int64_t *M_rowval,
+                      const double *M_nzval, const int64_t *G_colptr, const int64_t *G_rowval,
+                      const double *G_nzval, const double *Rhs, int32_t flags) {
+    if (!ctx) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        Context &c = ctx->c;
+        STK_CHECK((flags & ~(STK_ASM_FP64 | STK_ASM_FP32)) == 0, STK_E_ARG,
+                  "stk_set_operators takes STK_ASM_FP64 and STK_ASM_FP32 only");
+        const bool fp32 = flags & STK_ASM_FP32;
+        std::vector<double> K, G, GT, M, F;
+        import_operators(c, K_lower != 0, CscIn{K_colptr, K_rowval, K_nzval}, M_lower != 0,
+                         CscIn{M_colptr, M_rowval, M_nzval}, CscIn{G_colptr, G_rowval, G_nzval}, Rhs, K, G, GT, M,
+                         F);
+        c.assembled = false;
+        c.mg_built = false;
+        c.have_mg64 = c.have_mg32 = false;
+        c.pic.valid = false;
+        c.K32.release(); c.G32.release(); c.GT32.release(); c.M32.release(); c.F32.release(); c.pc32.release();
+        c.RK64.release(); c.RK32.release();
+        c.S64.release(); c.S32.release(); c.S32s.release(); c.Sdinv64.release(); c.Sdinv32.release();
+        c.K64.upload(K, c.stream);
+        c.G64.upload(G, c.stream);
+        c.GT64.upload(GT, c.stream);
+        c.M64.upload(M, c.stream);
+        c.F64.upload(F, c.stream);
+        fix_diag<double>(c, c.KN, (int64_t)c.fixed_list.size(), c.d_fixlist.p, c.K64.p);
+        derive_from_fp64(c, fp32);
+        STK_HIP(hipStreamSynchronize(c.stream));
+        // the caller's operators, not this library's assembly: no multigrid (it is built from the
+        // element matrices of stk_assemble)
+        c.loaded = true;
+        c.asm_flags = STK_ASM_FP64 | (fp32 ? STK_ASM_FP32 : 0);
+        setup_solver_data<double>(c);
+        if (fp32) setup_solver_data<float>(c);
+        c.assembled = true;
+        STK_HIP(hipStreamSynchronize(c.stream));
+        return STK_OK;
     });
 }
 

@@ -408,6 +408,183 @@ bool fixed_set_is_free_slip(const Context &c) {
 
 namespace {
 
+// Index k of block column `col` in row `row` of p (columns sorted), or -1.
+int64_t find_block(const Pattern &p, int64_t row, int64_t col) {
+    const int32_t *b = p.col.data() + p.ptr[row], *e = p.col.data() + p.ptr[row + 1];
+    const int32_t *it = std::lower_bound(b, e, (int32_t)col);
+    return (it == e || *it != (int32_t)col) ? -1 : (int64_t)(it - b);
+}
+
+// Walks the entries of a 1-based CSC matrix after checking its structure.
+template <typename F>
+void for_each_csc(const CscIn &a, int64_t nrows, int64_t ncols, const char *name, F f) {
+    STK_CHECK(a.colptr && a.rowval && a.nzval, STK_E_ARG, std::string("NULL array of ") + name);
+    STK_CHECK(a.colptr[0] == 1, STK_E_SHAPE, std::string(name) + ": colptr[0] must be 1");
+    for (int64_t j = 0; j < ncols; ++j) {
+        STK_CHECK(a.colptr[j + 1] >= a.colptr[j], STK_E_SHAPE, std::string(name) + ": colptr decreases");
+        for (int64_t k = a.colptr[j] - 1; k < a.colptr[j + 1] - 1; ++k) {
+            const int64_t i = a.rowval[k] - 1;
+            STK_CHECK(i >= 0 && i < nrows, STK_E_SHAPE, std::string(name) + ": row index out of range");
+            f(i, j, a.nzval[k]);
+        }
+    }
+}
+
+}  // namespace
+
+void import_operators(const Context &c, bool K_lower, CscIn K, bool M_lower, CscIn M, CscIn G, const double *Rhs,
+                      std::vector<double> &Kv, std::vector<double> &Gv, std::vector<double> &GTv,
+                      std::vector<double> &Mv, std::vector<double> &Fv) {
+    auto outside = [](const char *name, int64_t i, int64_t j) {
+        return Error(STK_E_SHAPE, std::string(name) + " entry (" + std::to_string(i + 1) + ", " +
+                                      std::to_string(j + 1) + ") lies outside the mesh's pattern");
+    };
+    Kv.assign(4 * (size_t)c.KN.nstore(), 0.0);
+    auto put_k = [&](int64_t i, int64_t j, double v) {
+        const int64_t k = find_block(c.KN, i / 2, j / 2);
+        if (k < 0) throw outside("K", i, j);
+        if (c.fixed[i] || c.fixed[j]) return;          // K[ifree, ifree] only; identity set later
+        Kv[4 * c.KN.pos(i / 2, k) + (i % 2) * 2 + (j % 2)] += v;
+    };
+    for_each_csc(K, c.nU, c.nU, "K", [&](int64_t i, int64_t j, double v) {
+        if (K_lower) {
+            STK_CHECK(i >= j, STK_E_SHAPE, "K: entry above the diagonal with K_lower");
+            if (i != j) put_k(j, i, v);
+        }
+        put_k(i, j, v);
+    });
+    Mv.assign((size_t)c.MN.nnz(), 0.0);
+    auto put_m = [&](int64_t i, int64_t j, double v) {
+        const int64_t k = find_block(c.MN, i, j);
+        if (k < 0) throw outside("M", i, j);
+        Mv[c.MN.ptr[i] + k] += v;
+    };
+    for_each_csc(M, c.nP, c.nP, "M", [&](int64_t i, int64_t j, double v) {
+        if (M_lower) {
+            STK_CHECK(i >= j, STK_E_SHAPE, "M: entry above the diagonal with M_lower");
+            if (i != j) put_m(j, i, v);
+        }
+        put_m(i, j, v);
+    });
+    Gv.assign(2 * (size_t)c.GN.nstore(), 0.0);
+    GTv.assign(2 * (size_t)c.GTN.nstore(), 0.0);
+    for_each_csc(G, c.nU, c.nP, "G", [&](int64_t i, int64_t j, double v) {
+        const int64_t k = find_block(c.GN, i / 2, j), kt = find_block(c.GTN, j, i / 2);
+        if (k < 0 || kt < 0) throw outside("G", i, j);
+        if (c.fixed[i]) return;                          // rows of fixed dofs leave the solve
+        Gv[2 * c.GN.pos(i / 2, k) + i % 2] += v;
+        GTv[2 * c.GTN.pos(j, kt) + i % 2] += v;
+    });
+    STK_CHECK(Rhs, STK_E_ARG, "NULL Rhs");
+    Fv.assign(Rhs, Rhs + c.nU);
+}
+
+void build_pressure_ic(Context &c, const std::vector<double> &M) {
+    const Pattern &p = c.MN;
+    const int64_t n = p.nrows;
+    // greedy colouring of the vertex graph in natural order (3 colours on the 2-triangle split)
+    std::vector<int32_t> colour(n, -1);
+    int nc = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        uint64_t used = 0;
+        for (int32_t k = p.ptr[i]; k < p.ptr[i + 1]; ++k)
+            if (colour[p.col[k]] >= 0 && colour[p.col[k]] < 64) used |= 1ull << colour[p.col[k]];
+        int ci = 0;
+        while (used >> ci & 1ull) ++ci;
+        STK_CHECK(ci < 64, STK_E_SHAPE, "pressure colouring needs more than 64 colours");
+        colour[i] = ci;
+        nc = std::max(nc, ci + 1);
+    }
+    std::vector<int32_t> order;
+    order.reserve(n);
+    c.pic.colour_off.assign(nc + 1, 0);
+    for (int ci = 0; ci < nc; ++ci) {
+        for (int64_t i = 0; i < n; ++i)
+            if (colour[i] == ci) order.push_back((int32_t)i);
+        c.pic.colour_off[ci + 1] = (int64_t)order.size();
+    }
+    std::vector<int64_t> pos(n);
+    for (int64_t q = 0; q < n; ++q) pos[order[q]] = q;
+    // up-looking IC(0): for row i (in colour order) and each earlier neighbour j,
+    //   L_ij D_j = M_ij - sum_k L_ik D_k L_jk,   D_i = M_ii - sum_k L_ik^2 D_k,
+    // k over earlier vertices in the pattern of both rows.  A pivot that is not positive restarts
+    // the factorisation of M + alpha diag(M) with a larger alpha (the shift rule of limited-memory
+    // incomplete Cholesky).
+    std::vector<int32_t> lptr(n + 1, 0), lcol;
+    std::vector<double> lval, D(n);
+    std::vector<double> mark(n, 0.0);
+    std::vector<uint8_t> has(n, 0);
+    double alpha = 0.0;
+    for (int attempt = 0;; ++attempt) {
+        STK_CHECK(attempt < 40, STK_E_SHAPE, "incomplete Cholesky of MM: no positive pivots with any shift");
+        lcol.clear();
+        lval.clear();
+        bool ok = true;
+        for (int64_t q = 0; q < n && ok; ++q) {
+            const int64_t i = order[q];
+            lptr[q] = (int32_t)lcol.size();
+            double dii = 0.0;
+            std::vector<std::pair<int64_t, int32_t>> lower;   // (position, vertex) of earlier neighbours
+            for (int32_t k = p.ptr[i]; k < p.ptr[i + 1]; ++k) {
+                const int32_t j = p.col[k];
+                if (j == i) dii = M[k] * (1.0 + alpha);
+                else if (pos[j] < q) lower.push_back({pos[j], j});
+            }
+            std::sort(lower.begin(), lower.end());
+            const size_t first = lcol.size();
+            for (const auto &pj : lower) {
+                const int32_t j = pj.second;
+                double v = M[p.ptr[i] + find_block(p, i, j)];
+                // sum over the entries k of row j (all earlier than j) that row i already holds;
+                // row j is complete: its position precedes q, so lptr[pos_j + 1] is set
+                for (int32_t t = lptr[pj.first]; t < lptr[pj.first + 1]; ++t) {
+                    const int32_t kk = lcol[t];
+                    if (has[kk]) v -= mark[kk] * D[kk] * lval[t];
+                }
+                const double lij = v / D[j];
+                lcol.push_back(j);
+                lval.push_back(lij);
+                mark[j] = lij;
+                has[j] = 1;
+                dii -= lij * lij * D[j];
+            }
+            for (size_t t = first; t < lcol.size(); ++t) has[lcol[t]] = 0;
+            if (!(dii > 1e-12 * std::fabs(M[p.ptr[i] + find_block(p, i, i)]))) ok = false;
+            D[i] = dii;
+        }
+        if (ok) break;
+        alpha = alpha > 0 ? 2.0 * alpha : 1e-3;
+    }
+    lptr[n] = (int32_t)lcol.size();
+    // L^T by rows (position of the column vertex), for the backward solve
+    std::vector<int32_t> uptr(n + 1, 0), ucol(lcol.size());
+    std::vector<double> uval(lcol.size());
+    for (size_t t = 0; t < lcol.size(); ++t) uptr[pos[lcol[t]] + 1]++;
+    for (int64_t q = 0; q < n; ++q) uptr[q + 1] += uptr[q];
+    std::vector<int32_t> fill(uptr.begin(), uptr.end() - 1);
+    for (int64_t q = 0; q < n; ++q)
+        for (int32_t t = lptr[q]; t < lptr[q + 1]; ++t) {
+            const int64_t qc = pos[lcol[t]];
+            ucol[fill[qc]] = order[q];
+            uval[fill[qc]++] = lval[t];
+        }
+    std::vector<double> dinv(n);
+    for (int64_t i = 0; i < n; ++i) dinv[i] = 1.0 / D[i];
+    c.pic.order.upload(order, c.stream);
+    c.pic.lptr.upload(lptr, c.stream);
+    c.pic.lcol.upload(lcol, c.stream);
+    c.pic.lval.upload(lval, c.stream);
+    c.pic.uptr.upload(uptr, c.stream);
+    c.pic.ucol.upload(ucol, c.stream);
+    c.pic.uval.upload(uval, c.stream);
+    c.pic.dinv.upload(dinv, c.stream);
+    STK_HIP(hipStreamSynchronize(c.stream));
+    c.pic.shift = alpha;
+    c.pic.valid = true;
+}
+
+namespace {
+
 // Local vertex lattice coordinates (i, j) of the 3 vertices of element type t in cell (I, J).
 inline void tri_vertices(int t, int I, int J, int vi[3], int vj[3]) {
     if (t == 0) {

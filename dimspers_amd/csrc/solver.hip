@@ -1059,6 +1059,37 @@ double vdot(Context &c, const T *a, const T *b, int64_t n) {
     return red.fetch(SC_TMP);
 }
 
+// z = (L D L^T)^-1 r with the incomplete Cholesky of M (STK_PPC_IC): forward substitution colour by
+// colour (each row needs only rows of earlier colours), then the diagonal and the backward
+// substitution in reverse colour order.  y is an nP work vector.
+void apply_pressure_ic(Context &c, const double *r, double *y, double *z) {
+    const Context::PressureIC &ic = c.pic;
+    const int32_t *ord = ic.order.p;
+    const int32_t *lp = ic.lptr.p, *lc = ic.lcol.p, *up = ic.uptr.p, *uc = ic.ucol.p;
+    const double *lv = ic.lval.p, *uv = ic.uval.p, *di = ic.dinv.p;
+    const int nc = (int)ic.colour_off.size() - 1;
+    for (int k = 0; k < nc; ++k) {
+        const int64_t q0 = ic.colour_off[k];
+        launch_map(c, ic.colour_off[k + 1] - q0, [=] __device__(int64_t t) {
+            const int64_t q = q0 + t;
+            const int32_t i = ord[q];
+            double v = r[i];
+            for (int32_t e = lp[q]; e < lp[q + 1]; ++e) v -= lv[e] * y[lc[e]];
+            y[i] = v;
+        });
+    }
+    for (int k = nc - 1; k >= 0; --k) {
+        const int64_t q0 = ic.colour_off[k];
+        launch_map(c, ic.colour_off[k + 1] - q0, [=] __device__(int64_t t) {
+            const int64_t q = q0 + t;
+            const int32_t i = ord[q];
+            double v = di[i] * y[i];
+            for (int32_t e = up[q]; e < up[q + 1]; ++e) v -= uv[e] * z[uc[e]];
+            z[i] = v;
+        });
+    }
+}
+
 // Generalised StokesPcCG: solve K du - G dp = fu, G^T du = h (h may be null = 0)
 // with du, dp starting at zero.  Returns outer iterations; inner iterations accumulate.
 template <typename T>
@@ -1117,8 +1148,12 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
                 spmv<1, 1, 8, T>(c, c.MN, S, dout, es, nullptr);
                 launch_map(c, nP, [=] __device__(int64_t i) { dout[i] += T(0.7) * sd[i] * (rin[i] - tmp[i]); });
             }
-        } else {
+        } else if (pcvec) {
             launch_map(c, nP, [=] __device__(int64_t i) { dout[i] = pcvec[i] * rin[i]; });
+        } else {
+            // no vector: the incomplete Cholesky of M (FP64 plain method only)
+            if constexpr (sizeof(T) == 8) apply_pressure_ic(c, rin, w.Sq, dout);
+            else throw Error(STK_E_ARG, "the incomplete Cholesky pressure preconditioner is FP64 only");
         }
     };
     *conv = false;
@@ -1262,13 +1297,20 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
     std::memset(&rep, 0, sizeof(rep));
     const bool use_mg = (o.inner == STK_INNER_MG) && c.mg_built && !c.loaded;
     if (o.inner == STK_INNER_MG && !use_mg)
-        throw Error(STK_E_NOMESH, "multigrid not available for this system" + (c.mg_note.empty() ? std::string()
-                                                                                             : ": " + c.mg_note));
+        throw Error(STK_E_NOMESH,
+                    "multigrid not available for this system" +
+                        (c.loaded ? std::string(": it is built from the element matrices of stk_assemble, and the "
+                                                "operators were set by stk_set_operators")
+                                  : c.mg_note.empty() ? std::string() : ": " + c.mg_note));
     const int64_t nU = c.nU, nP = c.nP;
-    c.U.alloc(nU);
-    c.P.alloc(nP);
-    c.U.zero(c.stream);
-    c.P.zero(c.stream);
+    // warm start: keep the context's solution (the last solve's or stk_set_solution's)
+    const bool warm = o.warm_start != 0 && c.U.n == (size_t)nU && c.P.n == (size_t)nP;
+    if (!warm) {
+        c.U.alloc(nU);
+        c.P.alloc(nP);
+        c.U.zero(c.stream);
+        c.P.zero(c.stream);
+    }
     c.work64.alloc(std::max(nU, nP));
     EventPair ev_;
     hipEvent_t e0 = ev_.a, e1 = ev_.b;
@@ -1298,20 +1340,46 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
         // with STK_STOP_FULL it stops as soon as the continuity residual |r| is below
         // half the full-residual target (the momentum part is the inner-solve error),
         // and continues in restarted sweeps while the full residual is not met.
-        const double *pc = c.pc64.p;
+        if (o.pressure_pc == STK_PPC_IC && !c.pic.valid) {
+            std::vector<double> Mh(c.M64.n);
+            c.M64.download(Mh.data(), Mh.size(), c.stream);
+            STK_HIP(hipStreamSynchronize(c.stream));
+            build_pressure_ic(c, Mh);
+            if (trace_on()) std::fprintf(stderr, "[stk] pressure IC(0): %d colours, shift %.1e\n",
+                                         (int)c.pic.colour_off.size() - 1, c.pic.shift);
+        }
+        const double *pc = o.pressure_pc == STK_PPC_IC ? nullptr : c.pc64.p;
         const double abs_tol = (o.stop == STK_STOP_FULL) ? 0.5 * o.rtol * c.fnorm : 0.0;
-        int its = schur_pcg<double>(c, o, use_mg, Fm.p, nullptr, w, o.rtol, o.itmax, o.itmin, o.inner_rtol,
-                                    &iits, &sres, &conv, pc, false, deflate, nullptr, abs_tol);
-        {
+        DBuf<double> ru(nU), hc(nP);
+        // from zero: the system itself; warm: the correction of (U, P), whose initial Schur
+        // residual -G^T K^-1 (F + G P) is StokesPcCG_pardiso's (Rhs .+= GG*P, then U = K^-1 Rhs)
+        const double *fu = Fm.p, *hh = nullptr;
+        int its = 0;
+        conv = false;
+        sres = 1.0;
+        if (warm) {
+            rep.full_residual = full_residual(c, c.U.p, c.P.p, ru.p, hc.p, project);
+            fu = ru.p;
+            hh = hc.p;
+            // the judged residual is already met: nothing to do
+            if (o.stop == STK_STOP_FULL && rep.full_residual < o.rtol) {
+                conv = true;
+                sres = 0.0;
+                fu = nullptr;
+            }
+        }
+        if (fu) {
+            its = schur_pcg<double>(c, o, use_mg, fu, hh, w, o.rtol, o.itmax, o.itmin, o.inner_rtol, &iits, &sres,
+                                    &conv, pc, false, deflate, nullptr, abs_tol);
             double *U = c.U.p, *P = c.P.p;
-            launch_map(c, nU, [=] __device__(int64_t i) { U[i] = w.du[i]; });
-            launch_map(c, nP, [=] __device__(int64_t i) { P[i] = w.dp[i]; });
+            const double keep = warm ? 1.0 : 0.0;
+            launch_map(c, nU, [=] __device__(int64_t i) { U[i] = keep * U[i] + w.du[i]; });
+            launch_map(c, nP, [=] __device__(int64_t i) { P[i] = keep * P[i] + w.dp[i]; });
         }
         rep.outer_iterations = its;
         rep.inner_iterations = iits;
         rep.schur_residual = sres;
         rep.refinements = 0;
-        DBuf<double> ru(nU), hc(nP);
         rep.full_residual = full_residual(c, c.U.p, c.P.p, ru.p, hc.p, project);
         if (trace_on()) std::fprintf(stderr, "[stk] plain first pass: %d its, full residual %.3e\n", its, rep.full_residual);
         if (o.stop == STK_STOP_FULL) {

@@ -239,6 +239,18 @@ struct Context {
         bool filled = false;                   // values and dinv of the current FP32 K
     } gs;
     int fine_smoother = 0;                     // STK_SMOOTH_* of the last stk_solve
+    // Incomplete Cholesky M ~ L D L^T of the pressure mass matrix (STK_PPC_IC), IC(0) on M's
+    // pattern in a multicolour vertex order: vertices of one colour are not coupled, so each
+    // triangular solve is one launch per colour.  Rows of L (forward) and of L^T (backward) are
+    // stored by position in that order; their columns are vertex ids.
+    struct PressureIC {
+        bool valid = false;
+        double shift = 0;                      // diagonal shift alpha of the last factorisation
+        std::vector<int64_t> colour_off;       // ncolour + 1 offsets into order
+        DBuf<int32_t> order;                   // vertices by colour
+        DBuf<int32_t> lptr, lcol, uptr, ucol;
+        DBuf<double> lval, uval, dinv;         // dinv per vertex
+    } pic;
     DBuf<double> rot64; DBuf<float> rot32;     // unit near-null vector of K (discrete rigid rotation)
     double null_lambda = 0;                    // its Rayleigh quotient v^T K v
     DBuf<int64_t> d_fixlist;
@@ -312,6 +324,16 @@ void build_slot_maps(Context &c);
 void build_gs_layout(Context &c);
 void detect_structured(Context &c);
 bool fixed_set_is_free_slip(const Context &c);
+// IC(0) factorisation of the full pressure mass matrix M (values on MN in CSR order) for STK_PPC_IC.
+void build_pressure_ic(Context &c, const std::vector<double> &M);
+// Values of caller operators (1-based CSC) scattered into the layouts of K, G, G^T, M and F.
+struct CscIn {
+    const int64_t *colptr, *rowval;
+    const double *nzval;
+};
+void import_operators(const Context &c, bool K_lower, CscIn K, bool M_lower, CscIn M, CscIn G, const double *Rhs,
+                      std::vector<double> &Kv, std::vector<double> &Gv, std::vector<double> &GTv,
+                      std::vector<double> &Mv, std::vector<double> &Fv);
 void build_mg_structure(Context &c);
 
 // ---------------------------------------------------------------- kernels (assembly.hip)

@@ -3,14 +3,16 @@
 # Keeps the reference call signatures:
 #   assembly_stokes_cylindric(gr, coordinates, g, ρ, η, 𝓒, εII, T, P, PhaseID, KKidx, GGidx, MMidx)
 #       (src/Stokes/StokesSolver.jl:247-404)  -> (KK, GG, MM, Fb, η)
-#   StokesPcCG(U, P, KK, MM, GG, Rhs, ifree)  (src/IterativeSolvers.jl:94-168), here on the
-#       device-resident operators of the last `solveStokes` call
+#   StokesPcCG(U, P, KK, MM, GG, Rhs, ifree)  (src/IterativeSolvers.jl:94-168) and
+#   StokesPcCG_pardiso(U, P, KK, MM, GG, Rhs, ifree)  (:1-92) on the caller's operators
+#       (keywords gr, coordinates, UBC name the mesh whose pattern they live on)
 #   solveStokes(U, P, T, gr, Ucartesian, Upolar, g, ρ, η, 𝓒, εII, coordinates, RotationMatrices,
 #               PhaseID, UBC, KKidx, GGidx, MMidx, to; solver=:mi355x)
 #       (src/Stokes/StokesSolver.jl:7-54)
 # Every call passes plain pointers (GC.@preserve) and turns a non-zero status into an error.
 module StokesMI355X
 
+using LinearAlgebra: istril
 using SparseArrays
 
 const LIB = get(ENV, "DIMSPERS_STOKES_LIB", joinpath(@__DIR__, "..", "libdimspers_stokes.so"))
@@ -20,9 +22,9 @@ mutable struct Opts
     method::Int32; inner::Int32; stop::Int32; itmax::Int32; itmin::Int32; inner_itmax::Int32
     refine_max::Int32; cheb_degree::Int32; null_mode::Int32; pc_sweeps::Int32
     rtol::Float64; inner_rtol::Float64; mixed_inner_rtol::Float64; mixed_schur_rtol::Float64; stall_ratio::Float64
-    inner_warm::Int32; fine_smoother::Int32
+    inner_warm::Int32; fine_smoother::Int32; warm_start::Int32; pressure_pc::Int32
 end
-Opts() = (o = Opts(zeros(Int32, 10)..., zeros(5)..., Int32(0), Int32(0));
+Opts() = (o = Opts(zeros(Int32, 10)..., zeros(5)..., zeros(Int32, 4)...);
           ccall((:stk_default_opts, LIB), Cvoid, (Ref{Opts},), o); o)
 
 mutable struct Report
@@ -33,6 +35,9 @@ Report() = Report(0, 0, 0, 0, 0, 0.0, 0.0, 0.0)
 
 const SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED = Int32(0), Int32(1), Int32(2)
 const STOP_SCHUR, STOP_FULL = Int32(0), Int32(1)
+const INNER_MG, INNER_JACOBI = Int32(0), Int32(1)
+const NULL_PROJECT, NULL_DEFLATE = Int32(0), Int32(1)
+const PPC_LUMPED, PPC_IC = Int32(0), Int32(1)
 const ASM_FP64, ASM_FP32, ASM_CARTESIAN, ASM_NO_BC = Int32(1), Int32(2), Int32(4), Int32(8)
 const MAT_K, MAT_G, MAT_GT, MAT_M = Int32(0), Int32(1), Int32(2), Int32(3)
 const FSE_MODE = Dict(:NoHealing => Int32(0), :FabricDestructionOnly => Int32(1),
@@ -49,27 +54,33 @@ check(ctx::Union{Context,Nothing}, rc) = rc == 0 ? nothing :
     error("libdimspers_stokes: ", unsafe_string(ccall((:stk_last_error, LIB), Cstring, (Ptr{Cvoid},),
                                                       ctx === nothing ? C_NULL : ctx.h)), " (", rc, ")")
 
-const CONTEXTS = IdDict{Any,Context}()
+# Contexts per grid, one for every (fixed dofs, element coordinates) pair used with it.
+const CONTEXTS = IdDict{Any,Vector{Tuple{Vector{Int64},Any,Context}}}()
 
-"One device context per `Grid` (patterns, colouring and multigrid structure are built once)."
+"""One device context per (`Grid`, fixed dofs `UBC.Ω`, element `coordinates`): patterns, colouring
+and multigrid structure are built once.  `UBC === nothing` stands for no fixed dof (the assembly
+export, which applies no boundary condition)."""
 function context(gr, coordinates, UBC)
-    get!(CONTEXTS, gr) do
-        h = Ref{Ptr{Cvoid}}(C_NULL)
-        e2n = Matrix{Int64}(gr.e2n)                 # 6 x nel, 1-based, column-major
-        th = Matrix{Float64}(coordinates.θ); r = Matrix{Float64}(coordinates.r)
-        fixed = Vector{Int64}(UBC.Ω)
-        GC.@preserve e2n th r fixed gr begin
-            rc = ccall((:stk_create, LIB), Cint,
-                       (Ref{Ptr{Cvoid}}, Int64, Int64, Int64, Ptr{Int64}, Ptr{Float64}, Ptr{Float64},
-                        Ptr{Float64}, Ptr{Float64}, Ptr{Int64}, Int64),
-                       h, size(e2n, 2), length(gr.θ), maximum(view(e2n, 1:3, :)), e2n, th, r,
-                       gr.θ, gr.r, fixed, length(fixed))
-        end
-        check(nothing, rc)
-        ctx = Context(h[], 2 * length(gr.θ), maximum(view(e2n, 1:3, :)))
-        finalizer(c -> ccall((:stk_destroy, LIB), Cvoid, (Ptr{Cvoid},), c.h), ctx)
-        ctx
+    fixed = UBC === nothing ? Int64[] : Vector{Int64}(UBC.Ω)
+    entries = get!(() -> Tuple{Vector{Int64},Any,Context}[], CONTEXTS, gr)
+    for (f, coords, ctx) in entries
+        f == fixed && coords === coordinates && return ctx
     end
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    e2n = Matrix{Int64}(gr.e2n)                 # 6 x nel, 1-based, column-major
+    th = Matrix{Float64}(coordinates.θ); r = Matrix{Float64}(coordinates.r)
+    GC.@preserve e2n th r fixed gr begin
+        rc = ccall((:stk_create, LIB), Cint,
+                   (Ref{Ptr{Cvoid}}, Int64, Int64, Int64, Ptr{Int64}, Ptr{Float64}, Ptr{Float64},
+                    Ptr{Float64}, Ptr{Float64}, Ptr{Int64}, Int64),
+                   h, size(e2n, 2), length(gr.θ), maximum(view(e2n, 1:3, :)), e2n, th, r,
+                   gr.θ, gr.r, fixed, length(fixed))
+    end
+    check(nothing, rc)
+    ctx = Context(h[], 2 * length(gr.θ), maximum(view(e2n, 1:3, :)))
+    finalizer(c -> ccall((:stk_destroy, LIB), Cvoid, (Ptr{Cvoid},), c.h), ctx)
+    push!(entries, (fixed, coordinates, ctx))
+    ctx
 end
 
 function set_fields!(ctx, g, ρ, η, 𝓒)
@@ -95,8 +106,10 @@ function export_csc(ctx, which; lower=false)
     SparseMatrixCSC(nr[], nc[], colptr, rowval, nzval)
 end
 
-"GPU `assembly_stokes_cylindric`: lower-triangular KK and MM, GG, Fb in the Cartesian frame."
-function assembly_stokes_cylindric(gr, coordinates, g, ρ, η, 𝓒, εII, T, P, PhaseID, KKidx, GGidx, MMidx; UBC)
+"""GPU `assembly_stokes_cylindric`: lower-triangular KK and MM, GG, Fb in the Cartesian frame, with the
+reference's positional signature (the boundary conditions are applied later, so `UBC` is optional)."""
+function assembly_stokes_cylindric(gr, coordinates, g, ρ, η, 𝓒, εII, T, P, PhaseID, KKidx, GGidx, MMidx;
+                                   UBC=nothing)
     ctx = context(gr, coordinates, UBC)
     set_fields!(ctx, g, ρ, η, 𝓒)
     check(ctx, ccall((:stk_assemble, LIB), Cint, (Ptr{Cvoid}, Int32), ctx.h, ASM_FP64 | ASM_CARTESIAN | ASM_NO_BC))
@@ -105,15 +118,34 @@ function assembly_stokes_cylindric(gr, coordinates, g, ρ, η, 𝓒, εII, T, P,
     return export_csc(ctx, MAT_K; lower=true), export_csc(ctx, MAT_G), export_csc(ctx, MAT_M; lower=true), Fb, η
 end
 
-"GPU `solveStokes`: assemble (rotated, free slip), Schur PCG, net-rotation removal."
+"""GPU `solveStokes`: assemble (rotated, with UBC's fixed dofs), Schur PCG, net-rotation removal.
+`solver` picks the method of the reference's dispatch (StokesSolver.jl:41-46) or a fast one:
+`:suitesparse` = StokesPcCG (FP64, zero start, lumped MM, Schur rtol 1e-8); `:pardiso` =
+StokesPcCG_pardiso (FP64, starts from the incoming P, incomplete Cholesky of MM, Schur rtol 1e-6);
+`:mixed` / `:learned` = FP32 Schur PCG under FP64 refinement to a 1e-8 full residual (`:learned`
+needs `train!` first).  A solver name it does not know is an error."""
 function solveStokes(U, P, T, gr, Ucartesian, Upolar, g, ρ, η, 𝓒, εII, coordinates, RotationMatrices,
-                     PhaseID, UBC, KKidx, GGidx, MMidx, to; solver=:mixed, rtol=1e-8)
+                     PhaseID, UBC, KKidx, GGidx, MMidx, to; solver=:pardiso, rtol=nothing)
     ctx = context(gr, coordinates, UBC)
     set_fields!(ctx, g, ρ, η, 𝓒)
     check(ctx, ccall((:stk_assemble, LIB), Cint, (Ptr{Cvoid}, Int32), ctx.h, ASM_FP64 | ASM_FP32))
     o = Opts()
-    o.method = solver === :plain ? SOLVE_PLAIN : solver === :learned ? SOLVE_LEARNED : SOLVE_MIXED
-    o.stop = STOP_FULL; o.rtol = rtol
+    if solver === :suitesparse
+        o.method = SOLVE_PLAIN; o.stop = STOP_SCHUR; o.rtol = something(rtol, 1e-8)
+    elseif solver === :pardiso
+        o.method = SOLVE_PLAIN; o.stop = STOP_SCHUR; o.rtol = something(rtol, 1e-6)
+        o.warm_start = 1; o.pressure_pc = PPC_IC
+        Pin = Vector{Float64}(P)
+        GC.@preserve Pin begin
+            check(ctx, ccall((:stk_set_solution, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+                             ctx.h, C_NULL, Pin))
+        end
+    elseif solver === :mixed || solver === :learned
+        o.method = solver === :mixed ? SOLVE_MIXED : SOLVE_LEARNED
+        o.stop = STOP_FULL; o.rtol = something(rtol, 1e-8)
+    else
+        error("solveStokes: unknown solver ", solver)
+    end
     rep = Report()
     check(ctx, ccall((:stk_solve, LIB), Cint, (Ptr{Cvoid}, Ref{Opts}, Ref{Report}), ctx.h, o, rep))
     Ucart = similar(U); ω = Ref(0.0)
@@ -126,6 +158,52 @@ function solveStokes(U, P, T, gr, Ucartesian, Upolar, g, ρ, η, 𝓒, εII, coo
     end
     return Ucartesian, Upolar, U, Ucart, P, to
 end
+
+function _pcg_on_operators(U, P, KK, MM, GG, Rhs, ifree, gr, coordinates, UBC; warm, ppc, rtol)
+    ctx = context(gr, coordinates, UBC)
+    nU = length(Rhs)
+    setdiff(1:nU, ifree) == sort(Vector{Int64}(UBC.Ω)) ||
+        error("StokesPcCG: ifree is not the complement of UBC.Ω")
+    K = SparseMatrixCSC{Float64,Int64}(KK); M = SparseMatrixCSC{Float64,Int64}(MM)
+    G = SparseMatrixCSC{Float64,Int64}(GG); F = Vector{Float64}(Rhs)
+    GC.@preserve K M G F begin
+        check(ctx, ccall((:stk_set_operators, LIB), Cint,
+                         (Ptr{Cvoid}, Int32, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}, Int32, Ptr{Int64}, Ptr{Int64},
+                          Ptr{Float64}, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Int32),
+                         ctx.h, Int32(istril(K)), K.colptr, K.rowval, K.nzval, Int32(istril(M)), M.colptr,
+                         M.rowval, M.nzval, G.colptr, G.rowval, G.nzval, F, ASM_FP64))
+    end
+    if warm
+        Pin = Vector{Float64}(P)
+        GC.@preserve Pin begin
+            check(ctx, ccall((:stk_set_solution, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+                             ctx.h, C_NULL, Pin))
+        end
+    end
+    o = Opts()
+    o.method = SOLVE_PLAIN; o.inner = INNER_JACOBI; o.inner_itmax = 100000; o.stop = STOP_SCHUR; o.rtol = rtol
+    o.warm_start = warm ? 1 : 0; o.pressure_pc = ppc
+    # with only radial dofs fixed, K[ifree, ifree] has the near-null rotation: deflating it gives
+    # the LU answer of the reference's inner solve; otherwise there is nothing to deflate
+    o.null_mode = all(iseven, UBC.Ω) ? NULL_DEFLATE : NULL_PROJECT
+    rep = Report()
+    check(ctx, ccall((:stk_solve, LIB), Cint, (Ptr{Cvoid}, Ref{Opts}, Ref{Report}), ctx.h, o, rep))
+    Uo = Vector{Float64}(undef, nU); Po = Vector{Float64}(undef, length(P))
+    check(ctx, ccall((:stk_get_solution, LIB), Cint, (Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), ctx.h, Uo, Po))
+    copyto!(U, Uo); copyto!(P, Po)
+    return U, P
+end
+
+"""GPU `StokesPcCG(U, P, KK, MM, GG, Rhs, ifree)` (src/IterativeSolvers.jl:94-168) on the caller's
+operators: U and P start at zero, lumped MM preconditioner, Schur rtol 1e-8, point-Jacobi CG inner
+solves on K[ifree, ifree].  `gr`, `coordinates` and `UBC` name the mesh the operators live on."""
+StokesPcCG(U, P, KK, MM, GG, Rhs, ifree; gr, coordinates, UBC) =
+    _pcg_on_operators(U, P, KK, MM, GG, Rhs, ifree, gr, coordinates, UBC; warm=false, ppc=PPC_LUMPED, rtol=1e-8)
+
+"""GPU `StokesPcCG_pardiso(U, P, KK, MM, GG, Rhs, ifree)` (src/IterativeSolvers.jl:1-92): keeps the
+incoming P (Rhs .+= GG*P), incomplete Cholesky of MM, Schur rtol 1e-6."""
+StokesPcCG_pardiso(U, P, KK, MM, GG, Rhs, ifree; gr, coordinates, UBC) =
+    _pcg_on_operators(U, P, KK, MM, GG, Rhs, ifree, gr, coordinates, UBC; warm=true, ppc=PPC_IC, rtol=1e-6)
 
 """GPU `stress!` (src/Stress/StressCalculation.jl:244-249): fills ε, εII, τ, τII (nel × nip,
 nip = size(ε.xx, 2), 7 in MAIN.jl:94,164) from the Cartesian velocity `U` (MAIN passes Ucart).

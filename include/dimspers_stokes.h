@@ -123,7 +123,22 @@ typedef struct stk_opts {
                             /* beta * z of the previous one (q_k = d_k + beta q_k-1) */
     int32_t fine_smoother;  /* STK_SMOOTH_*: smoother of the fine multigrid level  */
                             /* in the FP32 inner solves                            */
+    int32_t warm_start;     /* 0: U and P start at zero (StokesPcCG, :101-102);    */
+                            /* 1: start from the context's solution (the last      */
+                            /* solve's, or stk_set_solution's): the pressure is    */
+                            /* kept as in StokesPcCG_pardiso (IterativeSolvers.jl: */
+                            /* 7-17, Rhs .+= GG*P), the first pass solves for the  */
+                            /* correction from the full residual of (U, P)         */
+    int32_t pressure_pc;    /* STK_PPC_*: pressure preconditioner of the plain     */
+                            /* method (_preconditioner, :189-217)                  */
 } stk_opts;
+
+/* Pressure preconditioner of the plain FP64 Schur PCG (stk_opts.pressure_pc). */
+enum {
+    STK_PPC_LUMPED = 0,     /* inverse row sums of MM (:jacobi = lumpedpreconditioner, StokesPcCG) */
+    STK_PPC_IC = 1          /* incomplete Cholesky of MM (:incomplete_chol = lldl, the            */
+                            /* preconditioner of StokesPcCG_pardiso, :32, :242-244)               */
+};
 
 typedef struct stk_report {
     int32_t converged;          /* 1 if the stopping rule was met               */
@@ -209,6 +224,30 @@ int stk_solve(stk_ctx *ctx, const stk_opts *opts, stk_report *report);
 
 /* Copy the last solution to the host (U: nU, P: nP; polar dof frame). */
 int stk_get_solution(stk_ctx *ctx, double *U, double *P);
+
+/* Set the solution a solve with stk_opts.warm_start = 1 starts from: the incoming U and P of
+ * StokesPcCG_pardiso (src/IterativeSolvers.jl:1-17), which keeps P and solves for U.  U (nU) and
+ * P (nP) are host FP64 in the polar dof frame; either may be NULL (zero). */
+int stk_set_solution(stk_ctx *ctx, const double *U, const double *P);
+
+/* StokesPcCG's operators from the caller instead of stk_assemble: the matrices StokesPcCG(U, P, KK,
+ * MM, GG, Rhs, ifree) receives (src/IterativeSolvers.jl:94; solveStokes passes them at
+ * src/Stokes/StokesSolver.jl:41-46, the threaded assembler at src/Stokes/AssemblerStokes.jl:62).
+ * K: the rotated velocity block of _prepare_matrices (nU x nU, full symmetric, or its lower
+ * triangle with K_lower = 1); MM: the pressure mass matrix (nP x nP) as assembly returns it, lower
+ * triangle with M_lower = 1 (_preconditioner symmetrises it, :190); GG: nU x nP; all 1-based CSC
+ * (colptr of length ncols + 1, rowval, nzval), with every entry inside the mesh's P2-P1 pattern
+ * (STK_E_SHAPE otherwise; explicit zeros are allowed).  Rhs (nU) is the right-hand side after
+ * _apply_bcs.  The fixed dofs are those given to stk_create (UBC.Omega, the complement of ifree):
+ * their rows and columns of K and rows of GG are dropped, as the inner solve on K[ifree, ifree]
+ * (src/PardisoSolvers.jl:247-256) and the free-dof solution leave them out.  flags: STK_ASM_FP64
+ * and/or STK_ASM_FP32 (rounded copies for the mixed method).  Then stk_solve runs on these
+ * operators; the multigrid, which is built from the element matrices of stk_assemble, is not
+ * available (STK_INNER_JACOBI, plain method). */
+int stk_set_operators(stk_ctx *ctx, int32_t K_lower, const int64_t *K_colptr, const int64_t *K_rowval,
+                      const double *K_nzval, int32_t M_lower, const int64_t *M_colptr, const int64_t *M_rowval,
+                      const double *M_nzval, const int64_t *G_colptr, const int64_t *G_rowval,
+                      const double *G_nzval, const double *Rhs, int32_t flags);
 
 /* Net-rotation removal and Cartesian velocity.  Replaces updatevelocity2
  * (StokesSolver.jl:170-184) with volume_integral (src/Stress/FSE.jl:412-464).

@@ -291,11 +291,16 @@ def inner_projected(K, ifree, v):
     return solve
 
 
-def stokes_pcg(K, MM, G, F, ifree, rtol=1e-8, itmax=200, itmin=10, inner=None, pc=None):
-    """StokesPcCG (IterativeSolvers.jl:94-168).  Returns U, P, PCGInfo."""
+def stokes_pcg(K, MM, G, F, ifree, rtol=1e-8, itmax=200, itmin=10, inner=None, pc=None, P0=None):
+    """StokesPcCG (IterativeSolvers.jl:94-168).  Returns U, P, PCGInfo.
+
+    ``pc``: the lumped inverse (default), another vector (d = r * pc) or a callable d = pc(r).
+    ``P0``: the incoming pressure of StokesPcCG_pardiso (IterativeSolvers.jl:1-17), which keeps P
+    (its fill! is commented out) and adds GG * P to the right-hand side; None starts from zero.
+    """
     nU, nP = G.shape
     U = np.zeros(nU)
-    P = np.zeros(nP)
+    P = np.zeros(nP) if P0 is None else np.array(P0, dtype=np.float64)
     Rhs = F + G @ P
     solve = inner if inner is not None else inner_direct(K, ifree)
     U = solve(Rhs)
@@ -305,7 +310,8 @@ def stokes_pcg(K, MM, G, F, ifree, rtol=1e-8, itmax=200, itmin=10, inner=None, p
     tol = np.linalg.norm(r) * rtol
     if pc is None:
         pc = lumped_preconditioner(MM)
-    d = r * pc
+    apply_pc = pc if callable(pc) else (lambda res, v=pc: res * v)
+    d = apply_pc(r)
     q = d.copy()
     info = PCGInfo(residuals=[np.linalg.norm(r)])
     for it in range(1, itmax + 1):
@@ -327,10 +333,84 @@ def stokes_pcg(K, MM, G, F, ifree, rtol=1e-8, itmax=200, itmin=10, inner=None, p
         if nr < tol and it >= itmin:
             info.converged = True
             break
-        d = r * pc
+        d = apply_pc(r)
         beta = ((r - rlast) @ d) / rd
         q = d + beta * q
     return U, P, info
+
+
+def greedy_colour_order(A):
+    """Vertices grouped by a greedy first-fit colouring in natural order (colour 0 first)."""
+    A = sp.csr_matrix(A)
+    n = A.shape[0]
+    colour = -np.ones(n, dtype=np.int64)
+    for i in range(n):
+        used = {colour[j] for j in A.indices[A.indptr[i]:A.indptr[i + 1]] if colour[j] >= 0}
+        c = 0
+        while c in used:
+            c += 1
+        colour[i] = c
+    return np.argsort(colour, kind="stable"), colour
+
+
+def incomplete_cholesky(A, order):
+    """IC(0) of the symmetric matrix A in the elimination order ``order``: A ~ L D L^T with L unit
+    lower triangular (in that order) on A's pattern.  A pivot that is not positive restarts the
+    factorisation of A + alpha diag(A) with alpha = 1e-3, doubled on every restart.
+
+    The pressure preconditioner of StokesPcCG_pardiso is ``lldl(MM)`` (IterativeSolvers.jl:32,
+    :242-244) of LimitedLDLFactorizations 0.4.1, an incomplete LDL^T with a fill-reducing
+    permutation and limited memory; its factors are parity unpinned (no Julia).  This incomplete
+    factorisation of the same matrix is what the library computes; its defining property
+    (L D L^T equals A on the pattern) pins it.  Returns (L, D) with L in the original numbering
+    (a scipy CSR whose row i holds the entries L_ij of earlier vertices j) and D per vertex."""
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    n = A.shape[0]
+    pos = np.empty(n, dtype=np.int64)
+    pos[order] = np.arange(n)
+    alpha = 0.0
+    while True:
+        rows = {}
+        D = np.zeros(n)
+        ok = True
+        for i in order:
+            cols = A.indices[A.indptr[i]:A.indptr[i + 1]]
+            vals = A.data[A.indptr[i]:A.indptr[i + 1]]
+            row = {}
+            dii = 0.0
+            earlier = sorted((pos[j], j, v) for j, v in zip(cols, vals) if pos[j] < pos[i])
+            for j, v in zip(cols, vals):
+                if j == i:
+                    dii = v * (1.0 + alpha)
+            for _, j, v in earlier:
+                s_ = v - sum(row[k] * D[k] * lk for k, lk in rows[j].items() if k in row)
+                row[j] = s_ / D[j]
+                dii -= row[j] ** 2 * D[j]
+            rows[i] = row
+            D[i] = dii
+            if not dii > 1e-12 * abs(A[i, i]):
+                ok = False
+                break
+        if ok:
+            break
+        alpha = 2.0 * alpha if alpha > 0 else 1e-3
+    I = [i for i in rows for _ in rows[i]]
+    J = [j for i in rows for j in rows[i]]
+    V = [rows[i][j] for i in rows for j in rows[i]]
+    L = sp.csr_matrix((V, (I, J)), shape=(n, n))
+    return L, D
+
+
+def ic_preconditioner(MM):
+    """d = (L D L^T)^-1 r with the IC(0) of the symmetric MM in the greedy colour order."""
+    Mfull = symmetric_from_lower(MM)
+    order, _ = greedy_colour_order(Mfull)
+    L, D = incomplete_cholesky(Mfull, order)
+    Lu = (L + sp.identity(L.shape[0])).tocsr()
+    A = (Lu @ sp.diags(D) @ Lu.T).tocsc()
+    lu = spla.splu(A)
+    return lu.solve
 
 
 def volume_integral(V, mesh):

@@ -28,12 +28,35 @@ def test_binding_covers_the_header():
     assert set(_lib.EXPORTED) == set(declared_functions())
 
 
-def test_struct_layouts():
-    # int32 fields first, doubles 8-byte aligned
-    assert ctypes.sizeof(_lib.StkOpts) == 10 * 4 + 5 * 8 + 2 * 4
-    assert ctypes.sizeof(_lib.StkReport) == 8 + 8 + 8 + 3 * 8
+def test_struct_layouts(tmp_path):
+    # every field of the ctypes mirrors sits where gcc puts it from the header
+    import subprocess
+    structs = {"stk_opts": _lib.StkOpts, "stk_report": _lib.StkReport, "stk_train_opts": _lib.StkTrainOpts,
+               "stk_train_report": _lib.StkTrainReport}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dimspers_stokes.h"', 'int main(void) {']
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    root = Path(__file__).resolve().parent.parent
+    res = subprocess.run(["gcc", "-std=c11", "-I", str(root / "include"), str(src), "-o", str(exe)],
+                         capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        cname, field, val = line.split()
+        got[(cname, field)] = int(val)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
     o = _lib.default_opts()
     assert o.itmax == 200 and o.itmin == 10 and o.rtol == 1e-8
+    assert o.warm_start == 0 and o.pressure_pc == _lib.PPC_LUMPED
 
 
 def test_last_error_without_context():

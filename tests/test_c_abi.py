@@ -75,3 +75,70 @@ def test_c_driver_matches_the_ctypes_path(tmp_path, method):
     ctx.close()
     assert rep.outer_iterations == int(outer) and rep.inner_iterations == int(inner)
     assert np.array_equal(U, U_c) and np.array_equal(P, P_c)
+
+
+def _write_operators(d: Path, K, MM, G, F, P0, k_lower, warm, ppc, stop, null_mode, itmax, rtol, inner_rtol):
+    import scipy.sparse as sp
+
+    def put(name, A):
+        A = sp.csc_matrix(A)
+        A.sort_indices()
+        (d / f"{name}colptr.bin").write_bytes(np.asarray(A.indptr + 1, dtype=np.int64).tobytes())
+        (d / f"{name}rowval.bin").write_bytes(np.asarray(A.indices + 1, dtype=np.int64).tobytes())
+        (d / f"{name}nzval.bin").write_bytes(np.asarray(A.data, dtype=np.float64).tobytes())
+        return A.nnz
+    Ks = sp.tril(K, format="csc") if k_lower else K
+    nK, nM, nG = put("K", Ks), put("M", MM), put("G", G)
+    (d / "Rhs.bin").write_bytes(np.asarray(F, dtype=np.float64).tobytes())
+    if P0 is not None:
+        (d / "P0.bin").write_bytes(np.asarray(P0, dtype=np.float64).tobytes())
+    (d / "ops.txt").write_text(f"{nK} {nM} {nG} {int(k_lower)} 1 {int(warm)} {ppc} {stop} {null_mode} {itmax} "
+                               f"{rtol!r} {inner_rtol!r}\n")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", ["StokesPcCG", "StokesPcCG_pardiso"])
+def test_c_driver_solves_operators_the_caller_assembled(tmp_path, variant):
+    """StokesPcCG(U, P, KK, MM, GG, Rhs, ifree) on operators the caller built (here: the oracle's
+    assembly, _prepare_matrices and MM), handed over through stk_set_operators from C.
+
+    StokesPcCG (IterativeSolvers.jl:94-168): zero start, lumped preconditioner, Schur rtol 1e-8 -> the
+    oracle's literal algorithm (LU inner solve); the null-space deflation reproduces the LU solve of
+    K[ifree, ifree], whose near-null rotation FP64 still resolves at this size.
+    StokesPcCG_pardiso (:1-92): the incoming pressure kept (Rhs .+= GG*P), the incomplete Cholesky of
+    MM, rtol 1e-6 -> the oracle's same iteration from the same P0 with IC(0) of MM: the same iteration
+    count (within one) and the same iterate."""
+    from oracle import stokes_oracle as so
+    inp = make_inputs(scaled_workload(1, 8, 32))
+    m = inp.mesh
+    print(f"[case] C ABI driver, caller operators 8x32, {variant}", flush=True)
+    KK, GG, MM, Fb = so.assembly_stokes_cylindric(inp)
+    K, G, F = so.prepare_matrices(KK, GG, Fb, so.rotation_matrix(m.theta))
+    _, ifree = m.free_slip()
+    inner = so.inner_direct(K, ifree)
+    if variant == "StokesPcCG":
+        Uo, Po, info = so.stokes_pcg(K, MM, G, F, ifree, rtol=1e-8, inner=inner)
+        P0, warm, ppc, rtol = None, 0, _lib.PPC_LUMPED, 1e-8
+    else:
+        Ul, Pl, _ = so.stokes_pcg(K, MM, G, F, ifree, rtol=1e-12, itmax=400, inner=inner)
+        P0 = 0.9 * Pl + 0.05 * np.abs(Pl).max() * np.cos(3 * m.theta[:m.nvert])   # a previous step's pressure
+        Uo, Po, info = so.stokes_pcg(K, MM, G, F, ifree, rtol=1e-6, inner=inner, pc=so.ic_preconditioner(MM), P0=P0)
+        warm, ppc, rtol = 1, _lib.PPC_IC, 1e-6
+    assert info.converged
+    exe = _compile(tmp_path)
+    _write_inputs(inp, tmp_path)
+    _write_operators(tmp_path, K, MM, G, F, P0, k_lower=(variant == "StokesPcCG"), warm=warm, ppc=ppc,
+                     stop=_lib.STOP_SCHUR, null_mode=_lib.NULL_DEFLATE, itmax=200, rtol=rtol, inner_rtol=1e-13)
+    res = subprocess.run([str(exe), str(tmp_path), str(_lib.SOLVE_PLAIN), "operators"], capture_output=True,
+                         text=True, timeout=120, env=dict(os.environ))
+    print(res.stdout.strip(), res.stderr.strip()[-2000:])
+    assert res.returncode == 0
+    conv, outer, inner_its, resid, rc_bad, msg = (tmp_path / "report.txt").read_text().split()
+    assert int(conv) == 1 and int(rc_bad) == _lib.STK_E_ARG
+    U = np.fromfile(tmp_path / "U.bin", dtype=np.float64)
+    P = np.fromfile(tmp_path / "P.bin", dtype=np.float64)
+    eu = np.linalg.norm(U - Uo) / np.linalg.norm(Uo)
+    ep = np.linalg.norm((P - P.mean()) - (Po - Po.mean())) / np.linalg.norm(Po - Po.mean())
+    print(f"[result] {variant}: outer {outer} (oracle {info.iterations}), errU {eu:.2e} errP {ep:.2e}", flush=True)
+    assert abs(int(outer) - info.iterations) <= 1
+    assert eu < 1e-7 and ep < 1e-7

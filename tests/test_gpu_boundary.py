@@ -51,3 +51,81 @@ def test_other_velocity_bcs_match_the_literal_oracle(kind):
     assert rep.converged and rep.full_residual < 1e-11
     assert eu < 1e-8 and ep < 1e-8
     ctx.close()
+
+
+def _oracle_operators(inp):
+    KK, GG, MM, Fb = so.assembly_stokes_cylindric(inp)
+    K, G, F = so.prepare_matrices(KK, GG, Fb, so.rotation_matrix(inp.mesh.theta))
+    return K, MM, G, F
+
+
+def test_caller_operators_equal_the_assembled_ones_and_are_checked():
+    """stk_set_operators with the oracle's operators leaves the same FP64 operators as the
+    library's own assembly (K[ifree, ifree], G on free rows, full M) and solves to the same answer;
+    the lower-triangle form of K is accepted, an entry outside the mesh pattern is refused
+    (STK_E_SHAPE), and the multigrid methods are refused with the reason (STK_E_NOMESH)."""
+    import scipy.sparse as sp
+    inp = make_inputs(scaled_workload(3, 8, 32))
+    m = inp.mesh
+    print("[case] caller operators 8x32 workload 3", flush=True)
+    K, MM, G, F = _oracle_operators(inp)
+    own = _lib.Context.from_mesh(m)
+    own.set_fields(inp.g, inp.rho, inp.eta_ip, inp.C6)
+    own.assemble(_lib.ASM_FP64)
+    opts = _lib.default_opts(method=_lib.SOLVE_PLAIN, inner=_lib.INNER_JACOBI, stop=_lib.STOP_FULL, rtol=1e-10,
+                             itmax=400, inner_itmax=50000)
+    rep_own = own.solve(opts)
+    Uown, Pown = own.solution()
+    for k_lower in (False, True):
+        ctx = _lib.Context.from_mesh(m)
+        ctx.set_operators(sp.tril(K) if k_lower else K, MM, G, F, K_lower=k_lower)
+        for which in (_lib.MAT_K, _lib.MAT_G, _lib.MAT_GT, _lib.MAT_M):
+            a, b = ctx.export_csc(which), own.export_csc(which)
+            assert abs(a - b).max() <= 1e-12 * abs(b).max(), which
+        rep = ctx.solve(opts)
+        U, P = ctx.solution()
+        eu, ep = _errs(U, P, Uown, Pown)
+        print(f"[result] K_lower={k_lower}: outer {rep.outer_iterations} (own {rep_own.outer_iterations}) "
+              f"errU {eu:.2e} errP {ep:.2e}", flush=True)
+        assert rep.converged and eu < 1e-9 and ep < 1e-9
+        with pytest.raises(_lib.StkError) as e:
+            ctx.solve(_lib.default_opts(method=_lib.SOLVE_MIXED, stop=_lib.STOP_FULL))
+        assert e.value.code == _lib.STK_E_NOMESH and "stk_set_operators" in str(e.value)
+        ctx.close()
+    # an entry between two nodes that share no element
+    bad = K.tolil()
+    bad[0, m.nU - 1] = bad[m.nU - 1, 0] = 1.0
+    ctx = _lib.Context.from_mesh(m)
+    with pytest.raises(_lib.StkError) as e:
+        ctx.set_operators(bad.tocsc(), MM, G, F)
+    assert e.value.code == _lib.STK_E_SHAPE and "outside" in str(e.value)
+    ctx.close()
+    own.close()
+
+
+@pytest.mark.parametrize("method", [_lib.SOLVE_PLAIN, _lib.SOLVE_MIXED])
+def test_warm_start_from_the_solution_needs_no_iterations(method):
+    """A solve with warm_start = 1 starts from the context's solution: from the converged one it
+    meets the judged residual at once (no Schur iteration), and from a perturbed pressure it
+    converges back to the same solution."""
+    inp = make_inputs(scaled_workload(3, 16, 128))
+    print(f"[case] warm start 16x128 method {method}", flush=True)
+    ctx = _lib.Context.from_mesh(inp.mesh)
+    ctx.set_fields(inp.g, inp.rho, inp.eta_ip, inp.C6)
+    ctx.assemble(_lib.ASM_FP64 | _lib.ASM_FP32)
+    rep = ctx.solve(_lib.default_opts(method=method, stop=_lib.STOP_FULL, rtol=1e-9))
+    assert rep.converged and rep.outer_iterations > 0
+    U0, P0 = ctx.solution()
+    again = ctx.solve(_lib.default_opts(method=method, stop=_lib.STOP_FULL, rtol=1e-9, warm_start=1))
+    assert again.converged and again.outer_iterations == 0
+    U1, P1 = ctx.solution()
+    assert np.array_equal(P1, P0) and np.abs(U1 - U0).max() <= 1e-14 * np.abs(U0).max()
+    rng = np.random.default_rng(3)
+    ctx.set_solution(U0, P0 * (1 + 0.1 * rng.standard_normal(P0.size)))
+    rep2 = ctx.solve(_lib.default_opts(method=method, stop=_lib.STOP_FULL, rtol=1e-9, warm_start=1))
+    U2, P2 = ctx.solution()
+    eu, ep = _errs(U2, P2, U0, P0)
+    print(f"[result] warm from a perturbed pressure: outer {rep2.outer_iterations} (cold {rep.outer_iterations}) "
+          f"errU {eu:.2e} errP {ep:.2e}", flush=True)
+    assert rep2.converged and eu < 1e-7 and ep < 1e-7
+    ctx.close()

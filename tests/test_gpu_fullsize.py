@@ -11,27 +11,74 @@ from dimspers_amd.inputs import make_inputs
 pytestmark = pytest.mark.gpu
 
 
-def test_learned_mixed_matches_plain_fp64_at_full_size():
-    inp = make_inputs(4)
+def _errs(U, P, U0, P0):
+    eu = np.linalg.norm(U - U0) / np.linalg.norm(U0)
+    ep = np.linalg.norm((P - P.mean()) - (P0 - P0.mean())) / np.linalg.norm(P0 - P0.mean())
+    return eu, ep
+
+
+def _tight_fp64(ctx):
+    """Plain FP64 Schur PCG run to a 1e-11 full-system residual (inner solves to 1e-13): the reference
+    the fast solves are measured against, three orders tighter than the 1e-8 they are asked for."""
+    rep = ctx.solve(_lib.default_opts(method=_lib.SOLVE_PLAIN, stop=_lib.STOP_FULL, rtol=1e-11, itmax=800,
+                                      inner_rtol=1e-13, inner_itmax=4000))
+    print(f"[result] tight FP64 reference: {rep.solve_ms:.1f} ms ({rep.outer_iterations}/{rep.inner_iterations}), "
+          f"residual {rep.full_residual:.2e}", flush=True)
+    assert rep.converged and rep.full_residual < 1e-11
+    return ctx.solution()
+
+
+def _context(inp):
     m = inp.mesh
-    print(f"[case] full-size solve {m.ncell_r}x{m.ncell_t}", flush=True)
     ctx = _lib.Context.from_mesh(m)
     ctx.set_fields(inp.g, inp.rho, inp.eta_ip, inp.C6)
     ctx.assemble(_lib.ASM_FP64 | _lib.ASM_FP32)
+    return ctx
+
+
+def test_mixed_precision_at_full_size_against_a_tight_fp64_solve():
+    """Workload 3 (256 x 2048, contrast 1e4, ratio U[1, 100] per element): FP32 assembly copies,
+    SpMV and inner PCG under FP64 refinement reach the 1e-8 full residual; velocity and pressure
+    errors against the tight FP64 solve are below 1e-7."""
+    inp = make_inputs(3)
+    print(f"[case] workload 3 mixed {inp.mesh.ncell_r}x{inp.mesh.ncell_t}", flush=True)
+    ctx = _context(inp)
+    rep = ctx.solve(_lib.default_opts(method=_lib.SOLVE_MIXED, stop=_lib.STOP_FULL, rtol=1e-8))
+    assert rep.converged and rep.full_residual < 1e-8
+    U, P = ctx.solution()
+    U0, P0 = _tight_fp64(ctx)
+    eu, ep = _errs(U, P, U0, P0)
+    print(f"[result] mixed {rep.solve_ms:.1f} ms ({rep.outer_iterations}/{rep.inner_iterations}, "
+          f"{rep.refinements} sweeps), residual {rep.full_residual:.2e}: errU {eu:.2e} errP {ep:.2e}", flush=True)
+    assert eu < 1e-7 and ep < 1e-7
+    ctx.close()
+
+
+def test_learned_mixed_at_full_size_against_a_tight_fp64_solve_and_its_gain():
+    """Workload 4 (256 x 2048, learned): the learned + mixed solve the benchmark times and the mixed
+    solve without the model both reach the 1e-8 full residual with errors below 1e-7 against the tight
+    FP64 solve, and the model saves Schur iterations over the same solver without it."""
+    inp = make_inputs(4)
+    print(f"[case] workload 4 learned {inp.mesh.ncell_r}x{inp.mesh.ncell_t}", flush=True)
+    ctx = _context(inp)
     ctx.set_fabric(inp.phi, inp.delta)
     ctx.mlp_train(_lib.default_train_opts())
     ctx.learned_setup()
-    rep = ctx.solve(_lib.default_opts(method=_lib.SOLVE_LEARNED, stop=_lib.STOP_FULL, rtol=1e-8))
-    assert rep.converged and rep.full_residual < 1e-8
-    U, P = ctx.solution()
-    prep = ctx.solve(_lib.default_opts(method=_lib.SOLVE_PLAIN, stop=_lib.STOP_FULL, rtol=1e-8, itmax=400))
-    assert prep.converged and prep.full_residual < 1e-8
-    U0, P0 = ctx.solution()
-    eu = np.linalg.norm(U - U0) / np.linalg.norm(U0)
-    ep = np.linalg.norm((P - P.mean()) - (P0 - P0.mean())) / np.linalg.norm(P0 - P0.mean())
-    print(f"[result] learned {rep.solve_ms:.1f} ms ({rep.outer_iterations}/{rep.inner_iterations}), "
-          f"plain {prep.solve_ms:.1f} ms: errU {eu:.2e} errP {ep:.2e}")
-    assert eu < 1e-7 and ep < 1e-7
+    reps, sols = {}, {}
+    for name, method in (("learned", _lib.SOLVE_LEARNED), ("mixed", _lib.SOLVE_MIXED)):
+        rep = ctx.solve(_lib.default_opts(method=method, stop=_lib.STOP_FULL, rtol=1e-8))
+        assert rep.converged and rep.full_residual < 1e-8
+        reps[name], sols[name] = rep, ctx.solution()
+    U0, P0 = _tight_fp64(ctx)
+    for name in ("learned", "mixed"):
+        rep = reps[name]
+        eu, ep = _errs(*sols[name], U0, P0)
+        print(f"[result] {name} {rep.solve_ms:.1f} ms ({rep.outer_iterations}/{rep.inner_iterations}, "
+              f"{rep.refinements} sweeps, {rep.fallbacks} fallbacks), residual {rep.full_residual:.2e}: "
+              f"errU {eu:.2e} errP {ep:.2e}", flush=True)
+        assert eu < 1e-7 and ep < 1e-7
+    assert reps["learned"].fallbacks == 0
+    assert reps["learned"].outer_iterations < reps["mixed"].outer_iterations
     ctx.close()
 
 

@@ -493,3 +493,52 @@ def test_volume_integral_of_polynomials_in_r_is_exact():
     for k in range(3):
         want = 2 * np.pi * (R_OUT ** (k + 2) - R_IN ** (k + 2)) / (k + 2)
         assert so.volume_integral(m.r ** k, m) == pytest.approx(want, rel=1e-13)
+
+
+def test_incomplete_cholesky_reproduces_the_matrix_on_its_pattern():
+    # IC(0) is defined by (L D L^T)_ij = A_ij on A's pattern; where the elimination order creates no
+    # fill (a path graph in natural order) it is the exact Cholesky factorisation.
+    inp = _small_inputs(8, 32)
+    KK, GG, MM, Fb = so.assembly_stokes_cylindric(inp)
+    M = so.symmetric_from_lower(MM).tocsr()
+    order, colour = so.greedy_colour_order(M)
+    Mc = M.tocoo()
+    off = Mc.row != Mc.col
+    assert np.all(colour[Mc.row[off]] != colour[Mc.col[off]])         # colours are independent sets
+    L, D = so.incomplete_cholesky(M, order)
+    assert np.all(D > 0)
+    pos = np.empty(M.shape[0], dtype=np.int64)
+    pos[order] = np.arange(M.shape[0])
+    Lc = L.tocoo()
+    assert np.all(pos[Lc.col] < pos[Lc.row])                          # strictly earlier in the order
+    Lu = (L + sp.identity(M.shape[0])).tocsr()
+    A = (Lu @ sp.diags(D) @ Lu.T).tocsr()
+    on = np.asarray(A[Mc.row, Mc.col]).ravel()
+    assert np.abs(on - Mc.data).max() < 1e-14 * np.abs(Mc.data).max()
+    n = 9
+    T = sp.diags([np.full(n - 1, -1.0), np.full(n, 2.5), np.full(n - 1, -1.0)], [-1, 0, 1]).tocsr()
+    L, D = so.incomplete_cholesky(T, np.arange(n))
+    Lu = (L + sp.identity(n)).toarray()
+    assert np.allclose(Lu @ np.diag(D) @ Lu.T, T.toarray(), atol=1e-14)
+
+
+def test_pardiso_variant_keeps_the_incoming_pressure():
+    # StokesPcCG_pardiso (IterativeSolvers.jl:1-17) keeps P and adds GG * P to the right-hand side: from
+    # the converged pressure the initial Schur residual vanishes (it does not for a pressure subtracted
+    # instead of added, or one zeroed), and with the IC(0) preconditioner of MM the iteration converges
+    # to the solution of the dense saddle solve.
+    inp = _small_inputs(4, 16, w=1)
+    sol, _ = so.solve_stokes(inp, rtol=1e-13, itmax=400)
+    K, G, F, ifree, MM = sol.K, sol.G, sol.F, sol.ifree, sol.MM
+    inner = so.inner_direct(K, ifree)
+    _, _, cold = so.stokes_pcg(K, MM, G, F, ifree, rtol=1e-8, inner=inner)
+    _, _, warm = so.stokes_pcg(K, MM, G, F, ifree, rtol=1e-8, inner=inner, P0=sol.P)
+    assert warm.residuals[0] < 1e-9 * cold.residuals[0]
+    U, P, info = so.stokes_pcg(K, MM, G, F, ifree, rtol=1e-12, itmax=400, inner=inner,
+                               pc=so.ic_preconditioner(MM), P0=0.5 * sol.P)
+    assert info.converged
+    Ud, Pd = _dense_saddle_solution(sol)
+    Ud, _ = so.updatevelocity2(Ud, inp.rho, inp.mesh)
+    U, _ = so.updatevelocity2(U, inp.rho, inp.mesh)
+    assert np.linalg.norm(U - Ud) / np.linalg.norm(Ud) < 1e-9
+    assert np.linalg.norm((P - P.mean()) - Pd) / np.linalg.norm(Pd) < 1e-9
