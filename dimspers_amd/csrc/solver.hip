@@ -579,6 +579,7 @@ int64_t inner_cg(Context &c, MGView<T> &v, bool use_mg, T *y, T *z, double rtol,
     int64_t it = 0;
     *converged = false;
     double hist[8];
+    double best = bb;   // smallest |r|^2 seen at a check
     while (it < itmax) {
         const int rz_cur = SC_RZV + 2 * par, rz_new = SC_RZV + 2 * (1 - par);
         // Ap = K p with (p . Ap, v . Ap)
@@ -606,9 +607,16 @@ int64_t inner_cg(Context &c, MGView<T> &v, bool use_mg, T *y, T *z, double rtol,
             }
             if (bad) throw Error(STK_E_NAN, "non-finite residual in the inner CG");
             if (done) { *converged = true; break; }
-            bool grew = false;   // rounding-dominated right-hand side: stop instead of diverging
-            for (int k = 0; k < m; ++k) grew |= hist[k] > 1e6 * bb;
-            if (grew) break;
+            // A tolerance below the attainable accuracy: once the recursive residual has stagnated
+            // CG drifts and the residual climbs.  Stop when it is 100x (in norm) above the smallest
+            // one seen, so the returned iterate stays near the attainable floor.
+            bool grew = false;
+            for (int k = 0; k < m; ++k) grew |= hist[k] > 1e6 * bb || hist[k] > 1e4 * best;
+            if (grew) {
+                if (trace_on()) std::fprintf(stderr, "[stk] inner CG stagnated at %.3e\n", std::sqrt(best / bb));
+                break;
+            }
+            for (int k = 0; k < m; ++k) best = std::min(best, hist[k]);
         }
         // zz = M^-1 r with (r . zz, v . zz); p = P zz + be p
         T *zz = precond(r, rz_new);

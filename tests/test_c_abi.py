@@ -137,6 +137,11 @@ def test_c_driver_solves_operators_the_caller_assembled(tmp_path, variant):
     assert int(conv) == 1 and int(rc_bad) == _lib.STK_E_ARG
     U = np.fromfile(tmp_path / "U.bin", dtype=np.float64)
     P = np.fromfile(tmp_path / "P.bin", dtype=np.float64)
+    # solveStokes removes the net rotation right after the solve (updatevelocity2,
+    # StokesSolver.jl:48-50); before it, the LU answer carries an ill-determined multiple of the
+    # near-null rotation of K[ifree, ifree]
+    U, _ = so.updatevelocity2(U, inp.rho, m)
+    Uo, _ = so.updatevelocity2(Uo, inp.rho, m)
     eu = np.linalg.norm(U - Uo) / np.linalg.norm(Uo)
     ep = np.linalg.norm((P - P.mean()) - (Po - Po.mean())) / np.linalg.norm(Po - Po.mean())
     print(f"[result] {variant}: outer {outer} (oracle {info.iterations}), errU {eu:.2e} errP {ep:.2e}", flush=True)

@@ -1,7 +1,8 @@
-"""Full-size (256 x 2048, 4.73M dofs) solves: the learned + mixed path that the benchmark times
-agrees with the plain FP64 Schur PCG, and both meet the 1e-8 full-system residual.  The oracle cannot
-run at this size; the two GPU methods share only the assembled operators, so their agreement to
-the parity bar (1e-7 relative L2) is the size-independent check."""
+"""Full-size solves (256 x 2048, 4.73M dofs, and 128 x 1024, 512 x 4096): the mixed-precision and the
+learned + mixed paths meet the 1e-8 full-system residual and agree with a plain FP64 Schur PCG solved
+to a 1e-11 residual within the parity bar (1e-7 relative L2 on velocity and pressure).  The oracle
+cannot run at these sizes; the methods share only the assembled operators, so their agreement with the
+tight FP64 solve is the size-independent check."""
 import numpy as np
 import pytest
 
@@ -18,10 +19,11 @@ def _errs(U, P, U0, P0):
 
 
 def _tight_fp64(ctx):
-    """Plain FP64 Schur PCG run to a 1e-11 full-system residual (inner solves to 1e-13): the reference
-    the fast solves are measured against, three orders tighter than the 1e-8 they are asked for."""
-    rep = ctx.solve(_lib.default_opts(method=_lib.SOLVE_PLAIN, stop=_lib.STOP_FULL, rtol=1e-11, itmax=800,
-                                      inner_rtol=1e-13, inner_itmax=4000))
+    """Plain FP64 Schur PCG run to a 1e-11 full-system residual: the reference the fast solves are
+    measured against, three orders tighter than the 1e-8 they are asked for.  The FP64 floor of this
+    residual at 256 x 2048 is about 2e-12 (asked for 1e-12 or 1e-13 the solve stalls there), and the
+    solution at that floor differs from this one by 5e-13 in velocity."""
+    rep = ctx.solve(_lib.default_opts(method=_lib.SOLVE_PLAIN, stop=_lib.STOP_FULL, rtol=1e-11, itmax=800))
     print(f"[result] tight FP64 reference: {rep.solve_ms:.1f} ms ({rep.outer_iterations}/{rep.inner_iterations}), "
           f"residual {rep.full_residual:.2e}", flush=True)
     assert rep.converged and rep.full_residual < 1e-11
