@@ -31,6 +31,7 @@ EXPORTED = [
     "stk_mlp_params", "stk_mlp_forward", "stk_mlp_gradient", "stk_local_schur", "stk_learned_setup",
     "stk_set_stress_fields", "stk_stress", "stk_deformation_gradient", "stk_stats", "stk_thermal_solve",
     "stk_fse", "stk_set_dem", "stk_anisotropic_tensor", "stk_colouring", "stk_set_solution", "stk_set_operators",
+    "stk_stats_noalloc",
 ]
 N_PARAM = 9094
 KERNEL_SPMV_K32, KERNEL_SPMV_K64, KERNEL_CHEB_K32, KERNEL_CHEB_K64, KERNEL_ASSEMBLY, KERNEL_CHEB_KH16 = 0, 1, 2, 3, 4, 5
@@ -117,6 +118,7 @@ def load(path: Path | str | None = None):
     lib.stk_set_fields.argtypes = [vp, C.c_double, dp, dp, C.c_int64, C.POINTER(dp)]
     lib.stk_assemble.argtypes = [vp, C.c_int32]
     lib.stk_set_solution.argtypes = [vp, dp, dp]
+    lib.stk_stats_noalloc.argtypes = [vp, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.stk_set_operators.argtypes = [vp, C.c_int32, i64p, i64p, dp, C.c_int32, i64p, i64p, dp, i64p, i64p, dp, dp,
                                       C.c_int32]
     lib.stk_export_csc.argtypes = [vp, C.c_int32, C.c_int32, i64p, i64p, i64p, i64p, i64p, dp]

@@ -162,6 +162,15 @@ int stk_create(stk_ctx **out, int64_t nel, int64_t nnod, int64_t nvert, const in
         c.d_sc.alloc(SC_COUNT);
         c.d_sc.zero(c.stream);
         STK_HIP(hipHostMalloc(&c.h_pinned, SC_COUNT * sizeof(double), hipHostMallocDefault));
+        // statistics data (stk_stats, stk_stats_noalloc) once per mesh; a mesh without the radius
+        // levels of write_stats keeps the reason for those calls
+        try {
+            statistics_prepare(c);
+        } catch (const Error &e) {
+            c.stats_note = e.what();
+        }
+        // this device's copy of stk_stats_noalloc's scratch array (the device is current)
+        c.stats_scratch = stats_scratch_address(c.device);
         STK_HIP(hipStreamSynchronize(c.stream));
         return STK_OK;
     });
@@ -812,6 +821,14 @@ int stk_stats(stk_ctx *ctx, const double *U, const double *T, double out[4]) {
         STK_CHECK(Ud, STK_E_STATE, "no device solution: solve first or pass U");
         t.upload(T, c.nnod, c.stream);
         statistics(c, Ud, t.p, out);
+        return STK_OK;
+    });
+}
+
+int stk_stats_noalloc(stk_ctx *ctx, const double *U, const double *T, double *out) {
+    if (!ctx || !U || !T || !out) return STK_E_ARG;
+    return guarded(ctx, [&]() {
+        statistics_noalloc(ctx->c, U, T, out);
         return STK_OK;
     });
 }

@@ -279,6 +279,8 @@ struct Context {
     // statistics: nodes one radius level inside each wall, spacings, wall radii
     DBuf<int32_t> d_itop, d_ibot;
     double nu_dr = 0, nu_dth = 0, nu_rin = 0, nu_rout = 0;
+    std::string stats_note;                  // why the statistics data could not be prepared
+    double *stats_scratch = nullptr;         // this device's fixed scratch array (stk_stats_noalloc)
 
     // fabric: finite strain ellipses of the last stk_fse, the DEM table, the viscous tensor of
     // the last stk_anisotropic_tensor

@@ -13,5 +13,12 @@ void launch_stress(Context &c, const double *U, double *out);
 void launch_deformation_gradient(Context &c, const double *U, double dt, double *F);
 // write_stats: out = (Urms, <T>, Nusselt top, Nusselt bottom); U, T device arrays
 void statistics(Context &c, const double *U, const double *T, double out[4]);
+// Once per mesh: natural-order element data and the Nusselt node lists of statistics.
+void statistics_prepare(Context &c);
+// Device address of the fixed scratch array of statistics_noalloc on `device` (the current device).
+double *stats_scratch_address(int device);
+// statistics on device arrays into the device array out[4], enqueued on the context stream,
+// allocating nothing: its scratch is the fixed per-device array.
+void statistics_noalloc(Context &c, const double *U, const double *T, double *out);
 
 }  // namespace stk

@@ -18,6 +18,9 @@
 #include <cmath>
 #include <vector>
 
+#include <map>
+#include <mutex>
+
 #include "device_ops.h"
 #include "stress.h"
 
@@ -281,28 +284,65 @@ void launch_stress(Context &c, const double *U, double *out) {
 }
 
 // write_stats (src/IO/IOmanager.jl:277-322) without the file: Urms, <T>, Nusselt top/bottom.
-void statistics(Context &c, const double *U, const double *T, double out[4]) {
+// Once per mesh: the natural-order element data and ScratchNusselt's node lists (IOmanager.jl:338-354),
+// the nodes one radius level inside each wall.
+void statistics_prepare(Context &c) {
     stress_prepare(c);
-    if (!c.d_itop) {
-        // ScratchNusselt (IOmanager.jl:338-354): nodes one radius level inside each wall
-        std::vector<double> ru(c.node_r);
-        std::sort(ru.begin(), ru.end());
-        ru.erase(std::unique(ru.begin(), ru.end()), ru.end());
-        if (ru.size() < 3) throw Error(STK_E_SHAPE, "statistics need at least three radius levels");
-        std::vector<int32_t> top, bot;
-        for (int64_t n = 0; n < c.nnod; ++n) {
-            if (c.node_r[n] == ru[ru.size() - 2]) top.push_back((int32_t)n);
-            if (c.node_r[n] == ru[1]) bot.push_back((int32_t)n);
-        }
-        if (top.size() < 2) throw Error(STK_E_SHAPE, "statistics need two nodes below the outer wall");
-        c.nu_dr = ru[1] - ru[0];
-        c.nu_dth = c.node_th[top[0]] - c.node_th[top[1]];
-        c.nu_rin = ru.front();
-        c.nu_rout = ru.back();
-        c.d_itop.upload(top, c.stream);
-        c.d_ibot.upload(bot, c.stream);
-        STK_HIP(hipStreamSynchronize(c.stream));
+    if (c.d_itop) return;
+    std::vector<double> ru(c.node_r);
+    std::sort(ru.begin(), ru.end());
+    ru.erase(std::unique(ru.begin(), ru.end()), ru.end());
+    if (ru.size() < 3) throw Error(STK_E_SHAPE, "statistics need at least three radius levels");
+    std::vector<int32_t> top, bot;
+    for (int64_t n = 0; n < c.nnod; ++n) {
+        if (c.node_r[n] == ru[ru.size() - 2]) top.push_back((int32_t)n);
+        if (c.node_r[n] == ru[1]) bot.push_back((int32_t)n);
     }
+    if (top.size() < 2) throw Error(STK_E_SHAPE, "statistics need two nodes below the outer wall");
+    c.nu_dr = ru[1] - ru[0];
+    c.nu_dth = c.node_th[top[0]] - c.node_th[top[1]];
+    c.nu_rin = ru.front();
+    c.nu_rout = ru.back();
+    c.d_itop.upload(top, c.stream);
+    c.d_ibot.upload(bot, c.stream);
+    STK_HIP(hipStreamSynchronize(c.stream));
+}
+
+namespace {
+
+// The constants that turn the four sums into write_stats' numbers.
+struct StatsCombine {
+    double area;   // pi (r_out^2 - r_in^2) from the extreme node radii
+    double A;      // log(f) / (2 pi 2.22 (1 - f)), f = 1.22 / 2.22: radii fixed in ScratchNusselt (:341)
+    double dth, dr, rin, rout, nbot;
+};
+
+StatsCombine stats_combine(const Context &c) {
+    const double f = 1.22 / 2.22;
+    return StatsCombine{M_PI * (c.nu_rout * c.nu_rout - c.nu_rin * c.nu_rin),
+                        std::log(f) / (2.0 * M_PI * 2.22 * (1.0 - f)), c.nu_dth, c.nu_dr, c.nu_rin, c.nu_rout,
+                        (double)c.d_ibot.n};
+}
+
+__host__ __device__ inline void combine_stats(const StatsCombine &k, const double h[4], double out[4]) {
+    out[0] = sqrt(h[0] / k.area);
+    out[1] = h[1] / k.area;
+    out[2] = k.A * k.dth * (-h[2] / k.dr) * k.rout;
+    out[3] = k.A * k.dth * ((h[3] - k.nbot) / k.dr) * k.rin;
+}
+
+__global__ void k_stats_combine(StatsCombine k, const double *__restrict__ h, double *__restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double hv[4] = {h[0], h[1], h[2], h[3]}, o[4];
+        combine_stats(k, hv, o);
+        for (int i = 0; i < 4; ++i) out[i] = o[i];
+    }
+}
+
+// The four sums of write_stats into sums[0..3]: int |U|^2 and int T over the straight triangles
+// (volume_integral_cartesian, src/Stress/FSE.jl:466-514: 6-point rule, P2 interpolation), and the sums
+// of T over the two Nusselt node lists.  part holds 2 * kReduceBlocks partials.
+void statistics_launch(Context &c, const double *U, const double *T, double *part, double *sums) {
     // integrated P2 weights of the 6-point rule (ip_triangle6, Quadrature.jl:80-107)
     VolIntArgs a{};
     {
@@ -326,30 +366,66 @@ void statistics(Context &c, const double *U, const double *T, double out[4]) {
     a.nr = c.d_noder.p;
     a.U = U;
     a.T = T;
-    double *part = c.red64.p, *sc = c.d_sc.p;
-    const int nb = (int)std::min<int64_t>(kReduceBlocks, std::max<int64_t>(1, (c.nel + 255) / 256));
-    if ((int64_t)c.red64.n < 2 * (int64_t)nb) throw Error(STK_E_STATE, "reduction buffer too small");
+    auto blocks = [](int64_t n) { return (int)std::min<int64_t>(kReduceBlocks, std::max<int64_t>(1, (n + 255) / 256)); };
+    const int nb = blocks(c.nel);
     k_reduce_map<256, 2, VolInt><<<dim3(nb), dim3(256), 0, c.stream>>>(c.nel, VolInt{a}, part);
-    k_reduce_final<256, 2><<<dim3(1), dim3(256), 0, c.stream>>>(nb, part, sc + 0);
+    k_reduce_final<256, 2><<<dim3(1), dim3(256), 0, c.stream>>>(nb, part, sums + 0);
     ListSum lt{c.d_itop.p, T}, lb{c.d_ibot.p, T};
     const int64_t ntop = (int64_t)c.d_itop.n, nbot = (int64_t)c.d_ibot.n;
-    const int nbt = (int)std::min<int64_t>(kReduceBlocks, std::max<int64_t>(1, (ntop + 255) / 256));
+    const int nbt = blocks(ntop), nbb = blocks(nbot);
     k_reduce_map<256, 1, ListSum><<<dim3(nbt), dim3(256), 0, c.stream>>>(ntop, lt, part);
-    k_reduce_final<256, 1><<<dim3(1), dim3(256), 0, c.stream>>>(nbt, part, sc + 2);
-    const int nbb = (int)std::min<int64_t>(kReduceBlocks, std::max<int64_t>(1, (nbot + 255) / 256));
+    k_reduce_final<256, 1><<<dim3(1), dim3(256), 0, c.stream>>>(nbt, part, sums + 2);
     k_reduce_map<256, 1, ListSum><<<dim3(nbb), dim3(256), 0, c.stream>>>(nbot, lb, part);
-    k_reduce_final<256, 1><<<dim3(1), dim3(256), 0, c.stream>>>(nbb, part, sc + 3);
+    k_reduce_final<256, 1><<<dim3(1), dim3(256), 0, c.stream>>>(nbb, part, sums + 3);
     STK_HIP(hipGetLastError());
+}
+
+// Scratch of stats_noalloc: the reduction partials (at most 2 per block and kReduceBlocks blocks:
+// every reduction runs a grid-stride loop over at most that many blocks, whatever the mesh size)
+// and the four sums.  One copy per device, held until the process ends and shared by every call on
+// that device; nothing in it has to start at zero (each launch writes what the next one reads).
+constexpr int64_t kStatsScratch = 2 * kReduceBlocks + 8;
+static __device__ __align__(256) double scratch[kStatsScratch];
+
+}  // namespace
+
+void statistics(Context &c, const double *U, const double *T, double out[4]) {
+    statistics_prepare(c);
+    double *part = c.red64.p, *sc = c.d_sc.p;
+    if ((int64_t)c.red64.n < 2 * (int64_t)kReduceBlocks) throw Error(STK_E_STATE, "reduction buffer too small");
+    statistics_launch(c, U, T, part, sc);
     double h[4];
     STK_HIP(hipMemcpyAsync(h, sc, 4 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     STK_HIP(hipStreamSynchronize(c.stream));
-    const double area = M_PI * (c.nu_rout * c.nu_rout - c.nu_rin * c.nu_rin);
-    const double f = 1.22 / 2.22;                       // radii fixed in ScratchNusselt (:341)
-    const double A = std::log(f) / (2.0 * M_PI * 2.22 * (1.0 - f));
-    out[0] = std::sqrt(h[0] / area);
-    out[1] = h[1] / area;
-    out[2] = A * c.nu_dth * (-h[2] / c.nu_dr) * c.nu_rout;
-    out[3] = A * c.nu_dth * ((h[3] - (double)nbot) / c.nu_dr) * c.nu_rin;
+    combine_stats(stats_combine(c), h, out);
+}
+
+// Device address of this device's scratch array; the current device must be the context's.
+// hipGetSymbolAddress is called once per device (each device holds its own copy at its own address).
+double *stats_scratch_address(int device) {
+    static std::mutex mu;
+    static std::map<int, double *> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) return it->second;
+    double *p = nullptr;
+    STK_HIP(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(STK_E_HIP, "hipGetSymbolAddress returned a null scratch address");
+    addr[device] = p;
+    return p;
+}
+
+void statistics_noalloc(Context &c, const double *U, const double *T, double *out) {
+    if (!c.d_itop) throw Error(STK_E_STATE, "statistics data not prepared for this mesh" +
+                                                (c.stats_note.empty() ? std::string() : ": " + c.stats_note));
+    if (!c.stats_scratch) throw Error(STK_E_STATE, "no scratch address for this device");
+    // every reduction uses at most kReduceBlocks partials of two columns, then four sums
+    const int64_t need = 2 * (int64_t)kReduceBlocks + 4;
+    if (need > kStatsScratch) throw Error(STK_E_SHAPE, "statistics need more scratch than the fixed array holds");
+    double *part = c.stats_scratch, *sums = c.stats_scratch + 2 * kReduceBlocks;
+    statistics_launch(c, U, T, part, sums);
+    k_stats_combine<<<dim3(1), dim3(64), 0, c.stream>>>(stats_combine(c), sums, out);
+    STK_HIP(hipGetLastError());
 }
 
 void launch_deformation_gradient(Context &c, const double *U, double dt, double *F) {

@@ -285,6 +285,17 @@ int stk_deformation_gradient(stk_ctx *ctx, const double *U, double dt, double *F
  * (NULL = the device solution); T: temperature at every node (nnod, host). */
 int stk_stats(stk_ctx *ctx, const double *U, const double *T, double out[4]);
 
+/* stk_stats that allocates nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight time-step loops, harnesses that forbid
+ * hipMalloc).  U (nU: 2 per node), T (nnod) and out (4 doubles, receiving what stk_stats returns) are
+ * DEVICE pointers on the context's device; the work is enqueued on the context stream and the call
+ * returns without waiting.  Its reduction partials and sums live in one fixed device array of
+ * 2 * 1024 + 8 doubles (16 KiB) per device, held until the process ends and shared by every call on
+ * that device: calls that overlap on one device (other streams, other threads) race.  The scratch need
+ * is bounded whatever the mesh size (grid-stride reductions over at most 1024 blocks); a larger need is
+ * refused with STK_E_SHAPE.  STK_E_STATE if the mesh lacks the radius levels write_stats needs. */
+int stk_stats_noalloc(stk_ctx *ctx, const double *U, const double *T, double *out);
+
 /* One thermal diffusion step: solveDiffusion_threaded (src/ThermalDiffusion/
  * AssemblerThermal.jl:1-62) with assemble_element! (:117-209; 7-point rule,
  * MAIN.jl:167-169), _prepare_matrices! (ThermalSolver.jl:54-63: Rhs = M T0 + F,
