@@ -16,6 +16,7 @@ STK_E_HIP, STK_E_SHAPE, STK_E_STATE, STK_E_NOTCONV, STK_E_NAN, STK_E_ARG, STK_E_
 
 ASM_FP64, ASM_FP32, ASM_CARTESIAN, ASM_NO_BC, ASM_ISOTROPIC, ASM_NO_MG = 1, 2, 4, 8, 16, 32
 ASM_FP16_SMOOTHER = 64
+ASM_7POINT, ASM_CLAMP = 128, 256
 SMOOTH_GS, SMOOTH_CHEB = 0, 1
 MAT_K, MAT_G, MAT_GT, MAT_M, MAT_S, MAT_KT, MAT_MT, MAT_R, MAT_RK = 0, 1, 2, 3, 4, 5, 6, 7, 8
 SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED = 0, 1, 2

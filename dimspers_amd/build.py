@@ -9,7 +9,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libdimspers_stokes.so"
-SOURCES = ["setup.cpp", "assembly.hip", "solver.hip", "mlp.hip", "stress.hip", "thermal.hip", "fabric.hip", "api.hip"]
+SOURCES = ["setup.cpp", "assembly.hip", "element.hip", "solver.hip", "mlp.hip", "stress.hip", "thermal.hip", "fabric.hip", "api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -144,6 +144,7 @@ int stk_create(stk_ctx **out, int64_t nel, int64_t nnod, int64_t nvert, const in
             build_gs_layout(c);
             c.gs.on = true;
         }
+        stress_prepare(c);           // natural-order element data (assembly, stress, statistics)
         if (!c.mg.empty()) {
             STK_CHECK(c.mg[0].pat.ptr == c.MN.ptr && c.mg[0].pat.col == c.MN.col, STK_E_SHAPE,
                       "vertex pattern mismatch between mesh and multigrid level 0");
@@ -220,13 +221,16 @@ int stk_set_fields(stk_ctx *ctx, double g, const double *rho, const double *eta_
         STK_CHECK(nip >= 6, STK_E_SHAPE, "eta.ip needs at least 6 integration-point columns");
         c.g = g;
         c.d_rho.upload(rho, c.nnod, c.stream);
-        // Julia nel x nip column-major: the first six columns are contiguous
-        c.d_eta.upload(eta_ip, 6 * c.nel, c.stream);
+        // Julia nel x nip column-major: the first columns are contiguous; up to 7 are kept (the
+        // 6-point rule reads 6, the 7-point rule 7)
+        const int64_t ncol = std::min<int64_t>(nip, 7);
+        c.field_cols = ncol;
+        c.d_eta.upload(eta_ip, ncol * c.nel, c.stream);
         if (C) {
-            std::vector<double> buf(36 * c.nel);
+            std::vector<double> buf(6 * ncol * c.nel);
             for (int k = 0; k < 6; ++k) {
                 STK_CHECK(C[k], STK_E_ARG, "NULL tensor component");
-                std::copy(C[k], C[k] + 6 * c.nel, buf.begin() + (int64_t)k * 6 * c.nel);
+                std::copy(C[k], C[k] + ncol * c.nel, buf.begin() + (int64_t)k * ncol * c.nel);
             }
             c.d_C.upload(buf, c.stream);
             c.have_C = true;
@@ -247,11 +251,12 @@ namespace {
 template <typename T>
 void assemble_prec(Context &c, int flags, bool mg, DBuf<T> &K, DBuf<T> &G, DBuf<T> &GT, DBuf<T> &M, DBuf<T> &F,
                    DBuf<T> &RK, bool f64) {
-    K.alloc(4 * c.KN.nstore()); K.zero(c.stream);
-    G.alloc(2 * c.GN.nstore()); G.zero(c.stream);
-    GT.alloc(2 * c.GTN.nstore()); GT.zero(c.stream);
-    M.alloc(c.MN.nnz()); M.zero(c.stream);
-    F.alloc(c.nU); F.zero(c.stream);
+    // the gathers of the element assembly write every stored position of these
+    K.alloc(4 * c.KN.nstore());
+    G.alloc(2 * c.GN.nstore());
+    GT.alloc(2 * c.GTN.nstore());
+    M.alloc(c.MN.nnz());
+    F.alloc(c.nU);
     T *A1 = nullptr, *E1 = nullptr;
     auto Abuf = [&](MGLevel &L) -> DBuf<T> & {
         if constexpr (sizeof(T) == 8) return L.A64; else return L.A32;
@@ -260,15 +265,15 @@ void assemble_prec(Context &c, int flags, bool mg, DBuf<T> &K, DBuf<T> &G, DBuf<
         if constexpr (sizeof(T) == 8) return L.E64; else return L.E32;
     };
     if (mg) {
-        for (MGLevel &L : c.mg) {
+        for (size_t l = 0; l < c.mg.size(); ++l) {
+            MGLevel &L = c.mg[l];
             Abuf(L).alloc(4 * L.pat.nstore());
-            Abuf(L).zero(c.stream);
+            if (l > 0) Abuf(L).zero(c.stream);   // the coarse Galerkin levels accumulate; level 0 is gathered
             Ebuf(L).alloc(36 * L.nel);
         }
         A1 = Abuf(c.mg[0]).p;
         E1 = Ebuf(c.mg[0]).p;
         RK.alloc(4 * c.GTN.nstore());
-        RK.zero(c.stream);
     } else {
         RK.release();
     }
@@ -305,6 +310,9 @@ int stk_assemble(stk_ctx *ctx, int32_t flags) {
     return guarded(ctx, [&]() {
         Context &c = ctx->c;
         STK_CHECK(c.have_fields, STK_E_STATE, "stk_set_fields must precede stk_assemble");
+        STK_CHECK(!(flags & STK_ASM_7POINT) || c.field_cols >= 7, STK_E_STATE,
+                  "the 7-point rule needs 7 columns of eta.ip and of the tensor in stk_set_fields");
+        if (!c.gK.npos) build_gather_lists(c);   // once per mesh, at its first assembly
         const bool fp32 = flags & STK_ASM_FP32;
         const bool fp64 = (flags & STK_ASM_FP64) || !fp32;
         const bool solve_form = !(flags & (STK_ASM_CARTESIAN | STK_ASM_NO_BC));

@@ -69,28 +69,6 @@ __device__ __forceinline__ void p2_shape(T r, T s, T N[6], T dr[6], T ds[6]) {
 // index of lower block (a >= b) among the 21 node-pair blocks
 __device__ __forceinline__ int lblk(int a, int b) { return a * (a + 1) / 2 + b; }
 
-template <typename T>
-struct AsmArgs {
-    int64_t nel, off, cnt;
-    const int32_t *perm, *e2n;          // e2n: 6 x nel, permuted SoA
-    const double *th, *r;               // 3 x nel permuted SoA (vertex angles after fixangles)
-    const double *nodecs;               // 2 x nnod
-    const uint8_t *fixed;               // per dof
-    const double *rho, *eta, *C;        // rho: nnod; eta: 6 x nel (ip-major, element id); C: 6 x 6 x nel
-    double g;
-    const uint8_t *sK, *sG, *sGT, *sM;  // slot offsets, SoA by position
-    const int32_t *Kptr, *Gptr, *GTptr, *Mptr;
-    const int64_t *Ksptr;               // sliced layout of K (null: CSR order)
-    const int64_t *A1sptr;              // sliced layout of the P1 level operator (null: CSR)
-    const int64_t *Gsptr, *GTsptr;      // sliced layouts of G and G^T (null: CSR)
-    T *K, *G, *GT, *M, *F;
-    // finest multigrid level (vertex space)
-    T *A1, *E1;                         // A1: BSR 2x2 over the M pattern; E1: 36 x nel by element id
-    T *RK;                              // R K: BSR 2x2 over the G^T pattern (null: not built)
-    int cart, bc, iso;
-    QuadTables quad;
-};
-
 // Integrates one element with the 6-point rule (StokesSolver.jl:543-845): the lower
 // node-pair 2x2 blocks of K_e into sK[.][tid] (LDS), M_e (upper triangle, 6) into
 // sM[.][tid] (LDS), G_e (node a, pressure i, component) and f_e into registers.
@@ -185,250 +163,6 @@ __device__ __forceinline__ void integrate_element(const QuadTables &qt, const T 
         }
     }
 
-}
-
-template <typename T>
-__global__ __launch_bounds__(kAsmThreads) void k_assemble(AsmArgs<T> a) {
-    __shared__ T sK[84][kAsmThreads];
-    __shared__ T sM[6][kAsmThreads];
-    T sG[36], sF[12];
-    const int tid = threadIdx.x;
-    const int64_t j = (int64_t)blockIdx.x * kAsmThreads + tid;
-    if (j >= a.cnt) return;   // no block-level synchronisation below: early exit is safe
-    const int64_t p = a.off + j;
-    const int64_t nel = a.nel;
-    const int64_t e = a.perm[p];
-
-    int32_t n[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) n[k] = a.e2n[k * nel + p];
-    T th[3], rr[3], rhov[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        th[k] = (T)a.th[k * nel + p];
-        rr[k] = (T)a.r[k * nel + p];
-        rhov[k] = (T)a.rho[n[k]];
-    }
-    // _fill_R_J! geometry is recomputed inside integrate_element
-    auto eta_at = [&](int q) { return (T)a.eta[(int64_t)q * nel + e]; };
-    auto tensor_at = [&](int q, T d[6]) {
-        // StiffnessTensor order eta11, eta33, eta55, eta13, eta15, eta35
-        const double *C = a.C + (int64_t)q * nel + e;
-        const int64_t cstr = 6 * nel;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) d[k] = (T)C[k * cstr];
-    };
-    integrate_element<T>(a.quad, th, rr, rhov, (T)a.g, a.iso != 0, eta_at, tensor_at, sK, sM, tid, sG, sF);
-
-    // rotate to the node frames: blk(a,b) <- Q_a blk Q_b^T, Q = [[c, -s], [s, c]]
-    if (!a.cart) {
-        T c[6], s[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            c[k] = (T)a.nodecs[2 * (int64_t)n[k]];
-            s[k] = (T)a.nodecs[2 * (int64_t)n[k] + 1];
-        }
-        for (int aa = 0; aa < 6; ++aa)
-            for (int b = 0; b <= aa; ++b) {
-                const int k = lblk(aa, b) * 4;
-                const T k00 = sK[k][tid], k01 = sK[k + 1][tid], k10 = sK[k + 2][tid], k11 = sK[k + 3][tid];
-                // t = Q_a blk
-                const T t00 = c[aa] * k00 - s[aa] * k10, t01 = c[aa] * k01 - s[aa] * k11;
-                const T t10 = s[aa] * k00 + c[aa] * k10, t11 = s[aa] * k01 + c[aa] * k11;
-                // t Q_b^T, Q_b^T = [[c, s], [-s, c]]
-                sK[k + 0][tid] = t00 * c[b] - t01 * s[b];
-                sK[k + 1][tid] = t00 * s[b] + t01 * c[b];
-                sK[k + 2][tid] = t10 * c[b] - t11 * s[b];
-                sK[k + 3][tid] = t10 * s[b] + t11 * c[b];
-            }
-#pragma unroll
-        for (int aa = 0; aa < 6; ++aa) {
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-                const int k = (aa * 3 + i) * 2;
-                const T g0 = sG[k], g1 = sG[k + 1];
-                sG[k] = c[aa] * g0 - s[aa] * g1;
-                sG[k + 1] = s[aa] * g0 + c[aa] * g1;
-            }
-            const T f0 = sF[2 * aa], f1 = sF[2 * aa + 1];
-            sF[2 * aa] = c[aa] * f0 - s[aa] * f1;
-            sF[2 * aa + 1] = s[aa] * f0 + c[aa] * f1;
-        }
-    }
-
-    uint8_t fx[12];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        fx[2 * k] = a.bc ? a.fixed[2 * (int64_t)n[k]] : 0;
-        fx[2 * k + 1] = a.bc ? a.fixed[2 * (int64_t)n[k] + 1] : 0;
-    }
-
-    // scatter K (both triangles of the symmetric operator)
-    for (int aa = 0; aa < 6; ++aa) {
-        const int64_t na = n[aa];
-        const int64_t rowbase = a.Ksptr ? a.Ksptr[na / Pattern::kSlice] + na % Pattern::kSlice : (int64_t)a.Kptr[na];
-        const int64_t kstride = a.Ksptr ? Pattern::kSlice : 1;
-        for (int b = 0; b < 6; ++b) {
-            const int64_t blk = rowbase + kstride * a.sK[(aa * 6 + b) * nel + p];
-            T v[4];
-            if (b <= aa) {
-                const int k = lblk(aa, b) * 4;
-                v[0] = sK[k][tid]; v[1] = sK[k + 1][tid]; v[2] = sK[k + 2][tid]; v[3] = sK[k + 3][tid];
-            } else {
-                const int k = lblk(b, aa) * 4;
-                v[0] = sK[k][tid]; v[1] = sK[k + 2][tid]; v[2] = sK[k + 1][tid]; v[3] = sK[k + 3][tid];
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int rc = q >> 1, cc = q & 1;
-                if (!(fx[2 * aa + rc] | fx[2 * b + cc])) a.K[4 * blk + q] += v[q];
-            }
-        }
-    }
-    // scatter G (rows: velocity dofs) and G^T (rows: pressure vertices)
-#pragma unroll
-    for (int aa = 0; aa < 6; ++aa)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            const int k = (aa * 3 + i) * 2;
-            const T g0 = fx[2 * aa] ? T(0) : sG[k];
-            const T g1 = fx[2 * aa + 1] ? T(0) : sG[k + 1];
-            const int64_t sg = a.sG[(aa * 3 + i) * nel + p], st = a.sGT[(i * 6 + aa) * nel + p];
-            const int64_t bg = a.Gsptr ? a.Gsptr[n[aa] / Pattern::kSlice] + sg * Pattern::kSlice + n[aa] % Pattern::kSlice
-                                       : (int64_t)a.Gptr[n[aa]] + sg;
-            a.G[2 * bg] += g0;
-            a.G[2 * bg + 1] += g1;
-            const int64_t bt = a.GTsptr ? a.GTsptr[n[i] / Pattern::kSlice] + st * Pattern::kSlice + n[i] % Pattern::kSlice
-                                        : (int64_t)a.GTptr[n[i]] + st;
-            a.GT[2 * bt] += g0;
-            a.GT[2 * bt + 1] += g1;
-        }
-    // scatter M (full 3x3) and F
-    {
-        const T m[3][3] = {{sM[0][tid], sM[1][tid], sM[2][tid]},
-                           {sM[1][tid], sM[3][tid], sM[4][tid]},
-                           {sM[2][tid], sM[4][tid], sM[5][tid]}};
-        for (int i = 0; i < 3; ++i)
-            for (int jj = 0; jj < 3; ++jj) {
-                const int64_t bm = (int64_t)a.Mptr[n[i]] + a.sM[(i * 3 + jj) * nel + p];
-                a.M[bm] += m[i][jj];
-            }
-    }
-#pragma unroll
-    for (int aa = 0; aa < 6; ++aa) {
-        a.F[2 * (int64_t)n[aa]] += sF[2 * aa];
-        a.F[2 * (int64_t)n[aa] + 1] += sF[2 * aa + 1];
-    }
-
-    if (a.A1 == nullptr) return;
-    // ---- P1 projection: E1 = Ph^T K Ph, Ph: 12 x 6 (rows: P2 dofs, cols: vertex dofs)
-    // vertex rows: identity; midpoint 3+k of edge (ea, eb): 0.5 R(+h) from ea, 0.5 R(-h) from eb,
-    // h = (theta_eb - theta_ea) / 2; rows / columns of fixed dofs are zero.
-    const int ea[3] = {0, 1, 2}, eb[3] = {1, 2, 0};
-    T Ph[6][3][4];   // block (fine node, coarse vertex)
-#pragma unroll
-    for (int f = 0; f < 6; ++f)
-#pragma unroll
-        for (int v = 0; v < 3; ++v)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) Ph[f][v][q] = T(0);
-#pragma unroll
-    for (int v = 0; v < 3; ++v) { Ph[v][v][0] = T(1); Ph[v][v][3] = T(1); }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const T h = T(0.5) * (th[eb[k]] - th[ea[k]]);
-        T sh, ch;
-        if constexpr (sizeof(T) == 8) sincos(h, &sh, &ch);
-        else sincosf(h, &sh, &ch);
-        T *A = Ph[3 + k][ea[k]], *B = Ph[3 + k][eb[k]];
-        A[0] = T(0.5) * ch; A[1] = -T(0.5) * sh; A[2] = T(0.5) * sh; A[3] = T(0.5) * ch;
-        B[0] = T(0.5) * ch; B[1] = T(0.5) * sh; B[2] = -T(0.5) * sh; B[3] = T(0.5) * ch;
-    }
-    // masks (always applied: the multigrid acts on the constrained space)
-    uint8_t fxa[12];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        fxa[2 * k] = a.fixed[2 * (int64_t)n[k]];
-        fxa[2 * k + 1] = a.fixed[2 * (int64_t)n[k] + 1];
-    }
-#pragma unroll
-    for (int f = 0; f < 6; ++f)
-#pragma unroll
-        for (int v = 0; v < 3; ++v)
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (fxa[2 * f + (q >> 1)] | fxa[2 * v + (q & 1)]) Ph[f][v][q] = T(0);
-    // T1 = K Ph (12 x 6), accumulated column by column into E1 = Ph^T T1
-    T E[6][6];
-#pragma unroll
-    for (int i = 0; i < 6; ++i)
-#pragma unroll
-        for (int jj = 0; jj < 6; ++jj) E[i][jj] = T(0);
-    for (int f = 0; f < 6; ++f) {          // fine row node of T1
-        T t1[2][6];
-#pragma unroll
-        for (int rc = 0; rc < 2; ++rc)
-#pragma unroll
-            for (int cd = 0; cd < 6; ++cd) t1[rc][cd] = T(0);
-        for (int b = 0; b < 6; ++b) {      // fine column node of K
-            T kb[4];
-            if (b <= f) {
-                const int k = lblk(f, b) * 4;
-                kb[0] = sK[k][tid]; kb[1] = sK[k + 1][tid]; kb[2] = sK[k + 2][tid]; kb[3] = sK[k + 3][tid];
-            } else {
-                const int k = lblk(b, f) * 4;
-                kb[0] = sK[k][tid]; kb[1] = sK[k + 2][tid]; kb[2] = sK[k + 1][tid]; kb[3] = sK[k + 3][tid];
-            }
-#pragma unroll
-            for (int v = 0; v < 3; ++v) {
-                const T *P = Ph[b][v];
-                t1[0][2 * v + 0] += kb[0] * P[0] + kb[1] * P[2];
-                t1[0][2 * v + 1] += kb[0] * P[1] + kb[1] * P[3];
-                t1[1][2 * v + 0] += kb[2] * P[0] + kb[3] * P[2];
-                t1[1][2 * v + 1] += kb[2] * P[1] + kb[3] * P[3];
-            }
-        }
-        if (a.RK) {
-            // (Ph^T K_c)(vertex u, node f) = T1(f, u)^T with K_c the constrained K: its
-            // columns of fixed dofs are zero (the unit diagonal of a fixed dof meets a
-            // zero column of the masked Ph^T)
-#pragma unroll
-            for (int u = 0; u < 3; ++u) {
-                const int64_t st = a.sGT[(u * 6 + f) * nel + p];
-                const int64_t bt = a.GTsptr ? a.GTsptr[n[u] / Pattern::kSlice] + st * Pattern::kSlice +
-                                                  n[u] % Pattern::kSlice
-                                            : (int64_t)a.GTptr[n[u]] + st;
-#pragma unroll
-                for (int cu = 0; cu < 2; ++cu)
-#pragma unroll
-                    for (int rc = 0; rc < 2; ++rc)
-                        if (!fxa[2 * f + rc]) a.RK[4 * bt + cu * 2 + rc] += t1[rc][2 * u + cu];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < 3; ++u) {
-            const T *P = Ph[f][u];   // rows (f, rc), cols (u, cu): Ph^T entry = P[rc*2+cu]
-#pragma unroll
-            for (int cd = 0; cd < 6; ++cd) {
-                E[2 * u + 0][cd] += P[0] * t1[0][cd] + P[2] * t1[1][cd];
-                E[2 * u + 1][cd] += P[1] * t1[0][cd] + P[3] * t1[1][cd];
-            }
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 6; ++i)
-#pragma unroll
-        for (int jj = 0; jj < 6; ++jj) a.E1[(int64_t)(i * 6 + jj) * nel + e] = E[i][jj];
-    for (int u = 0; u < 3; ++u)
-        for (int v = 0; v < 3; ++v) {
-            const int64_t sl = a.sM[(u * 3 + v) * nel + p];
-            const int64_t bm = a.A1sptr ? a.A1sptr[n[u] / Pattern::kSlice] + sl * Pattern::kSlice + n[u] % Pattern::kSlice
-                                        : (int64_t)a.Mptr[n[u]] + sl;
-            a.A1[4 * bm + 0] += E[2 * u][2 * v];
-            a.A1[4 * bm + 1] += E[2 * u][2 * v + 1];
-            a.A1[4 * bm + 2] += E[2 * u + 1][2 * v];
-            a.A1[4 * bm + 3] += E[2 * u + 1][2 * v + 1];
-        }
 }
 
 // ------------------------------------------------------------------------ local Schur blocks
@@ -839,33 +573,6 @@ void launch_learned_scatter(Context &c, const float *Y, const double *scale, dou
 }
 
 template <typename T>
-void assemble_impl(Context &c, int flags, T *K, T *G, T *GT, T *M, T *F, T *A1, T *E1, T *RK) {
-    AsmArgs<T> a{};
-    a.quad = make_quad();
-    a.nel = c.nel;
-    a.perm = c.d_perm.p; a.e2n = c.d_e2n.p; a.th = c.d_elth.p; a.r = c.d_elr.p;
-    a.nodecs = c.d_nodecs.p; a.fixed = c.d_fixed.p;
-    a.rho = c.d_rho.p; a.eta = c.d_eta.p; a.C = c.d_C.p; a.g = c.g;
-    a.sK = c.d_sK.p; a.sG = c.d_sG.p; a.sGT = c.d_sGT.p; a.sM = c.d_sM.p;
-    a.Kptr = c.KN.dptr.p; a.Gptr = c.GN.dptr.p; a.GTptr = c.GTN.dptr.p; a.Mptr = c.MN.dptr.p;
-    a.Ksptr = c.KN.sell ? c.KN.dsptr.p : nullptr;
-    a.A1sptr = (A1 && c.mg[0].pat.sell) ? c.mg[0].pat.dsptr.p : nullptr;
-    a.Gsptr = c.GN.sell ? c.GN.dsptr.p : nullptr;
-    a.GTsptr = c.GTN.sell ? c.GTN.dsptr.p : nullptr;
-    a.K = K; a.G = G; a.GT = GT; a.M = M; a.F = F; a.A1 = A1; a.E1 = E1; a.RK = A1 ? RK : nullptr;
-    a.cart = (flags & STK_ASM_CARTESIAN) ? 1 : 0;
-    a.bc = (flags & STK_ASM_NO_BC) ? 0 : 1;
-    a.iso = ((flags & STK_ASM_ISOTROPIC) || !c.have_C) ? 1 : 0;
-    for (size_t k = 0; k + 1 < c.col_off.size(); ++k) {
-        a.off = c.col_off[k];
-        a.cnt = c.col_off[k + 1] - c.col_off[k];
-        const int nb = (int)((a.cnt + kAsmThreads - 1) / kAsmThreads);
-        k_assemble<T><<<dim3(nb), dim3(kAsmThreads), 0, c.stream>>>(a);
-        STK_HIP(hipGetLastError());
-    }
-}
-
-template <typename T>
 void galerkin_impl(Context &c, size_t l, T *Ef, T *Ec, T *A) {
     MGLevel &F = c.mg[l], &C = c.mg[l + 1];
     GalArgs<T> a{};
@@ -892,10 +599,6 @@ void fix_diag(Context &c, const Pattern &p, int64_t nfix, const int64_t *dofs, T
     STK_HIP(hipGetLastError());
 }
 
-template void assemble_impl<double>(Context &, int, double *, double *, double *, double *, double *, double *, double *,
-                                    double *);
-template void assemble_impl<float>(Context &, int, float *, float *, float *, float *, float *, float *, float *,
-                                   float *);
 template void galerkin_impl<double>(Context &, size_t, double *, double *, double *);
 template void galerkin_impl<float>(Context &, size_t, float *, float *, float *);
 template void fix_diag<double>(Context &, const Pattern &, int64_t, const int64_t *, double *);

@@ -368,6 +368,98 @@ void build_slot_maps(Context &c) {
     STK_HIP(hipStreamSynchronize(c.stream));
 }
 
+namespace {
+
+// CSR of contributions over npos stored positions: for each element e (in order) and each of its
+// `per_el` local blocks j, pos(e, j) names the position and code(e, j) the source block.  Filling
+// element by element keeps every position's list in element order (a fixed summation order).
+template <typename PosF, typename CodeF>
+void build_list(Context::GatherList &g, int64_t npos, int64_t nel, int per_el, PosF pos, CodeF code,
+                hipStream_t s) {
+    std::vector<int32_t> ptr(npos + 1, 0);
+    std::vector<int64_t> where((size_t)nel * per_el);
+    for (int64_t e = 0; e < nel; ++e)
+        for (int j = 0; j < per_el; ++j) {
+            const int64_t q = pos(e, j);
+            if (q < 0 || q >= npos) throw Error(STK_E_STATE, "gather list position out of range");
+            where[(size_t)e * per_el + j] = q;
+            ptr[q + 1]++;
+        }
+    for (int64_t q = 0; q < npos; ++q) ptr[q + 1] += ptr[q];
+    std::vector<uint32_t> list(ptr[npos]);
+    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t e = 0; e < nel; ++e)
+        for (int j = 0; j < per_el; ++j) list[fill[where[(size_t)e * per_el + j]]++] = code(e, j);
+    g.npos = npos;
+    g.ptr.upload(ptr, s);
+    g.list.upload(list, s);
+}
+
+}  // namespace
+
+void build_gather_lists(Context &c) {
+    const int64_t nel = c.nel;
+    // codes index element blocks [block][nel]; K's carry a transposition bit
+    if ((uint64_t)2 * 21 * (uint64_t)nel >= (uint64_t)UINT32_MAX) throw Error(STK_E_SHAPE, "mesh too large for the gather codes");
+    const int32_t *n = c.e2n.data();   // element-major: n[e * 6 + a]
+    auto lb = [](int a, int b) { return a * (a + 1) / 2 + b; };
+    // K: node pair (a, b) -> block row n_a, column n_b; the lower block (max, min) of K_e, transposed
+    // when a < b
+    build_list(c.gK, c.KN.nstore(), nel, 36,
+               [&](int64_t e, int j) {
+                   const int a = j / 6, b = j % 6;
+                   const int32_t na = n[e * 6 + a], nb = n[e * 6 + b];
+                   return c.KN.pos(na, find_offset(c.KN, na, nb));
+               },
+               [&](int64_t e, int j) {
+                   const int a = j / 6, b = j % 6;
+                   const int k = a >= b ? lb(a, b) : lb(b, a);
+                   return (uint32_t)(((uint64_t)k * nel + e) * 2 + (a < b ? 1 : 0));
+               },
+               c.stream);
+    // G (node n_a, vertex n_i) and G^T (vertex n_i, node n_a): block a * 3 + i of G_e
+    auto gcode = [&](int64_t e, int j) { return (uint32_t)((uint64_t)j * nel + e); };
+    build_list(c.gG, c.GN.nstore(), nel, 18,
+               [&](int64_t e, int j) {
+                   const int32_t na = n[e * 6 + j / 3], ni = n[e * 6 + j % 3];
+                   return c.GN.pos(na, find_offset(c.GN, na, ni));
+               },
+               gcode, c.stream);
+    build_list(c.gGT, c.GTN.nstore(), nel, 18,
+               [&](int64_t e, int j) {
+                   const int32_t na = n[e * 6 + j / 3], ni = n[e * 6 + j % 3];
+                   return c.GTN.pos(ni, find_offset(c.GTN, ni, na));
+               },
+               gcode, c.stream);
+    // M (vertex i, vertex j): the upper entry (min, max) of M_e
+    static const int mpair[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    build_list(c.gM, c.MN.nnz(), nel, 9,
+               [&](int64_t e, int j) {
+                   const int32_t ni = n[e * 6 + j / 3], nj = n[e * 6 + j % 3];
+                   return (int64_t)c.MN.ptr[ni] + find_offset(c.MN, ni, nj);
+               },
+               [&](int64_t e, int j) { return (uint32_t)((uint64_t)mpair[j / 3][j % 3] * nel + e); }, c.stream);
+    // F (node n_a): block a of f_e
+    build_list(c.gF, c.nnod, nel, 6, [&](int64_t e, int j) { return (int64_t)n[e * 6 + j]; }, gcode, c.stream);
+    if (!c.mg.empty()) {
+        // the P1 operator (vertex u, vertex v) on the level-0 pattern and R K (vertex u, node f) on G^T's
+        const Pattern &P0 = c.mg[0].pat;
+        build_list(c.gA1, P0.nstore(), nel, 9,
+                   [&](int64_t e, int j) {
+                       const int32_t nu = n[e * 6 + j / 3], nv = n[e * 6 + j % 3];
+                       return P0.pos(nu, find_offset(P0, nu, nv));
+                   },
+                   gcode, c.stream);
+        build_list(c.gRK, c.GTN.nstore(), nel, 18,
+                   [&](int64_t e, int j) {
+                       const int32_t nu = n[e * 6 + j / 6], nf = n[e * 6 + j % 6];
+                       return c.GTN.pos(nu, find_offset(c.GTN, nu, nf));
+                   },
+                   gcode, c.stream);
+    }
+    STK_HIP(hipStreamSynchronize(c.stream));
+}
+
 // The multigrid needs the structured annulus of Grid_split1 (src/Grid/mesher.jl:152-233):
 // vertex v(i, j) = i + nr*j (radial fastest), cell q = i + Nr*j, elements 2q and 2q+1 with
 // vertices [v(i,j), v(i,j+1), v(i+1,j)] and [v(i,j+1), v(i+1,j+1), v(i+1,j)].

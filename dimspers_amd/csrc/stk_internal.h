@@ -171,6 +171,17 @@ struct Context {
     // patterns
     Pattern KN, GN, GTN, MN;
 
+    // Gather lists of the element assembly: for every stored position of an operator, the element
+    // blocks that sum into it, in element order (CSR over positions; codes name the element block).
+    struct GatherList {
+        int64_t npos = 0;
+        DBuf<int32_t> ptr;
+        DBuf<uint32_t> list;
+    };
+    GatherList gK, gG, gGT, gM, gF, gA1, gRK;
+    DBuf<double> el64; DBuf<float> el32;      // per-element blocks of the last assembly
+    int64_t field_cols = 6;                   // columns of eta.ip and of the tensor arrays on the device
+
     // device mesh data (colour-permuted, SoA: [j * nel + p])
     DBuf<int32_t> d_perm, d_e2n;
     DBuf<double> d_elth, d_elr, d_nodecs, d_noder;
@@ -325,6 +336,8 @@ void build_colouring(Context &c);
 void build_slot_maps(Context &c);
 void build_gs_layout(Context &c);
 void detect_structured(Context &c);
+// Gather lists of the element assembly (after the patterns and the multigrid structure).
+void build_gather_lists(Context &c);
 bool fixed_set_is_free_slip(const Context &c);
 // IC(0) factorisation of the full pressure mass matrix M (values on MN in CSR order) for STK_PPC_IC.
 void build_pressure_ic(Context &c, const std::vector<double> &M);

@@ -43,7 +43,11 @@ enum {
     STK_ASM_NO_BC = 8,       /* keep rows/columns of fixed dofs (no _apply_bcs)          */
     STK_ASM_ISOTROPIC = 16,  /* isotropic D (StokesSolver.jl:688-742), C6 ignored       */
     STK_ASM_NO_MG = 32,      /* do not build the multigrid hierarchy                     */
-    STK_ASM_FP16_SMOOTHER = 64 /* fine-level smoother on an FP16 copy of S^-1 K S^-1       */
+    STK_ASM_FP16_SMOOTHER = 64,/* fine-level smoother on an FP16 copy of S^-1 K S^-1       */
+    STK_ASM_7POINT = 128,    /* 7-point rule (ip_triangle7) of the threaded assembler,   */
+                             /* AssemblerStokes.jl:174 (needs 7 columns in stk_set_fields) */
+    STK_ASM_CLAMP = 256      /* clamp every eta * C entry to [1e18, 1e25] as the top-level */
+                             /* _fill_Kblk! does for dimensional runs (StokesSolver.jl:756) */
 };
 
 /* Which operator stk_export_csc / stk_spmv address. */
@@ -190,9 +194,10 @@ int stk_colouring(stk_ctx *ctx, int32_t which, int32_t *colour, int32_t *ncolour
 
 /* Upload the per-step fields assembly_stokes_cylindric reads
  * (StokesSolver.jl:247-249): g, rho.rho (nnod), eta.ip (nel x nip column-major,
- * nip >= 6, first 6 columns used, :608) and the six StiffnessTensor arrays
- * (src/Rheology/anisotropy.jl:1-8) in the order eta11, eta33, eta55, eta13,
- * eta15, eta35, each nel x nip.  C may be NULL for the isotropic law. */
+ * nip >= 6; the 6-point rule uses the first 6 columns, :608, STK_ASM_7POINT the first 7)
+ * and the six StiffnessTensor arrays (src/Rheology/anisotropy.jl:1-8) in the order
+ * eta11, eta33, eta55, eta13, eta15, eta35, each nel x nip.  C may be NULL for the
+ * isotropic law. */
 int stk_set_fields(stk_ctx *ctx, double g, const double *rho_node,
                    const double *eta_ip, int64_t nip, const double *const C[6]);
 

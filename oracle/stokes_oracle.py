@@ -51,7 +51,7 @@ import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
-from dimspers_amd.quadrature import ip_triangle6, sf_dN_tri6, sf_N_tri3, sf_N_tri6
+from dimspers_amd.quadrature import ip_triangle6, ip_triangle7, sf_dN_tri6, sf_N_tri3, sf_N_tri6
 
 
 # ----------------------------------------------------------------------------- element level
@@ -65,12 +65,16 @@ def geometry(el_theta, el_r):
     return R21, R31, Th21, Th31, detJ
 
 
-def ip_derivatives(el_theta, el_r, q):
-    """_derivative_weights! (StokesSolver.jl:641-686) at integration point q.
+def _rule(nip):
+    return ip_triangle7() if nip == 7 else ip_triangle6()
+
+
+def ip_derivatives(el_theta, el_r, q, nip=6):
+    """_derivative_weights! (StokesSolver.jl:641-686) at integration point q of the nip-point rule.
 
     Returns (dNdx, dNdz) of shape (nel, 6), omega (nel,), theta_q, r_q.
     """
-    x_ip, w_ip = ip_triangle6()
+    x_ip, w_ip = _rule(nip)
     N3 = sf_N_tri3(x_ip[q, 0], x_ip[q, 1])
     dNds = sf_dN_tri6(x_ip[q, 0], x_ip[q, 1])      # (2, 6)
     R21, R31, Th21, Th31, detJ = geometry(el_theta, el_r)
@@ -88,8 +92,13 @@ def ip_derivatives(el_theta, el_r, q):
     return dNdx, dNdz, omega, th, r
 
 
-def element_matrices(inp, isotropic: bool = False):
+def element_matrices(inp, isotropic: bool = False, nip: int = 6, clamp: bool = False):
     """Per-element K_e (12x12), G_e (12x3), M_e (3x3), f_e (12) in the Cartesian frame.
+
+    ``nip`` = 6 is the live blocked assembly's rule (StokesSolver.jl:262); 7 the threaded
+    assembler's (src/Stokes/AssemblerStokes.jl:174), which reads column q of eta.ip and of the
+    tensor at point q.  ``clamp`` applies applybounds(eta C, 1e25, 1e18) to every component of the
+    anisotropic D as the top-level _fill_Kblk! does (StokesSolver.jl:756,770-804; src/PiC/PiC.jl:834).
 
     Local velocity dofs are x1 z1 x2 z2 ... x6 z6.  K_e = sum_q omega_q B^T D_q B
     with B_j = [[dN_j/dx, 0], [0, dN_j/dz], [dN_j/dz, dN_j/dx]] and
@@ -100,14 +109,14 @@ def element_matrices(inp, isotropic: bool = False):
     """
     mesh = inp.mesh
     nel = mesh.nel
-    x_ip, _ = ip_triangle6()
+    x_ip, _ = _rule(nip)
     K = np.zeros((nel, 12, 12))
     G = np.zeros((nel, 12, 3))
     M = np.zeros((nel, 3, 3))
     f = np.zeros((nel, 12))
     rho_v = inp.rho[mesh.e2n[:3]]                   # (3, nel) vertex densities
-    for q in range(6):
-        dNdx, dNdz, om, th, r = ip_derivatives(mesh.el_theta, mesh.el_r, q)
+    for q in range(nip):
+        dNdx, dNdz, om, th, r = ip_derivatives(mesh.el_theta, mesh.el_r, q, nip)
         NP = sf_N_tri3(x_ip[q, 0], x_ip[q, 1])      # (3,)
         Nq = sf_N_tri6(x_ip[q, 0], x_ip[q, 1])      # (6,)
         eta = inp.eta_ip[:, q]
@@ -128,7 +137,10 @@ def element_matrices(inp, isotropic: bool = False):
                 np.stack([c13, c33, c35], -1),
                 np.stack([c15, c35, c55], -1),
             ], -2)
-        D = D * (eta * om)[:, None, None]
+        if clamp and not isotropic:
+            D = np.maximum(np.minimum(D * eta[:, None, None], 1e25), 1e18) * om[:, None, None]
+        else:
+            D = D * (eta * om)[:, None, None]
         K += np.einsum("eia,eij,ejb->eab", B, D, B)
         # G_e[(x_j | z_j), i] += omega NP_i dN_j/d(x | z)   (StokesSolver.jl:812-823)
         G[:, 0::2, :] += (om[:, None, None] * dNdx[:, :, None]) * NP[None, None, :]
@@ -152,14 +164,14 @@ def el2dof(e2n):
 
 
 # ----------------------------------------------------------------------------- global level
-def assembly_stokes_cylindric(inp, isotropic: bool = False):
+def assembly_stokes_cylindric(inp, isotropic: bool = False, nip: int = 6, clamp: bool = False):
     """Global KK (lower triangle), GG, MM (lower triangle), Fb; Cartesian frame.
 
     Returns scipy CSC matrices (0-based) and Fb, like StokesSolver.jl:247-404.
     """
     mesh = inp.mesh
     nU, nP = mesh.nU, mesh.nP
-    Ke, Ge, Me, fe = element_matrices(inp, isotropic)
+    Ke, Ge, Me, fe = element_matrices(inp, isotropic, nip, clamp)
     dof = el2dof(mesh.e2n)                           # (12, nel)
     I = np.broadcast_to(dof[:, None, :], (12, 12, mesh.nel))
     J = np.broadcast_to(dof[None, :, :], (12, 12, mesh.nel))

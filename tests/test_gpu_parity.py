@@ -155,8 +155,8 @@ def test_assembly_is_bitwise_deterministic(medium):
     ctx.close()
 
 
-def _oracle_projected(inp):
-    KK, GG, MM, Fb = so.assembly_stokes_cylindric(inp)
+def _oracle_projected(inp, nip=6):
+    KK, GG, MM, Fb = so.assembly_stokes_cylindric(inp, nip=nip)
     K, G, F = so.prepare_matrices(KK, GG, Fb, so.rotation_matrix(inp.mesh.theta))
     _, ifree = inp.mesh.free_slip()
     v, lam = so.null_vector(K, ifree, inp.mesh)
@@ -400,3 +400,61 @@ def test_colourings_are_valid(size):
     assert nnc == nc.max() + 1
     print(f"[result] {nec} element colours, {nnc} node colours, {off.sum()} coupled node pairs checked")
     ctx.close()
+
+
+def _fields7(inp):
+    """eta.ip at the 7 points of ip_triangle7 (the workload's law at each point, as make_inputs does at
+    the 6 points) and the tensor in 7 columns."""
+    from dimspers_amd.quadrature import ip_triangle7, sf_N_tri6
+    m = inp.mesh
+    x7, _ = ip_triangle7()
+    N = sf_N_tri6(x7[:, 0], x7[:, 1])                       # (7 points, 6 nodes)
+    Tq = (N @ inp.T[m.e2n]).T                               # (nel, 7)
+    eta7 = np.asfortranarray(np.exp(-np.log(inp.workload.contrast) * Tq))
+    C7 = tuple(np.asfortranarray(np.repeat(np.asarray(c)[:, :1], 7, axis=1)) for c in inp.C6)
+    return StokesInputs(m, inp.workload, inp.T, inp.rho, eta7, C7, inp.phi, inp.delta)
+
+
+@pytest.mark.parametrize("variant", ["7point", "clamp", "clamp_dimensional"])
+def test_assembly_variants_match_oracle(small, variant):
+    """The 7-point rule of the threaded assembler (AssemblerStokes.jl:174) and the clamp of the top-level
+    _fill_Kblk! (StokesSolver.jl:756): Cartesian operators against the oracle within 1e-12; the clamp on
+    non-dimensional and on dimensional (eta ~ 1e21) viscosities."""
+    inp, nip, clamp = small, 6, False
+    if variant == "7point":
+        inp, nip = _fields7(small), 7
+    elif variant.startswith("clamp"):
+        clamp = True
+        if variant == "clamp_dimensional":
+            inp = StokesInputs(small.mesh, small.workload, small.T, small.rho, small.eta_ip * 1e21, small.C6,
+                               small.phi, small.delta)
+    flags = _lib.ASM_FP64 | _lib.ASM_CARTESIAN | _lib.ASM_NO_BC
+    flags |= _lib.ASM_7POINT if nip == 7 else 0
+    flags |= _lib.ASM_CLAMP if clamp else 0
+    ctx = _ctx(inp, flags)
+    KK, GG, MM, Fb = so.assembly_stokes_cylindric(inp, nip=nip, clamp=clamp)
+    errs = (_rel(ctx.export_csc(_lib.MAT_K, lower_only=True), KK), _rel(ctx.export_csc(_lib.MAT_G), GG),
+            _rel(ctx.export_csc(_lib.MAT_M, lower_only=True), MM),
+            np.abs(ctx.export_rhs() - Fb).max() / np.abs(Fb).max())
+    print(f"[result] assembly {variant}: K {errs[0]:.1e} G {errs[1]:.1e} M {errs[2]:.1e} F {errs[3]:.1e}", flush=True)
+    assert max(errs) < 1e-12
+    ctx.close()
+
+
+def test_seven_point_operators_solve_like_the_oracle():
+    """The 7-point rule through the whole path (rotation, boundary conditions, the multigrid built
+    from the 7-point element matrices, the mixed method) at 8 x 64 against the projected oracle."""
+    inp = _fields7(make_inputs(scaled_workload(1, 8, 64)))
+    ctx = _ctx(inp, _lib.ASM_FP64 | _lib.ASM_FP32 | _lib.ASM_7POINT)
+    Uo, Po, _, _ = _oracle_projected(inp, nip=7)
+    rep = ctx.solve(_lib.default_opts(method=_lib.SOLVE_MIXED, stop=_lib.STOP_FULL, rtol=1e-10, itmax=400))
+    U, P = ctx.solution()
+    eu, ep = _errs(U, P, Uo, Po)
+    print(f"[result] 7-point mixed: outer {rep.outer_iterations} inner {rep.inner_iterations} errU {eu:.2e} "
+          f"errP {ep:.2e}", flush=True)
+    assert rep.converged and eu < 1e-7 and ep < 1e-7
+    # the 6-point assembly of the same fields is a different operator
+    ctx6 = _ctx(inp, _lib.ASM_FP64)
+    assert _rel(ctx6.export_csc(_lib.MAT_K), ctx.export_csc(_lib.MAT_K)) > 1e-8
+    ctx.close()
+    ctx6.close()

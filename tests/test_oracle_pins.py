@@ -407,7 +407,8 @@ def _point_weights(m, w_ip):
     return np.abs(det)[:, None] * w_ip[None, :]
 
 
-def test_velocity_block_is_the_viscous_energy_of_the_rotated_tensor():
+@pytest.mark.parametrize("nip", [6, 7])
+def test_velocity_block_is_the_viscous_energy_of_the_rotated_tensor(nip):
     # _fill_Kblk!{Anisotropic} (src/src/Stokes/StokesSolver.jl:744-808): u^T K_e u must equal
     #   sum_q omega_q eta_q eps_ij C_ijkl eps_kl
     # with C the fourth-order tensor whose Voigt entries (xx, zz, xz) are eta11 = C_xxxx,
@@ -419,7 +420,7 @@ def test_velocity_block_is_the_viscous_energy_of_the_rotated_tensor():
     inp = _small_inputs(4, 16)
     m = inp.mesh
     nel = m.nel
-    x_ip, w_ip = ip_triangle6()
+    x_ip, w_ip = ip_triangle7() if nip == 7 else ip_triangle6()
     Cf = _fabric_fourth_order(4 / 3, 0.6, 0.3, -0.5)
     phi = rng.uniform(0, np.pi, nel)
     R = np.stack([np.stack([np.cos(phi), -np.sin(phi)], -1), np.stack([np.sin(phi), np.cos(phi)], -1)], -2)
@@ -427,10 +428,10 @@ def test_velocity_block_is_the_viscous_energy_of_the_rotated_tensor():
     comps = [C4[:, 0, 0, 0, 0], C4[:, 1, 1, 1, 1], C4[:, 0, 1, 0, 1], C4[:, 0, 0, 1, 1], C4[:, 0, 0, 0, 1],
              C4[:, 1, 1, 0, 1]]
     assert np.abs(comps[4] - comps[5]).min() > 1e-3 or np.abs(comps[4] - comps[5]).max() > 0.1
-    C6 = tuple(np.asfortranarray(np.repeat(c[:, None], 6, axis=1)) for c in comps)
-    eta = np.asfortranarray(0.5 + rng.random((nel, 6)))
+    C6 = tuple(np.asfortranarray(np.repeat(c[:, None], nip, axis=1)) for c in comps)
+    eta = np.asfortranarray(0.5 + rng.random((nel, nip)))
     inp = StokesInputs(m, inp.workload, inp.T, inp.rho, eta, C6, phi, inp.delta)
-    Ke, Ge, _, _ = so.element_matrices(inp)
+    Ke, Ge, _, _ = so.element_matrices(inp, nip=nip)
 
     cx, cz, val, d_t, d_r = _quadratic_field(rng)
     tn, rn = _element_nodes_polar(m)
@@ -439,7 +440,7 @@ def test_velocity_block_is_the_viscous_energy_of_the_rotated_tensor():
     tq, rq = _points_polar(m, x_ip)
     uxx, uxz = _cartesian_gradient(cx, tq, rq, d_t, d_r)
     uzx, uzz = _cartesian_gradient(cz, tq, rq, d_t, d_r)
-    eps = np.empty((nel, 6, 2, 2))
+    eps = np.empty((nel, nip, 2, 2))
     eps[..., 0, 0], eps[..., 1, 1] = uxx, uzz
     eps[..., 0, 1] = eps[..., 1, 0] = 0.5 * (uxz + uzx)
     dens = np.einsum("eqij,eijkl,eqkl->eq", eps, C4, eps)
@@ -542,3 +543,38 @@ def test_pardiso_variant_keeps_the_incoming_pressure():
     U, _ = so.updatevelocity2(U, inp.rho, inp.mesh)
     assert np.linalg.norm(U - Ud) / np.linalg.norm(Ud) < 1e-9
     assert np.linalg.norm((P - P.mean()) - Pd) / np.linalg.norm(Pd) < 1e-9
+
+
+def test_clamped_tensor_is_the_all_ones_matrix_below_the_lower_bound():
+    # The top-level _fill_Kblk! (StokesSolver.jl:756,770-804) clamps every eta C entry to [1e18, 1e25]
+    # (applybounds, src/PiC/PiC.jl:834).  Non-dimensional viscosities (eta C <= 1e2) all become 1e18, so
+    # D = 1e18 [[1,1,1],[1,1,1],[1,1,1]] and u^T K_e u = 1e18 sum_q omega_q (u_x,x + u_z,z + u_x,z + u_z,x)^2;
+    # in the dimensional range the clamp is the identity.
+    rng = np.random.default_rng(5)
+    inp = _small_inputs(4, 16)
+    m = inp.mesh
+    x_ip, w_ip = ip_triangle6()
+    Ke, _, _, _ = so.element_matrices(inp, clamp=True)
+    cx, cz, val, d_t, d_r = _quadratic_field(rng)
+    tn, rn = _element_nodes_polar(m)
+    ue = np.zeros((m.nel, 12))
+    ue[:, 0::2], ue[:, 1::2] = val(cx, tn, rn).T, val(cz, tn, rn).T
+    tq, rq = _points_polar(m, x_ip)
+    uxx, uxz = _cartesian_gradient(cx, tq, rq, d_t, d_r)
+    uzx, uzz = _cartesian_gradient(cz, tq, rq, d_t, d_r)
+    omega = rq * _point_weights(m, w_ip)
+    want = 1e18 * (omega * (uxx + uzz + uxz + uzx) ** 2).sum(1)
+    got = np.einsum("ea,eab,eb->e", ue, Ke, ue)
+    assert np.allclose(got, want, rtol=1e-11, atol=1e-12 * np.abs(want).max())
+    dim = StokesInputs(m, inp.workload, inp.T, inp.rho, inp.eta_ip * 1e21, inp.C6, inp.phi, inp.delta)
+    C = np.stack([np.asarray(c) for c in inp.C6]) * dim.eta_ip[None]
+    inside = (C > 1e18) & (C < 1e25)
+    if inside.all():
+        Kc, _, _, _ = so.element_matrices(dim, clamp=True)
+        Kd, _, _, _ = so.element_matrices(dim)
+        assert np.allclose(Kc, Kd, rtol=1e-14, atol=0)
+    else:
+        # negative eta13 entries hit the lower bound: the clamp changes K
+        Kc, _, _, _ = so.element_matrices(dim, clamp=True)
+        Kd, _, _, _ = so.element_matrices(dim)
+        assert not np.allclose(Kc, Kd)
