@@ -320,6 +320,7 @@ int stk_assemble(stk_ctx *ctx, int32_t flags) {
         c.assembled = false;
         c.loaded = false;
         c.mg_built = false;
+        c.coarse64_valid = false;
         c.have_mg64 = c.have_mg32 = false;
         // stale products of an earlier assembly
         c.K32.release(); c.G32.release(); c.GT32.release(); c.M32.release(); c.F32.release(); c.pc32.release();
@@ -351,13 +352,11 @@ int stk_assemble(stk_ctx *ctx, int32_t flags) {
         c.mg_built = mg;
         c.asm_flags = flags;
         if (solve_form) {
-            if (fp64) {
-                setup_solver_data<double>(c);
-                c.have_mg64 = mg;
-            }
-            if (fp32) {
-                setup_solver_data<float>(c);
-                c.have_mg32 = mg;
+            if (fp64 && fp32 && mg) {
+                setup_fast(c);
+            } else {
+                if (fp64) setup_solver_data<double>(c);
+                if (fp32) setup_solver_data<float>(c);
             }
         }
         c.asm_flags = flags;
@@ -524,6 +523,7 @@ int stk_set_operators(stk_ctx *ctx, int32_t K_lower, const int64_t *K_colptr, co
                          F);
         c.assembled = false;
         c.mg_built = false;
+        c.coarse64_valid = false;
         c.have_mg64 = c.have_mg32 = false;
         c.pic.valid = false;
         c.K32.release(); c.G32.release(); c.GT32.release(); c.M32.release(); c.F32.release(); c.pc32.release();

@@ -37,6 +37,10 @@ struct Reducer {
 
 template <typename T>
 void setup_solver_data(Context &c);
+// Set-up of the FP32 methods after an FP64 + FP32 assembly with the multigrid; the FP64 hierarchy
+// follows on demand (ensure_mg64).
+void setup_fast(Context &c);
+void ensure_mg64(Context &c);
 void build_coarse_inverse(Context &c, bool f64);
 int run_solve(Context &c, const stk_opts &o, stk_report &rep);
 double full_residual(Context &c, const double *U, const double *P, double *ru, double *hc, bool project_null);

@@ -721,20 +721,23 @@ __global__ void k_symmetrize_lower_cm(int64_t n, const double *__restrict__ L, d
 void build_coarse_inverse(Context &c, bool f64) {
     MGLevel &L = c.mg.back();
     const int64_t n = 2 * L.nvert;
-    if (!f64 && (c.asm_flags & STK_ASM_FP64) && c.coarse_inv64.n == (size_t)(n * n)) {
-        // the FP64 set-up of this assembly (it runs first) already inverted the exact
-        // operator: its rounding serves the FP32 preconditioner as well
+    if (c.coarse64_valid && c.coarse_inv64.n == (size_t)(n * n)) {
+        // this assembly's coarsest operator is already inverted in FP64: its rounding serves the
+        // FP32 preconditioner as well
         c.coarse_n = n;
-        c.coarse_inv32.alloc(n * n);
-        convert_vec(c, c.coarse_inv64.p, c.coarse_inv32.p, n * n);
-        STK_HIP(hipStreamSynchronize(c.stream));
+        if (!f64) {
+            c.coarse_inv32.alloc(n * n);
+            convert_vec(c, c.coarse_inv64.p, c.coarse_inv32.p, n * n);
+            STK_HIP(hipStreamSynchronize(c.stream));
+        }
         return;
     }
     if (L.pat.sell) throw Error(STK_E_STATE, "the coarsest multigrid level must use CSR block order");
     const int64_t nb = L.pat.nnz();
     DBuf<double> blk;
     const double *bp;
-    if (f64) {
+    const bool have64 = (c.asm_flags & STK_ASM_FP64) && L.A64;
+    if (have64) {
         bp = L.A64.p;
     } else {
         blk.alloc(4 * nb);
@@ -776,11 +779,11 @@ void build_coarse_inverse(Context &c, bool f64) {
     STK_HIP(hipGetLastError());
     dA = std::move(full);
     c.coarse_n = n;
-    if (f64) {
-        c.coarse_inv64 = std::move(dA);
-    } else {
+    c.coarse_inv64 = std::move(dA);
+    c.coarse64_valid = true;
+    if (!f64) {
         c.coarse_inv32.alloc(n * n);
-        convert_vec(c, dA.p, c.coarse_inv32.p, n * n);
+        convert_vec(c, c.coarse_inv64.p, c.coarse_inv32.p, n * n);
         STK_HIP(hipStreamSynchronize(c.stream));
     }
 }
@@ -849,10 +852,12 @@ struct PhaseTimer {
 };
 
 // After assembly: inverse diagonals, eigenvalue bounds, coarse inverse, rotation vector.
+// Per-precision solver data that every method needs: inner-CG work vectors, the inverse diagonal of K,
+// the Gauss-Seidel smoother's values (FP32) and the unrefined near-null vector (the nodal rigid
+// rotation u_theta = r, u_r = 0, unit norm).
 template <typename T>
-void setup_solver_data(Context &c) {
-    const bool f64 = sizeof(T) == 8;
-    PhaseTimer pt(c, f64 ? "set-up FP64" : "set-up FP32");
+void setup_base(Context &c, PhaseTimer &pt) {
+    (void)pt;
     MGLevelBufs<T> &kw = pick<T>(c.kw64, c.kw32);
     MGLevelBufs<T> &cg = pick<T>(c.cg64, c.cg32);
     alloc_bufs(kw, c.nU);
@@ -938,8 +943,16 @@ void setup_solver_data(Context &c) {
             rp[i] = (i % 2 == 0 && !fx[i]) ? (T)(nr[i / 2] * inv) : T(0);
         });
     }
-    // the hierarchy's operators exist only when this assembly built them (not with
-    // STK_ASM_NO_MG, though the mesh's multigrid structure is there)
+}
+
+// The multigrid data of precision T: level work vectors, inverse diagonals, Chebyshev bounds, the
+// coarsest inverse (and the FP16 smoother copy in FP32).
+template <typename T>
+void setup_mg(Context &c, PhaseTimer &pt) {
+    const bool f64 = sizeof(T) == 8;
+    MGLevelBufs<T> &kw = pick<T>(c.kw64, c.kw32);
+    DBuf<T> &kd = pick<T>(c.Kdinv64, c.Kdinv32);
+    const T *K = (const T *)pick<T>(c.K64, c.K32).p;
     if (c.mg_built) {
         for (MGLevel &L : c.mg) {
             alloc_bufs(level_bufs<T>(L), 2 * L.nvert);
@@ -970,9 +983,14 @@ void setup_solver_data(Context &c) {
         power_lmax<T>(c, K0, kd.p, c.nU, kw.r.p, kw.d.p, &lm);
         (f64 ? c.Klmax64 : c.Klmax32) = lm;
     }
-    // Near-null vector: the nodal rigid rotation z0 corrected by one solve on its
-    // complement, v = z0 - Pi K^-1 Pi K z0 (Pi = I - z0 z0^T), which is the lowest
-    // eigenvector of K to rounding at the mesh sizes where it can be checked.
+}
+
+// Near-null vector of precision T from its own multigrid (or the rounding of the FP64 one).
+template <typename T>
+void null_vector(Context &c, PhaseTimer &pt) {
+    const bool f64 = sizeof(T) == 8;
+    DBuf<T> &rot = pick<T>(c.rot64, c.rot32);
+    const T *K = (const T *)pick<T>(c.K64, c.K32).p;
     if (!c.rotation_free) {
         if (f64) c.null_lambda = 0.0;
     } else if (!f64 && c.rot64) {
@@ -1023,26 +1041,140 @@ void setup_solver_data(Context &c) {
                          (int)conv, inv, c.null_lambda, f64 ? c.Klmax64 : c.Klmax32);
     }
     pt.mark("near-null vector");
-    if constexpr (sizeof(T) == 4) {
-        // the Gauss-Seidel smoother's copy of the near-null vector in coloured row order
-        Context::GSSmoother &g = c.gs;
-        if (g.on && c.mg_built) {
-            const int64_t nq = g.row_off.back();
-            const int32_t *rid = g.rowid.p;
-            const float *rp = c.rot32.p;
-            float *wq = g.wq.p;
-            launch_map(c, 2 * nq, [=] __device__(int64_t i) {
-                const int32_t n = rid[i / 2];
-                wq[i] = n >= 0 ? rp[2 * (int64_t)n + i % 2] : 0.0f;
-            });
-            g.filled = true;
-        }
+}
+
+// The Gauss-Seidel smoother's copy of the FP32 near-null vector in coloured row order.
+void fill_gs_rot(Context &c) {
+    Context::GSSmoother &g = c.gs;
+    if (g.on && c.mg_built) {
+        const int64_t nq = g.row_off.back();
+        const int32_t *rid = g.rowid.p;
+        const float *rp = c.rot32.p;
+        float *wq = g.wq.p;
+        launch_map(c, 2 * nq, [=] __device__(int64_t i) {
+            const int32_t n = rid[i / 2];
+            wq[i] = n >= 0 ? rp[2 * (int64_t)n + i % 2] : 0.0f;
+        });
+        g.filled = true;
     }
+}
+
+template <typename T>
+void setup_solver_data(Context &c) {
+    const bool f64 = sizeof(T) == 8;
+    PhaseTimer pt(c, f64 ? "set-up FP64" : "set-up FP32");
+    setup_base<T>(c, pt);
+    setup_mg<T>(c, pt);
+    null_vector<T>(c, pt);
+    if constexpr (sizeof(T) == 4) fill_gs_rot(c);
+    if (f64) c.have_mg64 = c.mg_built;
+    else c.have_mg32 = c.mg_built;
     STK_HIP(hipStreamSynchronize(c.stream));
 }
 
 template void setup_solver_data<double>(Context &);
 template void setup_solver_data<float>(Context &);
+
+// Near-null vector for the FP32 methods: Jacobi-Davidson steps v <- z - d with (Pi K Pi) d = Pi K z,
+// Pi = I - z z^T, the correction from FP64 iterative refinement around FP32 multigrid-CG solves (the
+// multigrid the FP32 solves use).  d is needed to 1e-6 of its own size: the first correction of the
+// nodal rotation is already ~1e-8 of it at 256 x 2048, so v ends accurate to ~1e-14.
+void null_vector_mixed(Context &c, PhaseTimer &pt) {
+    Reducer red(c);
+    const int64_t n = c.nU;
+    double *z = c.rot64.p;
+    float *z32 = c.rot32.p;
+    DBuf<double> y(n), d(n), r(n), keep(n);
+    DBuf<float> r32(n), e32(n);
+    double *yp = y.p, *dp = d.p, *rp = r.p, *kp = keep.p, *sc = c.d_sc.p;
+    float *r32p = r32.p, *e32p = e32.p;
+    Op22<double> K{&c.KN, c.K64.p};
+    auto project = [&](double *w) {
+        red.map_reduce(n, [=] __device__(int64_t i) { return z[i] * w[i]; }, SC_TMP);
+        launch_map(c, n, [=] __device__(int64_t i) { w[i] -= sc[SC_TMP] * z[i]; });
+    };
+    auto norm = [&](const double *w) {
+        red.map_reduce(n, [=] __device__(int64_t i) { return w[i] * w[i]; }, SC_TMP);
+        return std::sqrt(red.fetch(SC_TMP));
+    };
+    double prev = 1.0;
+    int sweeps = 0;
+    for (int step = 0; step < 4; ++step) {
+        convert_vec(c, z, z32, n);
+        fill_gs_rot(c);
+        MGView<float> view = make_view<float>(c);
+        launch_map(c, n, [=] __device__(int64_t i) { kp[i] = z[i]; dp[i] = 0.0; });
+        EpStore<double> ey{yp};
+        apply22<8>(c, K, z, ey);
+        project(yp);
+        const double yn = norm(yp);
+        if (!(yn > 0)) break;
+        for (int sweep = 0; sweep < 8; ++sweep, ++sweeps) {
+            // r = Pi (y - K d), d orthogonal to z
+            EpStore<double> er{rp};
+            apply22<8>(c, K, dp, er);
+            launch_map(c, n, [=] __device__(int64_t i) { rp[i] = yp[i] - rp[i]; });
+            project(rp);
+            const double rn = norm(rp);
+            if (!std::isfinite(rn)) throw Error(STK_E_NAN, "non-finite residual in the near-null vector set-up");
+            if (rn <= 1e-6 * yn) break;
+            const double scale = 1.0 / rn;
+            launch_map(c, n, [=] __device__(int64_t i) { r32p[i] = (float)(rp[i] * scale); });
+            bool conv;
+            inner_cg<float>(c, view, true, r32p, e32p, 1e-3, 500, 2, &conv);
+            launch_map(c, n, [=] __device__(int64_t i) { dp[i] += rn * (double)e32p[i]; });
+            project(dp);
+        }
+        const double dn = norm(dp);
+        if (trace_on()) std::fprintf(stderr, "[stk] null vector step %d: |correction| %.3e after %d sweeps\n", step, dn, sweeps);
+        // a step is kept only while the (quadratically convergent) correction still shrinks
+        if (step > 0 && (!std::isfinite(dn) || dn > 0.1 * prev)) {
+            launch_map(c, n, [=] __device__(int64_t i) { z[i] = kp[i]; });
+            break;
+        }
+        launch_map(c, n, [=] __device__(int64_t i) { z[i] -= dp[i]; });
+        const double inv = 1.0 / norm(z);
+        launch_map(c, n, [=] __device__(int64_t i) { z[i] *= inv; });
+        prev = dn;
+        if (dn < 1e-6) break;
+    }
+    convert_vec(c, z, z32, n);
+    fill_gs_rot(c);
+    EpStoreDot<double> ed{yp, z};
+    const int64_t nb = apply22<8>(c, K, z, ed, c.red64.p);
+    red.finish(nb, SC_TMP);
+    c.null_lambda = red.fetch(SC_TMP);
+    if (trace_on()) std::fprintf(stderr, "[stk] null vector (mixed): lambda %.6e\n", c.null_lambda);
+    pt.mark("near-null vector (FP64 refinement around FP32 multigrid)");
+}
+
+// Set-up after an assembly with FP64 and FP32 operators and the multigrid: what the FP32 methods
+// use (the FP32 hierarchy, the coarsest inverse, the near-null vector).  The FP64 hierarchy waits
+// for the first solve that needs it (ensure_mg64).
+void setup_fast(Context &c) {
+    PhaseTimer pt(c, "set-up");
+    setup_base<double>(c, pt);
+    setup_base<float>(c, pt);
+    setup_mg<float>(c, pt);
+    if (c.rotation_free) {
+        null_vector_mixed(c, pt);
+    } else {
+        c.null_lambda = 0.0;
+        convert_vec(c, c.rot64.p, c.rot32.p, c.nU);
+        fill_gs_rot(c);
+    }
+    c.have_mg32 = true;
+    c.have_mg64 = false;
+    STK_HIP(hipStreamSynchronize(c.stream));
+}
+
+void ensure_mg64(Context &c) {
+    if (!c.mg_built || c.have_mg64 || !c.K64) return;
+    PhaseTimer pt(c, "set-up FP64 multigrid");
+    setup_mg<double>(c, pt);
+    c.have_mg64 = true;
+    STK_HIP(hipStreamSynchronize(c.stream));
+}
 
 // ------------------------------------------------------------------ Schur PCG
 // Pressure-space vectors (nP) and the velocity work vectors of one Schur solve.
@@ -1340,6 +1472,7 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
         throw Error(STK_E_ARG, "null-space deflation needs the plain FP64 method and a near-null rotation "
                                "(no fixed tangential dof)");
     if (o.method == STK_SOLVE_PLAIN) {
+        if (use_mg) ensure_mg64(c);
         SchurWork<double> w = schur_work<double>(c);
         bool conv;
         double sres;
@@ -1479,6 +1612,7 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
             bool conv;
             double sres;
             int64_t it64 = 0;
+            if (use_mg) ensure_mg64(c);
             const bool mg64 = use_mg && c.K64 && c.mg_built && c.have_mg64;
             int k = schur_pcg<double>(c, o, mg64, ru.p, hc.p, w64, o.rtol, o.itmax, 1, o.inner_rtol, &it64, &sres,
                                       &conv, c.pc64.p, false);

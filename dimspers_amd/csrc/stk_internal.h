@@ -278,6 +278,7 @@ struct Context {
     DBuf<double> coarse_inv64;                // dense inverse of the coarsest operator
     DBuf<float> coarse_inv32;
     int64_t coarse_n = 0;
+    bool coarse64_valid = false;              // coarse_inv64 belongs to the current assembly
 
     // stress post-processor: fields at the stress rule, natural-order element data,
     // device Cartesian velocity of the last stk_updatevelocity
