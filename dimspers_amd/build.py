@@ -9,7 +9,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libdimspers_stokes.so"
-SOURCES = ["setup.cpp", "assembly.hip", "element.hip", "solver.hip", "mlp.hip", "stress.hip", "thermal.hip", "fabric.hip", "api.hip"]
+SOURCES = ["setup.cpp", "assembly.hip", "element.hip", "dense.hip", "solver.hip", "mlp.hip", "stress.hip", "thermal.hip", "fabric.hip", "api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -39,7 +39,7 @@ def build(verbose: bool = False, jobs: int = 4) -> Path:
     _wait(procs)
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", *map(str, objs), "-o", str(tmp),
-           "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+           "-Wl,-rpath,/opt/rocm/lib"]
     if verbose:
         print(" ".join(cmd), flush=True)
     out = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)

@@ -4,7 +4,6 @@
 #include <cstring>
 #include <numeric>
 
-#include <rocblas/rocblas.h>
 
 #include "device_ops.h"
 #include "fabric.h"
@@ -22,7 +21,6 @@ double update_velocity(Context &c, double *Ucart_dev);
 
 Context::~Context() {
     delete mlp;
-    if (blas) (void)rocblas_destroy_handle((rocblas_handle)blas);
     if (h_pinned) (void)hipHostFree(h_pinned);
     if (stream) (void)hipStreamDestroy(stream);
 }

@@ -110,7 +110,7 @@ __device__ __forceinline__ void kblock(const T *L, int f, int g, T v[4]) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256, 4) void k_element(ElemArgs<T> a) {
+__global__ __launch_bounds__(256) void k_element(ElemArgs<T> a) {
     __shared__ T lds[kElPerBlock][kElLds];
     __shared__ T ltab[7][9];                    // per point: P2 N (6), P1 NP (3)
     __shared__ int32_t lnode[kElPerBlock][6];

@@ -26,8 +26,6 @@
 #include <memory>
 #include <type_traits>
 
-#include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include "device_ops.h"
 #include "solver.h"
@@ -690,21 +688,34 @@ void diag_inverse(Context &c, const Pattern &p, const T *A, T *dinv) {
 
 // Dense inverse (Gauss-Jordan, partial pivoting) of the coarsest operator, on the host.
 // Dense row-major copy of a CSR 2x2 block operator (unsliced), zero-filled by the caller.
-__global__ void k_dense_from_blocks(int64_t nrows, int64_t n, const int32_t *__restrict__ ptr,
+__global__ void k_dense_from_blocks(int64_t nrows, int64_t ld, const int32_t *__restrict__ ptr,
                                     const int32_t *__restrict__ col, const double *__restrict__ blk,
                                     double *__restrict__ A) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrows) return;
     for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) {
         const int64_t cl = col[k];
-        A[(2 * r) * n + 2 * cl] = blk[4 * (int64_t)k];
-        A[(2 * r) * n + 2 * cl + 1] = blk[4 * (int64_t)k + 1];
-        A[(2 * r + 1) * n + 2 * cl] = blk[4 * (int64_t)k + 2];
-        A[(2 * r + 1) * n + 2 * cl + 1] = blk[4 * (int64_t)k + 3];
+        A[(2 * r) * ld + 2 * cl] = blk[4 * (int64_t)k];
+        A[(2 * r) * ld + 2 * cl + 1] = blk[4 * (int64_t)k + 1];
+        A[(2 * r + 1) * ld + 2 * cl] = blk[4 * (int64_t)k + 2];
+        A[(2 * r + 1) * ld + 2 * cl + 1] = blk[4 * (int64_t)k + 3];
     }
 }
 
-// The lower triangle of a column-major symmetric matrix (rocSOLVER's potri output) to a
+// Identity on the diagonal of the padding rows n <= i < np.
+__global__ void k_pad_identity(int64_t n, int64_t np, double *__restrict__ A) {
+    const int64_t i = n + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) A[i * np + i] = 1.0;
+}
+
+// The leading n x n block of an np x np matrix, into a dense n x n array.
+__global__ void k_copy_leading(int64_t n, int64_t np, const double *__restrict__ src, double *__restrict__ dst) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = blockIdx.y;
+    if (j < n) dst[i * n + j] = src[i * np + j];
+}
+
+// The lower triangle of a column-major symmetric matrix to a
 // full row-major copy.
 __global__ void k_symmetrize_lower_cm(int64_t n, const double *__restrict__ L, double *__restrict__ A) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -715,9 +726,9 @@ __global__ void k_symmetrize_lower_cm(int64_t n, const double *__restrict__ L, d
     A[i * n + j] = L[hi + lo * n];
 }
 
-// Dense inverse of the coarsest multigrid operator, computed on the device in FP64 from
-// the operator of the given precision and stored in that precision: Cholesky factorisation
-// and inverse of the SPD matrix (rocSOLVER potrf + potri on the library stream).
+// Dense inverse of the coarsest multigrid operator, computed on the device in FP64 (from the FP64
+// operator when the assembly has one) and stored in the precision asked: blocked Cholesky
+// factorisation and inverse of the SPD matrix (spd_inverse, dense.hip) on the library stream.
 void build_coarse_inverse(Context &c, bool f64) {
     MGLevel &L = c.mg.back();
     const int64_t n = 2 * L.nvert;
@@ -752,30 +763,24 @@ void build_coarse_inverse(Context &c, bool f64) {
         for (double v : hb)
             if (!std::isfinite(v)) throw Error(STK_E_NAN, "non-finite coarsest multigrid operator");
     }
-    DBuf<double> dA(n * n);
+    // padded to a multiple of 64 with the identity: the blocked inverse works on 64 x 64 tiles
+    const int64_t np = (n + 63) / 64 * 64;
+    DBuf<double> dA(np * np), Y(np * np);
     dA.zero(c.stream);
     k_dense_from_blocks<<<dim3((unsigned)((L.nvert + 255) / 256)), dim3(256), 0, c.stream>>>(
-        L.nvert, n, L.pat.dptr.p, L.pat.dcol.p, bp, dA.p);
+        L.nvert, np, L.pat.dptr.p, L.pat.dcol.p, bp, dA.p);
+    if (np > n) k_pad_identity<<<dim3((unsigned)((np - n + 255) / 256)), dim3(256), 0, c.stream>>>(n, np, dA.p);
     STK_HIP(hipGetLastError());
-    if (!c.blas) {
-        rocblas_handle h;
-        if (rocblas_create_handle(&h) != rocblas_status_success) throw Error(STK_E_HIP, "rocblas_create_handle failed");
-        c.blas = h;
-    }
-    rocblas_handle h = (rocblas_handle)c.blas;
-    if (rocblas_set_stream(h, c.stream) != rocblas_status_success) throw Error(STK_E_HIP, "rocblas_set_stream failed");
-    DBuf<int32_t> info(2);
+    DBuf<int32_t> info(1);
     info.zero(c.stream);
     // symmetric: the row-major matrix is its own column-major copy
-    if (rocsolver_dpotrf(h, rocblas_fill_lower, (rocblas_int)n, dA.p, (rocblas_int)n, info.p) != rocblas_status_success ||
-        rocsolver_dpotri(h, rocblas_fill_lower, (rocblas_int)n, dA.p, (rocblas_int)n, info.p + 1) != rocblas_status_success)
-        throw Error(STK_E_HIP, "rocSOLVER potrf/potri failed");
-    int32_t hinfo[2];
-    info.download(hinfo, 2, c.stream);
+    spd_inverse(c.stream, dA.p, Y.p, np, info.p);
+    int32_t hinfo = 0;
+    info.download(&hinfo, 1, c.stream);
     STK_HIP(hipStreamSynchronize(c.stream));
-    if (hinfo[0] != 0 || hinfo[1] != 0) throw Error(STK_E_SHAPE, "coarsest multigrid operator is not positive definite");
+    if (hinfo != 0) throw Error(STK_E_SHAPE, "coarsest multigrid operator is not positive definite");
     DBuf<double> full(n * n);
-    k_symmetrize_lower_cm<<<dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, c.stream>>>(n, dA.p, full.p);
+    k_copy_leading<<<dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, c.stream>>>(n, np, dA.p, full.p);
     STK_HIP(hipGetLastError());
     dA = std::move(full);
     c.coarse_n = n;

@@ -153,7 +153,6 @@ struct MGLevel {
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
-    void *blas = nullptr;                     // rocblas_handle (dense coarsest inverse), lazily
     std::string err;
 
     int64_t nel = 0, nnod = 0, nvert = 0, nU = 0, nP = 0;
@@ -340,6 +339,9 @@ void detect_structured(Context &c);
 // Gather lists of the element assembly (after the patterns and the multigrid structure).
 void build_gather_lists(Context &c);
 bool fixed_set_is_free_slip(const Context &c);
+// In-place inverse of the SPD matrix A (np x np column-major, np a multiple of 64) with np x np work
+// space Y; *info (device) receives the first non-positive pivot (1-based) or stays 0 (dense.hip).
+void spd_inverse(hipStream_t s, double *A, double *Y, int64_t np, int *info);
 // IC(0) factorisation of the full pressure mass matrix M (values on MN in CSR order) for STK_PPC_IC.
 void build_pressure_ic(Context &c, const std::vector<double> &M);
 // Values of caller operators (1-based CSC) scattered into the layouts of K, G, G^T, M and F.
