@@ -34,9 +34,29 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(workload):
-    """The FP64 oracle (reference StokesPcCG: SuperLU inner solve, lumped Schur PCG) on a
-    bounded sample of the benchmarked workload: the same field definition on a 64 x 512 mesh."""
+# Inner iterations of the plain FP64 method with the point-Jacobi inner CG (the algorithm of the CPU
+# restatement) to the 1e-8 full residual, per workload index, measured on the GPU by
+# `python bench.py --jacobi-count --workload W` (FP64 throughout, so the count is the algorithm's).
+JACOBI_INNER_ITERATIONS = {}
+
+
+def cpu_baseline(inp, workload, budget_s):
+    """The C++/OpenMP CPU restatement (oracle/cpu_stokes.cpp: element assembly, node frames, free
+    slip, StokesPcCG with the lumped Schur preconditioner and a projected point-Jacobi CG inner
+    solve) on all host cores, on the benchmarked inputs: the whole assembly, then ``budget_s`` of
+    the solve.  The solve time is the measured time per inner iteration times the iteration
+    count of the same algorithm at this size."""
+    from oracle import cpu_port
+
+    rep, _, _ = cpu_port.run(inp, budget_s=budget_s, inner_rtol=1e-10)
+    n_it = JACOBI_INNER_ITERATIONS.get(workload)
+    value = None if n_it is None else (rep["assembly_s"] + n_it * rep["s_per_inner_iteration"]) * 1000.0
+    return rep, n_it, value
+
+
+def superlu_aside(workload):
+    """The FP64 numpy/SciPy oracle (StokesPcCG with the SuperLU inner solve, one core) on the
+    same workload definition at 64 x 512 cells."""
     from dimspers_amd.inputs import make_inputs, scaled_workload
     from oracle import stokes_oracle as so
 
@@ -44,8 +64,9 @@ def cpu_baseline(workload):
     t0 = time.perf_counter()
     sol, _ = so.solve_stokes(inp, rtol=1e-8)
     dt = time.perf_counter() - t0
-    ndof = inp.mesh.nU + inp.mesh.nP
-    return dt, ndof, sol.info.iterations
+    return {"s": round(dt, 2), "cells": "64x512", "dofs": inp.mesh.nU + inp.mesh.nP,
+            "outer_iterations": sol.info.iterations, "cores": 1,
+            "note": "numpy/SciPy oracle, SuperLU inner solve; not scaled"}
 
 
 def main():
@@ -63,6 +84,11 @@ def main():
                     help="time every kernel family of the solve with its algorithmic bytes (GB/s, fraction of 8 TB/s)")
     ap.add_argument("--diagnostics", action="store_true",
                     help="only time the SpMV, one V-cycle and one inner CG iteration")
+    ap.add_argument("--jacobi-count", action="store_true",
+                    help="only count the inner iterations of the plain FP64 method with the point-Jacobi inner CG "
+                         "(the CPU restatement's algorithm) to the 1e-8 full residual")
+    ap.add_argument("--cpu-budget", type=float, default=20.0,
+                    help="seconds of CPU-restatement solve time in the CPU baseline sample")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path is not sharded across GPUs.",
@@ -110,6 +136,20 @@ def main():
             log(f"{name}: {kms * 1e3:.1f} us, {kb / kms / 1e6:.0f} GB/s")
         ctx.close()
         print(json.dumps({"rates": res}), flush=True)
+        return
+
+    if args.jacobi_count:
+        # per-outer-iteration progress on stderr while the long solve runs
+        os.environ["STK_TRACE"] = "1"
+        jopts = _lib.default_opts(method=_lib.SOLVE_PLAIN, inner=_lib.INNER_JACOBI, stop=_lib.STOP_FULL,
+                                  rtol=1e-8, itmax=400, inner_rtol=1e-10, inner_itmax=10_000_000)
+        rep = ctx.solve(jopts)
+        ctx.close()
+        print(json.dumps({"jacobi_count": {"workload": w.index, "outer_iterations": rep.outer_iterations,
+                                           "inner_iterations": int(rep.inner_iterations),
+                                           "refinements": rep.refinements, "converged": bool(rep.converged),
+                                           "full_residual": rep.full_residual, "solve_ms": round(rep.solve_ms, 1)}}),
+              flush=True)
         return
 
     if args.kernels or args.diagnostics:
@@ -236,15 +276,47 @@ def main():
         f"CG product K p {pms * 1e3:.1f} us, {pbytes / pms / 1e6:.0f} GB/s; "
         f"plain FP32 SpMV {sms * 1e3:.1f} us, {sbytes / sms / 1e6:.0f} GB/s")
 
+    # one time step of each method as a time loop runs it: re-assembly with the solver set-up it
+    # triggers, then the solve.  The fast methods assemble FP64 + FP32 (FP32 multigrid, coarsest
+    # inverse, near-null vector); the plain method alone assembles FP64 only (FP64 multigrid and
+    # near-null vector).  MLP training is excluded: the network is trained once and reused.
+    per_step = {"learned_or_mixed_ms": round(resetup_ms + ms, 1), "reassembly_and_setup_fast_ms": round(resetup_ms, 1)}
+    if mixed is not None:
+        per_step["mixed_ms"] = round(resetup_ms + mixed["ms"], 1)
+    if not args.no_plain:
+        t0 = time.perf_counter()
+        ctx.assemble(_lib.ASM_FP64)
+        asm64_ms = (time.perf_counter() - t0) * 1000.0
+        prep2 = ctx.solve(plain_opts)
+        per_step["plain_ms"] = round(asm64_ms + prep2.solve_ms, 1)
+        per_step["reassembly_and_setup_fp64_ms"] = round(asm64_ms, 1)
+        per_step["speedup_vs_plain"] = round(per_step["plain_ms"] / per_step["learned_or_mixed_ms"], 2)
+        log(f"per time step: fast {per_step['learned_or_mixed_ms']:.1f} ms, plain {per_step['plain_ms']:.1f} ms "
+            f"(FP64-only re-assembly + set-up {asm64_ms:.1f} ms)")
+
     cpu = None
+    aside = None
     if not args.no_cpu:
-        dt, ndof, its = cpu_baseline(w.index)
-        scaled_ms = dt * 1000.0 * (m.nU + m.nP) / ndof
-        cpu = {"value": round(scaled_ms, 1), "unit": "ms (scaled)", "cores": 1, "kind": "port",
-               "sample": f"FP64 oracle (reference StokesPcCG: SuperLU inner solve, lumped Schur PCG, {its} its, "
-                         f"Schur rtol 1e-8) on the same workload definition at 64x512 cells ({ndof} dofs) took "
-                         f"{dt:.2f} s on one host core; value scales it linearly in dofs to {w.ncell_r}x{w.ncell_t} "
-                         f"(a lower bound: the LU factorisation grows faster than linearly)"}
+        crep, n_it, cval = cpu_baseline(inp, w.index, args.cpu_budget)
+        log(f"CPU restatement: {crep['threads']} threads, assembly {crep['assembly_s']:.2f} s, "
+            f"{crep['inner_iterations']} inner iterations in {crep['solve_s']:.1f} s "
+            f"({crep['s_per_inner_iteration'] * 1e3:.2f} ms each)")
+        cpu = {"value": None if cval is None else round(cval, 1), "unit": "ms", "cores": crep["threads"],
+               "kind": "port",
+               "sample": (f"C++/OpenMP FP64 restatement of assembly_stokes_cylindric + _prepare_matrices + StokesPcCG "
+                          f"(lumped Schur PCG, projected point-Jacobi CG inner solve to 1e-10) on the benchmarked "
+                          f"{w.ncell_r}x{w.ncell_t} inputs, {crep['threads']} OpenMP threads of {os.cpu_count()} "
+                          f"host CPUs: whole assembly {crep['assembly_s']:.2f} s, then {crep['solve_s']:.1f} s of "
+                          f"the solve ({crep['inner_iterations']} inner iterations, "
+                          f"{crep['s_per_inner_iteration'] * 1e3:.2f} ms each); value = assembly + that time per "
+                          f"iteration x " + (f"{n_it} inner iterations (the count of the same algorithm to the "
+                                              f"1e-8 full residual at this size, measured by bench.py --jacobi-count)"
+                                              if n_it is not None else "the iteration count, not measured for "
+                                                                      "this workload: value left null")),
+               "assembly_s": crep["assembly_s"], "s_per_inner_iteration": crep["s_per_inner_iteration"],
+               "sample_inner_iterations": crep["inner_iterations"], "total_inner_iterations": n_it}
+        aside = superlu_aside(w.index)
+        log(f"SciPy/SuperLU oracle at 64x512 (aside): {aside['s']:.2f} s")
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_gs_backward_k32_256x2048.json")
@@ -286,6 +358,7 @@ def main():
         "mixed_without_mlp": mixed,
         "plain_fp64": plain,
         "speedup_vs_plain": round(plain["ms"] / ms, 2) if plain else None,
+        "per_time_step": per_step,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "kernel": "k_gs_colour<256,false,2> (backward multicolour block Gauss-Seidel sweep on K, FP32; "
@@ -295,6 +368,7 @@ def main():
                           "avg_us": round(pms * 1e3, 2), "bytes_per_launch": pbytes,
                           "GB_per_s": round(pbytes / pms / 1e6, 1)},
         "cpu_baseline": cpu,
+        "cpu_aside_superlu_64x512": aside,
         "thermal_step": {"ms": round(trep.solve_ms, 2), "cg_iterations": trep.outer_iterations,
                          "dt": 1e-4, "relative_residual": trep.schur_residual},
         "stress_postprocessor": {

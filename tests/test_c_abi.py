@@ -10,6 +10,7 @@ import pytest
 
 from dimspers_amd import _lib
 from dimspers_amd.inputs import make_inputs, scaled_workload
+from oracle import cpu_port
 
 ROOT = Path(__file__).resolve().parent.parent
 SRC = ROOT / "tests" / "c_abi" / "abi_driver.c"
@@ -32,23 +33,7 @@ def test_header_compiles_as_c_and_the_driver_links(tmp_path):
 
 
 def _write_inputs(inp, d: Path):
-    m = inp.mesh
-    fixed, _ = m.free_slip()
-    arrays = {
-        "e2n": np.ascontiguousarray(np.asarray(m.e2n + 1, dtype=np.int64).T),
-        "el_theta": np.ascontiguousarray(np.asarray(m.el_theta, dtype=np.float64).T),
-        "el_r": np.ascontiguousarray(np.asarray(m.el_r, dtype=np.float64).T),
-        "node_theta": np.ascontiguousarray(m.theta, dtype=np.float64),
-        "node_r": np.ascontiguousarray(m.r, dtype=np.float64),
-        "fixed": np.ascontiguousarray(fixed + 1, dtype=np.int64),
-        "rho": np.ascontiguousarray(inp.rho, dtype=np.float64),
-        "eta": np.asfortranarray(np.asarray(inp.eta_ip)[:, :6], dtype=np.float64),
-    }
-    for k, c in enumerate(inp.C6):
-        arrays[f"C{k}"] = np.asfortranarray(np.asarray(c)[:, :6], dtype=np.float64)
-    for name, a in arrays.items():
-        (d / f"{name}.bin").write_bytes(a.tobytes(order="A"))
-    (d / "meta.txt").write_text(f"{m.nel} {m.nnod} {m.nvert} {fixed.size} {inp.g!r}\n")
+    cpu_port.write_inputs(inp, d)
 
 
 @pytest.mark.gpu
