@@ -211,8 +211,9 @@ def main():
     ms = float(np.mean(times)) + setup_ms
 
     mixed = None
+    mixed_rep = plain_rep = None
     if method == _lib.SOLVE_LEARNED:
-        mrep = ctx.solve(mixed_opts)
+        mrep = mixed_rep = ctx.solve(mixed_opts)
         mixed = {"ms": round(mrep.solve_ms, 2), "outer": mrep.outer_iterations, "inner": int(mrep.inner_iterations),
                  "residual": mrep.full_residual}
         log(f"mixed without the MLP: {mrep.solve_ms:.1f} ms, {mrep.outer_iterations} outer, "
@@ -220,7 +221,7 @@ def main():
 
     plain = None
     if not args.no_plain:
-        prep = ctx.solve(plain_opts)
+        prep = plain_rep = ctx.solve(plain_opts)
         U_pl, P_pl = ctx.solution()
         eu = float(np.linalg.norm(U_fast - U_pl) / np.linalg.norm(U_pl))
         ep = float(np.linalg.norm((P_fast - P_fast.mean()) - (P_pl - P_pl.mean())) / np.linalg.norm(P_pl - P_pl.mean()))
@@ -321,6 +322,20 @@ def main():
         aside = superlu_aside(w.index)
         log(f"SciPy/SuperLU oracle at 64x512 (aside): {aside['s']:.2f} s")
 
+    def bandwidth(rep_):
+        # the solve's own algorithmic bytes (kernel operands, as the library counts them) over its time
+        gbs = rep_.bytes / (rep_.solve_ms * 1e-3) / 1e9
+        return {"bytes": rep_.bytes, "GB_per_s": round(gbs, 1), "frac_of_8TBps": round(gbs / PEAK_HBM_GBS, 4),
+                "launches": int(rep_.launches), "launches_at_8B_floor": int(rep_.launches_floor)}
+
+    solve_bw = {args.method: bandwidth(reps[-1])}
+    if mixed_rep is not None:
+        solve_bw["mixed"] = bandwidth(mixed_rep)
+    if plain_rep is not None:
+        solve_bw["plain_fp64"] = bandwidth(plain_rep)
+    log("solve bandwidth: " + ", ".join(f"{k} {v['GB_per_s']:.0f} GB/s ({v['launches']} launches)"
+                                         for k, v in solve_bw.items()))
+
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_gs_backward_k32_256x2048.json")
     if os.path.exists(tfile):
@@ -361,9 +376,14 @@ def main():
         "mixed_without_mlp": mixed,
         "plain_fp64": plain,
         "speedup_vs_plain": round(plain["ms"] / ms, 2) if plain else None,
+        "value_per_step": per_step["learned_or_mixed_ms"],
         "per_time_step": per_step,
+        "solve_bandwidth": solve_bw,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "traffic_source": "carried over from profiles/traffic_gs_backward_k32_256x2048.json "
+                                       "(rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes over bench.py --kernels, "
+                                       "scaled by a 256 MiB copy); not measured in this run",
                      "kernel": "k_gs_colour<256,false,2> (backward multicolour block Gauss-Seidel sweep on K, FP32; "
                                "seven colour launches per sweep, figures per launch)",
                      "avg_us": round(kms * 1e3, 2), "bytes_per_launch": kbytes},

@@ -61,7 +61,8 @@ class StkReport(C.Structure):
     _fields_ = [
         ("converged", C.c_int32), ("outer_iterations", C.c_int32), ("inner_iterations", C.c_int64),
         ("refinements", C.c_int32), ("fallbacks", C.c_int32), ("schur_residual", C.c_double),
-        ("full_residual", C.c_double), ("solve_ms", C.c_double),
+        ("full_residual", C.c_double), ("solve_ms", C.c_double), ("bytes", C.c_double),
+        ("launches", C.c_int64), ("launches_floor", C.c_int64),
     ]
 
     def as_dict(self):

@@ -88,18 +88,21 @@ __device__ __forceinline__ void block_sum_n(double (&v)[ND]) {
 // eighth of the rows instead measured 25% slower on K (two XCDs held all vertex rows).
 
 // ------------------------------------------------------------------ node-block SpMV
-// Epilogue interface: operator()(row_dof, Ax value) -> contribution to a dot (double).
+// Epilogue interface: operator()(row_dof, Ax value) -> contribution to a dot (double);
+// rowvec() (host) counts the row-length vectors it reads or writes, y included.
 template <typename T>
 struct EpStore {                       // y = A x
     T *y;
     static constexpr int kDots = 1;
     __device__ double operator()(int64_t i, T ax) const { y[i] = ax; return 0.0; }
+    int rowvec() const { return 1; }
 };
 template <typename T>
 struct EpStoreDot {                    // y = A x, dot += p_i (A x)_i
     T *y; const T *p;
     static constexpr int kDots = 1;
     __device__ double operator()(int64_t i, T ax) const { y[i] = ax; return (double)p[i] * (double)ax; }
+    int rowvec() const { return 2; }
 };
 template <typename T>
 struct EpResidual {                    // r = b - A x, dot += r_i^2
@@ -108,24 +111,28 @@ struct EpResidual {                    // r = b - A x, dot += r_i^2
     __device__ double operator()(int64_t i, T ax) const {
         const T v = b[i] - ax; r[i] = v; return (double)v * (double)v;
     }
+    int rowvec() const { return 2; }
 };
 template <typename T>
 struct EpAxpyOut {                     // y = beta*y + alpha*A x  (alpha, beta scalars)
     T *y; T alpha, beta;
     static constexpr int kDots = 1;
     __device__ double operator()(int64_t i, T ax) const { y[i] = beta * y[i] + alpha * ax; return 0.0; }
+    int rowvec() const { return 2; }
 };
 template <typename T>
 struct EpJacobi {                      // damped Jacobi: dout = din + w Dinv (r - A din)
     static constexpr int kDots = 1;
     const T *r, *dinv, *din; T *dout; T w;
     __device__ double operator()(int64_t i, T ax) const { dout[i] = din[i] + w * dinv[i] * (r[i] - ax); return 0.0; }
+    int rowvec() const { return 4; }
 };
 template <typename T>
 struct EpStorePlus {                   // y = A x + a
     static constexpr int kDots = 1;
     T *y; const T *a;
     __device__ double operator()(int64_t i, T ax) const { y[i] = ax + a[i]; return 0.0; }
+    int rowvec() const { return 2; }
 };
 template <typename T>
 struct EpStoreDot2 {                   // y = A x, dots (p . A x, w . A x)
@@ -135,6 +142,7 @@ struct EpStoreDot2 {                   // y = A x, dots (p . A x, w . A x)
         y[i] = ax;
         return Dots<2>{{(double)p[i] * (double)ax, (double)w[i] * (double)ax}};
     }
+    int rowvec() const { return 3; }
 };
 // Chebyshev step that also returns the dots (b . xo, w . xo) of its result.
 template <typename T>
@@ -149,6 +157,7 @@ struct EpChebDot2 {
         xo[i] = v;
         return Dots<2>{{(double)b[i] * (double)v, (double)w[i] * (double)v}};
     }
+    int rowvec() const { return 5 + (rd ? 1 : 0); }
 };
 // Chebyshev step with point-Jacobi: d = c1 d + c2 Dinv (b - A x); xo = x + d.
 // rd = false: first step (c1 = 0), d is not read; wd = false: last step, d is not stored.
@@ -162,6 +171,7 @@ struct EpCheb {
         if (wd) d[i] = dn;
         xo[i] = x[i] + dn; return 0.0;
     }
+    int rowvec() const { return 4 + (rd ? 1 : 0) + (wd ? 1 : 0); }
 };
 
 // First Chebyshev step from a zero start, whose iterate x0 = d0 = s D^-1 b is formed here
@@ -178,6 +188,7 @@ struct EpChebFirst {
         if (wd) d[i] = dn;
         xo[i] = x0 + dn; return 0.0;
     }
+    int rowvec() const { return 3 + (wd ? 1 : 0); }
 };
 
 // Chebyshev step in the symmetrically scaled variables of A^ = S^-1 A S^-1 (S^2 = diag A),
@@ -194,6 +205,7 @@ struct EpChebScaled {
         if (xu) xu[i] = v * sinv[i];
         return 0.0;
     }
+    int rowvec() const { return 5 + (xu ? 2 : 0); }
 };
 // xu += A x (prolongation) and its scaled copy xh = S xu.
 struct EpAxpyScaled {
@@ -205,6 +217,7 @@ struct EpAxpyScaled {
         xh[i] = v * s[i];
         return 0.0;
     }
+    int rowvec() const { return 4; }
 };
 
 template <int BR, int BC, int L, int NT, typename T, typename Ep>
@@ -266,6 +279,7 @@ template <> struct Vec2<double> { typedef double type __attribute__((ext_vector_
 template <typename T>
 struct XPlain {
     const T *x;
+    int xvec() const { return 1; }   // gathered vectors (host, for the byte count)
     __device__ __forceinline__ typename Vec2<T>::type operator()(int64_t c) const {
         return reinterpret_cast<const typename Vec2<T>::type *>(x)[c];
     }
@@ -273,6 +287,7 @@ struct XPlain {
 template <typename T>
 struct XJacobi {
     const T *b, *dinv; T s;
+    int xvec() const { return 0; }   // b and D^-1: counted once, by the epilogue that reads them
     __device__ __forceinline__ typename Vec2<T>::type operator()(int64_t c) const {
         typedef typename Vec2<T>::type V2;
         const V2 bv = reinterpret_cast<const V2 *>(b)[c], dv = reinterpret_cast<const V2 *>(dinv)[c];

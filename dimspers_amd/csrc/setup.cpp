@@ -204,6 +204,14 @@ void build_gs_layout(Context &c) {
         if (nl > 255) throw Error(STK_E_SHAPE, "row with more than 255 lower-colour blocks");
         rlow[i] = nl;
     }
+    g.rows.assign(nc, 0);
+    g.blocks.assign(nc, 0);
+    g.blocks_low.assign(nc, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        g.rows[colour[i]] += 1;
+        g.blocks[colour[i]] += len((int32_t)i);
+        g.blocks_low[colour[i]] += rlow[i];
+    }
     // slice s of colour k holds rows row_off[k] + (s - slice_off[k]) * 64 + j, j < 64;
     // block b of its row j at sptr[s] + b * 64 + j
     for (int k = 0; k < nc; ++k) {

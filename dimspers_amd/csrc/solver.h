@@ -24,12 +24,13 @@ enum : int {
 };
 
 // Two-pass deterministic reductions into device scalar slots.  map_reduce<ND> takes a
-// device function returning a double (ND = 1) or Dots<ND> (slots slot .. slot+ND-1).
+// device function returning a double (ND = 1) or Dots<ND> (slots slot .. slot+ND-1);
+// bpe = the bytes per element it reads and writes (0: not stated, 8 assumed).
 struct Reducer {
     Context &c;
     explicit Reducer(Context &c);
     template <int ND = 1, typename F>
-    void map_reduce(int64_t n, F f, int slot);
+    void map_reduce(int64_t n, F f, int slot, int bpe = 0);
     void finish(int64_t npart, int slot, int nd = 1);
     double fetch(int slot);
     void fetch_many(int slot0, int n, double *out);

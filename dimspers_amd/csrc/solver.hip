@@ -90,22 +90,39 @@ void Reducer::finish(int64_t npart, int slot, int nd) {
     else if (nd == 2) k_reduce_final<NF, 2><<<dim3(1), dim3(NF), 0, c.stream>>>(npart, c.red64.p, c.d_sc.p + slot);
     else throw Error(STK_E_STATE, "reduction of more than two columns");
     STK_HIP(hipGetLastError());
+    c.ledger.add(8.0 * (double)(npart + 1) * nd);
+}
+
+// Element-wise kernels: bpe = bytes per element read and written (0: not stated, 8 assumed).
+inline void ledger_elementwise(Context &c, int64_t n, int bpe) {
+    if (bpe > 0) c.ledger.add((double)n * bpe);
+    else c.ledger.add_floor(n);
 }
 
 template <int ND, typename F>
-void Reducer::map_reduce(int64_t n, F f, int slot) {
+void Reducer::map_reduce(int64_t n, F f, int slot, int bpe) {
     const int nb = std::min<int64_t>(kReduceBlocks, std::max<int64_t>(1, nblk(n)));
     if ((int64_t)c.red64.n < (int64_t)nb * ND) throw Error(STK_E_STATE, "reduction buffer too small");
     k_reduce_map<NT, ND, F><<<dim3(nb), dim3(NT), 0, c.stream>>>(n, f, c.red64.p);
     STK_HIP(hipGetLastError());
+    ledger_elementwise(c, n, bpe);
     finish(nb, slot, ND);
 }
 
 template <typename F>
-void launch_map(Context &c, int64_t n, F f) {
+void launch_map(Context &c, int64_t n, F f, int bpe = 0) {
     if (n == 0) return;
     k_map<NT, F><<<dim3(nblk(n)), dim3(NT), 0, c.stream>>>(n, f);
     STK_HIP(hipGetLastError());
+    ledger_elementwise(c, n, bpe);
+}
+
+// Bytes of a block sparse product: structural blocks (values and a 32-bit column each), row or
+// slice pointers, the gathered vector once (xvec vectors of ncols * BC), the epilogue's row vectors.
+inline double spmv_ledger_bytes(const Pattern &p, int BR, int BC, int vsize, int tsize, int xvec, int rowvec) {
+    const double ptrs = p.sell ? 8.0 * ((p.nrows + Pattern::kSlice - 1) / Pattern::kSlice + 1) : 4.0 * (p.nrows + 1);
+    return (double)p.nnz() * (BR * BC * vsize + 4) + ptrs + (double)xvec * p.ncols * BC * tsize +
+           (double)rowvec * p.nrows * BR * tsize;
 }
 
 template <typename S, typename D>
@@ -121,6 +138,7 @@ int64_t spmv(Context &c, const Pattern &p, const T *val, const T *x, Ep ep, doub
     if (partial && (int64_t)c.red64.n < nb * Ep::kDots) throw Error(STK_E_STATE, "reduction buffer too small");
     k_spmv_block<BR, BC, L, NT, T, Ep><<<dim3(nb), dim3(NT), 0, c.stream>>>(p.nrows, p.dptr.p, p.dcol.p, val, x, ep, partial, nb);
     STK_HIP(hipGetLastError());
+    if (c.ledger.on) c.ledger.add(spmv_ledger_bytes(p, BR, BC, sizeof(T), sizeof(T), 1, ep.rowvec()));
     return nb;
 }
 
@@ -141,6 +159,7 @@ int64_t spmv_sell_x(Context &c, const Pattern &p, const VT *val, XL xl, Ep ep, d
     k_spmv_sell22<NT, VT, T, Ep, XL><<<dim3(nb), dim3(NT), 0, c.stream>>>(p.nrows, p.dsptr.p, p.dscol.p, val, xl, ep,
                                                                        partial, nb);
     STK_HIP(hipGetLastError());
+    if (c.ledger.on) c.ledger.add(spmv_ledger_bytes(p, 2, 2, sizeof(VT), sizeof(T), xl.xvec(), ep.rowvec()));
     return nb;
 }
 
@@ -160,6 +179,7 @@ int64_t apply_rc(Context &c, const Pattern &p, const T *val, const T *x, Ep ep, 
     k_spmv_sell_rc<BR, BC, NT, T, Ep><<<dim3(nb), dim3(NT), 0, c.stream>>>(p.nrows, p.dsptr.p, p.dscol.p, val, x,
                                                                             ep, partial, nb);
     STK_HIP(hipGetLastError());
+    if (c.ledger.on) c.ledger.add(spmv_ledger_bytes(p, BR, BC, sizeof(T), sizeof(T), 1, ep.rowvec()));
     return nb;
 }
 
@@ -173,6 +193,7 @@ int64_t spmv_sell11(Context &c, const Pattern &p, const T *val, const T *x, Ep e
     k_spmv_sell11<NT, T, Ep><<<dim3(nb), dim3(NT), 0, c.stream>>>(p.nrows, p.dsptr.p, p.dscol.p, val, x, ep,
                                                                    partial, nb);
     STK_HIP(hipGetLastError());
+    if (c.ledger.on) c.ledger.add(spmv_ledger_bytes(p, 1, 1, sizeof(T), sizeof(T), 1, ep.rowvec()));
     return nb;
 }
 
@@ -404,6 +425,14 @@ void gs_sweep(Context &c, GsMode mode, const float *b, float *z, bool dots, int6
         else k_gs_colour<NTG, false, 0><<<grid, block, 0, c.stream>>>(STK_GS_ARGS);
 #undef STK_GS_ARGS
         STK_HIP(hipGetLastError());
+        if (c.ledger.on) {
+            // blocks (four values and a column); per real row from zero: b read, b, x written in
+            // coloured order, D^-1, row id, lower-block count; backward: b, x read, x written in
+            // both orders, D^-1, row id (and w with the dots); slice pointers (and lengths)
+            const double nsl = (double)(s1 - s0);
+            c.ledger.add(zero ? (double)g.blocks_low[k] * 20 + (double)g.rows[k] * 45 + nsl * 12
+                              : (double)g.blocks[k] * 20 + (double)g.rows[k] * (part ? 60 : 52) + (nsl + 1) * 8);
+        }
     };
     if (mode == GS_FROM_ZERO) {
         for (int k = 0; k < g.ncolour; ++k) launch(k, true, nullptr, 0);
@@ -459,6 +488,7 @@ T *vcycle(Context &c, MGView<T> &v, size_t l, const T *b, int deg, const T *dot_
         k_dense_matvec<T><<<dim3((unsigned)((v.coarse_n + 3) / 4)), dim3(256), 0, c.stream>>>(v.coarse_n, v.coarse_inv,
                                                                                            b, x);
         STK_HIP(hipGetLastError());
+        c.ledger.add(((double)v.coarse_n * v.coarse_n + 2.0 * v.coarse_n) * sizeof(T));
         return x;
     }
     T *x;
@@ -510,29 +540,30 @@ int64_t inner_cg(Context &c, MGView<T> &v, bool use_mg, T *y, T *z, double rtol,
     Op22<T> K{&c.KN, (const T *)pick<T>(c.K64, c.K32).p};
 
     // project the near-null component out of y; r = y; z = 0 (or the given z); |y|^2
-    red.map_reduce(n, [=] __device__(int64_t i) { return (double)rot[i] * (double)y[i]; }, SC_VY);
+    constexpr int B = (int)sizeof(T);
+    red.map_reduce(n, [=] __device__(int64_t i) { return (double)rot[i] * (double)y[i]; }, SC_VY, 2 * B);
     red.map_reduce(n, [=] __device__(int64_t i) {
         const T v = y[i] - (T)sc[SC_VY] * rot[i];
         y[i] = v;
         r[i] = v;
         if (!warm) z[i] = T(0);
         return (double)v * (double)v;
-    }, SC_BB);
+    }, SC_BB, (warm ? 4 : 5) * B);
     if (warm) {
         // start from the given z (orthogonal to v): r = P (y - K z)
         EpResidual<T> er{r, y};
         apply22<8>(c, K, z, er);
-        red.map_reduce(n, [=] __device__(int64_t i) { return (double)rot[i] * (double)r[i]; }, SC_TMP);
-        launch_map(c, n, [=] __device__(int64_t i) { r[i] -= (T)sc[SC_TMP] * rot[i]; });
+        red.map_reduce(n, [=] __device__(int64_t i) { return (double)rot[i] * (double)r[i]; }, SC_TMP, 2 * B);
+        launch_map(c, n, [=] __device__(int64_t i) { r[i] -= (T)sc[SC_TMP] * rot[i]; }, 3 * B);
     }
     const double lam = c.null_lambda;
     auto finish_null = [&]() {
         // z <- z - (v.z) v  (+ (v.y / lambda) v when deflating)
-        red.map_reduce(n, [=] __device__(int64_t i) { return (double)rot[i] * (double)z[i]; }, SC_VZ);
+        red.map_reduce(n, [=] __device__(int64_t i) { return (double)rot[i] * (double)z[i]; }, SC_VZ, 2 * B);
         const double dl = deflate ? 1.0 / lam : 0.0;
         launch_map(c, n, [=] __device__(int64_t i) {
             z[i] += (T)(dl * sc[SC_VY] - sc[SC_VZ]) * rot[i];
-        });
+        }, 3 * B);
     };
     const double bb = red.fetch(SC_BB);
     if (!std::isfinite(bb)) throw Error(STK_E_NAN, "non-finite right-hand side in the inner CG");
@@ -554,7 +585,7 @@ int64_t inner_cg(Context &c, MGView<T> &v, bool use_mg, T *y, T *z, double rtol,
             } else {
                 red.map_reduce<2>(n, [=] __device__(int64_t i) {
                     return Dots<2>{{(double)rin[i] * (double)res[i], (double)rot[i] * (double)res[i]}};
-                }, slot);
+                }, slot, 3 * (int)sizeof(T));
             }
             return res;
         }
@@ -563,14 +594,14 @@ int64_t inner_cg(Context &c, MGView<T> &v, bool use_mg, T *y, T *z, double rtol,
             const T zv = kdinv[i] * rin[i];
             zo[i] = zv;
             return Dots<2>{{(double)rin[i] * (double)zv, (double)rot[i] * (double)zv}};
-        }, slot);
+        }, slot, 4 * (int)sizeof(T));
         return zo;
     };
     // first direction: p = zz - (v . zz) v, r . z taken with the projected zz
     {
         T *zz = precond(r, SC_RZV);
-        launch_map(c, n, [=] __device__(int64_t i) { p[i] = zz[i] - (T)sc[SC_RZV + 1] * rot[i]; });
-        red.map_reduce(n, [=] __device__(int64_t i) { return (double)r[i] * (double)p[i]; }, SC_RZV);
+        launch_map(c, n, [=] __device__(int64_t i) { p[i] = zz[i] - (T)sc[SC_RZV + 1] * rot[i]; }, 3 * B);
+        red.map_reduce(n, [=] __device__(int64_t i) { return (double)r[i] * (double)p[i]; }, SC_RZV, 2 * B);
     }
     int par = 0;
     const int check_every = 2;
@@ -592,7 +623,7 @@ int64_t inner_cg(Context &c, MGView<T> &v, bool use_mg, T *y, T *z, double rtol,
             const T rv = r[i] - al * (Ap[i] - (T)sc[SC_PAP2 + 1] * rot[i]);
             r[i] = rv;
             return (double)rv * (double)rv;
-        }, slot_rr);
+        }, slot_rr, 7 * B);
         ++it;
         if (it % check_every == 0 || it == itmax) {
             if (trace_on() && it == itmax) std::fprintf(stderr, "[stk] inner CG hit its cap %d\n", itmax);
@@ -621,7 +652,7 @@ int64_t inner_cg(Context &c, MGView<T> &v, bool use_mg, T *y, T *z, double rtol,
         launch_map(c, n, [=] __device__(int64_t i) {
             const T be = (T)(sc[rz_new] / sc[rz_cur]);
             p[i] = zz[i] - (T)sc[rz_new + 1] * rot[i] + be * p[i];
-        });
+        }, 4 * B);
         par = 1 - par;
     }
     finish_null();
@@ -1129,6 +1160,10 @@ void null_vector_mixed(Context &c, PhaseTimer &pt) {
             inner_cg<float>(c, view, true, r32p, e32p, 1e-3, 500, 2, &conv);
             launch_map(c, n, [=] __device__(int64_t i) { dp[i] += rn * (double)e32p[i]; });
             project(dp);
+            // a correction this small (z has unit norm) leaves z far inside the step's 1e-6
+            // criterion even if the sweep's solve were off by a factor of ten: the usual case
+            // when z is the previous assembly's vector
+            if (norm(dp) < 1e-7) break;
         }
         const double dn = norm(dp);
         if (trace_on()) std::fprintf(stderr, "[stk] null vector step %d: |correction| %.3e after %d sweeps\n", step, dn, sweeps);
@@ -1192,15 +1227,15 @@ struct SchurWork {
 template <typename T>
 void vmean_remove(Context &c, T *a, int64_t n) {
     Reducer red(c);
-    red.map_reduce(n, [=] __device__(int64_t i) { return (double)a[i]; }, SC_TMP);
+    red.map_reduce(n, [=] __device__(int64_t i) { return (double)a[i]; }, SC_TMP, (int)sizeof(T));
     const double *sc = c.d_sc.p;
-    launch_map(c, n, [=] __device__(int64_t i) { a[i] -= (T)(sc[SC_TMP] / (double)n); });
+    launch_map(c, n, [=] __device__(int64_t i) { a[i] -= (T)(sc[SC_TMP] / (double)n); }, 2 * (int)sizeof(T));
 }
 
 template <typename T>
 double vdot(Context &c, const T *a, const T *b, int64_t n) {
     Reducer red(c);
-    red.map_reduce(n, [=] __device__(int64_t i) { return (double)a[i] * (double)b[i]; }, SC_TMP);
+    red.map_reduce(n, [=] __device__(int64_t i) { return (double)a[i] * (double)b[i]; }, SC_TMP, 2 * (int)sizeof(T));
     return red.fetch(SC_TMP);
 }
 
@@ -1276,7 +1311,7 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
                     T *buf[2] = {dout, w.Sq};
                     int cur = nsweep % 2;   // the last sweep writes buf[0] = dout
                     T *d0 = buf[cur];
-                    launch_map(c, nP, [=] __device__(int64_t i) { d0[i] = sd[i] * rin[i]; });
+                    launch_map(c, nP, [=] __device__(int64_t i) { d0[i] = sd[i] * rin[i]; }, 3 * (int)sizeof(T));
                     for (int s = 0; s < nsweep; ++s) {
                         EpJacobi<T> ej{rin, sd, buf[cur], buf[1 - cur], T(0.7)};
                         spmv_sell11<T>(c, c.mg[0].pat, c.S32s.p, buf[cur], ej, nullptr);
@@ -1287,14 +1322,15 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
             }
             const T *S = (const T *)pick<T>(c.S64, c.S32).p;
             T *tmp = w.Sq;
-            launch_map(c, nP, [=] __device__(int64_t i) { dout[i] = sd[i] * rin[i]; });
+            launch_map(c, nP, [=] __device__(int64_t i) { dout[i] = sd[i] * rin[i]; }, 3 * (int)sizeof(T));
             for (int s = 0; s < nsweep; ++s) {
                 EpStore<T> es{tmp};
                 spmv<1, 1, 8, T>(c, c.MN, S, dout, es, nullptr);
-                launch_map(c, nP, [=] __device__(int64_t i) { dout[i] += T(0.7) * sd[i] * (rin[i] - tmp[i]); });
+                launch_map(c, nP, [=] __device__(int64_t i) { dout[i] += T(0.7) * sd[i] * (rin[i] - tmp[i]); },
+                           5 * (int)sizeof(T));
             }
         } else if (pcvec) {
-            launch_map(c, nP, [=] __device__(int64_t i) { dout[i] = pcvec[i] * rin[i]; });
+            launch_map(c, nP, [=] __device__(int64_t i) { dout[i] = pcvec[i] * rin[i]; }, 3 * (int)sizeof(T));
         } else {
             // no vector: the incomplete Cholesky of M (FP64 plain method only)
             if constexpr (sizeof(T) == 8) apply_pressure_ic(c, rin, w.Sq, dout);
@@ -1314,9 +1350,10 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
     // residual norm and the next (r . d) and beta numerator.
     Reducer red(c);
     double *sc = c.d_sc.p;
+    constexpr int B = (int)sizeof(T);
     apply_pc(w.r, w.d);
-    launch_map(c, nP, [=] __device__(int64_t i) { w.q[i] = w.d[i]; });
-    red.map_reduce(nP, [=] __device__(int64_t i) { return (double)w.r[i] * (double)w.d[i]; }, SC_SRD);
+    launch_map(c, nP, [=] __device__(int64_t i) { w.q[i] = w.d[i]; }, 2 * B);
+    red.map_reduce(nP, [=] __device__(int64_t i) { return (double)w.r[i] * (double)w.d[i]; }, SC_SRD, 2 * B);
     double rd = red.fetch(SC_SRD);
     int it = 0;
     double beta_prev = 0.0;
@@ -1336,12 +1373,12 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
         if (start) {
             const T bp = (T)beta_prev;
             T *zz = w.z;
-            launch_map(c, nU, [=] __device__(int64_t i) { zz[i] *= bp; });
+            launch_map(c, nU, [=] __device__(int64_t i) { zz[i] *= bp; }, 2 * B);
         }
         *inner_its += inner_cg<T>(c, v, use_mg, w.y, w.z, inner_rtol, o.inner_itmax, deg, &ic, deflate, start);
         EpStore<T> es{w.Sq};
         apply_rc<1, 2, T>(c, c.GTN, GT, w.z, es, nullptr);
-        red.map_reduce(nP, [=] __device__(int64_t i) { return (double)w.q[i] * (double)w.Sq[i]; }, SC_SQSQ);
+        red.map_reduce(nP, [=] __device__(int64_t i) { return (double)w.q[i] * (double)w.Sq[i]; }, SC_SQSQ, 2 * B);
         // r . d of this direction: from before the loop, then from the previous iteration's fetch
         const int rds = it == 1 ? SC_SRD : SC_SRN + 1;
         launch_map(c, nP, [=] __device__(int64_t i) {
@@ -1349,17 +1386,17 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
             w.rl[i] = w.r[i];
             w.dp[i] += al * w.q[i];
             w.r[i] -= al * w.Sq[i];
-        });
-        launch_map(c, nU, [=] __device__(int64_t i) { w.du[i] += (T)(sc[rds] / sc[SC_SQSQ]) * w.z[i]; });
+        }, 7 * B);
+        launch_map(c, nU, [=] __device__(int64_t i) { w.du[i] += (T)(sc[rds] / sc[SC_SQSQ]) * w.z[i]; }, 3 * B);
         vmean_remove<T>(c, w.r, nP);
-        red.map_reduce(nP, [=] __device__(int64_t i) { return (double)w.r[i] * (double)w.r[i]; }, SC_SRN);
+        red.map_reduce(nP, [=] __device__(int64_t i) { return (double)w.r[i] * (double)w.r[i]; }, SC_SRN, B);
         // the next direction's preconditioned residual and its dots (unused after the last
         // iteration: one preconditioner application more than needed)
         apply_pc(w.r, w.d);
         red.map_reduce<2>(nP, [=] __device__(int64_t i) {
             const double d = (double)w.d[i];
             return Dots<2>{{(double)w.r[i] * d, ((double)w.r[i] - (double)w.rl[i]) * d}};
-        }, SC_SRN + 1);
+        }, SC_SRN + 1, 3 * B);
         double h[3];
         red.fetch_many(SC_SRN, 3, h);
         const double rn = std::sqrt(h[0]);
@@ -1378,7 +1415,7 @@ int schur_pcg(Context &c, const stk_opts &o, bool use_mg, const T *fu, const T *
         rd = h[1];
         beta_prev = beta;
         const T be = (T)beta;
-        launch_map(c, nP, [=] __device__(int64_t i) { w.q[i] = w.d[i] + be * w.q[i]; });
+        launch_map(c, nP, [=] __device__(int64_t i) { w.q[i] = w.d[i] + be * w.q[i]; }, 3 * B);
     }
     return std::min(it, itmax);
 }
@@ -1398,21 +1435,21 @@ double full_residual(Context &c, const double *U, const double *P, double *ru, d
     double *t = c.work64.p;
     EpStore<double> eg{t};
     apply_rc<2, 1, double>(c, c.GN, c.G64.p, P, eg, nullptr);
-    launch_map(c, nU, [=] __device__(int64_t i) { ru[i] = fx[i] ? 0.0 : F[i] + t[i] - ru[i]; });
+    launch_map(c, nU, [=] __device__(int64_t i) { ru[i] = fx[i] ? 0.0 : F[i] + t[i] - ru[i]; }, 33);
     if (project_null) {
         // the near-null component is outside the solved space (STK_NULL_PROJECT)
         const double *v = c.rot64.p;
-        red.map_reduce(nU, [=] __device__(int64_t i) { return v[i] * ru[i]; }, SC_TMP + 1);
-        launch_map(c, nU, [=] __device__(int64_t i) { ru[i] -= sc[SC_TMP + 1] * v[i]; });
+        red.map_reduce(nU, [=] __device__(int64_t i) { return v[i] * ru[i]; }, SC_TMP + 1, 16);
+        launch_map(c, nU, [=] __device__(int64_t i) { ru[i] -= sc[SC_TMP + 1] * v[i]; }, 24);
     }
-    red.map_reduce(nU, [=] __device__(int64_t i) { return ru[i] * ru[i]; }, SC_RU);
+    red.map_reduce(nU, [=] __device__(int64_t i) { return ru[i] * ru[i]; }, SC_RU, 8);
     // hc = -G^T U with its mean removed: the Schur iteration works modulo constants
     // (rm0space!, src/IterativeSolvers.jl:131,153), so the continuity residual does too.
     EpStore<double> ec{hc};
     apply_rc<1, 2, double>(c, c.GTN, c.GT64.p, U, ec, nullptr);
-    red.map_reduce(nP, [=] __device__(int64_t i) { return hc[i]; }, SC_TMP);
-    launch_map(c, nP, [=] __device__(int64_t i) { hc[i] = sc[SC_TMP] / (double)nP - hc[i]; });
-    red.map_reduce(nP, [=] __device__(int64_t i) { return hc[i] * hc[i]; }, SC_RC);
+    red.map_reduce(nP, [=] __device__(int64_t i) { return hc[i]; }, SC_TMP, 8);
+    launch_map(c, nP, [=] __device__(int64_t i) { hc[i] = sc[SC_TMP] / (double)nP - hc[i]; }, 16);
+    red.map_reduce(nP, [=] __device__(int64_t i) { return hc[i] * hc[i]; }, SC_RC, 8);
     double v[2];
     red.fetch_many(SC_RU, 2, v);
     if (trace_on())
@@ -1460,6 +1497,12 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
     EventPair ev_;
     hipEvent_t e0 = ev_.a, e1 = ev_.b;
     STK_HIP(hipEventRecord(e0, c.stream));
+    // count the algorithmic bytes of every kernel launched until e1 (off again on any exit)
+    struct LedgerScope {
+        Context::Ledger &l;
+        explicit LedgerScope(Context::Ledger &l) : l(l) { l = Context::Ledger{}; l.on = true; }
+        ~LedgerScope() { l.on = false; }
+    } ledger_scope(c.ledger);
     // masked right-hand side
     DBuf<double> Fm(nU);
     {
@@ -1578,8 +1621,8 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
             {
                 double *a = ru.p, *b = hc.p;
                 float *a32 = ru32.p, *b32 = hc32.p;
-                launch_map(c, nU, [=] __device__(int64_t i) { a32[i] = (float)(a[i] * scale); });
-                launch_map(c, nP, [=] __device__(int64_t i) { b32[i] = (float)(b[i] * scale); });
+                launch_map(c, nU, [=] __device__(int64_t i) { a32[i] = (float)(a[i] * scale); }, 12);
+                launch_map(c, nP, [=] __device__(int64_t i) { b32[i] = (float)(b[i] * scale); }, 12);
             }
             bool conv, stalled = false;
             double sres;
@@ -1604,8 +1647,8 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
             rep.schur_residual = sres;
             const double inv = 1.0 / scale;
             double *U = c.U.p, *P = c.P.p;
-            launch_map(c, nU, [=] __device__(int64_t i) { U[i] += inv * (double)w.du[i]; });
-            launch_map(c, nP, [=] __device__(int64_t i) { P[i] += inv * (double)w.dp[i]; });
+            launch_map(c, nU, [=] __device__(int64_t i) { U[i] += inv * (double)w.du[i]; }, 20);
+            launch_map(c, nP, [=] __device__(int64_t i) { P[i] += inv * (double)w.dp[i]; }, 20);
             rep.refinements++;
         }
         rep.inner_iterations = iits;
@@ -1640,6 +1683,10 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
         launch_map(c, nU, [=] __device__(int64_t i) { U[i] -= sc[SC_TMP] * v[i]; });
     }
     STK_HIP(hipEventRecord(e1, c.stream));
+    c.ledger.on = false;
+    rep.bytes = c.ledger.bytes;
+    rep.launches = c.ledger.launches;
+    rep.launches_floor = c.ledger.floor_launches;
     STK_HIP(hipEventSynchronize(e1));
     float ms = 0;
     STK_HIP(hipEventElapsedTime(&ms, e0, e1));

@@ -156,6 +156,17 @@ struct Context {
     std::string err;
 
     int64_t nel = 0, nnod = 0, nvert = 0, nU = 0, nP = 0;
+
+    // Algorithmic bytes of the kernels launched while `on` (one stk_solve): sparse and dense
+    // products and Gauss-Seidel sweeps by their operands, element-wise kernels by the bytes
+    // per element their call site states, or 8 bytes per element (`floor_launches`) if none.
+    struct Ledger {
+        bool on = false;
+        double bytes = 0.0;
+        int64_t launches = 0, floor_launches = 0;
+        void add(double b) { if (on) { bytes += b; ++launches; } }
+        void add_floor(int64_t n) { if (on) { bytes += 8.0 * (double)n; ++launches; ++floor_launches; } }
+    } ledger;
     // host mesh
     std::vector<int32_t> e2n;        // 6 x nel, 0-based, element-major (e*6 + a)
     std::vector<double> el_th, el_r; // 3 x nel, element-major
@@ -247,6 +258,9 @@ struct Context {
         DBuf<int32_t> pq_src;
         DBuf<float> pq_val;
         bool filled = false;                   // values and dinv of the current FP32 K
+        // per colour: rows, structural blocks, and blocks in lower colours (what a sweep from
+        // zero reads)
+        std::vector<int64_t> rows, blocks, blocks_low;
     } gs;
     int fine_smoother = 0;                     // STK_SMOOTH_* of the last stk_solve
     // Incomplete Cholesky M ~ L D L^T of the pressure mass matrix (STK_PPC_IC), IC(0) on M's

@@ -30,8 +30,9 @@ Opts() = (o = Opts(zeros(Int32, 10)..., zeros(5)..., zeros(Int32, 4)...);
 mutable struct Report
     converged::Int32; outer_iterations::Int32; inner_iterations::Int64; refinements::Int32; fallbacks::Int32
     schur_residual::Float64; full_residual::Float64; solve_ms::Float64
+    bytes::Float64; launches::Int64; launches_floor::Int64
 end
-Report() = Report(0, 0, 0, 0, 0, 0.0, 0.0, 0.0)
+Report() = Report(0, 0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0)
 
 const SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED = Int32(0), Int32(1), Int32(2)
 const STOP_SCHUR, STOP_FULL = Int32(0), Int32(1)

@@ -153,6 +153,14 @@ typedef struct stk_report {
     double schur_residual;      /* final ||r_k|| / ||r_0|| of the last Schur PCG */
     double full_residual;       /* FP64 full-system relative residual           */
     double solve_ms;            /* wall time of the solve on the stream         */
+    double bytes;               /* algorithmic bytes of the solve's kernels: sparse and dense */
+                                /* products and Gauss-Seidel sweeps by their operands (blocks, */
+                                /* indices, slice pointers, the gathered vector once, the     */
+                                /* vectors of the epilogue), element-wise kernels by the      */
+                                /* vectors they read and write; `launches_floor` of them state */
+                                /* no count and are taken at 8 bytes per element              */
+    int64_t launches;           /* kernel launches of the solve                   */
+    int64_t launches_floor;     /* of which counted at the 8-byte floor           */
 } stk_report;
 
 /* Fill *opts with the defaults of the reference StokesPcCG. */

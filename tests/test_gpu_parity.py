@@ -217,6 +217,29 @@ def test_inner_solvers_without_multigrid_match_oracle():
         ctx.close()
 
 
+def test_solve_byte_count_matches_the_kernel_formulas():
+    """stk_report.bytes of a plain solve with the Jacobi inner CG: each inner iteration is one FP64
+    K p with its two dots (the K SpMV bytes of stk_time_kernel plus p and v read once more) and
+    three element-wise passes over 15 FP64 vectors; the inner iterations dominate, so bytes per
+    inner iteration sit just above that sum.  Almost every launch states its byte count."""
+    inp = make_inputs(scaled_workload(3, 16, 128))
+    ctx = _ctx(inp, _lib.ASM_FP64 | _lib.ASM_FP32)
+    rep = ctx.solve(_lib.default_opts(method=_lib.SOLVE_PLAIN, inner=_lib.INNER_JACOBI, stop=_lib.STOP_SCHUR,
+                                      rtol=1e-6, itmax=400, inner_rtol=1e-8, inner_itmax=100000))
+    _, kb = ctx.time_kernel(_lib.KERNEL_SPMV_K64, reps=1)
+    nU = inp.mesh.nU
+    per_it = kb + 2 * nU * 8 + 15 * nU * 8
+    ratio = rep.bytes / rep.inner_iterations / per_it
+    print(f"[result] bytes {rep.bytes:.4e} inner {rep.inner_iterations} launches {rep.launches} "
+          f"floor {rep.launches_floor} ratio {ratio:.4f}")
+    assert 1.0 <= ratio < 1.1
+    assert rep.launches > 4 * rep.inner_iterations and rep.launches_floor < 0.02 * rep.launches
+    # an FP32 multigrid solve counts its bytes too
+    rep2 = ctx.solve(_lib.default_opts(method=_lib.SOLVE_MIXED, stop=_lib.STOP_FULL, rtol=1e-8))
+    assert rep2.bytes > rep2.inner_iterations * kb / 2 and rep2.launches_floor < 0.1 * rep2.launches
+    ctx.close()
+
+
 def test_fp16_smoother_matches_fp32_smoother():
     """The FP16 scaled copy of K only changes the fine-level smoother: the mixed solve
     reaches the same full residual, the same solution (1e-7 of the plain FP64 one) and
