@@ -133,7 +133,7 @@ int stk_create(stk_ctx **out, int64_t nel, int64_t nnod, int64_t nvert, const in
         // of every node on the smallest and largest radius, free_slip in
         // src/Setup/boundary_conditions.jl:27-37); other fixed sets solve without it.
         if (!c.structured)
-            c.mg_note = "the multigrid needs the structured 2-triangle annulus mesh";
+            c.mg_note = "the multigrid needs a structured annulus mesh (Grid_split1 or Grid)";
         else if (!fixed_set_is_free_slip(c))
             c.mg_note = "the multigrid needs free slip on both walls (the fixed set differs)";
         else
@@ -148,9 +148,8 @@ int stk_create(stk_ctx **out, int64_t nel, int64_t nnod, int64_t nvert, const in
                       "vertex pattern mismatch between mesh and multigrid level 0");
             for (MGLevel &L : c.mg) {
                 std::vector<int64_t> fl;
-                const int64_t nr = L.ncr + 1;
-                for (int64_t v = 0; v < L.nvert; ++v)
-                    if (v % nr == 0 || v % nr == L.ncr) fl.push_back(2 * v + 1);
+                for (int64_t d = 0; d < 2 * L.nvert; ++d)
+                    if (L.h_fixed[d]) fl.push_back(d);
                 L.nfix = (int64_t)fl.size();
                 L.fixlist.upload(fl, c.stream);
             }
@@ -393,7 +392,23 @@ int stk_export_csc(stk_ctx *ctx, int32_t which, int32_t lower_only, int64_t *nro
                 STK_CHECK(c.mg_built, STK_E_STATE, "no multigrid hierarchy");
                 p = &c.GTN; br = bc = 2; nr = 2 * c.nP; nc = c.nU; v64 = c.RK64.p; v32 = c.RK32.p;
                 break;
-            default: throw Error(STK_E_ARG, "unknown operator");
+            default: {
+                // multigrid levels and prolongations (set-up checks of the hierarchy)
+                const bool level = which >= STK_MAT_LEVEL && which < STK_MAT_PROLONG;
+                const bool prol = which >= STK_MAT_PROLONG && which < STK_MAT_PROLONG + 32;
+                if (!level && !prol) throw Error(STK_E_ARG, "unknown operator");
+                STK_CHECK(c.mg_built, STK_E_STATE, "no multigrid hierarchy");
+                const int64_t l = which - (level ? STK_MAT_LEVEL : STK_MAT_PROLONG);
+                STK_CHECK(l < (int64_t)c.mg.size(), STK_E_ARG, "no such multigrid level");
+                MGLevel &L = c.mg[l];
+                br = bc = 2;
+                if (level) {
+                    p = &L.pat; nr = nc = 2 * L.nvert; v64 = L.A64.p; v32 = L.A32.p;
+                } else {
+                    p = &L.P_pat; nr = l == 0 ? c.nU : 2 * c.mg[l - 1].nvert; nc = 2 * L.nvert;
+                    v64 = L.P64.p; v32 = L.P32.p;
+                }
+            }
         }
         STK_CHECK(v64 || v32, STK_E_STATE, "operator not available");
         const int64_t nvals = p->nstore() * br * bc;
@@ -699,7 +714,7 @@ int stk_local_schur(stk_ctx *ctx, double *S, float *X) {
     return guarded(ctx, [&]() {
         Context &c = ctx->c;
         STK_CHECK(c.have_fields && c.have_fabric, STK_E_STATE, "fields and fabric needed");
-        STK_CHECK(c.structured, STK_E_NOMESH, "the learned model needs the structured annulus");
+        STK_CHECK(c.mesh_split == 2, STK_E_NOMESH, "the learned model needs the 2-triangle annulus (Grid_split1)");
         DBuf<double> s(6 * c.nel);
         DBuf<float> x(5 * c.nel);
         SchurSampleHost h;

@@ -431,6 +431,76 @@ __device__ __forceinline__ void tri_verts(int t, int I, int J, int vi[3], int vj
     else { vi[0] = I; vj[0] = J + 1; vi[1] = I + 1; vj[1] = J + 1; vi[2] = I + 1; vj[2] = J; }
 }
 
+// E += Ph^T Kf Ph for one child: Kf its 6x6 element matrix (element ef of the fine level), Ph[f][v]
+// the 2x2 interpolation block from coarse vertex v to the child's fine vertex f.
+template <typename T>
+__device__ __forceinline__ void galerkin_child(const GalArgs<T> &a, int64_t ef, const T (&Ph)[3][3][4],
+                                               T (&E)[6][6]) {
+    T Kf[6][6];
+#pragma unroll
+    for (int u = 0; u < 6; ++u)
+#pragma unroll
+        for (int v = 0; v < 6; ++v) Kf[u][v] = a.Ef[(int64_t)(u * 6 + v) * a.nel_f + ef];
+    T T1[6][6];   // Kf Ph
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int v = 0; v < 3; ++v)
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                T s = T(0);
+#pragma unroll
+                for (int f = 0; f < 3; ++f) s += Kf[r][2 * f] * Ph[f][v][cc] + Kf[r][2 * f + 1] * Ph[f][v][2 + cc];
+                T1[r][2 * v + cc] = s;
+            }
+#pragma unroll
+    for (int u = 0; u < 3; ++u)
+#pragma unroll
+        for (int cu = 0; cu < 2; ++cu)
+#pragma unroll
+            for (int c2 = 0; c2 < 6; ++c2) {
+                T s = T(0);
+#pragma unroll
+                for (int f = 0; f < 3; ++f) s += Ph[f][u][cu] * T1[2 * f][c2] + Ph[f][u][2 + cu] * T1[2 * f + 1][c2];
+                E[2 * u + cu][c2] += s;
+            }
+}
+
+// Interpolation block: weight w, rotation by the fine-minus-coarse angle d, radial row / column
+// zero on a fixed fine / coarse dof.
+template <typename T>
+__device__ __forceinline__ void galerkin_block(T w, T d, bool ffix, bool cfix, T (&blk)[4]) {
+    T sd, cd;
+    if constexpr (sizeof(T) == 8) sincos(d, &sd, &cd);
+    else sincosf(d, &sd, &cd);
+    blk[0] = w * cd;
+    blk[1] = cfix ? T(0) : -w * sd;
+    blk[2] = ffix ? T(0) : w * sd;
+    blk[3] = (ffix || cfix) ? T(0) : w * cd;
+}
+
+// Store the coarse element matrix (by element id) and add it into the coarse operator.
+template <typename T>
+__device__ __forceinline__ void galerkin_store(const GalArgs<T> &a, int64_t p, int64_t ec, const T (&E)[6][6]) {
+#pragma unroll
+    for (int u = 0; u < 6; ++u)
+#pragma unroll
+        for (int v = 0; v < 6; ++v) a.Ec[(int64_t)(u * 6 + v) * a.nel_c + ec] = E[u][v];
+    int32_t vid[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vid[k] = a.e2v[k * a.nel_c + p];
+    for (int u = 0; u < 3; ++u)
+        for (int v = 0; v < 3; ++v) {
+            const int64_t sl = a.slot[(u * 3 + v) * a.nel_c + p];
+            const int64_t bm = a.sptr ? a.sptr[vid[u] / Pattern::kSlice] + sl * Pattern::kSlice + vid[u] % Pattern::kSlice
+                                      : (int64_t)a.ptr[vid[u]] + sl;
+            a.A[4 * bm + 0] += E[2 * u][2 * v];
+            a.A[4 * bm + 1] += E[2 * u][2 * v + 1];
+            a.A[4 * bm + 2] += E[2 * u + 1][2 * v];
+            a.A[4 * bm + 3] += E[2 * u + 1][2 * v + 1];
+        }
+}
+
 template <typename T>
 __global__ __launch_bounds__(64) void k_galerkin(GalArgs<T> a) {
     const int64_t j = (int64_t)blockIdx.x * 64 + threadIdx.x;
@@ -466,66 +536,86 @@ __global__ __launch_bounds__(64) void k_galerkin(GalArgs<T> a) {
             const bool ffix = (fi[f] == 0) || (fi[f] == a.ncr_f);
 #pragma unroll
             for (int v = 0; v < 3; ++v) {
-                const T d = dthc * (T(0.5) * fj[f] - T(cj[v]));
-                T sd, cd;
-                if constexpr (sizeof(T) == 8) sincos(d, &sd, &cd);
-                else sincosf(d, &sd, &cd);
                 const bool cfix = (ci[v] == 0) || (ci[v] == a.ncr_c);
-                const T w = lam[v];
-                Ph[f][v][0] = w * cd;
-                Ph[f][v][1] = cfix ? T(0) : -w * sd;
-                Ph[f][v][2] = ffix ? T(0) : w * sd;
-                Ph[f][v][3] = (ffix || cfix) ? T(0) : w * cd;
+                galerkin_block<T>(lam[v], dthc * (T(0.5) * fj[f] - T(cj[v])), ffix, cfix, Ph[f][v]);
             }
         }
-        // Ef (6 x 6) of the child
-        T Kf[6][6];
-#pragma unroll
-        for (int u = 0; u < 6; ++u)
-#pragma unroll
-            for (int v = 0; v < 6; ++v) Kf[u][v] = a.Ef[(int64_t)(u * 6 + v) * a.nel_f + ef];
-        // E += Ph^T Kf Ph
-        T T1[6][6];   // Kf Ph
-#pragma unroll
-        for (int r = 0; r < 6; ++r)
-#pragma unroll
-            for (int v = 0; v < 3; ++v)
-#pragma unroll
-                for (int cc = 0; cc < 2; ++cc) {
-                    T s = T(0);
-#pragma unroll
-                    for (int f = 0; f < 3; ++f) s += Kf[r][2 * f] * Ph[f][v][cc] + Kf[r][2 * f + 1] * Ph[f][v][2 + cc];
-                    T1[r][2 * v + cc] = s;
-                }
-#pragma unroll
-        for (int u = 0; u < 3; ++u)
-#pragma unroll
-            for (int cu = 0; cu < 2; ++cu)
-#pragma unroll
-                for (int c2 = 0; c2 < 6; ++c2) {
-                    T s = T(0);
-#pragma unroll
-                    for (int f = 0; f < 3; ++f) s += Ph[f][u][cu] * T1[2 * f][c2] + Ph[f][u][2 + cu] * T1[2 * f + 1][c2];
-                    E[2 * u + cu][c2] += s;
-                }
+        galerkin_child<T>(a, ef, Ph, E);
     }
+    galerkin_store<T>(a, p, ec, E);
+}
+
+// Four triangles per cell (Grid): coarse element 4q + t = [K_t, K_t+1, centre] of cell (I, J) is the
+// union of two triangles in each of two fine cells (2I + a, 2J + b): c_child4[t] lists (a, b, type).
+// Corner k of a cell sits at the lattice offset (kCornerDi[k], kCornerDj[k]) (radial, azimuthal).
+__constant__ int8_t c_child4[4][4][3] = {
+    {{0, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 1, 3}},
+    {{0, 1, 1}, {0, 1, 2}, {1, 1, 0}, {1, 1, 1}},
+    {{1, 1, 2}, {1, 1, 3}, {1, 0, 1}, {1, 0, 2}},
+    {{1, 0, 3}, {1, 0, 0}, {0, 0, 2}, {0, 0, 3}},
+};
+__constant__ int8_t c_cdi[4] = {0, 0, 1, 1}, c_cdj[4] = {0, 1, 1, 0};
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_galerkin4(GalArgs<T> a) {
+    const int64_t j = (int64_t)blockIdx.x * 64 + threadIdx.x;
+    if (j >= a.cnt) return;
+    const int64_t p = a.off + j;
+    const int64_t ec = a.perm[p];
+    const int t = (int)(ec % 4);
+    const int64_t qc = ec / 4;
+    const int I = (int)(qc % a.ncr_c), J = (int)(qc / a.ncr_c);
+    // coarse vertices in cell-local (radial u, azimuthal v) units, their angles (coarse cell widths,
+    // unwrapped) and fixed radial dofs (wall corners only)
+    T cu[3], cv[3], cpos[3];
+    bool cfix[3];
+    for (int k = 0; k < 2; ++k) {
+        const int kk = (t + k) % 4;
+        cu[k] = T(c_cdi[kk]);
+        cv[k] = T(c_cdj[kk]);
+        cpos[k] = T(J + c_cdj[kk]);
+        const int ci = I + c_cdi[kk];
+        cfix[k] = (ci == 0) || (ci == a.ncr_c);
+    }
+    cu[2] = T(0.5); cv[2] = T(0.5); cpos[2] = T(J) + T(0.5); cfix[2] = false;
+    const T det = (cu[0] - cu[2]) * (cv[1] - cv[2]) - (cu[1] - cu[2]) * (cv[0] - cv[2]);
+    const T dthc = T(2.0 * M_PI) / T(a.nct_c);
+    T E[6][6];
 #pragma unroll
     for (int u = 0; u < 6; ++u)
 #pragma unroll
-        for (int v = 0; v < 6; ++v) a.Ec[(int64_t)(u * 6 + v) * a.nel_c + ec] = E[u][v];
-    int32_t vid[3];
+        for (int v = 0; v < 6; ++v) E[u][v] = T(0);
+    for (int ch = 0; ch < 4; ++ch) {
+        const int da = c_child4[t][ch][0], db = c_child4[t][ch][1], tt = c_child4[t][ch][2];
+        const int64_t ef = 4 * ((int64_t)(2 * I + da) + (int64_t)a.ncr_f * (2 * J + db)) + tt;
+        T Ph[3][3][4];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) vid[k] = a.e2v[k * a.nel_c + p];
-    for (int u = 0; u < 3; ++u)
-        for (int v = 0; v < 3; ++v) {
-            const int64_t sl = a.slot[(u * 3 + v) * a.nel_c + p];
-            const int64_t bm = a.sptr ? a.sptr[vid[u] / Pattern::kSlice] + sl * Pattern::kSlice + vid[u] % Pattern::kSlice
-                                      : (int64_t)a.ptr[vid[u]] + sl;
-            a.A[4 * bm + 0] += E[2 * u][2 * v];
-            a.A[4 * bm + 1] += E[2 * u][2 * v + 1];
-            a.A[4 * bm + 2] += E[2 * u + 1][2 * v];
-            a.A[4 * bm + 3] += E[2 * u + 1][2 * v + 1];
+        for (int f = 0; f < 3; ++f) {
+            // fine vertex f of the child: corners tt and tt+1 of its cell, then the centre
+            T fu, fv, fpos;
+            bool ffix = false;
+            if (f < 2) {
+                const int kk = (tt + f) % 4;
+                fu = T(0.5) * T(da + c_cdi[kk]);
+                fv = T(0.5) * T(db + c_cdj[kk]);
+                fpos = T(J) + fv;
+                const int fi = 2 * I + da + c_cdi[kk];
+                ffix = (fi == 0) || (fi == a.ncr_f);
+            } else {
+                fu = T(0.5) * (T(da) + T(0.5));
+                fv = T(0.5) * (T(db) + T(0.5));
+                fpos = T(J) + fv;
+            }
+            // barycentric coordinates of the fine vertex in the coarse triangle
+            const T l0 = ((fu - cu[2]) * (cv[1] - cv[2]) - (cu[1] - cu[2]) * (fv - cv[2])) / det;
+            const T l1 = ((cu[0] - cu[2]) * (fv - cv[2]) - (fu - cu[2]) * (cv[0] - cv[2])) / det;
+            const T lam[3] = {l0, l1, T(1) - l0 - l1};
+#pragma unroll
+            for (int v = 0; v < 3; ++v) galerkin_block<T>(lam[v], dthc * (fpos - cpos[v]), ffix, cfix[v], Ph[f][v]);
         }
+        galerkin_child<T>(a, ef, Ph, E);
+    }
+    galerkin_store<T>(a, p, ec, E);
 }
 
 // Unit diagonal on the fixed dofs of a BSR 2x2 operator (their rows/columns are zero).
@@ -585,7 +675,8 @@ void galerkin_impl(Context &c, size_t l, T *Ef, T *Ec, T *A) {
         a.off = C.col_off[k];
         a.cnt = C.col_off[k + 1] - C.col_off[k];
         const int nb = (int)((a.cnt + 63) / 64);
-        k_galerkin<T><<<dim3(nb), dim3(64), 0, c.stream>>>(a);
+        if (C.split == 4) k_galerkin4<T><<<dim3(nb), dim3(64), 0, c.stream>>>(a);
+        else k_galerkin<T><<<dim3(nb), dim3(64), 0, c.stream>>>(a);
         STK_HIP(hipGetLastError());
     }
 }

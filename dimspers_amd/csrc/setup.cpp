@@ -468,28 +468,57 @@ void build_gather_lists(Context &c) {
     STK_HIP(hipStreamSynchronize(c.stream));
 }
 
-// The multigrid needs the structured annulus of Grid_split1 (src/Grid/mesher.jl:152-233):
-// vertex v(i, j) = i + nr*j (radial fastest), cell q = i + Nr*j, elements 2q and 2q+1 with
-// vertices [v(i,j), v(i,j+1), v(i+1,j)] and [v(i,j+1), v(i+1,j+1), v(i+1,j)].
+// The multigrid needs one of the structured annuli of src/Grid/mesher.jl, both with corner
+// vertex v(i, j) = i + nr*j (radial fastest) and cell q = i + Nr*j:
+//  * Grid_split1 (:152-233): elements 2q and 2q+1 with vertices [v(i,j), v(i,j+1), v(i+1,j)]
+//    and [v(i,j+1), v(i+1,j+1), v(i+1,j)];
+//  * Grid (:59-150, init_grid(N; split=2), the mesh MAIN.jl:33 builds): centre vertex nr*nt + q,
+//    elements 4q + t = [K_t, K_t+1, centre] with the corners K = v(i,j), v(i,j+1), v(i+1,j+1),
+//    v(i+1,j) (:88-115).
 void detect_structured(Context &c) {
     c.structured = false;
+    c.mesh_split = 0;
     // number of radial vertices: run of equal angles at the start of the vertex list
     int64_t nr = 1;
     while (nr < c.nvert && c.node_th[nr] == c.node_th[0]) ++nr;
-    if (nr < 2 || c.nvert % nr != 0) return;
-    const int64_t nt = c.nvert / nr;
+    if (nr < 2) return;
     const int64_t ncr = nr - 1;
-    if (c.nel != 2 * ncr * nt) return;
-    for (int64_t q = 0; q < ncr * nt; ++q) {
-        const int64_t i = q % ncr, j = q / ncr, jp = (j + 1) % nt;
-        const int32_t v00 = (int32_t)(i + nr * j), v01 = (int32_t)(i + nr * jp);
-        const int32_t v10 = v00 + 1, v11 = v01 + 1;
-        const int32_t *a = &c.e2n[(2 * q) * 6], *b = &c.e2n[(2 * q + 1) * 6];
-        if (a[0] != v00 || a[1] != v01 || a[2] != v10) return;
-        if (b[0] != v01 || b[1] != v11 || b[2] != v10) return;
-    }
+    auto split2 = [&]() {
+        if (c.nvert % nr != 0) return false;
+        const int64_t nt = c.nvert / nr;
+        if (c.nel != 2 * ncr * nt) return false;
+        for (int64_t q = 0; q < ncr * nt; ++q) {
+            const int64_t i = q % ncr, j = q / ncr, jp = (j + 1) % nt;
+            const int32_t v00 = (int32_t)(i + nr * j), v01 = (int32_t)(i + nr * jp);
+            const int32_t v10 = v00 + 1, v11 = v01 + 1;
+            const int32_t *a = &c.e2n[(2 * q) * 6], *b = &c.e2n[(2 * q + 1) * 6];
+            if (a[0] != v00 || a[1] != v01 || a[2] != v10) return false;
+            if (b[0] != v01 || b[1] != v11 || b[2] != v10) return false;
+        }
+        c.ncell_t = (int32_t)nt;
+        return true;
+    };
+    auto split4 = [&]() {
+        if (c.nvert % (nr + ncr) != 0) return false;
+        const int64_t nt = c.nvert / (nr + ncr);
+        if (c.nel != 4 * ncr * nt) return false;
+        for (int64_t q = 0; q < ncr * nt; ++q) {
+            const int64_t i = q % ncr, j = q / ncr, jp = (j + 1) % nt;
+            const int32_t k[4] = {(int32_t)(i + nr * j), (int32_t)(i + nr * jp), (int32_t)(i + nr * jp + 1),
+                                  (int32_t)(i + nr * j + 1)};
+            const int32_t ctr = (int32_t)(nr * nt + q);
+            for (int t = 0; t < 4; ++t) {
+                const int32_t *a = &c.e2n[(4 * q + t) * 6];
+                if (a[0] != k[t] || a[1] != k[(t + 1) % 4] || a[2] != ctr) return false;
+            }
+        }
+        c.ncell_t = (int32_t)nt;
+        return true;
+    };
+    if (split2()) c.mesh_split = 2;
+    else if (split4()) c.mesh_split = 4;
+    else return;
     c.ncell_r = (int32_t)ncr;
-    c.ncell_t = (int32_t)nt;
     c.structured = true;
 }
 
@@ -685,7 +714,8 @@ void build_pressure_ic(Context &c, const std::vector<double> &M) {
 
 namespace {
 
-// Local vertex lattice coordinates (i, j) of the 3 vertices of element type t in cell (I, J).
+// Local vertex lattice coordinates (i, j) of the 3 vertices of element type t in cell (I, J)
+// (two triangles per cell).
 inline void tri_vertices(int t, int I, int J, int vi[3], int vj[3]) {
     if (t == 0) {
         vi[0] = I;     vj[0] = J;
@@ -698,13 +728,27 @@ inline void tri_vertices(int t, int I, int J, int vi[3], int vj[3]) {
     }
 }
 
+// Corner k = 0..3 of cell (I, J) in the order of Grid's quadrilaterals: (I,J), (I,J+1), (I+1,J+1),
+// (I+1,J) as radial / azimuthal lattice offsets.
+constexpr int kCornerDi[4] = {0, 0, 1, 1}, kCornerDj[4] = {0, 1, 1, 0};
+
 void fill_level_geometry(MGLevel &L, hipStream_t s) {
     const int ncr = L.ncr, nct = L.nct, nr = ncr + 1;
-    L.nvert = (int64_t)nr * nct;
-    L.nel = 2ll * ncr * nct;
+    const int64_t ncell = (int64_t)ncr * nct, ncorner = (int64_t)nr * nct;
+    L.nvert = L.split == 4 ? ncorner + ncell : ncorner;
+    L.nel = (int64_t)L.split * ncell;
     L.h_e2v.resize(3 * L.nel);
-    for (int64_t q = 0; q < (int64_t)ncr * nct; ++q) {
+    for (int64_t q = 0; q < ncell; ++q) {
         const int I = (int)(q % ncr), J = (int)(q / ncr);
+        if (L.split == 4) {
+            for (int t = 0; t < 4; ++t) {
+                const int k0 = t, k1 = (t + 1) % 4;
+                L.h_e2v[(4 * q + t) * 3 + 0] = (I + kCornerDi[k0]) + nr * ((J + kCornerDj[k0]) % nct);
+                L.h_e2v[(4 * q + t) * 3 + 1] = (I + kCornerDi[k1]) + nr * ((J + kCornerDj[k1]) % nct);
+                L.h_e2v[(4 * q + t) * 3 + 2] = (int32_t)(ncorner + q);
+            }
+            continue;
+        }
         for (int t = 0; t < 2; ++t) {
             int vi[3], vj[3];
             tri_vertices(t, I, J, vi, vj);
@@ -735,13 +779,18 @@ void fill_level_geometry(MGLevel &L, hipStream_t s) {
     std::vector<uint8_t> fixed(2 * L.nvert, 0);
     const double dth = 2.0 * M_PI / nct;
     for (int64_t v = 0; v < L.nvert; ++v) {
-        const int64_t i = v % nr, j = v / nr;
-        cs[2 * v] = std::cos(dth * j);
-        cs[2 * v + 1] = std::sin(dth * j);
-        if (i == 0 || i == ncr) fixed[2 * v + 1] = 1;
+        // corners at angle dth j; centres half a cell further (radial dofs of wall corners fixed)
+        const bool corner = v < ncorner;
+        const int64_t q = v - ncorner;
+        const int64_t i = corner ? v % nr : q % ncr;
+        const double jj = corner ? (double)(v / nr) : (double)(q / ncr) + 0.5;
+        cs[2 * v] = std::cos(dth * jj);
+        cs[2 * v + 1] = std::sin(dth * jj);
+        if (corner && (i == 0 || i == ncr)) fixed[2 * v + 1] = 1;
     }
     L.cs.upload(cs, s);
     L.fixed.upload(fixed, s);
+    L.h_fixed = std::move(fixed);
 }
 
 // 2x2 rotation block R(d) = [[cos d, -sin d], [sin d, cos d]] scaled by w.
@@ -825,7 +874,8 @@ void build_transfer(MGLevel &coarse, int64_t nfine, const std::vector<Triplet> &
 
 }  // namespace
 
-// Levels: mg[0] = P1 on the original mesh (vertices), mg[l+1] = 2x coarser P1 lattice.
+// Levels: mg[0] = P1 on the original mesh (vertices), mg[l+1] = 2x coarser P1 lattice of the same
+// kind (two or four triangles per cell; the coarse triangles are unions of four fine ones).
 // Transfer mg[0] -> P2 nodes interpolates midpoints from their edge ends; transfer
 // mg[l+1] -> mg[l] interpolates the P1 field of the nested coarse triangles.  Both act on
 // the rotated (u_theta, u_r) node frames, so every block is a rotation by the angle between
@@ -834,17 +884,18 @@ void build_mg_structure(Context &c) {
     c.mg.clear();
     if (!c.structured) return;
     int ncr = c.ncell_r, nct = c.ncell_t;
+    const int split = c.mesh_split;
+    auto nverts = [&](int64_t a, int64_t b) { return (a + 1) * b + (split == 4 ? a * b : 0); };
     {
         MGLevel L;
-        L.ncr = ncr; L.nct = nct;
+        L.ncr = ncr; L.nct = nct; L.split = split;
         c.mg.push_back(std::move(L));
     }
     // halve the lattice while it is above the size solved by a dense inverse
-    while (ncr % 2 == 0 && nct % 2 == 0 && ncr / 2 >= 2 && nct / 2 >= 8 &&
-           (int64_t)(ncr + 1) * nct > kDenseMaxVerts) {
+    while (ncr % 2 == 0 && nct % 2 == 0 && ncr / 2 >= 2 && nct / 2 >= 8 && nverts(ncr, nct) > kDenseMaxVerts) {
         ncr /= 2; nct /= 2;
         MGLevel L;
-        L.ncr = ncr; L.nct = nct;
+        L.ncr = ncr; L.nct = nct; L.split = split;
         c.mg.push_back(std::move(L));
     }
     for (MGLevel &L : c.mg) fill_level_geometry(L, c.stream);
@@ -876,44 +927,44 @@ void build_mg_structure(Context &c) {
                 t.push_back(xb);
             }
         }
-        std::vector<uint8_t> cfix(2 * L0.nvert, 0);
-        for (int64_t v = 0; v < L0.nvert; ++v) {
-            const int64_t i = v % (L0.ncr + 1);
-            if (i == 0 || i == L0.ncr) cfix[2 * v + 1] = 1;
-        }
-        build_transfer(L0, c.nnod, t, c.fixed, cfix, c.stream);
+        build_transfer(L0, c.nnod, t, c.fixed, L0.h_fixed, c.stream);
     }
     // mg[l] <- mg[l+1]
     for (size_t l = 0; l + 1 < c.mg.size(); ++l) {
         MGLevel &F = c.mg[l], &C = c.mg[l + 1];
         const int nrf = F.ncr + 1, nrc = C.ncr + 1;
+        const int64_t ncorner_f = (int64_t)nrf * F.nct, ncorner_c = (int64_t)nrc * C.nct;
         const double dthc = 2.0 * M_PI / C.nct;
         std::vector<Triplet> t;
-        std::vector<uint8_t> ffix(2 * F.nvert, 0), cfix(2 * C.nvert, 0);
-        for (int64_t v = 0; v < F.nvert; ++v) {
-            const int64_t i = v % nrf;
-            if (i == 0 || i == F.ncr) ffix[2 * v + 1] = 1;
-        }
-        for (int64_t v = 0; v < C.nvert; ++v) {
-            const int64_t i = v % nrc;
-            if (i == 0 || i == C.ncr) cfix[2 * v + 1] = 1;
-        }
-        for (int64_t v = 0; v < F.nvert; ++v) {
+        auto cv = [&](int ci, int cj) { return (int32_t)(ci + nrc * (cj % C.nct)); };
+        auto ccentre = [&](int I, int J) { return (int32_t)(ncorner_c + I + (int64_t)C.ncr * J); };
+        // fine angle minus coarse angle, both in units of the coarse cell width (unwrapped)
+        auto add = [&](int64_t v, int32_t cvid, double fpos, double cpos, double w) {
+            Triplet x{(int32_t)v, cvid, {}};
+            rot_block(w, dthc * (fpos - cpos), x.blk);
+            t.push_back(x);
+        };
+        for (int64_t v = 0; v < ncorner_f; ++v) {
             const int i = (int)(v % nrf), j = (int)(v / nrf);
             const int I = i / 2, J = j / 2;
-            auto cv = [&](int ci, int cj) { return (int32_t)(ci + nrc * (cj % C.nct)); };
-            // fine angle minus coarse angle, in units of the coarse cell width
-            auto add = [&](int ci, int cj, double w) {
-                Triplet x{(int32_t)v, cv(ci, cj), {}};
-                rot_block(w, dthc * (0.5 * j - cj), x.blk);
-                t.push_back(x);
-            };
-            if (i % 2 == 0 && j % 2 == 0) add(I, J, 1.0);
-            else if (i % 2 == 1 && j % 2 == 0) { add(I, J, 0.5); add(I + 1, J, 0.5); }
-            else if (i % 2 == 0 && j % 2 == 1) { add(I, J, 0.5); add(I, J + 1, 0.5); }
-            else { add(I, J + 1, 0.5); add(I + 1, J, 0.5); }
+            const double fpos = 0.5 * j;
+            if (i % 2 == 0 && j % 2 == 0) add(v, cv(I, J), fpos, J, 1.0);
+            else if (i % 2 == 1 && j % 2 == 0) { add(v, cv(I, J), fpos, J, 0.5); add(v, cv(I + 1, J), fpos, J, 0.5); }
+            else if (i % 2 == 0 && j % 2 == 1) { add(v, cv(I, J), fpos, J, 0.5); add(v, cv(I, J + 1), fpos, J + 1, 0.5); }
+            else if (F.split == 4) add(v, ccentre(I, J), fpos, J + 0.5, 1.0);   // the coarse cell's centre
+            else { add(v, cv(I, J + 1), fpos, J + 1, 0.5); add(v, cv(I + 1, J), fpos, J, 0.5); }
         }
-        build_transfer(C, F.nvert, t, ffix, cfix, c.stream);
+        // fine centres (four triangles per cell): the centre of fine cell (2I + a, 2J + b) is the
+        // midpoint of the coarse centre and the coarse corner (I + a, J + b)
+        for (int64_t v = ncorner_f; v < F.nvert; ++v) {
+            const int64_t q = v - ncorner_f;
+            const int fi = (int)(q % F.ncr), fj = (int)(q / F.ncr);
+            const int I = fi / 2, J = fj / 2, a = fi % 2, b = fj % 2;
+            const double fpos = 0.5 * (fj + 0.5);
+            add(v, ccentre(I, J), fpos, J + 0.5, 0.5);
+            add(v, cv(I + a, J + b), fpos, J + b, 0.5);
+        }
+        build_transfer(C, F.nvert, t, F.h_fixed, C.h_fixed, c.stream);
     }
     STK_HIP(hipStreamSynchronize(c.stream));
 }

@@ -122,9 +122,12 @@ struct SchurBufs {
     DBuf<T> v;   // y, z, du (nU) then r, d, q, Sq, rl, pc, dp (nP)
 };
 
-// One multigrid level (P1 vertex lattice of the structured annulus, 2 dofs per vertex).
+// One multigrid level (P1 vertex lattice of the structured annulus, 2 dofs per vertex).  The
+// cells are split in two triangles (Grid_split1) or in four around a centre vertex (Grid);
+// corner vertex (i, j) is i + (ncr + 1) j, the centre of cell q = i + ncr j is (ncr + 1) nct + q.
 struct MGLevel {
     int32_t ncr = 0, nct = 0;        // cells radial / azimuthal
+    int32_t split = 2;               // triangles per cell: 2 or 4
     int64_t nvert = 0, nel = 0;
     Pattern pat;                     // vertex -> vertex (BSR 2x2)
     std::vector<int32_t> h_e2v;      // 3 x nel vertex ids (element-major per local vertex)
@@ -134,6 +137,7 @@ struct MGLevel {
     std::vector<int64_t> col_off;    // colour offsets into perm
     DBuf<double> cs;                 // vertex cos/sin (2 x nvert)
     DBuf<uint8_t> fixed;             // per dof
+    std::vector<uint8_t> h_fixed;    // host copy: the radial dofs of the wall corners
     DBuf<double> E64;                // 36 x nel element matrices (rotated frame, masked)
     DBuf<float> E32;
     DBuf<double> A64, dinv64;        // 4 x nnz blocks, inverse diagonal (2 nvert)
@@ -283,6 +287,7 @@ struct Context {
 
     // multigrid
     bool structured = false;
+    int32_t mesh_split = 0;                   // structured annulus: 2 (Grid_split1) or 4 (Grid) triangles per cell
     bool rotation_free = true;                // no tangential dof fixed: the rotation is admissible
     std::string mg_note;                      // why the multigrid is not built (when it is not)
     int32_t ncell_r = 0, ncell_t = 0;

@@ -58,6 +58,7 @@ class AnnulusMesh:
     nvert: int
     el_theta: np.ndarray = field(repr=False, default=None)  # (3, nel) vertex angles after fixangles
     el_r: np.ndarray = field(repr=False, default=None)      # (3, nel)
+    split: int = 2             # triangles per cell: 2 (Grid_split1) or 4 (Grid)
 
     @property
     def nel(self) -> int:
@@ -143,6 +144,48 @@ def build_annulus(ncell_r: int, ncell_t: int) -> AnnulusMesh:
     tri[:, 1::2] = np.stack([q2, q3, q4])   # local [2,3,4]
     nel = tri.shape[1]
 
+    return _with_midpoints(ncell_r, ncell_t, tri, tv, rv, nvert)
+
+
+def build_grid(ncell_r: int, ncell_t: int) -> AnnulusMesh:
+    """4-triangle-per-cell P2 annulus mesh: the connectivity of ``Grid`` (src/Grid/mesher.jl:59-150),
+    the mesh ``init_grid(N; split=2)`` builds and MAIN.jl:33 runs on.
+
+    Corner vertices as in ``build_annulus``; each quad [v(i,j) v(i,j+1) v(i+1,j+1) v(i+1,j)] gets a
+    centre vertex nr*nt + q (q = i + ncell_r*j, radial fastest; angle + dtheta/2, radius + dr/2 with
+    dr the first radial spacing, :101-118) and the four triangles [corner k, corner k+1, centre],
+    k = 1..4 (:108-115); edge midpoints follow in first-seen order."""
+    nr = ncell_r + 1
+    nt = ncell_t
+    ncorner = nr * nt
+    dth = 2.0 * np.pi / nt
+    rc = linrange(R_IN, R_OUT, nr)
+    rv_c = np.tile(rc, nt)
+    tv_c = np.repeat(dth * np.arange(nt, dtype=np.float64), nr)
+    i = np.tile(np.arange(ncell_r), nt)
+    j = np.repeat(np.arange(nt), ncell_r)
+    jp = (j + 1) % nt
+    q1 = i + nr * j
+    q2 = i + nr * jp
+    corners = np.stack([q1, q2, q2 + 1, q1 + 1])   # (4, ncell)
+    ncell = ncell_r * nt
+    centre = ncorner + np.arange(ncell)
+    tri = np.empty((3, 4 * ncell), dtype=np.int64)
+    for k in range(4):
+        tri[0, k::4] = corners[k]
+        tri[1, k::4] = corners[(k + 1) % 4]
+        tri[2, k::4] = centre
+    dr = rc[1] - rc[0]
+    tv = np.concatenate([tv_c, tv_c[q1] + dth / 2.0])
+    rv = np.concatenate([rv_c, rv_c[q1] + dr / 2.0])
+    mesh = _with_midpoints(ncell_r, ncell_t, tri, tv, rv, ncorner + ncell)
+    mesh.split = 4
+    return mesh
+
+
+def _with_midpoints(ncell_r, ncell_t, tri, tv, rv, nvert) -> AnnulusMesh:
+    """Append the edge midpoints of the P1 triangles ``tri`` (vertex angles tv, radii rv)."""
+    nel = tri.shape[1]
     # edges (1,2), (2,3), (3,1) of each element, numbered in first-seen order
     a = np.stack([tri[0], tri[1], tri[2]])        # (3, nel) edge start
     b = np.stack([tri[1], tri[2], tri[0]])        # (3, nel) edge end
@@ -169,7 +212,6 @@ def build_annulus(ncell_r: int, ncell_t: int) -> AnnulusMesh:
     e2n = np.concatenate([tri, edge_id + nvert], axis=0)
     theta = np.concatenate([tv, tmid])
     r = np.concatenate([rv, rmid])
-    assert theta.shape[0] == nvert + nedge == (4 * ncell_r + 2) * ncell_t
 
     el_theta = fixangles(tv[tri])
     el_r = rv[tri]
@@ -177,7 +219,7 @@ def build_annulus(ncell_r: int, ncell_t: int) -> AnnulusMesh:
 
 
 def structured_ids(mesh: AnnulusMesh):
-    """(i_r, i_theta, triangle type) of every element of ``build_annulus``."""
+    """(i_r, i_theta, triangle type) of every element of ``build_annulus`` / ``build_grid``."""
     e = np.arange(mesh.nel)
-    q = e // 2
-    return q % mesh.ncell_r, q // mesh.ncell_r, e % 2
+    q = e // mesh.split
+    return q % mesh.ncell_r, q // mesh.ncell_r, e % mesh.split

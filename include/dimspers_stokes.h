@@ -61,8 +61,14 @@ enum {
     STK_MAT_MT = 6,   /* thermal mass matrix rho Cp, nnod x nnod */
     STK_MAT_R = 7,    /* multigrid restriction P2 -> P1 (rotated */
                       /* frames, fixed dofs masked), 2 nP x nU   */
-    STK_MAT_RK = 8    /* R K (K with boundary conditions), the   */
+    STK_MAT_RK = 8,   /* R K (K with boundary conditions), the   */
                       /* coarse residual operator, 2 nP x nU     */
+    STK_MAT_LEVEL = 16,   /* + l: the operator of multigrid level l  */
+                          /* (l = 0: P1 on the mesh's vertices), */
+                          /* 2 nvert_l x 2 nvert_l                   */
+    STK_MAT_PROLONG = 48  /* + l: the prolongation into the next */
+                          /* finer space from level l (l = 0: into */
+                          /* the P2 velocity, = R^T), n_finer x 2 nvert_l */
 };
 
 /* Fine-level smoother of the FP32 multigrid V-cycle (stk_opts.fine_smoother). */

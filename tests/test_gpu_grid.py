@@ -1,0 +1,123 @@
+"""The fast path on the reference's own mesh: ``Grid`` (src/Grid/mesher.jl:59-150), four triangles
+per cell around a centre vertex, the mesh ``init_grid(5; split=2)`` of MAIN.jl:33 builds.
+
+The library recognises it, builds its multigrid (four-triangle P1 lattices, 2x2 cells per coarse
+cell), and the plain and mixed methods solve it like the oracle.  The hierarchy is pinned
+level by level: each level's operator is the Galerkin product of the finer one with the exported
+prolongation (unit diagonal on the fixed radial dofs of the wall corners), on both mesh kinds.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from dimspers_amd import _lib
+from dimspers_amd.inputs import make_inputs, scaled_workload
+from dimspers_amd.mesh import build_annulus, build_grid
+from oracle import stokes_oracle as so
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid_inputs(nr, nt, w=3):
+    return make_inputs(scaled_workload(w, nr, nt), mesh=build_grid(nr, nt))
+
+
+def _ctx(inp, flags):
+    m = inp.mesh
+    print(f"[case] grid split {m.split} {m.ncell_r}x{m.ncell_t} flags {flags}", flush=True)
+    ctx = _lib.Context.from_mesh(m)
+    ctx.set_fields(inp.g, inp.rho, inp.eta_ip, inp.C6)
+    ctx.assemble(flags)
+    return ctx
+
+
+def _rel(a, b):
+    a = a.toarray() if hasattr(a, "toarray") else a
+    b = b.toarray() if hasattr(b, "toarray") else b
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+def test_grid_assembly_matches_oracle():
+    inp = _grid_inputs(4, 32)
+    ctx = _ctx(inp, _lib.ASM_FP64 | _lib.ASM_NO_BC)
+    assert ctx.mg_levels == 1   # 288 vertices: the mesh's own P1 level is the dense coarsest one
+    KK, GG, MM, Fb = so.assembly_stokes_cylindric(inp)
+    K, G, F = so.prepare_matrices(KK, GG, Fb, so.rotation_matrix(inp.mesh.theta))
+    assert _rel(ctx.export_csc(_lib.MAT_K), K) < 1e-12
+    assert _rel(ctx.export_csc(_lib.MAT_G), G) < 1e-12
+    assert _rel(ctx.export_csc(_lib.MAT_M), so.symmetric_from_lower(MM)) < 1e-12
+    assert np.abs(ctx.export_rhs() - F).max() / np.abs(F).max() < 1e-12
+    ctx.close()
+
+
+@pytest.mark.parametrize("kind", ["grid", "split1"])
+def test_multigrid_levels_are_galerkin_products(kind):
+    """A_0 = P_0^T K P_0 (P_0 = R^T: P1 vertices -> P2 nodes) and A_l = P_l^T A_{l-1} P_l, plus the
+    unit diagonal of the fixed dofs, to rounding; the hierarchy reaches the dense coarsest level."""
+    nr, nt = (32, 256)
+    inp = _grid_inputs(nr, nt) if kind == "grid" else make_inputs(scaled_workload(3, nr, nt), mesh=build_annulus(nr, nt))
+    ctx = _ctx(inp, _lib.ASM_FP64)
+    # lattices halve while above 2,300 vertices: 32x256 has 3 levels with four triangles, 2 with two
+    assert ctx.mg_levels == (3 if kind == "grid" else 2)
+    fine = ctx.export_csc(_lib.MAT_K)
+    P0 = ctx.export_csc(_lib.MAT_PROLONG)
+    assert _rel(P0, ctx.export_csc(_lib.MAT_R).T) == 0.0
+    for l in range(ctx.mg_levels):
+        P = ctx.export_csc(_lib.MAT_PROLONG + l)
+        A = ctx.export_csc(_lib.MAT_LEVEL + l)
+        gal = (P.T @ fine @ P).tocsc()
+        fixed = np.flatnonzero(np.asarray(abs(P).sum(axis=0)).ravel() == 0)   # masked coarse dofs
+        D = sp.csc_matrix((np.ones(fixed.size), (fixed, fixed)), shape=A.shape)
+        err = _rel(A, gal + D)
+        print(f"[result] {kind} level {l}: n {A.shape[0]}, fixed {fixed.size}, rel err {err:.2e}")
+        assert err < 1e-12
+        # symmetric, and every fixed dof is a radial dof (odd index)
+        assert _rel(A, A.T) < 1e-13 and np.all(fixed % 2 == 1)
+        fine = A
+    ctx.close()
+
+
+@pytest.mark.parametrize("mesh", [(4, 32), (16, 128)])
+@pytest.mark.parametrize("method", ["plain", "mixed"])
+def test_grid_solves_match_oracle(mesh, method):
+    nr, nt = mesh
+    inp = _grid_inputs(nr, nt)
+    KK, GG, MM, Fb = so.assembly_stokes_cylindric(inp)
+    K, G, F = so.prepare_matrices(KK, GG, Fb, so.rotation_matrix(inp.mesh.theta))
+    _, ifree = inp.mesh.free_slip()
+    v, _ = so.null_vector(K, ifree, inp.mesh)
+    Uo, Po, _ = so.stokes_pcg(K, MM, G, F, ifree, rtol=1e-12, itmax=400, inner=so.inner_projected(K, ifree, v))
+    ctx = _ctx(inp, _lib.ASM_FP64 | _lib.ASM_FP32)
+    m = _lib.SOLVE_PLAIN if method == "plain" else _lib.SOLVE_MIXED
+    rep = ctx.solve(_lib.default_opts(method=m, stop=_lib.STOP_FULL, rtol=1e-10, itmax=400))
+    assert rep.converged and rep.full_residual < 1e-10
+    U, P = ctx.solution()
+    eu = np.linalg.norm(U - Uo) / np.linalg.norm(Uo)
+    ep = np.linalg.norm((P - P.mean()) - (Po - Po.mean())) / np.linalg.norm(Po - Po.mean())
+    print(f"[result] grid {nr}x{nt} {method}: outer {rep.outer_iterations} inner {rep.inner_iterations} "
+          f"errU {eu:.2e} errP {ep:.2e}")
+    assert eu < 1e-7 and ep < 1e-7
+    ctx.close()
+
+
+def test_grid_multigrid_is_as_effective_as_on_the_two_triangle_mesh():
+    """Same fields, same cells: the mixed method's inner iterations per outer iteration on the
+    four-triangle mesh stay within 1.5x of the two-triangle mesh's (a broken coarse level or
+    transfer would not converge in a few V-cycles); the learned model is refused with its reason."""
+    nr, nt = 32, 256
+    per = {}
+    for kind, mesh in (("grid", build_grid(nr, nt)), ("split1", build_annulus(nr, nt))):
+        inp = make_inputs(scaled_workload(3, nr, nt), mesh=mesh)
+        ctx = _ctx(inp, _lib.ASM_FP64 | _lib.ASM_FP32)
+        rep = ctx.solve(_lib.default_opts(method=_lib.SOLVE_MIXED, stop=_lib.STOP_FULL, rtol=1e-8))
+        assert rep.converged
+        per[kind] = rep.inner_iterations / rep.outer_iterations
+        print(f"[result] {kind} {nr}x{nt}: outer {rep.outer_iterations} inner {rep.inner_iterations} "
+              f"{rep.solve_ms:.1f} ms", flush=True)
+        if kind == "grid":
+            ctx.set_fabric(inp.phi, inp.delta)
+            with pytest.raises(_lib.StkError) as e:
+                ctx.mlp_train(_lib.default_train_opts())
+            assert e.value.code == _lib.STK_E_NOMESH
+        ctx.close()
+    assert per["grid"] < 1.5 * per["split1"]

@@ -15,7 +15,7 @@ import scipy.sparse as sp
 
 from dimspers_amd.inputs import (StokesInputs, direct_rotation_2d, make_inputs, rotmatrix, scaled_workload,
                                  splitmix64_uniform)
-from dimspers_amd.mesh import R_IN, R_OUT, build_annulus, fixangles
+from dimspers_amd.mesh import R_IN, R_OUT, build_annulus, build_grid, fixangles
 from dimspers_amd.quadrature import ip_triangle6, ip_triangle7, sf_dN_tri6, sf_N_tri3, sf_N_tri6
 from oracle import stokes_oracle as so
 
@@ -105,6 +105,42 @@ def test_mesh_connectivity_follows_grid_split1():
     # fixangles: every element spans less than pi after unwrapping
     span = th.max(0) - th.min(0)
     assert np.all(span < 1.0) and np.all(span > 0)
+
+
+def test_grid_connectivity_follows_the_four_triangle_grid():
+    """``Grid`` (mesher.jl:59-150): corners radial fastest, centre nr*nt + q at angle + dtheta/2 and
+    radius + dr/2, triangles [corner k, corner k+1, centre] of the quad [v(i,j) v(i,j+1) v(i+1,j+1)
+    v(i+1,j)], midpoints in first-seen order; counts; every element positively oriented in (theta, r);
+    the annulus area from the quadrature; free slip fixes only wall corners and wall midpoints."""
+    nr, nt = 4, 16
+    m = build_grid(nr, nt)
+    v = lambda i, j: i + (nr + 1) * (j % nt)
+    ncorner = (nr + 1) * nt
+    assert m.split == 4 and m.nel == 4 * nr * nt and m.nvert == ncorner + nr * nt
+    assert m.nnod == m.nvert + nr * nt * 4 + nr * nt + (nr + 1) * nt   # vertices + internal, radial, angular edges
+    q = 2 + nr * 15                                 # cell (i, j) = (2, 15), the last column
+    quad = [v(2, 15), v(2, 16), v(3, 16), v(3, 15)]
+    for k in range(4):
+        assert list(m.e2n[:3, 4 * q + k]) == [quad[k], quad[(k + 1) % 4], ncorner + q]
+    dth, dr = 2 * np.pi / nt, (R_OUT - R_IN) / nr
+    assert m.theta[ncorner + q] == pytest.approx(15 * dth + dth / 2, abs=1e-15)
+    assert m.r[ncorner + q] == pytest.approx(R_IN + 2 * dr + dr / 2, abs=1e-15)
+    assert list(m.e2n[3:, 0]) == [m.nvert, m.nvert + 1, m.nvert + 2]
+    th, r = m.el_theta, m.el_r
+    det = (th[1] - th[0]) * (r[2] - r[0]) - (th[2] - th[0]) * (r[1] - r[0])
+    assert np.all(det > 0)
+    for k, (a, b) in enumerate([(0, 1), (1, 2), (2, 0)]):
+        mid = m.e2n[3 + k]
+        assert np.allclose(m.r[mid], 0.5 * (r[a] + r[b]), atol=1e-15)
+        assert np.allclose(np.cos(m.theta[mid]), np.cos(0.5 * (th[a] + th[b])), atol=1e-14)
+    total = 0.0
+    for qq in range(6):
+        _, _, om, _, _ = so.ip_derivatives(m.el_theta, m.el_r, qq)
+        total += om.sum()
+    assert total == pytest.approx(np.pi * (R_OUT ** 2 - R_IN ** 2), rel=1e-13)
+    fixed, _ = m.free_slip()
+    assert fixed.size == 4 * nt and np.all(fixed % 2 == 1)
+    assert not np.any((fixed // 2 >= ncorner) & (fixed // 2 < m.nvert))   # no centre on a wall
 
 
 def test_fixangles_wrap_rule():
