@@ -1,12 +1,14 @@
 // Per-element MLP of the learned local-Schur preconditioner (no counterpart in
 // the reference): 5 -> 64 -> 64 -> 64 -> 6 with GELU, BF16 weights and BF16
-// activations as MFMA operands, FP32 accumulation (v_mfma_f32_32x32x8_bf16),
+// activations as MFMA operands, FP32 accumulation (gfx950's v_mfma_f32_32x32x16_bf16),
 // FP32 master weights trained with Adam.
 //
 // One wavefront handles a tile of 32 samples.  A layer Y[32 x N] = H[32 x K] W[K x N]
-// is a sum of 32x32x8 MFMA products: lane l feeds A[i][k..k+3] and B[k..k+3][j] with
-// i = j = l % 32, k = 4 (l / 32) (+ 8 per k-step), and receives C[i][j] for
-// j = l % 32, i = 8 (r / 4) + 4 (l / 32) + r % 4 in accumulator register r.
+// is a sum of 32x32x16 MFMA products: lane l feeds A[i][k..k+7] and B[k..k+7][j] with
+// i = j = l % 32, k = 8 (l / 32) (+ 16 per k-step), each operand one 16-byte LDS load,
+// and receives C[i][j] for j = l % 32, i = 8 (r / 4) + 4 (l / 32) + r % 4 in accumulator
+// register r.  (Which k a lane's eight values stand for only has to agree between A and
+// B: every k enters the sum once.)
 //
 // Inputs: (mean log eta, cos 2(phi - theta_c), sin 2(phi - theta_c), log delta,
 // area / mean area); outputs: the log-Cholesky factor (log L00, L10, log L11, L20,
@@ -24,16 +26,17 @@ namespace stk {
 
 namespace {
 
-typedef short v4i16 __attribute__((ext_vector_type(4)));
+typedef short v8i16 __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf16 __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 constexpr int NTW = 64;   // threads per workgroup (one wavefront)
 
-// packed BF16 weight layouts (uint16 offsets)
-// forward, transposed [out][in_pad8]: WT1 64x8, WT2 64x64, WT3 64x64, WT4 32x64 (rows >= 6 zero)
-constexpr int tW1 = 0, tW2 = 512, tW3 = 4608, tW4 = 8704, kWT = 10752;
-// backward, natural [in][out_pad8]: WN4 64x8, WN3 64x64, WN2 64x64
-constexpr int nW4 = 0, nW3 = 512, nW2 = 4608, kWN = 8704;
+// packed BF16 weight layouts (uint16 offsets; K of every product a multiple of 16)
+// forward, transposed [out][in_pad16]: WT1 64x16, WT2 64x64, WT3 64x64, WT4 32x64 (rows >= 6 zero)
+constexpr int tW1 = 0, tW2 = 1024, tW3 = 5120, tW4 = 9216, kWT = 11264;
+// backward, natural [in][out_pad16]: WN4 64x16, WN3 64x64, WN2 64x64
+constexpr int nW4 = 0, nW3 = 1024, nW2 = 5120, kWN = 9216;
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
     uint32_t u = __float_as_uint(f);
@@ -48,17 +51,17 @@ __device__ __forceinline__ float gelu_d(float x) {
 }
 
 // acc += A[32 x K] B[K x 32]: A rows i from sA (row stride lda), B column j = jbase + i
-// from row (jbase + i) of sW (stride ldb, K contiguous).  K a multiple of 8.
+// from row (jbase + i) of sW (stride ldb, K contiguous).  K, lda, ldb multiples of 16 and the
+// LDS arrays 16-byte aligned, so each lane's eight operands are one ds_read_b128.
 __device__ __forceinline__ v16f mm_tile(const uint16_t *sA, int lda, const uint16_t *sW, int ldb, int K, int jbase,
                                         v16f acc) {
     const int l = threadIdx.x & 63;
-    const int i = l & 31, kq = (l >> 5) * 4;
-    for (int k0 = 0; k0 < K; k0 += 8) {
-        const uint16_t *pa = sA + i * lda + k0 + kq;
-        const uint16_t *pb = sW + (jbase + i) * ldb + k0 + kq;
-        v4i16 a = {(short)pa[0], (short)pa[1], (short)pa[2], (short)pa[3]};
-        v4i16 b = {(short)pb[0], (short)pb[1], (short)pb[2], (short)pb[3]};
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    const int i = l & 31, kq = (l >> 5) * 8;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const v8i16 a = *reinterpret_cast<const v8i16 *>(sA + i * lda + k0 + kq);
+        const v8i16 b = *reinterpret_cast<const v8i16 *>(sW + (jbase + i) * ldb + k0 + kq);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf16, a), __builtin_bit_cast(v8bf16, b),
+                                                      acc, 0, 0, 0);
     }
     return acc;
 }
@@ -83,9 +86,9 @@ struct FwdArgs {
 };
 
 __global__ __launch_bounds__(NTW) void k_mlp_forward(FwdArgs a) {
-    __shared__ uint16_t sW[kWT];
-    __shared__ uint16_t sH0[32 * 8];
-    __shared__ uint16_t sH[2][32 * 64];
+    __shared__ __align__(16) uint16_t sW[kWT];
+    __shared__ __align__(16) uint16_t sH0[32 * 16];
+    __shared__ __align__(16) uint16_t sH[2][32 * 64];
     __shared__ float sb[3 * 64 + 8];
     const int l = threadIdx.x;
     for (int k = l; k < kWT; k += NTW) sW[k] = a.wt[k];
@@ -100,18 +103,18 @@ __global__ __launch_bounds__(NTW) void k_mlp_forward(FwdArgs a) {
     const int64_t ntiles = (a.n + 31) / 32;
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int64_t s0 = t * 32;
-        // normalised input (K padded to 8)
-        for (int k = l; k < 32 * 8; k += NTW) {
-            const int i = k / 8, f = k % 8;
+        // normalised input (K padded to 16; Xn keeps 8 columns)
+        for (int k = l; k < 32 * 16; k += NTW) {
+            const int i = k / 16, f = k % 16;
             const int64_t s = s0 + i;
             float v = 0.0f;
             if (f < 5 && s < a.n) v = (a.X[5 * s + f] - a.norm[f]) / a.norm[5 + f];
             sH0[k] = f2bf(v);
-            if (a.Xn && s < a.n) a.Xn[8 * s + f] = v;
+            if (a.Xn && s < a.n && f < 8) a.Xn[8 * s + f] = v;
         }
         __syncthreads();
         const uint16_t *hin = sH0;
-        int ldin = 8, K = 8;
+        int ldin = 16, K = 16;
         const int toff[3] = {tW1, tW2, tW3};
         float *Zs[3] = {a.Z1, a.Z2, a.Z3};
         for (int layer = 0; layer < 3; ++layer) {
@@ -170,23 +173,23 @@ struct BwdArgs {
 };
 
 __global__ __launch_bounds__(NTW) void k_mlp_backward(BwdArgs a) {
-    __shared__ uint16_t sW[kWN];
-    __shared__ uint16_t sD0[32 * 8];
-    __shared__ uint16_t sD[2][32 * 64];
+    __shared__ __align__(16) uint16_t sW[kWN];
+    __shared__ __align__(16) uint16_t sD0[32 * 16];
+    __shared__ __align__(16) uint16_t sD[2][32 * 64];
     const int l = threadIdx.x;
     for (int k = l; k < kWN; k += NTW) sW[k] = a.wn[k];
     __syncthreads();
     const int64_t ntiles = (a.n + 31) / 32;
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int64_t s0 = t * 32;
-        for (int k = l; k < 32 * 8; k += NTW) {
-            const int i = k / 8, f = k % 8;
+        for (int k = l; k < 32 * 16; k += NTW) {
+            const int i = k / 16, f = k % 16;
             const int64_t s = s0 + i;
             sD0[k] = f2bf((f < 6 && s < a.n) ? a.dY[8 * s + f] : 0.0f);
         }
         __syncthreads();
         const uint16_t *din = sD0;
-        int ldin = 8, K = 8;
+        int ldin = 16, K = 16;
         const int woff[3] = {nW4, nW3, nW2};
         const float *Zs[3] = {a.Z3, a.Z2, a.Z1};
         float *dZs[3] = {a.dZ3, a.dZ2, a.dZ1};
@@ -237,13 +240,13 @@ __global__ __launch_bounds__(NTW) void k_mlp_wgrad(WgradArgs a) {
     const float *D = layer == 0 ? a.dZ1 : layer == 1 ? a.dZ2 : layer == 2 ? a.dZ3 : a.dY;
     const int ldd = layer == 3 ? 8 : 64;
     const int l = threadIdx.x;
-    const int i = l & 31, kq = (l >> 5) * 4;
+    const int i = l & 31, kq = (l >> 5) * 8;
     const int64_t s_begin = slice * a.slice, s_end = std::min<int64_t>(a.n, s_begin + a.slice);
     v16f acc = {0};
-    for (int64_t k0 = s_begin; k0 < s_end; k0 += 8) {
-        short av[4], bv[4];
+    for (int64_t k0 = s_begin; k0 < s_end; k0 += 16) {
+        v8i16 A, B;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < 8; ++q) {
             const int64_t s = k0 + kq + q;
             float h = 0.0f, d = 0.0f;
             if (s < s_end) {
@@ -253,12 +256,11 @@ __global__ __launch_bounds__(NTW) void k_mlp_wgrad(WgradArgs a) {
                 }
                 if (ob + i < nout) d = D[ldd * s + ob + i];
             }
-            av[q] = (short)f2bf(h);
-            bv[q] = (short)f2bf(d);
+            A[q] = (short)f2bf(h);
+            B[q] = (short)f2bf(d);
         }
-        v4i16 A = {av[0], av[1], av[2], av[3]};
-        v4i16 B = {bv[0], bv[1], bv[2], bv[3]};
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A, B, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf16, A), __builtin_bit_cast(v8bf16, B),
+                                                      acc, 0, 0, 0);
     }
     const int woff = layer == 0 ? mlp::oW1 : layer == 1 ? mlp::oW2 : layer == 2 ? mlp::oW3 : mlp::oW4;
     float *P = a.part + slice * mlp::kNParam;
@@ -310,8 +312,8 @@ __global__ void k_pack(const float *w, uint16_t *wt, uint16_t *wn) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < kWT) {
         float v = 0.0f;
-        if (k < tW2) {                       // WT1 [64][8]
-            const int j = k / 8, i = k % 8;
+        if (k < tW2) {                       // WT1 [64][16]
+            const int j = k / 16, i = k % 16;
             if (i < 5) v = w[mlp::oW1 + i * 64 + j];
         } else if (k < tW3) {                // WT2 [64][64]
             const int q = k - tW2, j = q / 64, i = q % 64;
@@ -327,8 +329,8 @@ __global__ void k_pack(const float *w, uint16_t *wt, uint16_t *wn) {
     }
     if (k < kWN) {
         float v = 0.0f;
-        if (k < nW3) {                       // WN4 [64][8]
-            const int i = k / 8, j = k % 8;
+        if (k < nW3) {                       // WN4 [64][16]
+            const int i = k / 16, j = k % 16;
             if (j < 6) v = w[mlp::oW4 + i * 6 + j];
         } else if (k < nW2) {                // WN3 [64][64]
             const int q = k - nW3;
