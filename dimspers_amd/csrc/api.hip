@@ -714,7 +714,7 @@ int stk_local_schur(stk_ctx *ctx, double *S, float *X) {
     return guarded(ctx, [&]() {
         Context &c = ctx->c;
         STK_CHECK(c.have_fields && c.have_fabric, STK_E_STATE, "fields and fabric needed");
-        STK_CHECK(c.mesh_split == 2, STK_E_NOMESH, "the learned model needs the 2-triangle annulus (Grid_split1)");
+        STK_CHECK(c.structured, STK_E_NOMESH, "the learned model needs a structured annulus (Grid_split1 or Grid)");
         DBuf<double> s(6 * c.nel);
         DBuf<float> x(5 * c.nel);
         SchurSampleHost h;

@@ -256,8 +256,19 @@ __device__ __forceinline__ double splitmix_u(uint64_t seed, uint64_t k) {
     return (double)(z >> 11) * 0x1.0p-53;
 }
 
+// Canonical vertex order of the learned blocks by triangle type (element id modulo the
+// triangles per cell).  Two per cell: right-angle vertex, its azimuthal neighbour, its radial
+// neighbour.  Four per cell ([corner k, corner k+1, centre]): the corners swapped for types 2
+// and 3, so the mirror images of types 0/2 (in r) and 1/3 (in theta) get the same block.
+__device__ __forceinline__ const int *canon_order(int split, int64_t e) {
+    static constexpr int c2[2][3] = {{0, 1, 2}, {1, 2, 0}};
+    static constexpr int c4[4][3] = {{0, 1, 2}, {0, 1, 2}, {1, 0, 2}, {1, 0, 2}};
+    return split == 4 ? c4[e & 3] : c2[e & 1];
+}
+
 struct SchurSampleArgs {
     int64_t nel, nsamp, per_snap, first_snap;
+    int split;                   // triangles per cell (2 or 4)
     int mode;                    // 0: the current fields of every element; 1: seeded training draws
     int ratio_kind;              // 0: delta = 10, 1: U[1, 100], 2: log-uniform [1, 1e3]
     double mean_area;
@@ -273,11 +284,13 @@ struct SchurSampleArgs {
 };
 
 // Features (and, for training, exact targets) of the learned local-Schur model:
-// mean log eta, cos / sin 2(phi - theta_c), log delta and the element area relative
-// to the mean area of its cell (the two triangles of a cell differ by the curvature
-// of the polar map, which is the largest variation of the normalised blocks).
-// Canonical vertex order: right-angle vertex, its azimuthal neighbour, its radial
-// neighbour -- (0, 1, 2) for the first triangle of a cell, (1, 2, 0) for the second.
+// mean log eta, cos / sin 2(phi - theta_c), log delta, and the element area relative to the
+// mean area of its cell's triangles (they differ by the curvature of the polar map, which is
+// the largest variation of the normalised blocks: inner, side and outer triangles of a cell
+// fall in separate bands).  Four triangles per cell: the director is measured from the
+// corner-corner edge (phi - theta_c, less pi/2 on the radial edges of types 1 and 3) with the
+// sign of its sine flipped for the mirrored types 2 and 3, so a triangle and its rotated or
+// mirrored counterpart see the same inputs.  Blocks in the canonical order of canon_order.
 __global__ __launch_bounds__(kAsmThreads) void k_schur_samples(SchurSampleArgs a) {
     __shared__ double sK[84][kAsmThreads];
     __shared__ double sM[6][kAsmThreads];
@@ -315,20 +328,31 @@ __global__ __launch_bounds__(kAsmThreads) void k_schur_samples(SchurSampleArgs a
     const double detJ = (th[1] - th[0]) * (rr[2] - rr[0]) - (th[2] - th[0]) * (rr[1] - rr[0]);
     const double area = detJ * (rr[0] + rr[1] + rr[2]) / 6.0;
     const double thc = (th[0] + th[1] + th[2]) / 3.0;
-    // area relative to the mean of the two triangles of its cell (elements e and e ^ 1)
-    double parea;
+    // area relative to the mean of its cell's triangles (elements split * q .. split * q + split - 1)
+    double shape;
     {
-        const int64_t pp = a.iperm[e ^ 1];
-        double pt[3], pr[3];
-        for (int k = 0; k < 3; ++k) { pt[k] = a.th[k * a.nel + pp]; pr[k] = a.r[k * a.nel + pp]; }
-        parea = ((pt[1] - pt[0]) * (pr[2] - pr[0]) - (pt[2] - pt[0]) * (pr[1] - pr[0])) * (pr[0] + pr[1] + pr[2]) / 6.0;
+        const int64_t e0 = e - e % a.split;
+        double sum = 0.0;
+        for (int k = 0; k < a.split; ++k) {
+            const int64_t pp = a.iperm[e0 + k];
+            double pt[3], pr[3];
+            for (int v = 0; v < 3; ++v) { pt[v] = a.th[v * a.nel + pp]; pr[v] = a.r[v * a.nel + pp]; }
+            sum += ((pt[1] - pt[0]) * (pr[2] - pr[0]) - (pt[2] - pt[0]) * (pr[1] - pr[0])) * (pr[0] + pr[1] + pr[2]) / 6.0;
+        }
+        shape = area / (sum / a.split);
     }
     float *X = a.X + 5 * sidx;
+    double psi = phi - thc, sgn = 1.0;
+    if (a.split == 4) {
+        const int t = (int)(e & 3);
+        if (t & 1) psi -= 0.5 * M_PI;
+        if (t >= 2) sgn = -1.0;
+    }
     X[0] = (float)logeta;
-    X[1] = (float)cos(2.0 * (phi - thc));
-    X[2] = (float)sin(2.0 * (phi - thc));
+    X[1] = (float)cos(2.0 * psi);
+    X[2] = (float)(sgn * sin(2.0 * psi));
     X[3] = (float)log(delta);
-    X[4] = (float)(area / (0.5 * (area + parea)));
+    X[4] = (float)shape;
     const double se = area / exp(logeta);
     if (a.scale) a.scale[sidx] = se;
     if (!a.Tg && !a.S) return;
@@ -341,8 +365,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_schur_samples(SchurSampleArgs a
     }
     double Sl[3][3];
     local_schur<double>(sK, tid, sG, xn, zn, Sl);
-    const int pm[2][3] = {{0, 1, 2}, {1, 2, 0}};
-    const int *pp = pm[e & 1];
+    const int *pp = canon_order(a.split, e);
     double Sc[3][3];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) Sc[i][j] = Sl[pp[i]][pp[j]] / se;
@@ -366,6 +389,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_schur_samples(SchurSampleArgs a
 // Predicted element blocks -> assembled S (pattern of M), colour by colour.
 struct LearnedScatterArgs {
     int64_t nel, off, cnt;
+    int split;                      // triangles per cell (2 or 4)
     const int32_t *perm, *e2n;      // e2n by position
     const uint8_t *sM;
     const int32_t *Mptr;
@@ -390,8 +414,7 @@ __global__ void k_learned_scatter(LearnedScatterArgs a) {
     Sc[2][0] = Sc[0][2] = l20 * l00;
     Sc[2][1] = Sc[1][2] = l20 * l10 + l21 * l11;
     Sc[2][2] = l20 * l20 + l21 * l21 + l22 * l22;
-    const int pm[2][3] = {{0, 1, 2}, {1, 2, 0}};
-    const int *pp = pm[e & 1];
+    const int *pp = canon_order(a.split, e);
     double Sl[3][3];
     for (int i = 0; i < 3; ++i)
         for (int k = 0; k < 3; ++k) Sl[pp[i]][pp[k]] = Sc[i][k] * se;
@@ -640,7 +663,7 @@ void launch_schur_samples(Context &c, const SchurSampleHost &h) {
     SchurSampleArgs a{};
     a.quad = make_quad();
     a.nel = c.nel; a.nsamp = h.nsamp; a.per_snap = h.per_snap; a.first_snap = h.first_snap;
-    a.mode = h.mode; a.ratio_kind = h.ratio_kind; a.mean_area = c.mean_area;
+    a.mode = h.mode; a.ratio_kind = h.ratio_kind; a.mean_area = c.mean_area; a.split = c.mesh_split;
     a.iperm = c.d_iperm.p; a.th = c.d_elth.p; a.r = c.d_elr.p; a.eta = c.d_eta.p;
     a.phi = c.d_phi.p; a.delta = c.d_delta.p;
     a.X = h.X; a.Tg = h.Tg; a.S = h.S; a.scale = h.scale;
@@ -651,7 +674,7 @@ void launch_schur_samples(Context &c, const SchurSampleHost &h) {
 
 void launch_learned_scatter(Context &c, const float *Y, const double *scale, double *S64) {
     LearnedScatterArgs a{};
-    a.nel = c.nel; a.perm = c.d_perm.p; a.e2n = c.d_e2n.p; a.sM = c.d_sM.p; a.Mptr = c.MN.dptr.p;
+    a.nel = c.nel; a.split = c.mesh_split; a.perm = c.d_perm.p; a.e2n = c.d_e2n.p; a.sM = c.d_sM.p; a.Mptr = c.MN.dptr.p;
     a.Y = Y; a.scale = scale; a.S64 = S64;
     for (size_t k = 0; k + 1 < c.col_off.size(); ++k) {
         a.off = c.col_off[k];

@@ -456,7 +456,7 @@ void MLP::adam(float lr, hipStream_t s) {
 
 // Training on seeded snapshots of exact local Schur blocks (k_schur_samples, mode 1).
 void mlp_train(Context &c, const stk_train_opts &o, stk_train_report &rep) {
-    STK_CHECK(c.mesh_split == 2, STK_E_NOMESH, "the learned model needs the 2-triangle annulus (Grid_split1)");
+    STK_CHECK(c.structured, STK_E_NOMESH, "the learned model needs a structured annulus (Grid_split1 or Grid)");
     STK_CHECK(c.have_fields, STK_E_STATE, "stk_set_fields must precede training");
     STK_CHECK(o.snapshots > 0 && o.per_snapshot > 0 && o.epochs > 0 && o.batch >= 32, STK_E_ARG, "bad training options");
     if (!c.mlp) c.mlp = new MLP();
@@ -538,8 +538,8 @@ void mlp_train(Context &c, const stk_train_opts &o, stk_train_report &rep) {
 // Inference on every element and assembly of the learned Schur operator S (pattern of M).
 void learned_setup(Context &c) {
     STK_CHECK(c.mlp && c.mlp->trained, STK_E_STATE, "no trained model (stk_mlp_train or stk_mlp_params)");
-    // the features and the canonical vertex order of the blocks follow the two triangle types of a cell
-    STK_CHECK(c.mesh_split == 2, STK_E_NOMESH, "the learned model needs the 2-triangle annulus (Grid_split1)");
+    // the features and the canonical vertex order of the blocks follow the triangle types of a cell
+    STK_CHECK(c.structured, STK_E_NOMESH, "the learned model needs a structured annulus (Grid_split1 or Grid)");
     STK_CHECK(c.have_fabric && c.have_fields, STK_E_STATE, "stk_set_fabric and stk_set_fields must precede");
     STK_CHECK(c.assembled && c.K64, STK_E_STATE, "assemble first");
     DBuf<float> X(5 * c.nel), Y(6 * c.nel);

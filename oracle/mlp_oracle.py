@@ -8,8 +8,10 @@ restates the build's own definitions so the GPU kernels can be checked:
   canonical vertex order, divided by area / exp(mean log eta); K_e and G_e come
   from the FP64 element oracle (stokes_oracle.element_matrices).
 * ``features``: (mean log eta, cos 2(phi - theta_c), sin 2(phi - theta_c),
-  log delta, area / mean area of the element's cell -- elements 2q and 2q+1
-  share cell q).
+  log delta, area / mean area of the element's cell -- elements split*q ..
+  split*q + split - 1 share cell q); on the four-triangle ``Grid`` the director is
+  measured from the corner-corner edge (less pi/2 on the radial edges of types 1 and
+  3) with the sine's sign flipped on the mirrored types 2 and 3.
 * ``forward`` / ``gradient``: the 5-64-64-64-6 GELU network with the BF16
   rounding points of the GPU kernels (``bf16=True``) or in exact FP64
   (``bf16=False``, pinned against central finite differences).
@@ -101,6 +103,9 @@ def gradient(p, norm, X, T, bf=True):
 
 
 CANON = ((0, 1, 2), (1, 2, 0))
+# four triangles per cell, [corner k, corner k+1, centre]: corners swapped for types 2 and 3 (the
+# mirror images of types 0 and 1 in r and in theta)
+CANON4 = ((0, 1, 2), (0, 1, 2), (1, 0, 2), (1, 0, 2))
 
 
 def local_schur(inp, phi, delta, mean_area=None):
@@ -132,14 +137,22 @@ def local_schur(inp, phi, delta, mean_area=None):
     logeta = np.log(inp.eta_ip[:, :6]).mean(1)
     se = area / np.exp(logeta)
     out = np.zeros((m.nel, 6))
-    for t in (0, 1):
-        idx = np.arange(t, m.nel, 2)
-        pp = CANON[t]
+    split = getattr(m, "split", 2)
+    canon = CANON4 if split == 4 else CANON
+    for t in range(split):
+        idx = np.arange(t, m.nel, split)
+        pp = canon[t]
         Sc = S[idx][:, pp][:, :, pp] / se[idx, None, None]
         out[idx] = np.stack([Sc[:, 0, 0], Sc[:, 1, 0], Sc[:, 1, 1], Sc[:, 2, 0], Sc[:, 2, 1], Sc[:, 2, 2]], 1)
     thc = th.mean(0)
-    cell = 0.5 * (area + area[np.arange(m.nel) ^ 1])
-    X = np.stack([logeta, np.cos(2 * (phi - thc)), np.sin(2 * (phi - thc)), np.log(delta), area / cell], 1)
+    psi, sgn = phi - thc, np.ones(m.nel)
+    if split == 4:
+        # director from the corner-corner edge (radial on types 1 and 3), mirrored types 2 and 3
+        t = np.arange(m.nel) % 4
+        psi = psi - 0.5 * np.pi * (t % 2)
+        sgn = np.where(t >= 2, -1.0, 1.0)
+    shape = area / area.reshape(-1, split).mean(1).repeat(split)
+    X = np.stack([logeta, np.cos(2 * psi), sgn * np.sin(2 * psi), np.log(delta), shape], 1)
     return out, X, S
 
 
