@@ -212,6 +212,22 @@ void build_gs_layout(Context &c) {
         g.blocks[colour[i]] += len((int32_t)i);
         g.blocks_low[colour[i]] += rlow[i];
     }
+    // the distinct neighbour nodes a colour's launch gathers x from (every one at least once per
+    // launch: x changes between launches), all of them and those of lower colours (from zero)
+    {
+        std::vector<uint64_t> adj(n, 0);
+        for (int64_t i = 0; i < n; ++i)
+            for (int32_t k = K.ptr[i]; k < K.ptr[i + 1]; ++k)
+                if (K.col[k] != i) adj[K.col[k]] |= 1ull << colour[i];
+        g.nbr.assign(nc, 0);
+        g.nbr_low.assign(nc, 0);
+        for (int64_t j = 0; j < n; ++j)
+            for (int k = 0; k < nc; ++k)
+                if (adj[j] >> k & 1ull) {
+                    g.nbr[k] += 1;
+                    if (colour[j] < k) g.nbr_low[k] += 1;
+                }
+    }
     // slice s of colour k holds rows row_off[k] + (s - slice_off[k]) * 64 + j, j < 64;
     // block b of its row j at sptr[s] + b * 64 + j
     for (int k = 0; k < nc; ++k) {

@@ -408,6 +408,18 @@ float *vcycle0_half(Context &c, MGView<float> &v, const float *b, int deg) {
 // *dot_nb receives their count.  (Two or three sweeps each way cut the inner iterations by
 // 4-12% at 256x2048 and 512x4096 and cost more than they saved.)
 enum GsMode { GS_FROM_ZERO, GS_BACKWARD };
+
+// Algorithmic bytes of one colour launch: the blocks it reads (four values and a column), the x
+// of every neighbour node of its rows (x changes between launches, so each launch gathers each
+// once), and per real row from zero: b read, b and x written in coloured order, D^-1, row id,
+// lower-block count; backward: b and the row's own x read, x written in both orders, D^-1, row
+// id (and w with the dots); slice pointers (and lengths).
+double gs_launch_bytes(const Context::GSSmoother &g, int k, bool zero, bool dots, int64_t nslices) {
+    const double nsl = (double)nslices;
+    if (zero)
+        return (double)g.blocks_low[k] * 20 + (double)g.nbr_low[k] * 8 + (double)g.rows[k] * 45 + nsl * 12;
+    return (double)g.blocks[k] * 20 + (double)g.nbr[k] * 8 + (double)g.rows[k] * (dots ? 60 : 52) + (nsl + 1) * 8;
+}
 void gs_sweep(Context &c, GsMode mode, const float *b, float *z, bool dots, int64_t *dot_nb) {
     Context::GSSmoother &g = c.gs;
     constexpr int NTG = 256;
@@ -425,14 +437,7 @@ void gs_sweep(Context &c, GsMode mode, const float *b, float *z, bool dots, int6
         else k_gs_colour<NTG, false, 0><<<grid, block, 0, c.stream>>>(STK_GS_ARGS);
 #undef STK_GS_ARGS
         STK_HIP(hipGetLastError());
-        if (c.ledger.on) {
-            // blocks (four values and a column); per real row from zero: b read, b, x written in
-            // coloured order, D^-1, row id, lower-block count; backward: b, x read, x written in
-            // both orders, D^-1, row id (and w with the dots); slice pointers (and lengths)
-            const double nsl = (double)(s1 - s0);
-            c.ledger.add(zero ? (double)g.blocks_low[k] * 20 + (double)g.rows[k] * 45 + nsl * 12
-                              : (double)g.blocks[k] * 20 + (double)g.rows[k] * (part ? 60 : 52) + (nsl + 1) * 8);
-        }
+        if (c.ledger.on) c.ledger.add(gs_launch_bytes(g, k, zero, part != nullptr, s1 - s0));
     };
     if (mode == GS_FROM_ZERO) {
         for (int k = 0; k < g.ncolour; ++k) launch(k, true, nullptr, 0);
@@ -2116,12 +2121,12 @@ double time_kernel(Context &c, int kernel, int reps, double *bytes) {
                 int64_t nb = 0;
                 gs_sweep(c, GS_BACKWARD, nullptr, z, true, &nb);
             };
-            // structural blocks (four values and a column each), per real row: b, x read,
-            // x written twice (coloured and node order), D^-1 (4 values), w, row id; slice
-            // pointers; all over the colour launches
-            const int64_t nrow = c.nnod, nsl = c.gs.slice_off.back();
-            *bytes = ((double)c.KN.nnz() * 20 + (double)nrow * (8 + 8 + 16 + 16 + 8 + 4) + (double)(nsl + 1) * 8) /
-                     c.gs.ncolour;
+            // per launch: its blocks, the neighbour x it gathers, its rows' data (gs_launch_bytes),
+            // averaged over the colour launches of a sweep
+            double sum = 0.0;
+            for (int k = 0; k < c.gs.ncolour; ++k)
+                sum += gs_launch_bytes(c.gs, k, false, true, c.gs.slice_off[k + 1] - c.gs.slice_off[k]);
+            *bytes = sum / c.gs.ncolour;
             break;
         }
         case STK_KERNEL_INNER_IT32: {

@@ -263,8 +263,8 @@ struct Context {
         DBuf<float> pq_val;
         bool filled = false;                   // values and dinv of the current FP32 K
         // per colour: rows, structural blocks, and blocks in lower colours (what a sweep from
-        // zero reads)
-        std::vector<int64_t> rows, blocks, blocks_low;
+        // zero reads); distinct neighbour nodes of its rows (of lower colours)
+        std::vector<int64_t> rows, blocks, blocks_low, nbr, nbr_low;
     } gs;
     int fine_smoother = 0;                     // STK_SMOOTH_* of the last stk_solve
     // Incomplete Cholesky M ~ L D L^T of the pressure mass matrix (STK_PPC_IC), IC(0) on M's
