@@ -322,6 +322,7 @@ int stk_assemble(stk_ctx *ctx, int32_t flags) {
         // stale products of an earlier assembly
         c.K32.release(); c.G32.release(); c.GT32.release(); c.M32.release(); c.F32.release(); c.pc32.release();
         c.RK32.release();
+        for (MGLevel &L : c.mg) { L.RA64.release(); L.RA32.release(); }   // rebuilt by the solver set-up
         c.S64.release(); c.S32.release(); c.S32s.release(); c.Sdinv64.release(); c.Sdinv32.release();
         c.pic.valid = false;
         if (fp64) {
@@ -396,14 +397,17 @@ int stk_export_csc(stk_ctx *ctx, int32_t which, int32_t lower_only, int64_t *nro
                 // multigrid levels and prolongations (set-up checks of the hierarchy)
                 const bool level = which >= STK_MAT_LEVEL && which < STK_MAT_PROLONG;
                 const bool prol = which >= STK_MAT_PROLONG && which < STK_MAT_PROLONG + 32;
-                if (!level && !prol) throw Error(STK_E_ARG, "unknown operator");
+                const bool ra = which >= STK_MAT_LEVEL_RA && which < STK_MAT_LEVEL_RA + 32;
+                if (!level && !prol && !ra) throw Error(STK_E_ARG, "unknown operator");
                 STK_CHECK(c.mg_built, STK_E_STATE, "no multigrid hierarchy");
-                const int64_t l = which - (level ? STK_MAT_LEVEL : STK_MAT_PROLONG);
-                STK_CHECK(l < (int64_t)c.mg.size(), STK_E_ARG, "no such multigrid level");
+                const int64_t l = which - (level ? STK_MAT_LEVEL : prol ? STK_MAT_PROLONG : STK_MAT_LEVEL_RA);
+                STK_CHECK(l < (int64_t)c.mg.size() && (!ra || l >= 1), STK_E_ARG, "no such multigrid level");
                 MGLevel &L = c.mg[l];
                 br = bc = 2;
                 if (level) {
                     p = &L.pat; nr = nc = 2 * L.nvert; v64 = L.A64.p; v32 = L.A32.p;
+                } else if (ra) {
+                    p = &L.RA_pat; nr = 2 * L.nvert; nc = 2 * c.mg[l - 1].nvert; v64 = L.RA64.p; v32 = L.RA32.p;
                 } else {
                     p = &L.P_pat; nr = l == 0 ? c.nU : 2 * c.mg[l - 1].nvert; nc = 2 * L.nvert;
                     v64 = L.P64.p; v32 = L.P32.p;

@@ -347,6 +347,103 @@ __global__ __launch_bounds__(NT) void k_spmv_sell22(int64_t nrows, const int64_t
     }
 }
 
+// y = A1 x1 - A2 x2 for two 2x2-block operators with the same rows (the coarse right-hand side
+// R b - (R A) x of a multigrid level in one pass), both on the sliced layout: one lane per row
+// node, the wavefront's slice a scalar as in k_spmv_sell22.
+template <int NT, typename T>
+__global__ __launch_bounds__(NT) void k_spmv_dual_sell22(int64_t nrows, const int64_t *__restrict__ sptr1,
+                                                          const int32_t *__restrict__ col1, const T *__restrict__ v1,
+                                                          const T *__restrict__ x1, const int64_t *__restrict__ sptr2,
+                                                          const int32_t *__restrict__ col2, const T *__restrict__ v2,
+                                                          const T *__restrict__ x2, T *__restrict__ y) {
+    typedef typename Vec4<T>::type V4;
+    typedef typename Vec2<T>::type V2;
+    constexpr int C = kWave;
+    const int64_t slice = wave_slice<NT>();
+    if (slice * C >= nrows) return;
+    const int lane = threadIdx.x % C;
+    const int64_t row = slice * C + lane;
+    T a0 = T(0), a1 = T(0);
+    {
+        const int64_t base = sptr1[slice];
+        const int len = (int)((sptr1[slice + 1] - base) / C);
+        const V4 *__restrict__ vp = reinterpret_cast<const V4 *>(v1) + base + lane;
+        const int32_t *__restrict__ cp = col1 + base + lane;
+#pragma unroll 4
+        for (int k = 0; k < len; ++k) {
+            const V4 v = __builtin_nontemporal_load(&vp[(int64_t)k * C]);
+            const V2 xv = reinterpret_cast<const V2 *>(x1)[__builtin_nontemporal_load(&cp[(int64_t)k * C])];
+            a0 += v.x * xv.x + v.y * xv.y;
+            a1 += v.z * xv.x + v.w * xv.y;
+        }
+    }
+    {
+        const int64_t base = sptr2[slice];
+        const int len = (int)((sptr2[slice + 1] - base) / C);
+        const V4 *__restrict__ vp = reinterpret_cast<const V4 *>(v2) + base + lane;
+        const int32_t *__restrict__ cp = col2 + base + lane;
+#pragma unroll 4
+        for (int k = 0; k < len; ++k) {
+            const V4 v = __builtin_nontemporal_load(&vp[(int64_t)k * C]);
+            const V2 xv = reinterpret_cast<const V2 *>(x2)[__builtin_nontemporal_load(&cp[(int64_t)k * C])];
+            a0 -= v.x * xv.x + v.y * xv.y;
+            a1 -= v.z * xv.x + v.w * xv.y;
+        }
+    }
+    if (row < nrows) reinterpret_cast<V2 *>(y)[row] = V2{a0, a1};
+}
+
+// The same on node-block CSR (the small levels): L lanes per row node, shuffle reduction.
+template <int L, int NT, typename T>
+__global__ __launch_bounds__(NT) void k_spmv_dual_block22(int64_t nrows, const int32_t *__restrict__ ptr1,
+                                                           const int32_t *__restrict__ col1, const T *__restrict__ v1,
+                                                           const T *__restrict__ x1, const int32_t *__restrict__ ptr2,
+                                                           const int32_t *__restrict__ col2, const T *__restrict__ v2,
+                                                           const T *__restrict__ x2, T *__restrict__ y) {
+    typedef typename Vec4<T>::type V4;
+    typedef typename Vec2<T>::type V2;
+    const int64_t row = (int64_t)blockIdx.x * (NT / L) + threadIdx.x / L;
+    const int lane = threadIdx.x % L;
+    if (row >= nrows) return;   // whole L-lane groups leave together (NT / L rows per block)
+    T a0 = T(0), a1 = T(0);
+    for (int32_t k = ptr1[row] + lane; k < ptr1[row + 1]; k += L) {
+        const V4 v = reinterpret_cast<const V4 *>(v1)[k];
+        const V2 xv = reinterpret_cast<const V2 *>(x1)[col1[k]];
+        a0 += v.x * xv.x + v.y * xv.y;
+        a1 += v.z * xv.x + v.w * xv.y;
+    }
+    for (int32_t k = ptr2[row] + lane; k < ptr2[row + 1]; k += L) {
+        const V4 v = reinterpret_cast<const V4 *>(v2)[k];
+        const V2 xv = reinterpret_cast<const V2 *>(x2)[col2[k]];
+        a0 -= v.x * xv.x + v.y * xv.y;
+        a1 -= v.z * xv.x + v.w * xv.y;
+    }
+    a0 = group_sum<L>(a0);
+    a1 = group_sum<L>(a1);
+    if (lane == 0) reinterpret_cast<V2 *>(y)[row] = V2{a0, a1};
+}
+
+// out[q] = sum over the pairs (a, b) listed for stored block q of A[a] B[b] (2x2 block products,
+// in list order): a sparse product C = A B on a pattern and pair lists built once per mesh.
+template <typename T>
+__global__ __launch_bounds__(256) void k_pair_product(int64_t npos, const int32_t *__restrict__ ptr,
+                                                      const int32_t *__restrict__ pair, const T *__restrict__ A,
+                                                      const T *__restrict__ B, T *__restrict__ out) {
+    typedef typename Vec4<T>::type V4;
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= npos) return;
+    T c0 = T(0), c1 = T(0), c2 = T(0), c3 = T(0);
+    for (int32_t k = ptr[q]; k < ptr[q + 1]; ++k) {
+        const V4 a = reinterpret_cast<const V4 *>(A)[pair[2 * (int64_t)k]];
+        const V4 b = reinterpret_cast<const V4 *>(B)[pair[2 * (int64_t)k + 1]];
+        c0 += a.x * b.x + a.y * b.z;
+        c1 += a.x * b.y + a.y * b.w;
+        c2 += a.z * b.x + a.w * b.z;
+        c3 += a.z * b.y + a.w * b.w;
+    }
+    reinterpret_cast<V4 *>(out)[q] = V4{c0, c1, c2, c3};
+}
+
 // 2x1 / 1x2 block SpMV on the sliced layout (G: velocity node rows, pressure columns;
 // G^T: pressure rows, velocity node columns): one lane per row, two values per block.
 template <int BR, int BC, int NT, typename T, typename Ep>

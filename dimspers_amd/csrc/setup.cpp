@@ -982,7 +982,65 @@ void build_mg_structure(Context &c) {
         }
         build_transfer(C, F.nvert, t, F.h_fixed, C.h_fixed, c.stream);
     }
+    build_level_products(c);
     STK_HIP(hipStreamSynchronize(c.stream));
+}
+
+// Pattern of R A for every level from 1 on (R the restriction into it, A the operator of the
+// next finer P1 level; level 0 has R K from the element assembly) and, per stored block of it,
+// the pairs of stored positions (R block, A block) whose products sum there, in R's row order
+// and then A's: the numeric product (k_pair_product) is a fixed-order sum, so it is
+// deterministic.  The sliced layout follows R's.
+void build_level_products(Context &c) {
+    for (size_t k = 1; k < c.mg.size(); ++k) {
+        MGLevel &C = c.mg[k], &F = c.mg[k - 1];
+        const Pattern &R = C.R_pat, &A = F.pat;
+        if (R.ncols != A.nrows) throw Error(STK_E_STATE, "restriction and level operator do not match");
+        Pattern &Q = C.RA_pat;
+        Q = Pattern{};
+        Q.nrows = R.nrows;
+        Q.ncols = A.ncols;
+        Q.ptr.assign(Q.nrows + 1, 0);
+        std::vector<int32_t> cols;
+        for (int64_t u = 0; u < R.nrows; ++u) {
+            cols.clear();
+            for (int32_t i = R.ptr[u]; i < R.ptr[u + 1]; ++i) {
+                const int32_t g = R.col[i];
+                cols.insert(cols.end(), A.col.begin() + A.ptr[g], A.col.begin() + A.ptr[g + 1]);
+            }
+            std::sort(cols.begin(), cols.end());
+            cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+            Q.col.insert(Q.col.end(), cols.begin(), cols.end());
+            Q.ptr[u + 1] = (int32_t)Q.col.size();
+        }
+        Q.dptr.upload(Q.ptr, c.stream);
+        Q.dcol.upload(Q.col, c.stream);
+        if (R.sell) make_sliced(Q, c.stream);
+        // pair lists, counted then filled, over the stored positions (padding keeps an empty list)
+        const int64_t npos = Q.nstore();
+        std::vector<int32_t> ptr(npos + 1, 0);
+        auto each_pair = [&](auto &&fn) {
+            for (int64_t u = 0; u < R.nrows; ++u)
+                for (int32_t i = R.ptr[u]; i < R.ptr[u + 1]; ++i) {
+                    const int32_t g = R.col[i];
+                    const int64_t rpos = R.pos(u, i - R.ptr[u]);
+                    for (int32_t j = A.ptr[g]; j < A.ptr[g + 1]; ++j)
+                        fn(Q.pos(u, find_offset(Q, (int32_t)u, A.col[j])), rpos, A.pos(g, j - A.ptr[g]));
+                }
+        };
+        each_pair([&](int64_t q, int64_t, int64_t) { ptr[q + 1]++; });
+        for (int64_t q = 0; q < npos; ++q) ptr[q + 1] += ptr[q];
+        std::vector<int32_t> pair(2 * (size_t)ptr[npos]);
+        std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+        each_pair([&](int64_t q, int64_t rpos, int64_t apos) {
+            const int64_t at = fill[q]++;
+            pair[2 * at] = (int32_t)rpos;
+            pair[2 * at + 1] = (int32_t)apos;
+        });
+        C.ra_ptr.upload(ptr, c.stream);
+        C.ra_pair.upload(pair, c.stream);
+        STK_HIP(hipStreamSynchronize(c.stream));   // the host vectors go out of scope
+    }
 }
 
 }  // namespace stk

@@ -212,6 +212,42 @@ int64_t apply22_auto(Context &c, const Op22<T> &A, const T *x, Ep ep, double *pa
     return apply22<2>(c, A, x, ep, partial);
 }
 
+// y = A1 x1 - A2 x2 for two operators with the same rows in one launch (both sliced, or both
+// node-block CSR with lanes per row as apply22_auto picks them); other layout pairs (small
+// meshes only) take two launches.
+template <typename T>
+void apply22_restrict(Context &c, const Op22<T> &A1, const T *x1, const Op22<T> &A2, const T *x2, T *y) {
+    const Pattern &p1 = *A1.pat, &p2 = *A2.pat;
+    if (p1.nrows != p2.nrows) throw Error(STK_E_STATE, "restriction: operators with different rows");
+    if (p1.sell != p2.sell) {
+        EpStore<T> es{y};
+        apply22_auto<T>(c, A1, x1, es);
+        EpAxpyOut<T> ea{y, T(-1), T(1)};
+        apply22_auto<T>(c, A2, x2, ea);
+        return;
+    }
+    if (p1.sell) {
+        const int64_t nslice = (p1.nrows + Pattern::kSlice - 1) / Pattern::kSlice;
+        const int64_t nb = (nslice + NT / Pattern::kSlice - 1) / (NT / Pattern::kSlice);
+        k_spmv_dual_sell22<NT, T><<<dim3((unsigned)nb), dim3(NT), 0, c.stream>>>(
+            p1.nrows, p1.dsptr.p, p1.dscol.p, A1.val, x1, p2.dsptr.p, p2.dscol.p, A2.val, x2, y);
+    } else {
+        const double avg = (double)(p1.nnz() + p2.nnz()) / std::max<int64_t>(1, p1.nrows);
+        auto launch = [&](auto lanes) {
+            constexpr int L = decltype(lanes)::value;
+            const int64_t nb = (p1.nrows + NT / L - 1) / (NT / L);
+            k_spmv_dual_block22<L, NT, T><<<dim3((unsigned)nb), dim3(NT), 0, c.stream>>>(
+                p1.nrows, p1.dptr.p, p1.dcol.p, A1.val, x1, p2.dptr.p, p2.dcol.p, A2.val, x2, y);
+        };
+        if (avg > 10) launch(std::integral_constant<int, 8>{});
+        else launch(std::integral_constant<int, 4>{});
+    }
+    STK_HIP(hipGetLastError());
+    if (c.ledger.on)
+        c.ledger.add(spmv_ledger_bytes(p1, 2, 2, sizeof(T), sizeof(T), 1, 0) +
+                     spmv_ledger_bytes(p2, 2, 2, sizeof(T), sizeof(T), 1, 1));
+}
+
 // ------------------------------------------------------------------ multigrid
 template <typename T>
 struct MGView {
@@ -222,7 +258,7 @@ struct MGView {
         int64_t n;            // dofs
         T *b, *x, *xa, *r, *d;
         Op22<T> P, R;         // transfer to / from the next coarser level (P: this <- coarser)
-        Op22<T> RK;           // R A (level 0 only; pattern null when not built)
+        Op22<T> RK;           // R A: R K on level 0, R A of the P1 levels (pattern null when not built)
     };
     std::vector<Lv> lv;
     const T *coarse_inv = nullptr;
@@ -263,8 +299,11 @@ MGView<T> make_view(Context &c) {
         MGLevelBufs<T> &w = level_bufs<T>(L);
         l.b = w.b.p; l.x = w.x.p; l.xa = w.xa.p; l.r = w.r.p; l.d = w.d.p;
         if (i + 1 < c.mg.size()) {
-            l.P = {&c.mg[i + 1].P_pat, f64 ? (const T *)c.mg[i + 1].P64.p : (const T *)c.mg[i + 1].P32.p};
-            l.R = {&c.mg[i + 1].R_pat, f64 ? (const T *)c.mg[i + 1].R64.p : (const T *)c.mg[i + 1].R32.p};
+            MGLevel &C = c.mg[i + 1];
+            l.P = {&C.P_pat, f64 ? (const T *)C.P64.p : (const T *)C.P32.p};
+            l.R = {&C.R_pat, f64 ? (const T *)C.R64.p : (const T *)C.R32.p};
+            const T *ra = (const T *)pick<T>(C.RA64, C.RA32).p;
+            if (ra) l.RK = {&C.RA_pat, ra};
         }
         v.lv.push_back(l);
     }
@@ -465,10 +504,7 @@ float *vcycle0_gs(Context &c, MGView<float> &v, const float *b, int deg, const f
     float *z = L.x;
     gs_sweep(c, GS_FROM_ZERO, b, nullptr, false, nullptr);
     auto &C = v.lv[1];
-    EpStore<float> es{C.b};
-    apply22_auto<float>(c, L.R, b, es);
-    EpAxpyOut<float> er{C.b, -1.0f, 1.0f};
-    apply22_auto<float>(c, Op22<float>{&g.rkq, L.RK.val}, g.xq.p, er);
+    apply22_restrict<float>(c, L.R, b, Op22<float>{&g.rkq, L.RK.val}, g.xq.p, C.b);
     float *xc = vcycle<float>(c, v, 1, C.b, deg, nullptr, nullptr);
     EpAxpyOut<float> ea{g.xq.p, 1.0f, 1.0f};
     apply22_auto<float>(c, Op22<float>{&g.pq, g.pq_val.p}, xc, ea);
@@ -500,11 +536,8 @@ T *vcycle(Context &c, MGView<T> &v, size_t l, const T *b, int deg, const T *dot_
     chebyshev<T>(c, L, b, L.x, L.xa, L.d, deg, true, &x);
     auto &C = v.lv[l + 1];
     if (L.RK.pat) {
-        // b_c = R b - (R A) x: R A has the coarse level's rows, a quarter of A's
-        EpStore<T> es{C.b};
-        apply22_auto<T>(c, L.R, b, es);
-        EpAxpyOut<T> ea{C.b, T(-1), T(1)};
-        apply22_auto<T>(c, L.RK, x, ea);
+        // b_c = R b - (R A) x in one pass: R A has the coarse level's rows, a quarter of A's
+        apply22_restrict<T>(c, L.R, b, L.RK, x, C.b);
     } else {
         // r = b - A x; b_c = R r
         EpResidual<T> er{L.r, b};
@@ -1000,6 +1033,18 @@ void setup_mg(Context &c, PhaseTimer &pt) {
             DBuf<T> &di = pick<T>(L.dinv64, L.dinv32);
             di.alloc(2 * L.nvert);
             diag_inverse<T>(c, L.pat, (const T *)pick<T>(L.A64, L.A32).p, di.p);
+        }
+        // R A of the levels from 1 on, from this precision's restrictions and level operators
+        for (size_t k = 1; k < c.mg.size(); ++k) {
+            MGLevel &C = c.mg[k], &F = c.mg[k - 1];
+            const int64_t npos = C.RA_pat.nstore();
+            if (npos == 0 || !C.ra_ptr) continue;
+            DBuf<T> &ra = pick<T>(C.RA64, C.RA32);
+            ra.alloc(4 * npos);
+            k_pair_product<T><<<dim3((unsigned)((npos + 255) / 256)), dim3(256), 0, c.stream>>>(
+                npos, C.ra_ptr.p, C.ra_pair.p, (const T *)pick<T>(C.R64, C.R32).p,
+                (const T *)pick<T>(F.A64, F.A32).p, ra.p);
+            STK_HIP(hipGetLastError());
         }
         // eigenvalue bounds (level 0 = K, then the P1 levels except the coarsest)
         Op22<T> K0{&c.KN, K};

@@ -148,6 +148,13 @@ struct MGLevel {
     DBuf<double> P64;  DBuf<float> P32;
     Pattern R_pat;                   // coarse vertex -> fine node (transpose)
     DBuf<double> R64;  DBuf<float> R32;
+    // R A of the next finer P1 level (levels 1 and coarser): coarse vertex rows, fine vertex
+    // columns; the V-cycle forms this level's right-hand side R (b - A x) as R b - (R A) x in
+    // one launch.  Stored block q sums the (R block, A block) position pairs
+    // ra_pair[2 k], ra_pair[2 k + 1] for k in [ra_ptr[q], ra_ptr[q + 1]).
+    Pattern RA_pat;
+    DBuf<int32_t> ra_ptr, ra_pair;
+    DBuf<double> RA64; DBuf<float> RA32;
     DBuf<int64_t> fixlist;           // fixed dofs of this level
     int64_t nfix = 0;
     MGLevelBufs<double> w64;
@@ -372,6 +379,8 @@ void import_operators(const Context &c, bool K_lower, CscIn K, bool M_lower, Csc
                       std::vector<double> &Kv, std::vector<double> &Gv, std::vector<double> &GTv,
                       std::vector<double> &Mv, std::vector<double> &Fv);
 void build_mg_structure(Context &c);
+// R A pattern and pair lists of the levels from 1 on (called by build_mg_structure).
+void build_level_products(Context &c);
 
 // ---------------------------------------------------------------- kernels (assembly.hip)
 struct SchurSampleHost {

@@ -66,9 +66,13 @@ enum {
     STK_MAT_LEVEL = 16,   /* + l: the operator of multigrid level l  */
                           /* (l = 0: P1 on the mesh's vertices), */
                           /* 2 nvert_l x 2 nvert_l                   */
-    STK_MAT_PROLONG = 48  /* + l: the prolongation into the next */
+    STK_MAT_PROLONG = 48, /* + l: the prolongation into the next */
                           /* finer space from level l (l = 0: into */
                           /* the P2 velocity, = R^T), n_finer x 2 nvert_l */
+    STK_MAT_LEVEL_RA = 80 /* + l (l >= 1): R A, the restriction into */
+                          /* level l times the operator of level l-1, */
+                          /* the coarse residual operator of the V-cycle */
+                          /* (after a solver set-up), 2 nvert_l x 2 nvert_l-1 */
 };
 
 /* Fine-level smoother of the FP32 multigrid V-cycle (stk_opts.fine_smoother). */

@@ -123,6 +123,33 @@ int main(int argc, char **argv) {
             REQUIRE((int64_t)L.h_fixed.size() == 2 * L.nvert && L.pat.nrows == L.nvert);
             for (int64_t d = 0; d < 2 * L.nvert; ++d) REQUIRE(!L.h_fixed[d] || d % 2 == 1);
             if (l > 0) REQUIRE(L.P_pat.nrows == c.mg[l - 1].nvert && L.P_pat.ncols == L.nvert);
+            if (l > 0) {
+                // R A: one pair per (R block, A block) product, every pair inside both value arrays,
+                // and a stored block's pairs all end in its own column
+                const Pattern &Q = L.RA_pat, &R = L.R_pat, &A = c.mg[l - 1].pat;
+                REQUIRE(Q.nrows == L.nvert && Q.ncols == c.mg[l - 1].nvert && Q.sell == R.sell);
+                std::vector<int32_t> ptr(Q.nstore() + 1), pair;
+                std::memcpy(ptr.data(), L.ra_ptr.p, ptr.size() * sizeof(int32_t));
+                pair.resize(2 * (size_t)ptr.back());
+                std::memcpy(pair.data(), L.ra_pair.p, pair.size() * sizeof(int32_t));
+                int64_t want = 0;
+                for (int64_t u = 0; u < R.nrows; ++u)
+                    for (int32_t i = R.ptr[u]; i < R.ptr[u + 1]; ++i) want += A.ptr[R.col[i] + 1] - A.ptr[R.col[i]];
+                REQUIRE(ptr.back() == want);
+                std::vector<int32_t> acol(A.nstore());
+                for (int64_t g = 0; g < A.nrows; ++g)
+                    for (int32_t j = A.ptr[g]; j < A.ptr[g + 1]; ++j) acol[A.pos(g, j - A.ptr[g])] = A.col[j];
+                for (int64_t u = 0; u < Q.nrows; ++u)
+                    for (int32_t k = Q.ptr[u]; k < Q.ptr[u + 1]; ++k) {
+                        const int64_t q = Q.pos(u, k - Q.ptr[u]);
+                        REQUIRE(ptr[q] <= ptr[q + 1]);
+                        for (int32_t t = ptr[q]; t < ptr[q + 1]; ++t) {
+                            REQUIRE(pair[2 * t] >= 0 && pair[2 * t] < R.nstore());
+                            REQUIRE(pair[2 * t + 1] >= 0 && pair[2 * t + 1] < A.nstore());
+                            REQUIRE(acol[pair[2 * t + 1]] == Q.col[k]);
+                        }
+                    }
+            }
         }
     }
     // gather lists: every element block lands in exactly one stored position

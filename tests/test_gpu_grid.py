@@ -73,7 +73,21 @@ def test_multigrid_levels_are_galerkin_products(kind):
         assert err < 1e-12
         # symmetric, and every fixed dof is a radial dof (odd index)
         assert _rel(A, A.T) < 1e-13 and np.all(fixed % 2 == 1)
+        if l >= 1:
+            # the V-cycle's coarse residual operator R A = P_l^T A_{l-1} (pair-list product)
+            RA = ctx.export_csc(_lib.MAT_LEVEL_RA + l)
+            err = _rel(RA, (P.T @ fine).tocsc())
+            print(f"[result] {kind} level {l}: R A rel err {err:.2e}")
+            assert err < 1e-13
         fine = A
+    ctx.close()
+    # FP32 set-up: R A from the FP32 restriction and level operators
+    ctx = _ctx(inp, _lib.ASM_FP64 | _lib.ASM_FP32)
+    for l in range(1, ctx.mg_levels):
+        want = (ctx.export_csc(_lib.MAT_PROLONG + l).T @ ctx.export_csc(_lib.MAT_LEVEL + l - 1)).tocsc()
+        err = _rel(ctx.export_csc(_lib.MAT_LEVEL_RA + l), want)
+        print(f"[result] {kind} level {l}: FP32 R A rel err {err:.2e}")
+        assert err < 1e-6
     ctx.close()
 
 
