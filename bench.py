@@ -122,14 +122,16 @@ def main():
     log(f"element assembly kernels {asm_ms:.2f} ms ({asm_bytes / asm_ms / 1e6:.0f} GB/s compulsory)")
 
     if args.rates:
-        res = {"element_assembly_all_colours": {"avg_us": round(asm_ms * 1e3, 1), "bytes": asm_bytes,
-                                                "GB_per_s": round(asm_bytes / asm_ms / 1e6, 1)}}
+        res = {"element_assembly_fp64": {"avg_us": round(asm_ms * 1e3, 1), "bytes": asm_bytes,
+                                         "GB_per_s": round(asm_bytes / asm_ms / 1e6, 1),
+                                         "note": "k_element and the gathers, compulsory bytes"}}
         for name, kid, reps in (("spmv_k32", _lib.KERNEL_SPMV_K32, 50), ("spmv_k64", _lib.KERNEL_SPMV_K64, 50),
                                 ("cheb_k32", _lib.KERNEL_CHEB_K32, 50), ("cheb_k64", _lib.KERNEL_CHEB_K64, 50),
                                 ("cg_product_kp32", _lib.KERNEL_KP32, 50),
                                 ("gs_backward_sweep32_per_launch", _lib.KERNEL_GS_BWD32, 50),
                                 ("g32", _lib.KERNEL_G32, 50), ("gt32", _lib.KERNEL_GT32, 50),
                                 ("coarsest_dense32", _lib.KERNEL_COARSE32, 50),
+                                ("restriction32_level0", _lib.KERNEL_RESTRICT32, 50),
                                 ("cg_update_and_norm32", _lib.KERNEL_CGUPD32, 50), ("copy_256MiB", _lib.KERNEL_COPY, 20)):
             kms, kb = ctx.time_kernel(kid, reps=reps)
             res[name] = {"avg_us": round(kms * 1e3, 2), "bytes": kb, "GB_per_s": round(kb / kms / 1e6, 1),

@@ -39,7 +39,7 @@ N_PARAM = 9094
 KERNEL_SPMV_K32, KERNEL_SPMV_K64, KERNEL_CHEB_K32, KERNEL_CHEB_K64, KERNEL_ASSEMBLY, KERNEL_CHEB_KH16 = 0, 1, 2, 3, 4, 5
 KERNEL_COPY, KERNEL_VCYCLE32, KERNEL_INNER_IT32, KERNEL_STRESS, KERNEL_DEFGRAD, KERNEL_KP32 = 6, 7, 8, 9, 10, 11
 KERNEL_GS_BWD32, KERNEL_FSE, KERNEL_ANISO = 12, 13, 14
-KERNEL_G32, KERNEL_GT32, KERNEL_COARSE32, KERNEL_CGUPD32 = 15, 16, 17, 18
+KERNEL_G32, KERNEL_GT32, KERNEL_COARSE32, KERNEL_CGUPD32, KERNEL_RESTRICT32 = 15, 16, 17, 18, 19
 STRESS_FIELDS = ("exx", "ezz", "exz", "eII", "txx", "tzz", "txz", "tII")
 FSE_NO_HEALING, FSE_FABRIC_DESTRUCTION, FSE_ANNEALING_DESTRUCTION = 0, 1, 2
 INSTALL_ASSEMBLY, INSTALL_STRESS = 1, 2

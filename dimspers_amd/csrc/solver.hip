@@ -248,6 +248,9 @@ void apply22_restrict(Context &c, const Op22<T> &A1, const T *x1, const Op22<T> 
                      spmv_ledger_bytes(p2, 2, 2, sizeof(T), sizeof(T), 1, 1));
 }
 
+template <typename T>
+void power_lmax(Context &c, const Op22<T> &A, const T *dinv, int64_t n, T *v, T *w, double *out);
+
 // ------------------------------------------------------------------ multigrid
 template <typename T>
 struct MGView {
@@ -312,6 +315,15 @@ MGView<T> make_view(Context &c) {
     v.half0 = !f64 && c.ksm.on && v.lv.size() > 1;
     v.gs0 = !f64 && !v.half0 && c.gs.on && c.gs.filled && c.fine_smoother == STK_SMOOTH_GS && v.lv.size() > 1 &&
             v.lv[0].RK.pat != nullptr;
+    if (!v.gs0 && v.lv.size() > 1 && v.lv[0].lmax == 0.0 && v.lv[0].A.val) {
+        // level 0 smooths with Chebyshev, whose bound the set-up skipped: compute it now (the level-0
+        // work vectors are free between V-cycles)
+        double lm = 0.0;
+        MGLevelBufs<T> &kw = pick<T>(c.kw64, c.kw32);
+        power_lmax<T>(c, v.lv[0].A, v.lv[0].dinv, c.nU, kw.r.p, kw.d.p, &lm);
+        (f64 ? c.Klmax64 : c.Klmax32) = lm;
+        v.lv[0].lmax = lm;
+    }
     return v;
 }
 
@@ -716,24 +728,26 @@ void power_lmax(Context &c, const Op22<T> &A, const T *dinv, int64_t n, T *v, T 
         h ^= h >> 31; h *= 0xBF58476D1CE4E5B9ull; h ^= h >> 29;
         v[i] = (T)((double)(h >> 11) * 0x1.0p-53 - 0.5) * (dinv[i] != T(0) ? T(1) : T(0));
     });
-    double lam = 0;
+    // the iteration stays on the device: the scale of each step is read from the reduction slot,
+    // and the host fetches the last Rayleigh quotient only
+    const double *sc = c.d_sc.p;
     for (int k = 0; k < 20; ++k) {
         EpStore<T> es{w};
         apply22_auto<T>(c, A, v, es);
         // Rayleigh quotient in the D inner product: (v, A v) / (v, D v)
-        red.map_reduce(n, [=] __device__(int64_t i) { return (double)v[i] * (double)w[i]; }, SC_TMP);
-        red.map_reduce(n, [=] __device__(int64_t i) {
-            return dinv[i] != T(0) ? (double)v[i] * (double)v[i] / (double)dinv[i] : 0.0;
-        }, SC_TMP + 1);
-        double num, den;
-        num = red.fetch(SC_TMP);
-        den = red.fetch(SC_TMP + 1);
-        lam = num / den;
+        red.map_reduce<2>(n, [=] __device__(int64_t i) {
+            const double vi = (double)v[i];
+            return Dots<2>{{vi * (double)w[i], dinv[i] != T(0) ? vi * vi / (double)dinv[i] : 0.0}};
+        }, SC_TMP);
         // v <- D^-1 A v, normalised in the D norm
-        const double s = 1.0 / std::sqrt(std::max(den, 1e-300));
-        launch_map(c, n, [=] __device__(int64_t i) { v[i] = (T)s * dinv[i] * w[i]; });
+        launch_map(c, n, [=] __device__(int64_t i) {
+            const T s = (T)(1.0 / sqrt(fmax(sc[SC_TMP + 1], 1e-300)));
+            v[i] = s * dinv[i] * w[i];
+        });
     }
-    *out = lam;
+    double h[2];
+    red.fetch_many(SC_TMP, 2, h);
+    *out = h[0] / h[1];
 }
 
 template <typename T>
@@ -1046,10 +1060,13 @@ void setup_mg(Context &c, PhaseTimer &pt) {
                 (const T *)pick<T>(F.A64, F.A32).p, ra.p);
             STK_HIP(hipGetLastError());
         }
-        // eigenvalue bounds (level 0 = K, then the P1 levels except the coarsest)
+        // eigenvalue bounds (level 0 = K, then the P1 levels except the coarsest).  K's is needed by
+        // Chebyshev smoothing on level 0: every FP64 V-cycle, and FP32 ones without the Gauss-Seidel
+        // smoother (otherwise it is computed when a solve first asks for it, make_view)
         Op22<T> K0{&c.KN, K};
-        double lm;
-        power_lmax<T>(c, K0, kd.p, c.nU, kw.r.p, kw.d.p, &lm);
+        double lm = 0.0;
+        if (f64 || !c.gs.on || (c.asm_flags & STK_ASM_FP16_SMOOTHER))
+            power_lmax<T>(c, K0, kd.p, c.nU, kw.r.p, kw.d.p, &lm);
         (f64 ? c.Klmax64 : c.Klmax32) = lm;
         for (size_t i = 0; i + 1 < c.mg.size(); ++i) {
             MGLevel &L = c.mg[i];
@@ -2246,6 +2263,23 @@ double time_kernel(Context &c, int kernel, int reps, double *bytes) {
                 STK_HIP(hipGetLastError());
             };
             *bytes = (double)n * n * 4 + 2.0 * n * 4;
+            break;
+        }
+        case STK_KERNEL_RESTRICT32: {
+            STK_CHECK(c.mg_built && c.RK32 && c.gs.on && c.gs.filled, STK_E_STATE, "no FP32 multigrid with R K");
+            const int64_t nc = 2 * c.mg[0].nvert;
+            x32.alloc(nU); y32.alloc(nc);
+            float *b = x32.p, *y = y32.p, *xq = c.gs.xq.p;
+            launch_map(c, nU, [=] __device__(int64_t i) { b[i] = 1.0f + 1e-3f * (float)(i % 17); });
+            const Op22<float> R{&c.mg[0].R_pat, c.mg[0].R32.p}, RK{&c.gs.rkq, c.RK32.p};
+            launch = [&c, R, RK, b, xq, y]() { apply22_restrict<float>(c, R, b, RK, xq, y); };
+            // both operators' blocks (four values and a column), their slice pointers, b and the
+            // coloured iterate gathered once, y written
+            const Pattern &pr = *R.pat, &pk = *RK.pat;
+            auto ptrs = [](const Pattern &p) {
+                return p.sell ? 8.0 * ((p.nrows + Pattern::kSlice - 1) / Pattern::kSlice + 1) : 4.0 * (p.nrows + 1);
+            };
+            *bytes = (double)(pr.nnz() + pk.nnz()) * 20 + ptrs(pr) + ptrs(pk) + 2.0 * nU * 4 + (double)nc * 4;
             break;
         }
         case STK_KERNEL_CGUPD32: {

@@ -464,8 +464,10 @@ enum {
     STK_KERNEL_G32 = 15,        /* y = G p (pressure to velocity), FP32                   */
     STK_KERNEL_GT32 = 16,       /* y = G^T u (velocity to pressure), FP32                 */
     STK_KERNEL_COARSE32 = 17,   /* dense coarsest-level solve (inverse times b), FP32     */
-    STK_KERNEL_CGUPD32 = 18     /* inner CG update z += a p, r -= a P Ap with |r|^2 (the  */
+    STK_KERNEL_CGUPD32 = 18,    /* inner CG update z += a p, r -= a P Ap with |r|^2 (the  */
                                 /* map pass and the final reduction), FP32                */
+    STK_KERNEL_RESTRICT32 = 19  /* level-0 coarse right-hand side R b - (R K) x in one     */
+                                /* pass (the V-cycle's restriction), FP32                 */
 };
 
 /* Measurement hook with no counterpart in the reference: runs one of the hot
