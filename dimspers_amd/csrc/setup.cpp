@@ -290,6 +290,10 @@ void build_gs_layout(Context &c) {
         Q.sptr = GT.sptr;
         Q.scol.resize(GT.scol.size());
         for (size_t k = 0; k < GT.scol.size(); ++k) Q.scol[k] = qpos[GT.scol[k]];
+        // host CSR copy (the structural blocks: the byte counts of its products read it)
+        Q.ptr = GT.ptr;
+        Q.col.resize(GT.col.size());
+        for (size_t k = 0; k < GT.col.size(); ++k) Q.col[k] = qpos[GT.col[k]];
         Q.dsptr.upload(Q.sptr, c.stream);
         Q.dscol.upload(Q.scol, c.stream);
     }
@@ -326,6 +330,12 @@ void build_gs_layout(Context &c) {
         }
         for (int32_t x : Q.scol)
             if (x < 0 || x >= Q.ncols) throw Error(STK_E_STATE, "coloured prolongation column out of range");
+        Q.ptr.assign(nq + 1, 0);
+        for (int64_t q = 0; q < nq; ++q) Q.ptr[q + 1] = Q.ptr[q] + plen(rowid[q]);
+        Q.col.resize(Q.ptr[nq]);
+        for (int64_t q = 0; q < nq; ++q)
+            if (rowid[q] >= 0)
+                std::copy(P.col.begin() + P.ptr[rowid[q]], P.col.begin() + P.ptr[rowid[q] + 1], Q.col.begin() + Q.ptr[q]);
         Q.dsptr.upload(Q.sptr, c.stream);
         Q.dscol.upload(Q.scol, c.stream);
         g.pq_src.upload(psrc, c.stream);

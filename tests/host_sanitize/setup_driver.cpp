@@ -117,6 +117,8 @@ int main(int argc, char **argv) {
         int64_t rows = 0, blocks = 0;
         for (int k = 0; k < c.gs.ncolour; ++k) { rows += c.gs.rows[k]; blocks += c.gs.blocks[k]; }
         REQUIRE(rows == nnod && blocks == K.nnz());
+        // the coloured-column copies of R K and P keep every structural block
+        REQUIRE(c.gs.rkq.nnz() == c.GTN.nnz() && c.gs.pq.nnz() == c.mg[0].P_pat.nnz());
         // levels: sizes halve, fixed dofs are radial
         for (size_t l = 0; l < c.mg.size(); ++l) {
             const MGLevel &L = c.mg[l];
