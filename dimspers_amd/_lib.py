@@ -19,7 +19,7 @@ ASM_FP16_SMOOTHER = 64
 ASM_7POINT, ASM_CLAMP = 128, 256
 SMOOTH_GS, SMOOTH_CHEB = 0, 1
 MAT_K, MAT_G, MAT_GT, MAT_M, MAT_S, MAT_KT, MAT_MT, MAT_R, MAT_RK = 0, 1, 2, 3, 4, 5, 6, 7, 8
-MAT_LEVEL, MAT_PROLONG, MAT_LEVEL_RA = 16, 48, 80
+MAT_LEVEL, MAT_PROLONG, MAT_LEVEL_RA, MAT_COARSE_INV = 16, 48, 80, 112
 SOLVE_PLAIN, SOLVE_MIXED, SOLVE_LEARNED = 0, 1, 2
 INNER_MG, INNER_JACOBI = 0, 1
 STOP_SCHUR, STOP_FULL = 0, 1

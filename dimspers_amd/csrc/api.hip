@@ -369,6 +369,22 @@ int stk_export_csc(stk_ctx *ctx, int32_t which, int32_t lower_only, int64_t *nro
     if (!ctx) return STK_E_ARG;
     return guarded(ctx, [&]() {
         Context &c = ctx->c;
+        if (which == STK_MAT_COARSE_INV) {
+            // dense and symmetric: column j is row j of the row-major array
+            STK_CHECK(c.mg_built && c.coarse64_valid && c.coarse_n > 0, STK_E_STATE, "no coarsest inverse");
+            const int64_t n = c.coarse_n;
+            if (nrows) *nrows = n;
+            if (ncols) *ncols = n;
+            if (nnz) *nnz = n * n;
+            if (!colptr || !rowval || !nzval) return (int)STK_OK;
+            STK_HIP(hipMemcpyAsync(nzval, c.coarse_inv64.p, (size_t)(n * n) * sizeof(double), hipMemcpyDeviceToHost,
+                                   c.stream));
+            STK_HIP(hipStreamSynchronize(c.stream));
+            for (int64_t j = 0; j <= n; ++j) colptr[j] = j * n + 1;
+            for (int64_t j = 0; j < n; ++j)
+                for (int64_t i = 0; i < n; ++i) rowval[j * n + i] = i + 1;
+            return (int)STK_OK;
+        }
         const bool thermal = which == STK_MAT_KT || which == STK_MAT_MT;
         STK_CHECK(thermal ? c.have_thermal : c.assembled, STK_E_STATE, "nothing assembled");
         const Pattern *p = nullptr;
@@ -445,7 +461,7 @@ int stk_export_csc(stk_ctx *ctx, int32_t which, int32_t lower_only, int64_t *nro
         if (nrows) *nrows = nr;
         if (ncols) *ncols = nc;
         if (nnz) *nnz = cnt[nc];
-        if (!colptr || !rowval || !nzval) return STK_OK;
+        if (!colptr || !rowval || !nzval) return (int)STK_OK;
         std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
         // rows are visited in increasing order, so each column comes out sorted
         visit([&](int64_t row, int64_t col, double v) {
@@ -454,7 +470,7 @@ int stk_export_csc(stk_ctx *ctx, int32_t which, int32_t lower_only, int64_t *nro
             nzval[d] = v;
         });
         for (int64_t j = 0; j <= nc; ++j) colptr[j] = cnt[j] + 1;
-        return STK_OK;
+        return (int)STK_OK;
     });
 }
 

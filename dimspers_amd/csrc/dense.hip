@@ -29,61 +29,48 @@ __device__ __forceinline__ double *tile(double *A, int64_t ld, int I, int J) {
     return A + (int64_t)J * kT * ld + (int64_t)I * kT;
 }
 
-// Factor the lower triangle of diagonal tile k in place by four wavefronts: thread (t, q) keeps
-// columns 16q .. 16q + 15 of row t in registers (q is uniform over a wavefront); per column j the
-// wavefront that owns it forms the pivot and column j of L, which go through LDS as broadcasts.  The
-// upper triangle of the tile is set to zero.  A pivot that is not positive is reported in *info
-// (1-based column, the first one only) and replaced by 1 so that the factorisation finishes.
-__global__ __launch_bounds__(4 * kT) void k_potrf_tile(double *A, int64_t ld, int k, int *info) {
+// Factor the lower triangle of diagonal tile k in place by one wavefront: lane t keeps row t of
+// the tile in registers; per column j the pivot comes from lane j by a cross-lane read, the column
+// of L goes through LDS (a wavefront needs no barrier between its own LDS write and read, only the
+// compiler fence of __syncthreads), and each lane updates its row to the right of j.  The upper
+// triangle of the tile is set to zero.  A pivot that is not positive is reported in *info (1-based
+// column, the first one only) and replaced by 1 so that the factorisation finishes.
+__global__ __launch_bounds__(kT) void k_potrf_tile(double *A, int64_t ld, int k, int *info) {
     __shared__ double col[kT];
-    __shared__ double piv;
-    const int t = threadIdx.x % kT, q = threadIdx.x / kT;
+    const int t = threadIdx.x;
     double *D = tile(A, ld, k, k);
-    double a[16];
+    double a[kT];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int c = 16 * q + i;
-        a[i] = c <= t ? D[(int64_t)c * ld + t] : 0.0;
-    }
+    for (int c = 0; c < kT; ++c) a[c] = c <= t ? D[(int64_t)c * ld + t] : 0.0;
 #pragma unroll
     for (int j = 0; j < kT; ++j) {
-        const int qj = j / 16, ij = j % 16;
-        if (q == qj && t == j) {
-            double d = a[ij];
-            if (!(d > 0.0)) {
-                if (*info == 0) *info = kT * k + j + 1;
-                d = 1.0;
-            }
-            a[ij] = sqrt(d);
-            piv = a[ij];
+        double d = __shfl(a[j], j);
+        if (!(d > 0.0)) {
+            if (t == j && *info == 0) *info = kT * k + j + 1;
+            d = 1.0;
         }
-        __syncthreads();
-        if (q == qj) {
-            if (t > j) a[ij] /= piv;
-            col[t] = t > j ? a[ij] : 0.0;
-        }
+        const double s = sqrt(d), rs = 1.0 / s;
+        a[j] = t == j ? s : (t > j ? a[j] * rs : 0.0);
+        col[t] = t > j ? a[j] : 0.0;
         __syncthreads();
         if (t > j) {
-            const double l = col[t];
+            const double l = a[j];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const int c = 16 * q + i;
-                if (c > j && c <= t) a[i] -= l * col[c];
-            }
+            for (int c = j + 1; c < kT; ++c)
+                if (c <= t) a[c] -= l * col[c];
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int c = 16 * q + i;
-        D[(int64_t)c * ld + t] = c <= t ? a[i] : 0.0;
-    }
+    for (int c = 0; c < kT; ++c) D[(int64_t)c * ld + t] = c <= t ? a[c] : 0.0;
 }
 
 // L(I, k) = A(I, k) L(k, k)^-T for the tiles I = k + 1 + blockIdx.x: lane t solves row t in
-// registers, L(k, k) is read from LDS as broadcasts.
+// registers, L(k, k) and the reciprocals of its diagonal are read from LDS as broadcasts; each
+// forward step sums in two chains.
 __global__ __launch_bounds__(kT) void k_trsm_panel(double *A, int64_t ld, int k) {
     __shared__ double L[kT][kT + 1];
+    __shared__ double rdiag[kT];
     const int t = threadIdx.x;
     const int I = k + 1 + blockIdx.x;
     const double *D = tile(A, ld, k, k);
@@ -95,12 +82,17 @@ __global__ __launch_bounds__(kT) void k_trsm_panel(double *A, int64_t ld, int k)
         x[c] = P[(int64_t)c * ld + t];
     }
     __syncthreads();
+    rdiag[t] = 1.0 / L[t][t];
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < kT; ++j) {
-        double v = x[j];
+        double v0 = x[j], v1 = 0.0;
 #pragma unroll
-        for (int m = 0; m < j; ++m) v -= x[m] * L[j][m];
-        x[j] = v / L[j][j];
+        for (int m = 0; m < j; m += 2) {
+            v0 -= x[m] * L[j][m];
+            if (m + 1 < j) v1 -= x[m + 1] * L[j][m + 1];
+        }
+        x[j] = (v0 + v1) * rdiag[j];
     }
 #pragma unroll
     for (int c = 0; c < kT; ++c) P[(int64_t)c * ld + t] = x[c];
@@ -114,6 +106,7 @@ __global__ __launch_bounds__(kT) void k_trsm_rows(const double *A, double *Y, in
     const int j = blockIdx.x;
     const double *D = A + (int64_t)i * kT * ld + (int64_t)i * kT;
     double *B = Y + (int64_t)j * kT * ld + (int64_t)i * kT;
+    __shared__ double rdiag[kT];
     double x[kT];
 #pragma unroll
     for (int c = 0; c < kT; ++c) L[t][c] = D[(int64_t)c * ld + t];
@@ -121,12 +114,17 @@ __global__ __launch_bounds__(kT) void k_trsm_rows(const double *A, double *Y, in
 #pragma unroll
     for (int r = 0; r < kT; ++r) x[r] = B[(int64_t)t * ld + r];
     __syncthreads();
+    rdiag[t] = 1.0 / L[t][t];
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < kT; ++r) {
-        double v = x[r];
+        double v0 = x[r], v1 = 0.0;
 #pragma unroll
-        for (int m = 0; m < r; ++m) v -= L[r][m] * x[m];
-        x[r] = v / L[r][r];
+        for (int m = 0; m < r; m += 2) {
+            v0 -= L[r][m] * x[m];
+            if (m + 1 < r) v1 -= L[r][m + 1] * x[m + 1];
+        }
+        x[r] = (v0 + v1) * rdiag[r];
     }
 #pragma unroll
     for (int r = 0; r < kT; ++r) B[(int64_t)t * ld + r] = x[r];
@@ -266,7 +264,7 @@ void spd_inverse(hipStream_t s, double *A, double *Y, int64_t np, int *info) {
     const int nt = (int)(np / kT);
     GemmArgs g{A, Y, A, np, 0, nt};
     for (int k = 0; k < nt; ++k) {
-        k_potrf_tile<<<dim3(1), dim3(4 * kT), 0, s>>>(A, np, k, info);
+        k_potrf_tile<<<dim3(1), dim3(kT), 0, s>>>(A, np, k, info);
         const int m = nt - k - 1;
         if (m > 0) {
             k_trsm_panel<<<dim3(m), dim3(kT), 0, s>>>(A, np, k);

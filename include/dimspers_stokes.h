@@ -69,10 +69,13 @@ enum {
     STK_MAT_PROLONG = 48, /* + l: the prolongation into the next */
                           /* finer space from level l (l = 0: into */
                           /* the P2 velocity, = R^T), n_finer x 2 nvert_l */
-    STK_MAT_LEVEL_RA = 80 /* + l (l >= 1): R A, the restriction into */
+    STK_MAT_LEVEL_RA = 80, /* + l (l >= 1): R A, the restriction into */
                           /* level l times the operator of level l-1, */
                           /* the coarse residual operator of the V-cycle */
                           /* (after a solver set-up), 2 nvert_l x 2 nvert_l-1 */
+    STK_MAT_COARSE_INV = 112 /* the dense inverse of the coarsest level's */
+                          /* operator (after a solver set-up; FP64 as     */
+                          /* computed), every entry stored, n x n         */
 };
 
 /* Fine-level smoother of the FP32 multigrid V-cycle (stk_opts.fine_smoother). */

@@ -80,6 +80,19 @@ def test_multigrid_levels_are_galerkin_products(kind):
             print(f"[result] {kind} level {l}: R A rel err {err:.2e}")
             assert err < 1e-13
         fine = A
+    # the dense inverse of the coarsest level (blocked Cholesky, FP64): A A^-1 = I
+    A = ctx.export_csc(_lib.MAT_LEVEL + ctx.mg_levels - 1).toarray()
+    Ainv = ctx.export_csc(_lib.MAT_COARSE_INV).toarray()
+    assert Ainv.shape == A.shape
+    n, eps = A.shape[0], np.finfo(np.float64).eps
+    cond = np.linalg.cond(A)
+    err = np.abs(A @ Ainv - np.eye(n)).max()
+    ref = np.linalg.inv(A)
+    rel = np.abs(Ainv - ref).max() / np.abs(ref).max()
+    print(f"[result] {kind} coarsest n {n}: cond {cond:.2e}, |A A^-1 - I|_max {err:.2e}, vs LAPACK {rel:.2e}")
+    # backward-stable Cholesky inverse: errors of order eps * cond (with a generous factor)
+    assert err < 1e2 * eps * cond and rel < 1e2 * eps * cond
+    assert np.array_equal(Ainv, Ainv.T)
     ctx.close()
     # FP32 set-up: R A from the FP32 restriction and level operators
     ctx = _ctx(inp, _lib.ASM_FP64 | _lib.ASM_FP32)
