@@ -22,6 +22,7 @@ double update_velocity(Context &c, double *Ucart_dev);
 Context::~Context() {
     delete mlp;
     if (h_pinned) (void)hipHostFree(h_pinned);
+    if (aux_stream) (void)hipStreamDestroy(aux_stream);
     if (stream) (void)hipStreamDestroy(stream);
 }
 }  // namespace stk
@@ -98,6 +99,7 @@ int stk_create(stk_ctx **out, int64_t nel, int64_t nnod, int64_t nvert, const in
         STK_CHECK(nfixed == 0 || fixed_dofs, STK_E_ARG, "fixed_dofs is NULL");
         STK_HIP(hipGetDevice(&c.device));
         STK_HIP(hipStreamCreate(&c.stream));
+        STK_HIP(hipStreamCreate(&c.aux_stream));
         c.nel = nel; c.nnod = nnod; c.nvert = nvert; c.nU = 2 * nnod; c.nP = nvert;
         c.e2n.resize(6 * nel);
         for (int64_t e = 0; e < nel; ++e)

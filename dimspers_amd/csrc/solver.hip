@@ -24,7 +24,9 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <thread>
 #include <type_traits>
+#include <exception>
 
 
 #include "device_ops.h"
@@ -810,8 +812,66 @@ __global__ void k_symmetrize_lower_cm(int64_t n, const double *__restrict__ L, d
 }
 
 // Dense inverse of the coarsest multigrid operator, computed on the device in FP64 (from the FP64
-// operator when the assembly has one) and stored in the precision asked: blocked Cholesky
-// factorisation and inverse of the SPD matrix (spd_inverse, dense.hip) on the library stream.
+// operator when the assembly has one): blocked Cholesky factorisation and inverse of the SPD matrix
+// (spd_inverse, dense.hip), all work on stream s, into out64 (n x n, row-major = column-major) and,
+// when out32 is given, its FP32 rounding.  It reads the coarsest level's pattern and operator and
+// writes no state of the context, so it may run on another stream beside the context stream's work.
+void coarse_inverse_on(Context &c, hipStream_t s, DBuf<double> &out64, DBuf<float> *out32) {
+    MGLevel &L = c.mg.back();
+    const int64_t n = 2 * L.nvert;
+    if (L.pat.sell) throw Error(STK_E_STATE, "the coarsest multigrid level must use CSR block order");
+    const int64_t nb = L.pat.nnz();
+    DBuf<double> blk;
+    const double *bp;
+    const bool have64 = (c.asm_flags & STK_ASM_FP64) && L.A64;
+    if (have64) {
+        bp = L.A64.p;
+    } else {
+        blk.alloc(4 * nb);
+        const float *a32 = L.A32.p;
+        double *b = blk.p;
+        k_map<NT><<<dim3(nblk(4 * nb)), dim3(NT), 0, s>>>(4 * nb, [=] __device__(int64_t i) { b[i] = (double)a32[i]; });
+        STK_HIP(hipGetLastError());
+        bp = blk.p;
+    }
+    {
+        // non-finite entries (e.g. a NaN viscosity) are reported before the factorisation
+        std::vector<double> hb(4 * nb);
+        STK_HIP(hipMemcpyAsync(hb.data(), bp, 4 * nb * sizeof(double), hipMemcpyDeviceToHost, s));
+        STK_HIP(hipStreamSynchronize(s));
+        for (double v : hb)
+            if (!std::isfinite(v)) throw Error(STK_E_NAN, "non-finite coarsest multigrid operator");
+    }
+    // padded to a multiple of 64 with the identity: the blocked inverse works on 64 x 64 tiles
+    const int64_t np = (n + 63) / 64 * 64;
+    DBuf<double> dA(np * np), Y(np * np);
+    dA.zero(s);
+    k_dense_from_blocks<<<dim3((unsigned)((L.nvert + 255) / 256)), dim3(256), 0, s>>>(
+        L.nvert, np, L.pat.dptr.p, L.pat.dcol.p, bp, dA.p);
+    if (np > n) k_pad_identity<<<dim3((unsigned)((np - n + 255) / 256)), dim3(256), 0, s>>>(n, np, dA.p);
+    STK_HIP(hipGetLastError());
+    DBuf<int32_t> info(1);
+    info.zero(s);
+    // symmetric: the row-major matrix is its own column-major copy
+    spd_inverse(s, dA.p, Y.p, np, info.p);
+    int32_t hinfo = 0;
+    info.download(&hinfo, 1, s);
+    STK_HIP(hipStreamSynchronize(s));
+    if (hinfo != 0) throw Error(STK_E_SHAPE, "coarsest multigrid operator is not positive definite");
+    out64.alloc(n * n);
+    k_copy_leading<<<dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, s>>>(n, np, dA.p, out64.p);
+    STK_HIP(hipGetLastError());
+    if (out32) {
+        out32->alloc(n * n);
+        const double *src = out64.p;
+        float *dst = out32->p;
+        k_map<NT><<<dim3(nblk(n * n)), dim3(NT), 0, s>>>(n * n, [=] __device__(int64_t i) { dst[i] = (float)src[i]; });
+        STK_HIP(hipGetLastError());
+    }
+    STK_HIP(hipStreamSynchronize(s));
+}
+
+// This assembly's coarsest inverse on the context stream (FP64, and its FP32 rounding unless f64).
 void build_coarse_inverse(Context &c, bool f64) {
     MGLevel &L = c.mg.back();
     const int64_t n = 2 * L.nvert;
@@ -826,54 +886,11 @@ void build_coarse_inverse(Context &c, bool f64) {
         }
         return;
     }
-    if (L.pat.sell) throw Error(STK_E_STATE, "the coarsest multigrid level must use CSR block order");
-    const int64_t nb = L.pat.nnz();
-    DBuf<double> blk;
-    const double *bp;
-    const bool have64 = (c.asm_flags & STK_ASM_FP64) && L.A64;
-    if (have64) {
-        bp = L.A64.p;
-    } else {
-        blk.alloc(4 * nb);
-        convert_vec(c, L.A32.p, blk.p, 4 * nb);
-        bp = blk.p;
-    }
-    {
-        // non-finite entries (e.g. a NaN viscosity) are reported before the factorisation
-        std::vector<double> hb(4 * nb);
-        STK_HIP(hipMemcpyAsync(hb.data(), bp, 4 * nb * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-        STK_HIP(hipStreamSynchronize(c.stream));
-        for (double v : hb)
-            if (!std::isfinite(v)) throw Error(STK_E_NAN, "non-finite coarsest multigrid operator");
-    }
-    // padded to a multiple of 64 with the identity: the blocked inverse works on 64 x 64 tiles
-    const int64_t np = (n + 63) / 64 * 64;
-    DBuf<double> dA(np * np), Y(np * np);
-    dA.zero(c.stream);
-    k_dense_from_blocks<<<dim3((unsigned)((L.nvert + 255) / 256)), dim3(256), 0, c.stream>>>(
-        L.nvert, np, L.pat.dptr.p, L.pat.dcol.p, bp, dA.p);
-    if (np > n) k_pad_identity<<<dim3((unsigned)((np - n + 255) / 256)), dim3(256), 0, c.stream>>>(n, np, dA.p);
-    STK_HIP(hipGetLastError());
-    DBuf<int32_t> info(1);
-    info.zero(c.stream);
-    // symmetric: the row-major matrix is its own column-major copy
-    spd_inverse(c.stream, dA.p, Y.p, np, info.p);
-    int32_t hinfo = 0;
-    info.download(&hinfo, 1, c.stream);
-    STK_HIP(hipStreamSynchronize(c.stream));
-    if (hinfo != 0) throw Error(STK_E_SHAPE, "coarsest multigrid operator is not positive definite");
-    DBuf<double> full(n * n);
-    k_copy_leading<<<dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, c.stream>>>(n, np, dA.p, full.p);
-    STK_HIP(hipGetLastError());
-    dA = std::move(full);
+    DBuf<double> inv;
+    coarse_inverse_on(c, c.stream, inv, f64 ? nullptr : &c.coarse_inv32);
     c.coarse_n = n;
-    c.coarse_inv64 = std::move(dA);
+    c.coarse_inv64 = std::move(inv);
     c.coarse64_valid = true;
-    if (!f64) {
-        c.coarse_inv32.alloc(n * n);
-        convert_vec(c, c.coarse_inv64.p, c.coarse_inv32.p, n * n);
-        STK_HIP(hipStreamSynchronize(c.stream));
-    }
 }
 
 template <typename T>
@@ -1036,7 +1053,7 @@ void setup_base(Context &c, PhaseTimer &pt) {
 // The multigrid data of precision T: level work vectors, inverse diagonals, Chebyshev bounds, the
 // coarsest inverse (and the FP16 smoother copy in FP32).
 template <typename T>
-void setup_mg(Context &c, PhaseTimer &pt) {
+void setup_mg(Context &c, PhaseTimer &pt, bool coarse = true) {
     const bool f64 = sizeof(T) == 8;
     MGLevelBufs<T> &kw = pick<T>(c.kw64, c.kw32);
     DBuf<T> &kd = pick<T>(c.Kdinv64, c.Kdinv32);
@@ -1077,8 +1094,10 @@ void setup_mg(Context &c, PhaseTimer &pt) {
             if (trace_on()) std::fprintf(stderr, "[stk] level %zu lmax %.4e\n", i + 1, lm);
         }
         pt.mark("diagonals and eigenvalue bounds");
-        build_coarse_inverse(c, f64);
-        pt.mark("coarse inverse");
+        if (coarse) {
+            build_coarse_inverse(c, f64);
+            pt.mark("coarse inverse");
+        }
         if (!f64) build_half_smoother(c);
     } else {
         Op22<T> K0{&c.KN, K};
@@ -1260,9 +1279,44 @@ void null_vector_mixed(Context &c, PhaseTimer &pt) {
 // for the first solve that needs it (ensure_mg64).
 void setup_fast(Context &c) {
     PhaseTimer pt(c, "set-up");
-    setup_base<double>(c, pt);
-    setup_base<float>(c, pt);
-    setup_mg<float>(c, pt);
+    // The coarsest inverse, a latency-bound chain of small launches that reads only the coarsest
+    // operator, is factorised on the auxiliary stream (by a second host thread) while this thread
+    // issues the per-precision set-up (diagonals, smoother values, R A, eigenvalue bounds); it is
+    // joined before the near-null vector, whose multigrid solves use it.  The work and its order on
+    // each stream are those of a sequential set-up, so the results are the same bit for bit.
+    const bool overlap = c.aux_stream && !c.mg.empty();
+    DBuf<double> inv64;
+    DBuf<float> inv32;
+    std::exception_ptr err;
+    std::thread worker;
+    if (overlap) {
+        STK_HIP(hipStreamSynchronize(c.stream));   // the assembled operators the factorisation reads
+        worker = std::thread([&c, &inv64, &inv32, &err]() {
+            try {
+                STK_HIP(hipSetDevice(c.device));
+                coarse_inverse_on(c, c.aux_stream, inv64, &inv32);
+            } catch (...) {
+                err = std::current_exception();
+            }
+        });
+    }
+    try {
+        setup_base<double>(c, pt);
+        setup_base<float>(c, pt);
+        setup_mg<float>(c, pt, !overlap);
+    } catch (...) {
+        if (worker.joinable()) worker.join();
+        throw;
+    }
+    if (overlap) {
+        worker.join();
+        if (err) std::rethrow_exception(err);
+        c.coarse_n = 2 * c.mg.back().nvert;
+        c.coarse_inv64 = std::move(inv64);
+        c.coarse_inv32 = std::move(inv32);
+        c.coarse64_valid = true;
+        pt.mark("coarse inverse (auxiliary stream) joined");
+    }
     if (c.rotation_free) {
         null_vector_mixed(c, pt);
     } else {

@@ -164,6 +164,7 @@ struct MGLevel {
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t aux_stream = nullptr;   // set-up work that runs beside the context stream's
     std::string err;
 
     int64_t nel = 0, nnod = 0, nvert = 0, nU = 0, nP = 0;
