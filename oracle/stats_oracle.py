@@ -15,7 +15,8 @@ Restates ``write_stats`` (src/IO/IOmanager.jl:277-322) without the file output:
   Nusselt top / bottom of :286-290.
 
 Pinned (tests/test_stats_oracle.py): the integral of 1 equals the closed-form area of the
-inscribed polygonal annulus; a P2 vertex indicator integrates to 0 and a midpoint indicator
+inscribed polygonal annulus; Urms of a uniform velocity is its length times the square root of
+that area over the exact one, for either component; a P2 vertex indicator integrates to 0 and a midpoint indicator
 to a third of the areas of its triangles; the Nusselt numbers of a linear temperature profile
 follow from the node counts in closed form.
 """

@@ -38,7 +38,8 @@ sum_q omega_q eta_q eps:C:eps of a fourth-order tensor for a velocity P2
 holds exactly (fixes where eta15 and eta35 sit in D), and u^T G_e p equals
 sum_q omega_q p div u; ``prepare_matrices`` gives a rigid rotation near-zero
 energy and the field (x, z) divergence 2; ``volume_integral`` integrates
-1, r, r^2 exactly.  The Polak-Ribiere beta is not pinned apart from the
+1, r, r^2 exactly; ``full_residual`` vanishes on the dense LAPACK solution and
+moves with its pressure.  The Polak-Ribiere beta is not pinned apart from the
 Fletcher-Reeves form: with exact inner solves the two are equal.  The reference ships no tests or
 vectors and Julia is absent, so nothing is compared against a run of the
 reference itself.

@@ -17,7 +17,8 @@ the points of ``ip_triangle7`` (src/Algebra/Quadrature.jl:109-138), mapped by th
 "double Jacobian" (StressCalculation.jl:145-205).
 
 Pinning (tests/test_stress_oracle.py): a velocity linear in (theta, r) is reproduced
-exactly by P2, so its Cartesian gradient has a closed form at every point; the RK4 step of
+exactly by P2, so its Cartesian gradient has a closed form at every point, and the strain rate
+and its invariant are the symmetric part of that gradient and its norm; the RK4 step of
 F' = L F with constant L is the fourth-order Taylor polynomial of exp(L); the isotropic law
 is tau = 2 eta dev(eps) (three-dimensional deviator); with the initial isotropic tensor
 (anisotropy.jl:13-22) the anisotropic law differs from it by exactly -(2/9) eta tr(eps) on the

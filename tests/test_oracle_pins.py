@@ -280,6 +280,19 @@ def test_schur_pcg_matches_dense_saddle_solve():
     assert np.linalg.norm(Pd - P) / np.linalg.norm(P) < 1e-9
 
 
+def test_full_residual_of_the_dense_saddle_solution_vanishes():
+    """The judged residual ||[f + G p - K u ; -G^T u - c]|| / ||f|| (c the mean of -G^T u) is at
+    rounding level for the dense LAPACK solution of K u - G p = f, -G^T u = c 1 (the system the
+    reference iterates on), and a change of p shows in it: the signs of its terms are those of the
+    saddle system, independently of the Schur PCG."""
+    inp = _small_inputs(4, 16, w=1)
+    sol, _ = so.solve_stokes(inp, rtol=1e-13, itmax=400)
+    Ud, Pd = _dense_saddle_solution(sol)
+    assert so.full_residual(sol.K, sol.G, sol.F, Ud, Pd, sol.ifree) < 1e-10
+    bumped = Pd + 1e-3 * np.abs(Pd).max() * np.cos(np.arange(Pd.size))
+    assert so.full_residual(sol.K, sol.G, sol.F, Ud, bumped, sol.ifree) > 1e-6
+
+
 def test_updatevelocity2_removes_net_rotation():
     inp = _small_inputs(4, 16, w=1)
     sol, _ = so.solve_stokes(inp)

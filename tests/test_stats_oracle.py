@@ -20,6 +20,21 @@ def test_area_of_the_inscribed_polygonal_annulus(mesh):
     assert abs(got - want) < 1e-13 * want
 
 
+def test_rms_velocity_of_a_uniform_field(mesh):
+    """U = (ux, uz) constant in Cartesian node values: |U|^2 is constant, so Urms = |U| times the
+    square root of the ratio of the inscribed polygonal area to the exact annulus area; both
+    components count."""
+    nt = mesh.ncell_t
+    r_in, r_out = mesh.r.min(), mesh.r.max()
+    ratio = 0.5 * nt * np.sin(2 * np.pi / nt) * (r_out ** 2 - r_in ** 2) / (np.pi * (r_out ** 2 - r_in ** 2))
+    T = np.zeros(mesh.theta.size)
+    for ux, uz in ((0.0, 2.0), (3.0, 0.0), (1.5, -0.5)):
+        U = np.tile([ux, uz], mesh.theta.size)
+        urms = so.write_stats(mesh, U, T)[0]
+        want = np.hypot(ux, uz) * np.sqrt(ratio)
+        assert abs(urms - want) < 1e-13 * want
+
+
 def test_p2_indicator_integrals(mesh):
     """int N_vertex = 0 and int N_midpoint = area / 3 on a straight triangle (the 6-point
     rule is exact for quadratics)."""

@@ -47,6 +47,24 @@ def test_linear_field_gradient_is_exact(mesh):
     assert np.abs(uxx6[ok] - want6[ok]).max() < 1e-12
 
 
+def test_strain_rate_is_the_symmetric_part_of_the_gradient(mesh):
+    """eps = (grad u + grad u^T) / 2 and eII = sqrt((exx^2 + ezz^2) / 2 + exz^2), from the closed-form
+    Cartesian gradient of a field linear in (theta, r); for the rigid rotation u = (z, -x) in
+    node values the symmetric part of the exact gradient vanishes."""
+    a, b, c, d, e, f = 0.7, -1.3, 0.2, 2.1, 0.4, -0.5
+    U = _linear_field(mesh, a, b, c, d, e, f)
+    out = so.stress(mesh, U, np.ones((mesh.nel, 7)))
+    _, _, thq, rq = so.cartesian_gradients(mesh)
+    sn, cs = np.sin(thq), np.cos(thq)
+    uxx, uxz = b * sn + a * cs / rq, b * cs - a * sn / rq
+    uzx, uzz = e * sn + d * cs / rq, e * cs - d * sn / rq
+    ok = _interior(mesh)
+    exz = 0.5 * (uxz + uzx)
+    eII = np.sqrt(0.5 * (uxx ** 2 + uzz ** 2) + exz ** 2)
+    for k, want in (("exx", uxx), ("ezz", uzz), ("exz", exz), ("eII", eII)):
+        assert np.abs(out[k][ok] - want[ok]).max() < 1e-12 * (1 + np.abs(want).max()), k
+
+
 def test_rigid_translation_has_no_strain_rate(mesh):
     U = np.tile([0.3, -1.7], mesh.theta.size)
     out = so.stress(mesh, U, np.ones((mesh.nel, 7)))
