@@ -16,7 +16,8 @@ Restates ``solveDiffusion_threaded`` (src/ThermalDiffusion/AssemblerThermal.jl:1
 
 Pinned (tests/test_thermal_oracle.py): sum of M_T = area for rho = Cp = 1; K_T 1 = 0;
 u^T K_T u = K area for u = r (|grad r| = 1, held exactly by P2); the large-dt step equals
-the analytic steady conduction profile ln(r/r_out)/ln(r_in/r_out) to discretisation error.
+the analytic steady conduction profile ln(r/r_out)/ln(r_in/r_out) to discretisation error; a
+step with insulated walls gains exactly dQdT dt area of heat (1^T M_T (T - T0)).
 """
 from __future__ import annotations
 

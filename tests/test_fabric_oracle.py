@@ -80,6 +80,16 @@ def test_fabric_destruction_rule():
     assert np.isclose(fse0[0, 4] / fse0[0, 5], 2e3) and np.array_equal(F0, F)
 
 
+def test_fabric_destruction_at_exactly_the_ratio():
+    """The aggregate stays anisotropic only while a1/a2 < eps (FSE.jl:207, strict): F = diag(1000, 1)
+    has a1/a2 = 1000 exactly and is reset at eps = 1e3, and kept at eps = 1000.5."""
+    F = np.array([[[1000.0, 0.0], [0.0, 1.0]]])
+    fse, F1 = fo.get_fse(F, fo.FABRIC_DESTRUCTION, eps=1e3)
+    assert np.array_equal(fse[0], fo.ISOTROPIC_FSE) and np.array_equal(F1[0], np.eye(2))
+    fse, F1 = fo.get_fse(F, fo.FABRIC_DESTRUCTION, eps=1000.5)
+    assert fse[0, 4] == 1000.0 and fse[0, 5] == 1.0 and np.array_equal(F1, F)
+
+
 def test_annealing_destruction_rule():
     """_FSE(F, destruction, annealing) (FSE.jl:309-355): force_isotropy turns a1 <= 1 or a2 >= 1
     into (1, 0), which the ratio test then destroys; other points keep their axes."""

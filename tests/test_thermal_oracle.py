@@ -22,6 +22,21 @@ def test_mass_sums_to_the_area(mesh):
     assert abs(MT2.sum() - 3.0 * area) < 1e-12 * area
 
 
+def test_a_step_without_fixed_nodes_conserves_heat_plus_the_source(mesh):
+    """(M + dt K) T = M T0 + f with insulated walls: K 1 = 0, so 1^T M T = 1^T M T0 + sum f, and
+    sum f = dQdT dt area (the P2 functions sum to one; the 7-point rule is exact for r)."""
+    nn = mesh.theta.size
+    rng = np.random.default_rng(4)
+    T0 = 1.0 + 0.1 * rng.standard_normal(nn)
+    dt, dqdt = 0.01, 0.3
+    T = to.thermal_step(mesh, T0, np.ones(nn), 2.0, 1.0, dt, np.array([], dtype=np.int64), np.array([]), dQdT=dqdt)
+    _, MT, _ = to.thermal_matrices(mesh, np.ones(nn), 2.0, 1.0)
+    area = np.pi * (mesh.r.max() ** 2 - mesh.r.min() ** 2)
+    gained = MT.sum(axis=0) @ (T - T0)
+    assert abs(gained - dqdt * dt * area) < 1e-11 * dqdt * dt * area
+    assert np.abs(T - T0).max() > 1e-4          # the step did diffuse and heat
+
+
 def test_diffusion_annihilates_constants_and_measures_grad_r(mesh):
     """K_T 1 = 0; for u = r (held exactly by P2, |grad r| = 1 in Cartesian coordinates)
     u^T K_T u = K * area."""
