@@ -1,25 +1,17 @@
-// Element assembly of the anisotropic P2-P1 Stokes operators on gfx950.
+// Element-level kernels beside the assembly proper (csrc/element.hip):
 //
-// One thread per element, 64 elements per workgroup, elements of one colour per
-// launch (no two share a node, so the read-modify-write scatter is race free and
-// the summation order is fixed: assembly is bit-reproducible).  Per element the
-// kernel follows the reference block assembly (src/Stokes/StokesSolver.jl:247-404):
-//   _fill_R_J!            (:522-541)   straight-sided Jacobian in (theta, r),
-//   _derivative_weights!  (:641-686)   polar -> Cartesian "double Jacobian",
-//   _fill_Kblk! aniso     (src/src/Stokes/StokesSolver.jl:744-808, unclamped),
-//   _fill_Kblk! iso       (:688-742), _fill_Gblk! (:812-823), _fill_Mblk! (:825-836),
-//   _fill_Fbblk!          (:838-845) with _element_density (:1017-1022).
-// The 12x12 element matrix is accumulated as 21 node-pair 2x2 blocks in LDS,
-// rotated to the (u_theta, u_r) node frames (TT^T K TT of _prepare_matrices,
-// :192-201, applied per element since TT is block diagonal) and added into the
-// node-block CSR pattern at offsets precomputed per element.  Fixed (radial
-// boundary) dofs get zero rows/columns; a later pass puts 1 on their diagonal.
-//
-// The same kernel projects the rotated element matrix onto the P1 vertex space
-// (E1 = P^T K_e P with P the element's P1 -> P2 interpolation in rotated frames)
-// and adds it into the finest multigrid operator; coarser levels are built from
-// these element matrices by k_galerkin below (Galerkin products computed element
-// by element on nested meshes).
+//  * integrate_element: one element per thread with the 6-point rule, following the reference block
+//    assembly (src/Stokes/StokesSolver.jl:247-404): _fill_R_J! (:522-541), the polar -> Cartesian
+//    "double Jacobian" of _derivative_weights! (:641-686), _fill_Kblk! (aniso,
+//    src/src/Stokes/StokesSolver.jl:744-808; iso :688-742), _fill_Gblk! (:812-823), _fill_Mblk!
+//    (:825-836), _fill_Fbblk! (:838-845).  k_schur_samples uses it to form the exact local Schur
+//    blocks the learned preconditioner is trained on;
+//  * k_learned_scatter: the predicted local Schur blocks added into M's pattern, colour by colour
+//    (elements of one colour share no vertex, so the additions are race free and in a fixed order);
+//  * k_galerkin / k_galerkin4: the coarse multigrid levels, coarse element = sum over its four
+//    children of P^T E P (nested meshes of two or four triangles per cell, rotation-frame
+//    interpolation blocks, masked boundary dofs), colour scatter into the coarse operator;
+//  * k_fix_diag: the unit diagonal of the fixed dofs.
 #include <cmath>
 
 #include "device_ops.h"
