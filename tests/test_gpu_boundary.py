@@ -11,6 +11,7 @@ import pytest
 
 from dimspers_amd import _lib
 from dimspers_amd.inputs import make_inputs, scaled_workload
+from oracle import mlp_oracle as mo
 from oracle import stokes_oracle as so
 
 pytestmark = pytest.mark.gpu
@@ -128,4 +129,32 @@ def test_warm_start_from_the_solution_needs_no_iterations(method):
     print(f"[result] warm from a perturbed pressure: outer {rep2.outer_iterations} (cold {rep.outer_iterations}) "
           f"errU {eu:.2e} errP {ep:.2e}", flush=True)
     assert rep2.converged and eu < 1e-7 and ep < 1e-7
+    ctx.close()
+
+
+@pytest.mark.parametrize("method", [_lib.SOLVE_PLAIN, _lib.SOLVE_MIXED, _lib.SOLVE_LEARNED])
+@pytest.mark.parametrize("stop", [_lib.STOP_SCHUR, _lib.STOP_FULL])
+def test_zero_forcing_gives_the_zero_solution(method, stop):
+    """g = 0: the right-hand side vanishes.  StokesPcCG (src/IterativeSolvers.jl:114-137) would
+    divide 0 by 0 there (r = 0, so r.d = q.Sq = 0 and alpha is NaN); the drop-in reports a
+    converged solve without a Schur iteration and returns exactly U = 0, P = 0."""
+    inp = make_inputs(scaled_workload(3, 16, 128))
+    print(f"[case] zero forcing 16x128 method {method} stop {stop}", flush=True)
+    ctx = _lib.Context.from_mesh(inp.mesh)
+    ctx.set_fields(0.0, inp.rho, inp.eta_ip, inp.C6)
+    ctx.set_fabric(inp.phi, inp.delta)
+    ctx.assemble(_lib.ASM_FP64 | _lib.ASM_FP32)
+    if method == _lib.SOLVE_LEARNED:
+        rng = np.random.default_rng(5)
+        p = (rng.standard_normal(mo.N_PARAM) * 0.15).astype(np.float32)
+        norm = np.concatenate([np.zeros(5), np.ones(5), np.zeros(6), np.ones(6)]).astype(np.float32)
+        ctx.mlp_params(p, norm)
+        ctx.learned_setup()
+    rep = ctx.solve(_lib.default_opts(method=method, stop=stop, rtol=1e-8))
+    U, P = ctx.solution()
+    print(f"[result] outer {rep.outer_iterations} converged {rep.converged} "
+          f"max|U| {np.abs(U).max():.1e} max|P| {np.abs(P).max():.1e}", flush=True)
+    assert rep.converged and rep.outer_iterations == 0 and rep.fallbacks == 0
+    assert np.all(np.isfinite(U)) and np.all(np.isfinite(P))
+    assert not U.any() and not P.any()
     ctx.close()
