@@ -32,9 +32,12 @@ namespace {
 constexpr int kElLanes = 8;                     // lanes per element
 constexpr int kElPerBlock = 256 / kElLanes;     // elements per 256-thread block
 constexpr int kIpVals = 22;                     // per point: dx[6] dz[6] d[6] omega omega/eta fx fz
-// per element: 7 points x 22, or K_e (84) + T1 (72); an odd number of values, so the eight elements
-// of a wave that read the same offset fall in different LDS banks
-constexpr int kElLds = 161;
+// LDS values per element for an NQ-point rule: the point data (NQ x 22), then K_e (84) followed by
+// the table of the P1 projection (36), then T1 (72) over K_e; an odd number, so the eight elements of
+// a wave that read the same offset fall in different LDS banks.  With 6 points a 32-element block
+// takes 40.2 KB (FP64): four blocks, 16 waves, per CU.
+template <int NQ>
+constexpr int el_lds() { return (NQ * kIpVals > 84 + 36 ? NQ * kIpVals : 84 + 36) | 1; }
 
 // Integration rule of the assembly: ip_triangle6 (6 points, the live assembly, StokesSolver.jl:262)
 // or ip_triangle7 (7 points, the threaded assembler, AssemblerStokes.jl:174); weights include the 1/2
@@ -109,17 +112,20 @@ __device__ __forceinline__ void kblock(const T *L, int f, int g, T v[4]) {
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_element(ElemArgs<T> a) {
-    __shared__ T lds[kElPerBlock][kElLds];
-    __shared__ T ltab[7][9];                    // per point: P2 N (6), P1 NP (3)
-    __shared__ int32_t lnode[kElPerBlock][6];
+// NQ: points of the rule (6 or 7), sizing the LDS; 4 waves per SIMD (at most 128 VGPRs) where that
+// LDS allows it
+template <typename T, int NQ>
+__global__ __launch_bounds__(256, NQ == 6 ? 4 : 3) void k_element(ElemArgs<T> a) {
+    __shared__ T lds[kElPerBlock][el_lds<NQ>()];
+    __shared__ T ltab[NQ][9];                   // per point: P2 N (6), P1 NP (3)
+    __shared__ T lrho[kElPerBlock][3];          // rho at the three vertices
     __shared__ T lcs[kElPerBlock][6][2];        // node frame (cos, sin)
     __shared__ uint8_t lfix[kElPerBlock][12];   // fixed flags per local dof
     __shared__ T lph[kElPerBlock][3][2];        // (cos h, sin h) of the three edges, h = half the angle step
     const int le = threadIdx.x / kElLanes, lane = threadIdx.x % kElLanes;
-    const int64_t nel = a.nel;
-    const int64_t e = (int64_t)blockIdx.x * kElPerBlock + le;
+    // 32-bit element offsets (the host keeps 84 nel below 2^32): fewer registers per address
+    const uint32_t nel = (uint32_t)a.nel;
+    const uint32_t e = blockIdx.x * kElPerBlock + le;
     const bool live = e < nel;
     const int nq = a.rule.nq;
     T *L = lds[le];
@@ -137,9 +143,19 @@ __global__ __launch_bounds__(256) void k_element(ElemArgs<T> a) {
         th[k] = live ? (T)a.th[k * nel + e] : T(0);
         rr[k] = live ? (T)a.r[k * nel + e] : T(1);
     }
+    // every global load of the element is issued here, before the first barrier: the point's
+    // viscosity and tensor columns, and (through e2n) the node data
+    T eta_q = T(1), cq[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    if (live && lane < nq) {
+        eta_q = (T)a.eta[lane * nel + e];
+        if (!a.iso) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) cq[k] = (T)a.C[((uint32_t)k * (uint32_t)a.ccol + lane) * nel + e];
+        }
+    }
     if (live && lane < 6) {
         const int32_t n = a.e2n[lane * nel + e];
-        lnode[le][lane] = n;
+        if (lane < 3) lrho[le][lane] = (T)a.rho[n];
         lcs[le][lane][0] = (T)a.nodecs[2 * (int64_t)n];
         lcs[le][lane][1] = (T)a.nodecs[2 * (int64_t)n + 1];
         lfix[le][2 * lane] = a.fixed[2 * (int64_t)n];
@@ -180,7 +196,7 @@ __global__ __launch_bounds__(256) void k_element(ElemArgs<T> a) {
             P[b] = jx0 * dr[b] + jx1 * ds[b];
             P[6 + b] = jz0 * dr[b] + jz1 * ds[b];
         }
-        const T eta = (T)a.eta[(int64_t)q * nel + e];
+        const T eta = eta_q;
         T d[6];   // eta11, eta33, eta55, eta13, eta15, eta35 (the StiffnessTensor order)
         if (a.iso) {
             d[0] = d[1] = T(4) / T(3); d[2] = T(1); d[3] = -T(2) / T(3); d[4] = d[5] = T(0);
@@ -189,7 +205,7 @@ __global__ __launch_bounds__(256) void k_element(ElemArgs<T> a) {
         } else {
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
-                T v = (T)a.C[((int64_t)k * a.ccol + q) * nel + e];
+                T v = cq[k];
                 if (a.clamp) {
                     // applybounds(eta C, 1e25, 1e18) of the top-level _fill_Kblk! (StokesSolver.jl:756)
                     v = v * eta;
@@ -205,7 +221,7 @@ __global__ __launch_bounds__(256) void k_element(ElemArgs<T> a) {
         for (int k = 0; k < 6; ++k) P[12 + k] = d[k];
         P[18] = om;
         P[19] = om / eta;
-        const T rhoq = tq * (T)a.rho[lnode[le][0]] + xi * (T)a.rho[lnode[le][1]] + et * (T)a.rho[lnode[le][2]];
+        const T rhoq = tq * lrho[le][0] + xi * lrho[le][1] + et * lrho[le][2];
         const T cte = om * (T)a.g * rhoq;
         P[20] = cte * sn;
         P[21] = cte * cs;
@@ -280,7 +296,7 @@ __global__ __launch_bounds__(256) void k_element(ElemArgs<T> a) {
                     for (int j = 0; j < 4; ++j)
                         if (lfix[le][2 * aa + (j >> 1)] | lfix[le][2 * bb + (j & 1)]) v[j] = T(0);
                 }
-                T *o = a.EK + ((int64_t)k * nel + e) * 4;
+                T *o = a.EK + ((uint32_t)k * nel + e) * 4u;
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     o[j] = v[j];
@@ -301,13 +317,13 @@ __global__ __launch_bounds__(256) void k_element(ElemArgs<T> a) {
                     if (lfix[le][2 * an]) g0 = T(0);
                     if (lfix[le][2 * an + 1]) g1 = T(0);
                 }
-                T *o = a.EG + ((int64_t)pg * nel + e) * 2;
+                T *o = a.EG + ((uint32_t)pg * nel + e) * 2u;
                 o[0] = g0;
                 o[1] = g1;
             }
         }
         if (lane < 6) {
-            a.EM[(int64_t)lane * nel + e] = macc;
+            a.EM[lane * nel + e] = macc;
             T f0 = facc[0], f1 = facc[1];
             if (!a.cart) {
                 const T ca = lcs[le][lane][0], sa = lcs[le][lane][1];
@@ -315,13 +331,12 @@ __global__ __launch_bounds__(256) void k_element(ElemArgs<T> a) {
                 f0 = r0;
                 f1 = r1;
             }
-            T *o = a.EF + ((int64_t)lane * nel + e) * 2;
+            T *o = a.EF + (lane * nel + e) * 2u;
             o[0] = f0;
             o[1] = f1;
         }
     }
     if (!a.proj) return;   // uniform over the grid: no barrier below is skipped by part of a block
-    __syncthreads();
 
     // ---- 4. P1 projection.  Ph (fine node f, vertex v) is the identity for f = v and, for the
     // midpoint 3 + k of edge (ea, eb) = (k, k+1 mod 3), 0.5 R(+h) towards ea and 0.5 R(-h) towards eb;
@@ -341,35 +356,60 @@ __global__ __launch_bounds__(256) void k_element(ElemArgs<T> a) {
         for (int j = 0; j < 4; ++j)
             if (lfix[le][2 * f + (j >> 1)] | lfix[le][2 * v + (j & 1)]) P[j] = T(0);
     };
+    // the nine nonzero blocks Ph(g, v), g in {v, 3 + v, 3 + (v + 2) % 3}, tabulated once per element
+    // after K_e (the point data are dead): PH + 4 (3 v + gi)
+    T *PH = L + 84;
+    if (live) {
+        for (int idx = lane; idx < 9; idx += kElLanes) {
+            const int v = idx / 3, gi = idx % 3;
+            const int g = gi == 0 ? v : (gi == 1 ? 3 + v : 3 + (v + 2) % 3);
+            T P[4];
+            ph(g, v, P);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) PH[4 * idx + j] = P[j];
+        }
+    }
+    __syncthreads();
+
+    T t1[3][4];   // T1 blocks b = lane + 8 m, held until every lane of the element has read K_e
     if (live) {
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
             const int b = lane + kElLanes * m;
+            T *t = t1[m];
+            t[0] = t[1] = t[2] = t[3] = T(0);
             if (b < 18) {
                 const int f = b / 3, v = b % 3;
-                T t[4] = {T(0), T(0), T(0), T(0)};
                 // nonzero Ph(g, v): g = v and the midpoints of the two edges at v
                 const int gs[3] = {v, 3 + v, 3 + (v + 2) % 3};
 #pragma unroll
                 for (int gi = 0; gi < 3; ++gi) {
-                    T K[4], P[4];
+                    T K[4];
                     kblock(L, f, gs[gi], K);
-                    ph(gs[gi], v, P);
+                    const T *P = PH + 4 * (3 * v + gi);
                     t[0] += K[0] * P[0] + K[1] * P[2];
                     t[1] += K[0] * P[1] + K[1] * P[3];
                     t[2] += K[2] * P[0] + K[3] * P[2];
                     t[3] += K[2] * P[1] + K[3] * P[3];
                 }
-                T *o = L + 84 + 4 * b;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) o[j] = t[j];
                 // R K_e block (u = v, node f): entry (cu, rc) = T1(f, u)[rc][cu], zero on a fixed row dof of f
-                T *rk = a.ERK + ((int64_t)(v * 6 + f) * nel + e) * 4;
+                T *rk = a.ERK + ((uint32_t)(v * 6 + f) * nel + e) * 4u;
 #pragma unroll
                 for (int cu = 0; cu < 2; ++cu)
 #pragma unroll
                     for (int rc = 0; rc < 2; ++rc)
                         rk[cu * 2 + rc] = lfix[le][2 * f + rc] ? T(0) : t[rc * 2 + cu];
+            }
+        }
+    }
+    __syncthreads();   // K_e is read: T1 takes its place
+    if (live) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            const int b = lane + kElLanes * m;
+            if (b < 18) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) L[4 * b + j] = t1[m][j];
             }
         }
     }
@@ -381,9 +421,8 @@ __global__ __launch_bounds__(256) void k_element(ElemArgs<T> a) {
             const int fs[3] = {u, 3 + u, 3 + (u + 2) % 3};
 #pragma unroll
             for (int fi = 0; fi < 3; ++fi) {
-                T P[4];
-                ph(fs[fi], u, P);
-                const T *t = L + 84 + 4 * (fs[fi] * 3 + v);
+                const T *P = PH + 4 * (3 * u + fi);
+                const T *t = L + 4 * (fs[fi] * 3 + v);
                 // E(u, v)[cu][cv] += sum_rc Ph(f, u)[rc][cu] T1(f, v)[rc][cv]
                 E[0] += P[0] * t[0] + P[2] * t[2];
                 E[1] += P[0] * t[1] + P[2] * t[3];
@@ -394,7 +433,7 @@ __global__ __launch_bounds__(256) void k_element(ElemArgs<T> a) {
             for (int cu = 0; cu < 2; ++cu)
 #pragma unroll
                 for (int cv = 0; cv < 2; ++cv)
-                    a.E1[(int64_t)((2 * u + cu) * 6 + 2 * v + cv) * nel + e] = E[cu * 2 + cv];
+                    a.E1[(uint32_t)((2 * u + cu) * 6 + 2 * v + cv) * nel + e] = E[cu * 2 + cv];
         }
     }
 }
@@ -453,6 +492,8 @@ template <typename T>
 void assemble_impl(Context &c, int flags, T *K, T *G, T *GT, T *M, T *F, T *A1, T *E1, T *RK) {
     const int64_t nel = c.nel;
     const bool proj = A1 != nullptr;
+    if ((uint64_t)84 * (uint64_t)nel > (uint64_t)UINT32_MAX)
+        throw Error(STK_E_SHAPE, "mesh too large for the element kernel's 32-bit offsets");
     DBuf<T> *bufp;
     if constexpr (sizeof(T) == 8) bufp = &c.el64;
     else bufp = &c.el32;
@@ -478,7 +519,8 @@ void assemble_impl(Context &c, int flags, T *K, T *G, T *GT, T *M, T *F, T *A1, 
     a.ERK = proj ? p : nullptr;
     a.E1 = E1;
     const int nb = (int)((nel + kElPerBlock - 1) / kElPerBlock);
-    k_element<T><<<dim3(nb), dim3(256), 0, c.stream>>>(a);
+    if (a.rule.nq == 7) k_element<T, 7><<<dim3(nb), dim3(256), 0, c.stream>>>(a);
+    else k_element<T, 6><<<dim3(nb), dim3(256), 0, c.stream>>>(a);
     STK_HIP(hipGetLastError());
     gather<T, 4, 1>(c, c.gK, a.EK, K);
     gather<T, 2, 0>(c, c.gG, a.EG, G);
