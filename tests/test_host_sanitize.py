@@ -42,10 +42,13 @@ def _run(driver, d, *extra):
     return {k: int(v) for k, v in fields.items()}
 
 
-@pytest.mark.parametrize("kind,levels", [("split1", 2), ("grid", 3)])
-def test_setup_is_clean_under_sanitizers(driver, tmp_path, kind, levels):
-    mesh = build_annulus(32, 256) if kind == "split1" else build_grid(32, 256)
-    cpu_port.write_inputs(make_inputs(scaled_workload(3, 32, 256), mesh=mesh), tmp_path)
+@pytest.mark.parametrize("kind,size,levels", [("split1", (32, 256), 2), ("grid", (32, 256), 3),
+                                              ("split1", (7, 18), 1), ("grid", (5, 24), 1)])
+def test_setup_is_clean_under_sanitizers(driver, tmp_path, kind, size, levels):
+    """(7, 18) and (5, 24): odd radial counts and azimuthal counts that are not powers of two (partial
+    64-row slices, the P1 level only)."""
+    mesh = build_annulus(*size) if kind == "split1" else build_grid(*size)
+    cpu_port.write_inputs(make_inputs(scaled_workload(3, *size), mesh=mesh), tmp_path)
     out = _run(driver, tmp_path)
     assert out["split"] == (2 if kind == "split1" else 4) and out["levels"] == levels
 
