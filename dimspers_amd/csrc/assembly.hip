@@ -481,13 +481,10 @@ __device__ __forceinline__ void galerkin_child(const GalArgs<T> &a, int64_t ef, 
             }
 }
 
-// Interpolation block: weight w, rotation by the fine-minus-coarse angle d, radial row / column
-// zero on a fixed fine / coarse dof.
+// Interpolation block: weight w, rotation by the fine-minus-coarse angle (sine sd, cosine cd), radial
+// row / column zero on a fixed fine / coarse dof.
 template <typename T>
-__device__ __forceinline__ void galerkin_block(T w, T d, bool ffix, bool cfix, T (&blk)[4]) {
-    T sd, cd;
-    if constexpr (sizeof(T) == 8) sincos(d, &sd, &cd);
-    else sincosf(d, &sd, &cd);
+__device__ __forceinline__ void galerkin_block(T w, T sd, T cd, bool ffix, bool cfix, T (&blk)[4]) {
     blk[0] = w * cd;
     blk[1] = cfix ? T(0) : -w * sd;
     blk[2] = ffix ? T(0) : w * sd;
@@ -517,6 +514,23 @@ __device__ __forceinline__ void galerkin_store(const GalArgs<T> &a, int64_t p, i
 }
 
 template <typename T>
+__device__ __forceinline__ void sincos_g(T d, T *s, T *c) {
+    if constexpr (sizeof(T) == 8) sincos(d, s, c);
+    else sincosf(d, s, c);
+}
+
+// v[k] for a lane-dependent k in 0..4, by selects (a dynamic index would put v in scratch memory)
+template <typename T>
+__device__ __forceinline__ T pick5(const T (&v)[5], int k) {
+    T r = v[0];
+    r = k == 1 ? v[1] : r;
+    r = k == 2 ? v[2] : r;
+    r = k == 3 ? v[3] : r;
+    r = k == 4 ? v[4] : r;
+    return r;
+}
+
+template <typename T>
 __global__ __launch_bounds__(64) void k_galerkin(GalArgs<T> a) {
     const int64_t j = (int64_t)blockIdx.x * 64 + threadIdx.x;
     if (j >= a.cnt) return;
@@ -528,6 +542,11 @@ __global__ __launch_bounds__(64) void k_galerkin(GalArgs<T> a) {
     int ci[3], cj[3];
     tri_verts<T>(t, I, J, ci, cj);
     const T dthc = T(2.0 * M_PI) / T(a.nct_c);
+    // A fine vertex lies half a coarse cell from a coarse one at most: the fine-minus-coarse angle is
+    // dthc h with h = fj / 2 - cj in {-1, -1/2, 0, 1/2, 1}, so five sincos serve all 36 blocks.
+    T sh[5], chh[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) sincos_g(dthc * (T(0.5) * T(k) - T(1)), &sh[k], &chh[k]);
     T E[6][6];
 #pragma unroll
     for (int u = 0; u < 6; ++u)
@@ -552,7 +571,8 @@ __global__ __launch_bounds__(64) void k_galerkin(GalArgs<T> a) {
 #pragma unroll
             for (int v = 0; v < 3; ++v) {
                 const bool cfix = (ci[v] == 0) || (ci[v] == a.ncr_c);
-                galerkin_block<T>(lam[v], dthc * (T(0.5) * fj[f] - T(cj[v])), ffix, cfix, Ph[f][v]);
+                const int k = fj[f] - 2 * cj[v] + 2;   // 2 h + 2
+                galerkin_block<T>(lam[v], pick5(sh, k), pick5(chh, k), ffix, cfix, Ph[f][v]);
             }
         }
         galerkin_child<T>(a, ef, Ph, E);
@@ -626,7 +646,11 @@ __global__ __launch_bounds__(64) void k_galerkin4(GalArgs<T> a) {
             const T l1 = ((cu[0] - cu[2]) * (fv - cv[2]) - (fu - cu[2]) * (cv[0] - cv[2])) / det;
             const T lam[3] = {l0, l1, T(1) - l0 - l1};
 #pragma unroll
-            for (int v = 0; v < 3; ++v) galerkin_block<T>(lam[v], dthc * (fpos - cpos[v]), ffix, cfix[v], Ph[f][v]);
+            for (int v = 0; v < 3; ++v) {
+                T sd, cd;
+                sincos_g(dthc * (fpos - cpos[v]), &sd, &cd);
+                galerkin_block<T>(lam[v], sd, cd, ffix, cfix[v], Ph[f][v]);
+            }
         }
         galerkin_child<T>(a, ef, Ph, E);
     }
