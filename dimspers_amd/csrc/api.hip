@@ -22,6 +22,9 @@ double update_velocity(Context &c, double *Ucart_dev);
 Context::~Context() {
     delete mlp;
     if (h_pinned) (void)hipHostFree(h_pinned);
+    if (dense_ev) (void)hipEventDestroy(dense_ev);
+    if (dense_ev2) (void)hipEventDestroy(dense_ev2);
+    if (dense_stream) (void)hipStreamDestroy(dense_stream);
     if (aux_stream) (void)hipStreamDestroy(aux_stream);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -100,6 +103,9 @@ int stk_create(stk_ctx **out, int64_t nel, int64_t nnod, int64_t nvert, const in
         STK_HIP(hipGetDevice(&c.device));
         STK_HIP(hipStreamCreate(&c.stream));
         STK_HIP(hipStreamCreate(&c.aux_stream));
+        STK_HIP(hipStreamCreate(&c.dense_stream));
+        STK_HIP(hipEventCreateWithFlags(&c.dense_ev, hipEventDisableTiming));
+        STK_HIP(hipEventCreateWithFlags(&c.dense_ev2, hipEventDisableTiming));
         c.nel = nel; c.nnod = nnod; c.nvert = nvert; c.nU = 2 * nnod; c.nP = nvert;
         c.e2n.resize(6 * nel);
         for (int64_t e = 0; e < nel; ++e)

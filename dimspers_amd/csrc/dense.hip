@@ -2,11 +2,12 @@
 // n of a few thousand), FP64, in three stages over 64 x 64 tiles of the column-major matrix padded
 // to a multiple of 64 (identity in the padding):
 //
-//  1. right-looking blocked Cholesky A = L L^T: per tile column k, the diagonal tile by one
-//     wavefront (k_potrf_tile), the tiles below it by triangular solves (k_trsm_panel), then the
+//  1. right-looking blocked Cholesky A = L L^T: per tile column k, the diagonal tile by four
+//     wavefronts (k_potrf_tile), the tiles below it by triangular solves (k_trsm_panel), then the
 //     trailing lower tiles minus L(I,k) L(J,k)^T (k_tile_gemm, SYRK form);
 //  2. Y = L^-1 by blocked forward substitution: per tile row i, Y(i, j) = L(i,i)^-1 R(i, j) for j <= i
-//     (k_trsm_rows), then R(m, j) -= L(m, i) Y(i, j) for the rows m > i (k_tile_gemm);
+//     (k_trsm_rows), then R(m, j) -= L(m, i) Y(i, j) for the rows m > i (k_tile_gemm), on a second
+//     stream, each tile row as soon as the factorisation has finished its tile column;
 //  3. A^-1 = Y^T Y (k_tile_gemm, LAUUM form), written as the full symmetric matrix.
 //
 // Every stage is made of 64 x 64 x 64 tile products staged through LDS in chunks of 16, each of the
@@ -29,40 +30,59 @@ __device__ __forceinline__ double *tile(double *A, int64_t ld, int I, int J) {
     return A + (int64_t)J * kT * ld + (int64_t)I * kT;
 }
 
-// Factor the lower triangle of diagonal tile k in place by one wavefront: lane t keeps row t of
-// the tile in registers; per column j the pivot comes from lane j by a cross-lane read, the column
-// of L goes through LDS (a wavefront needs no barrier between its own LDS write and read, only the
-// compiler fence of __syncthreads), and each lane updates its row to the right of j.  The upper
-// triangle of the tile is set to zero.  A pivot that is not positive is reported in *info (1-based
-// column, the first one only) and replaced by 1 so that the factorisation finishes.
-__global__ __launch_bounds__(kT) void k_potrf_tile(double *A, int64_t ld, int k, int *info) {
+// Factor the lower triangle of diagonal tile k in place by four wavefronts: thread (t, q) =
+// (tid % 64, tid / 64) keeps the entries c = 4 m + q of row t in registers.  Per column j the pivot
+// comes through LDS from the thread that holds it, every thread forms its square root and reciprocal
+// (the same values), the threads holding column j scale it and publish it in LDS, and each thread
+// updates its entries of row t right of j.  Every entry of the lower triangle sees the same
+// operations in the same order as in a one-row-per-lane factorisation.  The upper triangle of the
+// tile is set to zero.
+// A pivot that is not positive is reported in *info (1-based column, the first one only) and
+// replaced by 1 so that the factorisation finishes.
+constexpr int kPotrfWaves = 4;
+constexpr int kPotrfPer = kT / kPotrfWaves;   // entries of a row per thread
+
+__global__ __launch_bounds__(kT * kPotrfWaves) void k_potrf_tile(double *A, int64_t ld, int k, int *info) {
     __shared__ double col[kT];
-    const int t = threadIdx.x;
+    __shared__ double piv;
+    const int t = threadIdx.x % kT, q = threadIdx.x / kT;   // q is uniform over a wavefront
     double *D = tile(A, ld, k, k);
-    double a[kT];
+    double a[kPotrfPer];
 #pragma unroll
-    for (int c = 0; c < kT; ++c) a[c] = c <= t ? D[(int64_t)c * ld + t] : 0.0;
+    for (int m = 0; m < kPotrfPer; ++m) {
+        const int c = kPotrfWaves * m + q;
+        a[m] = c <= t ? D[(int64_t)c * ld + t] : 0.0;
+    }
 #pragma unroll
     for (int j = 0; j < kT; ++j) {
-        double d = __shfl(a[j], j);
+        const int mj = j / kPotrfWaves, qj = j % kPotrfWaves;
+        if (q == qj && t == j) piv = a[mj];
+        __syncthreads();
+        double d = piv;
         if (!(d > 0.0)) {
-            if (t == j && *info == 0) *info = kT * k + j + 1;
+            if (q == qj && t == j && *info == 0) *info = kT * k + j + 1;
             d = 1.0;
         }
         const double s = sqrt(d), rs = 1.0 / s;
-        a[j] = t == j ? s : (t > j ? a[j] * rs : 0.0);
-        col[t] = t > j ? a[j] : 0.0;
-        __syncthreads();
-        if (t > j) {
-            const double l = a[j];
-#pragma unroll
-            for (int c = j + 1; c < kT; ++c)
-                if (c <= t) a[c] -= l * col[c];
+        if (q == qj) {
+            a[mj] = t == j ? s : (t > j ? a[mj] * rs : 0.0);
+            col[t] = t > j ? a[mj] : 0.0;
         }
         __syncthreads();
+        // l = L(t, j), zero on the rows t <= j; the test c > j is uniform over a wavefront.  Entries
+        // above the diagonal (c > t) take updates too: they are never read and are stored as zero.
+        const double l = col[t];
+#pragma unroll
+        for (int m = 0; m < kPotrfPer; ++m) {
+            const int c = kPotrfWaves * m + q;
+            if (c > j) a[m] -= l * col[c];
+        }
     }
 #pragma unroll
-    for (int c = 0; c < kT; ++c) D[(int64_t)c * ld + t] = c <= t ? a[c] : 0.0;
+    for (int m = 0; m < kPotrfPer; ++m) {
+        const int c = kPotrfWaves * m + q;
+        D[(int64_t)c * ld + t] = c <= t ? a[m] : 0.0;
+    }
 }
 
 // L(I, k) = A(I, k) L(k, k)^-T for the tiles I = k + 1 + blockIdx.x: lane t solves row t in
@@ -259,28 +279,41 @@ __global__ void k_identity_lower(double *Y, int64_t ld) {
 
 // In-place inverse of the SPD matrix A (np x np column-major, np a multiple of 64); Y is np x np
 // work space.  *info (device) receives the first non-positive pivot (1-based) or stays 0.
-void spd_inverse(hipStream_t s, double *A, double *Y, int64_t np, int *info) {
+// The forward stage runs on stream s2 beside the factorisation on s: its tile row i reads only
+// L(i, i) and the column L(m > i, i), which are final once the factorisation's step i has solved its
+// panel (the later trailing updates touch columns right of i), so after each panel solve s records
+// `ev` and s2 waits for it.  The inverse stage waits on s for the whole forward stage (`ev2`).  Each
+// tile is computed by the same kernels in the same order as on one stream.
+void spd_inverse(hipStream_t s, hipStream_t s2, hipEvent_t ev, hipEvent_t ev2, double *A, double *Y, int64_t np,
+                 int *info) {
     if (np % kT != 0) throw Error(STK_E_STATE, "spd_inverse needs a multiple of 64");
     const int nt = (int)(np / kT);
     GemmArgs g{A, Y, A, np, 0, nt};
+    GemmArgs gf = g;
+    // s2 starts after the work already on s (the matrix A is written there)
+    STK_HIP(hipEventRecord(ev, s));
+    STK_HIP(hipStreamWaitEvent(s2, ev, 0));
+    k_identity_lower<<<dim3((unsigned)((np + 255) / 256), (unsigned)np), dim3(256), 0, s2>>>(Y, np);
     for (int k = 0; k < nt; ++k) {
-        k_potrf_tile<<<dim3(1), dim3(kT), 0, s>>>(A, np, k, info);
+        k_potrf_tile<<<dim3(1), dim3(kT * kPotrfWaves), 0, s>>>(A, np, k, info);
         const int m = nt - k - 1;
+        if (m > 0) k_trsm_panel<<<dim3(m), dim3(kT), 0, s>>>(A, np, k);
+        STK_HIP(hipEventRecord(ev, s));
+        STK_HIP(hipStreamWaitEvent(s2, ev, 0));
         if (m > 0) {
-            k_trsm_panel<<<dim3(m), dim3(kT), 0, s>>>(A, np, k);
             g.k = k;
             k_tile_gemm<GEMM_SYRK><<<dim3((unsigned)((int64_t)m * (m + 1) / 2)), dim3(256), 0, s>>>(g);
         }
-    }
-    STK_HIP(hipGetLastError());
-    k_identity_lower<<<dim3((unsigned)((np + 255) / 256), (unsigned)np), dim3(256), 0, s>>>(Y, np);
-    for (int i = 0; i < nt; ++i) {
-        k_trsm_rows<<<dim3(i + 1), dim3(kT), 0, s>>>(A, Y, np, i);
-        if (i + 1 < nt) {
-            g.k = i;
-            k_tile_gemm<GEMM_FWD><<<dim3(i + 1, nt - i - 1), dim3(256), 0, s>>>(g);
+        // forward tile row k on s2
+        k_trsm_rows<<<dim3(k + 1), dim3(kT), 0, s2>>>(A, Y, np, k);
+        if (m > 0) {
+            gf.k = k;
+            k_tile_gemm<GEMM_FWD><<<dim3(k + 1, m), dim3(256), 0, s2>>>(gf);
         }
     }
+    STK_HIP(hipGetLastError());
+    STK_HIP(hipEventRecord(ev2, s2));
+    STK_HIP(hipStreamWaitEvent(s, ev2, 0));
     g.Out = A;
     k_tile_gemm<GEMM_LAUUM><<<dim3((unsigned)((int64_t)nt * (nt + 1) / 2)), dim3(256), 0, s>>>(g);
     STK_HIP(hipGetLastError());

@@ -813,9 +813,11 @@ __global__ void k_symmetrize_lower_cm(int64_t n, const double *__restrict__ L, d
 
 // Dense inverse of the coarsest multigrid operator, computed on the device in FP64 (from the FP64
 // operator when the assembly has one): blocked Cholesky factorisation and inverse of the SPD matrix
-// (spd_inverse, dense.hip), all work on stream s, into out64 (n x n, row-major = column-major) and,
-// when out32 is given, its FP32 rounding.  It reads the coarsest level's pattern and operator and
-// writes no state of the context, so it may run on another stream beside the context stream's work.
+// (spd_inverse, dense.hip), work on stream s (and, for the forward stage, on the context's dense
+// stream, joined back into s), into out64 (n x n, row-major = column-major) and, when out32 is given,
+// its FP32 rounding.  It reads the coarsest level's pattern and operator and writes no state of the
+// context, so it may run on another stream beside the context stream's work (one call at a time: the
+// dense stream and its events are the context's).
 void coarse_inverse_on(Context &c, hipStream_t s, DBuf<double> &out64, DBuf<float> *out32) {
     MGLevel &L = c.mg.back();
     const int64_t n = 2 * L.nvert;
@@ -853,7 +855,7 @@ void coarse_inverse_on(Context &c, hipStream_t s, DBuf<double> &out64, DBuf<floa
     DBuf<int32_t> info(1);
     info.zero(s);
     // symmetric: the row-major matrix is its own column-major copy
-    spd_inverse(s, dA.p, Y.p, np, info.p);
+    spd_inverse(s, c.dense_stream, c.dense_ev, c.dense_ev2, dA.p, Y.p, np, info.p);
     int32_t hinfo = 0;
     info.download(&hinfo, 1, s);
     STK_HIP(hipStreamSynchronize(s));

@@ -165,6 +165,8 @@ struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t aux_stream = nullptr;   // set-up work that runs beside the context stream's
+    hipStream_t dense_stream = nullptr; // second stream of the coarsest inverse (spd_inverse)
+    hipEvent_t dense_ev = nullptr, dense_ev2 = nullptr;
     std::string err;
 
     int64_t nel = 0, nnod = 0, nvert = 0, nU = 0, nP = 0;
@@ -368,7 +370,10 @@ void build_gather_lists(Context &c);
 bool fixed_set_is_free_slip(const Context &c);
 // In-place inverse of the SPD matrix A (np x np column-major, np a multiple of 64) with np x np work
 // space Y; *info (device) receives the first non-positive pivot (1-based) or stays 0 (dense.hip).
-void spd_inverse(hipStream_t s, double *A, double *Y, int64_t np, int *info);
+// Work on stream s and, for the forward stage, s2 (ordered by the events ev and ev2); the result
+// is ready in stream order on s.
+void spd_inverse(hipStream_t s, hipStream_t s2, hipEvent_t ev, hipEvent_t ev2, double *A, double *Y, int64_t np,
+                 int *info);
 // IC(0) factorisation of the full pressure mass matrix M (values on MN in CSR order) for STK_PPC_IC.
 void build_pressure_ic(Context &c, const std::vector<double> &M);
 // Values of caller operators (1-based CSC) scattered into the layouts of K, G, G^T, M and F.
