@@ -846,13 +846,16 @@ void coarse_inverse_on(Context &c, hipStream_t s, DBuf<double> &out64, DBuf<floa
     }
     // padded to a multiple of 64 with the identity: the blocked inverse works on 64 x 64 tiles
     const int64_t np = (n + 63) / 64 * 64;
-    DBuf<double> dA(np * np), Y(np * np);
+    DBuf<double> &dA = c.dense_A, &Y = c.dense_Y;
+    dA.alloc(np * np);
+    Y.alloc(np * np);
     dA.zero(s);
     k_dense_from_blocks<<<dim3((unsigned)((L.nvert + 255) / 256)), dim3(256), 0, s>>>(
         L.nvert, np, L.pat.dptr.p, L.pat.dcol.p, bp, dA.p);
     if (np > n) k_pad_identity<<<dim3((unsigned)((np - n + 255) / 256)), dim3(256), 0, s>>>(n, np, dA.p);
     STK_HIP(hipGetLastError());
-    DBuf<int32_t> info(1);
+    DBuf<int32_t> &info = c.dense_info;
+    info.alloc(1);
     info.zero(s);
     // symmetric: the row-major matrix is its own column-major copy
     spd_inverse(s, c.dense_stream, c.dense_ev, c.dense_ev2, dA.p, Y.p, np, info.p);
@@ -888,10 +891,8 @@ void build_coarse_inverse(Context &c, bool f64) {
         }
         return;
     }
-    DBuf<double> inv;
-    coarse_inverse_on(c, c.stream, inv, f64 ? nullptr : &c.coarse_inv32);
+    coarse_inverse_on(c, c.stream, c.coarse_inv64, f64 ? nullptr : &c.coarse_inv32);
     c.coarse_n = n;
-    c.coarse_inv64 = std::move(inv);
     c.coarse64_valid = true;
 }
 
@@ -1208,8 +1209,10 @@ void null_vector_mixed(Context &c, PhaseTimer &pt) {
     const int64_t n = c.nU;
     double *z = c.rot64.p;
     float *z32 = c.rot32.p;
-    DBuf<double> y(n), d(n), r(n), keep(n);
-    DBuf<float> r32(n), e32(n);
+    DBuf<double> &y = c.nv_y, &d = c.nv_d, &r = c.nv_r, &keep = c.nv_keep;
+    DBuf<float> &r32 = c.nv_r32, &e32 = c.nv_e32;
+    y.alloc(n); d.alloc(n); r.alloc(n); keep.alloc(n);
+    r32.alloc(n); e32.alloc(n);
     double *yp = y.p, *dp = d.p, *rp = r.p, *kp = keep.p, *sc = c.d_sc.p;
     float *r32p = r32.p, *e32p = e32.p;
     Op22<double> K{&c.KN, c.K64.p};
@@ -1286,17 +1289,17 @@ void setup_fast(Context &c) {
     // issues the per-precision set-up (diagonals, smoother values, R A, eigenvalue bounds); it is
     // joined before the near-null vector, whose multigrid solves use it.  The work and its order on
     // each stream are those of a sequential set-up, so the results are the same bit for bit.
+    // The worker writes the context's coarse inverse buffers, which this thread does not touch
+    // before the join (coarse64_valid was cleared by the assembly).
     const bool overlap = c.aux_stream && !c.mg.empty();
-    DBuf<double> inv64;
-    DBuf<float> inv32;
     std::exception_ptr err;
     std::thread worker;
     if (overlap) {
         STK_HIP(hipStreamSynchronize(c.stream));   // the assembled operators the factorisation reads
-        worker = std::thread([&c, &inv64, &inv32, &err]() {
+        worker = std::thread([&c, &err]() {
             try {
                 STK_HIP(hipSetDevice(c.device));
-                coarse_inverse_on(c, c.aux_stream, inv64, &inv32);
+                coarse_inverse_on(c, c.aux_stream, c.coarse_inv64, &c.coarse_inv32);
             } catch (...) {
                 err = std::current_exception();
             }
@@ -1314,8 +1317,6 @@ void setup_fast(Context &c) {
         worker.join();
         if (err) std::rethrow_exception(err);
         c.coarse_n = 2 * c.mg.back().nvert;
-        c.coarse_inv64 = std::move(inv64);
-        c.coarse_inv32 = std::move(inv32);
         c.coarse64_valid = true;
         pt.mark("coarse inverse (auxiliary stream) joined");
     }
@@ -1662,7 +1663,8 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
         }
         const double *pc = o.pressure_pc == STK_PPC_IC ? nullptr : c.pc64.p;
         const double abs_tol = (o.stop == STK_STOP_FULL) ? 0.5 * o.rtol * c.fnorm : 0.0;
-        DBuf<double> ru(nU), hc(nP);
+        DBuf<double> &ru = c.res_u, &hc = c.res_p;
+        ru.alloc(nU); hc.alloc(nP);
         // from zero: the system itself; warm: the correction of (U, P), whose initial Schur
         // residual -G^T K^-1 (F + G P) is StokesPcCG_pardiso's (Rhs .+= GG*P, then U = K^-1 Rhs)
         const double *fu = Fm.p, *hh = nullptr;
@@ -1720,8 +1722,9 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
         const bool learned = (o.method == STK_SOLVE_LEARNED);
         if (learned && !c.S32) throw Error(STK_E_STATE, "learned solve needs the learned Schur operator");
         SchurWork<float> w = schur_work<float>(c);
-        DBuf<double> ru(nU), hc(nP);
-        DBuf<float> ru32(nU), hc32(nP);
+        DBuf<double> &ru = c.res_u, &hc = c.res_p;
+        DBuf<float> &ru32 = c.res_u32, &hc32 = c.res_p32;
+        ru.alloc(nU); hc.alloc(nP); ru32.alloc(nU); hc32.alloc(nP);
         int64_t iits = 0;
         double res = 1.0, prev = 1.0;
         bool use_learned = learned, last_conv = true;

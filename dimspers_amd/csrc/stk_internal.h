@@ -50,7 +50,11 @@ struct EventPair {
     EventPair &operator=(const EventPair &) = delete;
 };
 
-// Owning device buffer.
+// Owning device buffer.  release() empties it (p null, n zero: its contents are gone) but keeps the
+// allocation, which the next alloc() of the same size takes back instead of a hipFree / hipMalloc
+// pair (each re-assembly releases and re-allocates the same operators).  Only this buffer reuses
+// it, and its new contents are written in stream order after the work that read the old ones.
+// purge() and the destructor free the memory.
 template <typename T>
 struct DBuf {
     T *p = nullptr;
@@ -59,22 +63,44 @@ struct DBuf {
     explicit DBuf(size_t count) { alloc(count); }
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
-    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), spare(o.spare), spare_n(o.spare_n) {
+        o.p = o.spare = nullptr;
+        o.n = o.spare_n = 0;
+    }
     DBuf &operator=(DBuf &&o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) {
+            purge();
+            p = o.p; n = o.n; spare = o.spare; spare_n = o.spare_n;
+            o.p = o.spare = nullptr;
+            o.n = o.spare_n = 0;
+        }
         return *this;
     }
-    ~DBuf() { release(); }
+    ~DBuf() { purge(); }
     void alloc(size_t count) {
         if (count == n && p) return;
-        release();
+        if (!p && count && count == spare_n) {
+            p = spare; n = count;
+            spare = nullptr; spare_n = 0;
+            return;
+        }
+        purge();
         n = count;
         if (count) STK_HIP(hipMalloc(&p, count * sizeof(T)));
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (spare) (void)hipFree(spare);
+            spare = p; spare_n = n;
+        }
         p = nullptr;
         n = 0;
+    }
+    void purge() {
+        if (p) (void)hipFree(p);
+        if (spare) (void)hipFree(spare);
+        p = spare = nullptr;
+        n = spare_n = 0;
     }
     void upload(const T *h, size_t count, hipStream_t s) {
         alloc(count);
@@ -86,6 +112,10 @@ struct DBuf {
     }
     void zero(hipStream_t s);
     explicit operator bool() const { return p != nullptr; }
+
+  private:
+    T *spare = nullptr;   // the allocation release() kept
+    size_t spare_n = 0;
 };
 
 // Node-level sparsity pattern: row i lists the sorted column nodes of its blocks.
@@ -305,6 +335,13 @@ struct Context {
     bool mg_built = false;
     DBuf<double> coarse_inv64;                // dense inverse of the coarsest operator
     DBuf<float> coarse_inv32;
+    // work space kept from one call to the next (same sizes every time step)
+    DBuf<double> dense_A, dense_Y;            // the padded coarsest matrix and L^-1 (spd_inverse)
+    DBuf<int32_t> dense_info;
+    DBuf<double> nv_y, nv_d, nv_r, nv_keep;   // near-null vector set-up
+    DBuf<float> nv_r32, nv_e32;
+    DBuf<double> res_u, res_p;                // residuals of the solve's refinement
+    DBuf<float> res_u32, res_p32;
     int64_t coarse_n = 0;
     bool coarse64_valid = false;              // coarse_inv64 belongs to the current assembly
 
