@@ -21,7 +21,8 @@ double update_velocity(Context &c, double *Ucart_dev);
 
 Context::~Context() {
     delete mlp;
-    if (h_pinned) (void)hipHostFree(h_pinned);
+    if (h_pub) (void)hipHostFree(h_pub);
+    if (h_seq) (void)hipHostFree(h_seq);
     if (dense_ev) (void)hipEventDestroy(dense_ev);
     if (dense_ev2) (void)hipEventDestroy(dense_ev2);
     if (dense_stream) (void)hipStreamDestroy(dense_stream);
@@ -167,7 +168,9 @@ int stk_create(stk_ctx **out, int64_t nel, int64_t nnod, int64_t nvert, const in
         c.red64.alloc(std::max<int64_t>(2 * kReduceBlocks, 2 * (maxrows / 32 + 64)));   // up to 2 dots
         c.d_sc.alloc(SC_COUNT);
         c.d_sc.zero(c.stream);
-        STK_HIP(hipHostMalloc(&c.h_pinned, SC_COUNT * sizeof(double), hipHostMallocDefault));
+        STK_HIP(hipHostMalloc(&c.h_pub, SC_COUNT * sizeof(double), hipHostMallocCoherent));
+        STK_HIP(hipHostMalloc(&c.h_seq, 256, hipHostMallocCoherent));
+        *(volatile unsigned *)c.h_seq = 0u;
         // statistics data (stk_stats, stk_stats_noalloc) once per mesh; a mesh without the radius
         // levels of write_stats keeps the reason for those calls
         try {

@@ -17,6 +17,7 @@
 // and the caller removes the net rotation afterwards (updatevelocity2,
 // src/Stokes/StokesSolver.jl:170-184), exactly as the reference does.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -73,17 +74,43 @@ template struct DBuf<uint16_t>;
 // ------------------------------------------------------------------ launch helpers
 Reducer::Reducer(Context &c) : c(c) {}
 
-double Reducer::fetch(int slot) {
-    STK_HIP(hipMemcpyAsync(c.h_pinned + slot, c.d_sc.p + slot, sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    STK_HIP(hipStreamSynchronize(c.stream));
-    return c.h_pinned[slot];
+// Copy n scalars to the host's coherent memory, then publish the sequence number (system-scope
+// fence between the two, so a host that sees the number sees the values).
+__global__ void k_publish(const double *__restrict__ src, int n, double *dst, unsigned *seq_ptr, unsigned seq) {
+    if (threadIdx.x != 0) return;
+    for (int i = 0; i < n; ++i) dst[i] = src[i];
+    __threadfence_system();
+    *(volatile unsigned *)seq_ptr = seq;
 }
 
+double Reducer::fetch(int slot) {
+    double v;
+    fetch_many(slot, 1, &v);
+    return v;
+}
+
+// The host polls the sequence number instead of waiting for the stream (a round trip of ~6 instead
+// of ~15 us, measured on MI355X): the values are read as soon as the kernel has written them.  After
+// 2 ms of polling (the stream still had much work queued, or a kernel failed) it waits for the
+// stream, which also reports a failed kernel.
 void Reducer::fetch_many(int slot0, int n, double *out) {
-    STK_HIP(hipMemcpyAsync(c.h_pinned + slot0, c.d_sc.p + slot0, n * sizeof(double), hipMemcpyDeviceToHost,
-                           c.stream));
-    STK_HIP(hipStreamSynchronize(c.stream));
-    for (int i = 0; i < n; ++i) out[i] = c.h_pinned[slot0 + i];
+    const unsigned seq = ++c.pub_seq;
+    k_publish<<<dim3(1), dim3(64), 0, c.stream>>>(c.d_sc.p + slot0, n, c.h_pub + slot0, c.h_seq, seq);
+    STK_HIP(hipGetLastError());
+    volatile unsigned *hs = c.h_seq;
+    const auto t0 = std::chrono::steady_clock::now();
+    bool seen = false;
+    for (int k = 0;; ++k) {
+        if (*hs == seq) { seen = true; break; }
+        if ((k & 255) == 255 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) break;
+    }
+    if (!seen) {
+        STK_HIP(hipStreamSynchronize(c.stream));
+        if (*hs != seq) throw Error(STK_E_HIP, "a published scalar did not reach the host");
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    const volatile double *hp = c.h_pub + slot0;
+    for (int i = 0; i < n; ++i) out[i] = hp[i];
 }
 
 void Reducer::finish(int64_t npart, int slot, int nd) {

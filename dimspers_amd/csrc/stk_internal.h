@@ -381,7 +381,11 @@ struct Context {
     DBuf<double> U, P, work64;
     DBuf<double> red64;                       // reduction partials
     DBuf<double> d_sc;                        // device scalars
-    double *h_pinned = nullptr;               // pinned staging for scalars
+    // Scalars the host reads (residual norms): a one-thread kernel copies them from d_sc into this
+    // fine-grained (coherent) host memory and then writes a sequence number, which the host polls
+    double *h_pub = nullptr;
+    unsigned *h_seq = nullptr;
+    unsigned pub_seq = 0;
     MGLevelBufs<double> kw64, cg64;
     MGLevelBufs<float> kw32, cg32;
     SchurBufs<double> sw64;
