@@ -814,7 +814,7 @@ int stk_stress(stk_ctx *ctx, const double *Ucart, double *const out[8]) {
     return guarded(ctx, [&]() {
         Context &c = ctx->c;
         STK_CHECK(c.have_sfields, STK_E_STATE, "stress fields not set");
-        DBuf<double> u;
+        DBuf<double> &u = c.staged("stress.u");
         const double *Ud = c.d_Ucart.p;
         if (Ucart) {
             u.upload(Ucart, c.nU, c.stream);
@@ -822,7 +822,8 @@ int stk_stress(stk_ctx *ctx, const double *Ucart, double *const out[8]) {
         }
         STK_CHECK(Ud, STK_E_STATE, "no device velocity: call stk_updatevelocity or pass Ucart");
         const int64_t plane = c.nel * (int64_t)c.s_nip;
-        DBuf<double> o(8 * plane);
+        DBuf<double> &o = c.staged("stress.out");
+        o.alloc(8 * plane);
         launch_stress(c, Ud, o.p);
         if (out)
             for (int k = 0; k < 8; ++k)
@@ -844,8 +845,8 @@ int stk_thermal_solve(stk_ctx *ctx, double *T, const double *rho, double K, doub
             STK_CHECK(fixed_nodes[k] >= 1 && fixed_nodes[k] <= c.nnod, STK_E_SHAPE, "fixed node out of range");
             fx[k] = fixed_nodes[k] - 1;
         }
-        DBuf<double> t, r, vf;
-        DBuf<int64_t> f;
+        DBuf<double> &t = c.thw_io.t, &r = c.thw_io.r, &vf = c.thw_io.vf;
+        DBuf<int64_t> &f = c.thw_io.f;
         t.upload(T, c.nnod, c.stream);
         r.upload(rho, c.nnod, c.stream);
         f.upload(fx, c.stream);
@@ -864,7 +865,7 @@ int stk_stats(stk_ctx *ctx, const double *U, const double *T, double out[4]) {
     if (!ctx || !T || !out) return STK_E_ARG;
     return guarded(ctx, [&]() {
         Context &c = ctx->c;
-        DBuf<double> u, t;
+        DBuf<double> &u = c.staged("stats.u"), &t = c.staged("stats.t");
         const double *Ud = c.U.p;
         if (U) {
             u.upload(U, c.nU, c.stream);
@@ -891,7 +892,7 @@ int stk_deformation_gradient(stk_ctx *ctx, const double *U, double dt, double *F
         Context &c = ctx->c;
         STK_CHECK(c.s_nip > 0, STK_E_STATE, "stress rule not set (stk_set_stress_fields)");
         stress_prepare(c);
-        DBuf<double> u;
+        DBuf<double> &u = c.staged("defgrad.u");
         const double *Ud = c.U.p;
         if (U) {
             u.upload(U, c.nU, c.stream);
@@ -899,7 +900,7 @@ int stk_deformation_gradient(stk_ctx *ctx, const double *U, double dt, double *F
         }
         STK_CHECK(Ud, STK_E_STATE, "no device solution: solve first or pass U");
         const int64_t n = 4 * c.nel * (int64_t)c.s_nip;
-        DBuf<double> f;
+        DBuf<double> &f = c.staged("defgrad.f");
         f.upload(F, n, c.stream);
         launch_deformation_gradient(c, Ud, dt, f.p);
         f.download(F, n, c.stream);
@@ -917,13 +918,13 @@ int stk_fse(stk_ctx *ctx, double *F, int32_t nip, int32_t mode, double eps, doub
                   "unknown healing mode");
         STK_CHECK(mode == STK_FSE_NO_HEALING || eps > 0, STK_E_ARG, "the destruction ratio eps must be positive");
         const int64_t npt = c.nel * (int64_t)nip;
-        DBuf<double> f;
+        DBuf<double> &f = c.staged("fse.f");
         f.upload(F, 4 * npt, c.stream);
         c.fse_nip = 0;
         c.d_fse.alloc(6 * npt);
         launch_fse(c, f.p, npt, mode, eps, c.d_fse.p);
         if (mode != STK_FSE_NO_HEALING) f.download(F, 4 * npt, c.stream);
-        DBuf<double> pts;
+        DBuf<double> &pts = c.staged("fse.points");
         if (fse) {
             pts.alloc(6 * npt);
             fse_layout(c, c.d_fse.p, pts.p, npt, true);
@@ -960,7 +961,8 @@ int stk_anisotropic_tensor(stk_ctx *ctx, const double *fse, int32_t nip, const d
         STK_CHECK(!(install & STK_INSTALL_STRESS) || c.have_sfields, STK_E_STATE,
                   "install into the stress fields needs stk_set_stress_fields first");
         const int64_t npt = c.nel * (int64_t)nip;
-        DBuf<double> f, fp, sn, s1, s2;
+        DBuf<double> &f = c.staged("aniso.fse"), &fp = c.staged("aniso.fse_points"), &sn = c.staged("aniso.a1"),
+                     &s1 = c.staged("aniso.r1"), &s2 = c.staged("aniso.r2");
         const double *dfse = c.d_fse.p;
         if (fse) {
             fp.upload(fse, 6 * npt, c.stream);

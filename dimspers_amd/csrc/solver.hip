@@ -2071,8 +2071,11 @@ void thermal_solve(Context &c, double *T, const double *rho, const ThermalStep &
         const double dt = s.dt;
         launch_map(c, ns, [=] __device__(int64_t q) { A[q] = Mt[q] + dt * Kt[q]; });
     }
-    DBuf<double> b(nn), xf(nn), t(nn), r(nn), z(nn), p(nn), Ap(nn), dinv(nn);
-    DBuf<uint8_t> flag(nn);
+    ThermalWork &w = c.thw;
+    DBuf<double> &b = w.b, &xf = w.xf, &t = w.t, &r = w.r, &z = w.z, &p = w.p, &Ap = w.Ap, &dinv = w.dinv;
+    DBuf<uint8_t> &flag = w.flag;
+    for (DBuf<double> *v : {&b, &xf, &t, &r, &z, &p, &Ap, &dinv}) v->alloc(nn);
+    flag.alloc(nn);
     xf.zero(c.stream);
     flag.zero(c.stream);
     {

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -191,6 +192,17 @@ struct MGLevel {
     MGLevelBufs<float> w32;
 };
 
+// Work vectors of the thermal step, kept from one step to the next.
+struct ThermalWork {
+    DBuf<double> b, xf, t, r, z, p, Ap, dinv;
+    DBuf<uint8_t> flag;
+};
+// Device copies of stk_thermal_solve's host arrays (temperature, density, fixed nodes and values).
+struct ThermalIo {
+    DBuf<double> t, r, vf;
+    DBuf<int64_t> f;
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -342,6 +354,12 @@ struct Context {
     DBuf<float> nv_r32, nv_e32;
     DBuf<double> res_u, res_p;                // residuals of the solve's refinement
     DBuf<float> res_u32, res_p32;
+    ThermalWork thw;
+    ThermalIo thw_io;
+    // Device copies of the host arrays that the API calls upload and download, kept from one call to
+    // the next by name (each name has one user; calls on a context do not overlap).
+    std::map<std::string, DBuf<double>> stage;
+    DBuf<double> &staged(const char *name) { return stage[name]; }
     int64_t coarse_n = 0;
     bool coarse64_valid = false;              // coarse_inv64 belongs to the current assembly
 
