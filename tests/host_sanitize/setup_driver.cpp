@@ -49,8 +49,37 @@ static bool has_block(const Pattern &p, int64_t row, int32_t col) {
         }                                                                      \
     } while (0)
 
+// DBuf keeps the allocation that release() drops and gives it back to an alloc() of the same size;
+// purge() and the destructor free it (the sanitizer's leak check covers the rest).
+static int check_dbuf() {
+    DBuf<double> b;
+    b.alloc(100);
+    REQUIRE(b && b.n == 100);
+    double *p0 = b.p;
+    b.release();
+    REQUIRE(!b && b.n == 0 && b.p == nullptr);
+    b.alloc(100);
+    REQUIRE(b.p == p0 && b.n == 100);
+    b.release();
+    b.alloc(50);
+    REQUIRE(b && b.n == 50);
+    b.p[49] = 1.0;
+    b.release();
+    b.release();
+    b.alloc(50);
+    REQUIRE(b && b.n == 50 && b.p[49] == 1.0);
+    DBuf<double> moved(std::move(b));
+    REQUIRE(!b && moved.n == 50);
+    moved.purge();
+    REQUIRE(!moved && moved.n == 0);
+    moved.alloc(50);
+    REQUIRE(moved && moved.n == 50);
+    return 0;
+}
+
 int main(int argc, char **argv) {
     if (argc < 2) return 2;
+    if (check_dbuf() != 0) return 1;
     const std::string dir = argv[1];
     const bool operators = argc > 2 && std::string(argv[2]) == "operators";
     long long nel, nnod, nvert, nfixed;
