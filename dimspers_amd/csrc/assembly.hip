@@ -530,9 +530,14 @@ __device__ __forceinline__ T pick5(const T (&v)[5], int k) {
     return r;
 }
 
+// Four lanes per coarse element, one child each (the four lanes of an element are consecutive in a
+// wavefront and share its range test); lane 0 sums the four child products in child order, as one
+// thread adding the children in turn would, and stores the coarse element.
 template <typename T>
-__global__ __launch_bounds__(64) void k_galerkin(GalArgs<T> a) {
-    const int64_t j = (int64_t)blockIdx.x * 64 + threadIdx.x;
+__global__ __launch_bounds__(256) void k_galerkin(GalArgs<T> a) {
+    const int64_t gid = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t j = gid / 4;
+    const int ch = (int)(gid % 4);
     if (j >= a.cnt) return;
     const int64_t p = a.off + j;
     const int64_t ec = a.perm[p];
@@ -543,7 +548,7 @@ __global__ __launch_bounds__(64) void k_galerkin(GalArgs<T> a) {
     tri_verts<T>(t, I, J, ci, cj);
     const T dthc = T(2.0 * M_PI) / T(a.nct_c);
     // A fine vertex lies half a coarse cell from a coarse one at most: the fine-minus-coarse angle is
-    // dthc h with h = fj / 2 - cj in {-1, -1/2, 0, 1/2, 1}, so five sincos serve all 36 blocks.
+    // dthc h with h = fj / 2 - cj in {-1, -1/2, 0, 1/2, 1}, so five sincos serve all the blocks.
     T sh[5], chh[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) sincos_g(dthc * (T(0.5) * T(k) - T(1)), &sh[k], &chh[k]);
@@ -552,7 +557,7 @@ __global__ __launch_bounds__(64) void k_galerkin(GalArgs<T> a) {
     for (int u = 0; u < 6; ++u)
 #pragma unroll
         for (int v = 0; v < 6; ++v) E[u][v] = T(0);
-    for (int ch = 0; ch < 4; ++ch) {
+    {
         const int di = c_child[t][ch][0], dj = c_child[t][ch][1], tt = c_child[t][ch][2];
         const int fi0 = 2 * I + di, fj0 = 2 * J + dj;
         const int64_t ef = 2 * ((int64_t)fi0 + (int64_t)a.ncr_f * fj0) + tt;
@@ -577,7 +582,15 @@ __global__ __launch_bounds__(64) void k_galerkin(GalArgs<T> a) {
         }
         galerkin_child<T>(a, ef, Ph, E);
     }
-    galerkin_store<T>(a, p, ec, E);
+    const int lb = (int)(threadIdx.x % 64) & ~3;   // lane of child 0 in the wavefront
+#pragma unroll
+    for (int u = 0; u < 6; ++u)
+#pragma unroll
+        for (int v = 0; v < 6; ++v) {
+            const T e1 = __shfl(E[u][v], lb + 1), e2 = __shfl(E[u][v], lb + 2), e3 = __shfl(E[u][v], lb + 3);
+            E[u][v] = ((E[u][v] + e1) + e2) + e3;
+        }
+    if (ch == 0) galerkin_store<T>(a, p, ec, E);
 }
 
 // Four triangles per cell (Grid): coarse element 4q + t = [K_t, K_t+1, centre] of cell (I, J) is the
@@ -713,9 +726,8 @@ void galerkin_impl(Context &c, size_t l, T *Ef, T *Ec, T *A) {
     for (size_t k = 0; k + 1 < C.col_off.size(); ++k) {
         a.off = C.col_off[k];
         a.cnt = C.col_off[k + 1] - C.col_off[k];
-        const int nb = (int)((a.cnt + 63) / 64);
-        if (C.split == 4) k_galerkin4<T><<<dim3(nb), dim3(64), 0, c.stream>>>(a);
-        else k_galerkin<T><<<dim3(nb), dim3(64), 0, c.stream>>>(a);
+        if (C.split == 4) k_galerkin4<T><<<dim3((unsigned)((a.cnt + 63) / 64)), dim3(64), 0, c.stream>>>(a);
+        else k_galerkin<T><<<dim3((unsigned)((4 * a.cnt + 255) / 256)), dim3(256), 0, c.stream>>>(a);
         STK_HIP(hipGetLastError());
     }
 }
