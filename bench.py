@@ -232,7 +232,7 @@ def main():
         log(f"plain FP64: {prep.solve_ms:.1f} ms, {prep.outer_iterations} outer, {prep.inner_iterations} inner, "
             f"residual {prep.full_residual:.2e}; fast-vs-plain errU {eu:.2e} errP {ep:.2e}")
         # errors of the fast solve against a plain FP64 solve three orders tighter (1e-11 full residual;
-        # untimed): the accuracy the 1e-7 bar is judged on
+        # untimed): the accuracy the 1e-7 bar is measured against
         trep = ctx.solve(_lib.default_opts(method=_lib.SOLVE_PLAIN, stop=_lib.STOP_FULL, rtol=1e-11, itmax=800))
         U_t, P_t = ctx.solution()
         plain["vs_tight_fp64"] = dict(

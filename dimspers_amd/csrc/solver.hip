@@ -1702,7 +1702,7 @@ int run_solve(Context &c, const stk_opts &o, stk_report &rep) {
             rep.full_residual = full_residual(c, c.U.p, c.P.p, ru.p, hc.p, project);
             fu = ru.p;
             hh = hc.p;
-            // the judged residual is already met: nothing to do
+            // the full residual target is already met: nothing to do
             if (o.stop == STK_STOP_FULL && rep.full_residual < o.rtol) {
                 conv = true;
                 sres = 0.0;
