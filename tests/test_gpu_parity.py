@@ -170,11 +170,13 @@ def _errs(U, P, Uo, Po):
     return eu, ep
 
 
-@pytest.mark.parametrize("mesh", [(4, 16, 3), (8, 64, 1), (32, 256, 1), (5, 24, 3), (7, 18, 1)])
+@pytest.mark.parametrize("mesh", [(4, 16, 3), (8, 64, 1), (32, 256, 1), (5, 24, 3), (7, 18, 1), (8, 64, 5)])
 @pytest.mark.parametrize("method", ["plain", "mixed"])
 def test_solve_matches_oracle_projected(mesh, method):
     """(5, 24) and (7, 18): odd radial counts and azimuthal counts that are not powers of two, so
-    partial 64-row slices at every level and no h-coarsening of the P1 level."""
+    partial 64-row slices at every level and no h-coarsening of the P1 level.  (8, 64) with the
+    fields of the stress workload: viscosity contrast 1e6, log-uniform anisotropy ratio up to 1e3
+    and a smooth director field."""
     nr, nt, w = mesh
     inp = make_inputs(scaled_workload(w, nr, nt))
     ctx = _ctx(inp, _lib.ASM_FP64 | _lib.ASM_FP32)
