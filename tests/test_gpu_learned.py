@@ -108,6 +108,29 @@ def test_learned_solve_converges_and_reports_its_gain():
     ctx.close()
 
 
+def test_learned_solve_on_the_stress_workload_fields():
+    """The learned + mixed solve with the stress workload's fields (contrast 1e6, anisotropy ratio up
+    to 1e3, smooth director) at 16x128, trained on those fields, against the plain FP64 solve."""
+    inp = make_inputs(scaled_workload(5, 16, 128))
+    ctx = _ctx(inp)
+    ctx.mlp_train(_lib.default_train_opts(snapshots=200))
+    ctx.learned_setup()
+    out = {}
+    for name, m in (("learned", _lib.SOLVE_LEARNED), ("plain", _lib.SOLVE_PLAIN)):
+        r = ctx.solve(_lib.default_opts(method=m, stop=_lib.STOP_FULL, rtol=1e-9, itmax=800))
+        U, P = ctx.solution()
+        out[name] = (U, P)
+        print(f"[result] {name}: outer {r.outer_iterations} inner {r.inner_iterations} sweeps {r.refinements} "
+              f"fallbacks {r.fallbacks} residual {r.full_residual:.2e}")
+        assert r.converged and r.full_residual < 1e-9
+    (U, P), (Upl, Ppl) = out["learned"], out["plain"]
+    eu = np.linalg.norm(U - Upl) / np.linalg.norm(Upl)
+    ep = np.linalg.norm((P - P.mean()) - (Ppl - Ppl.mean())) / np.linalg.norm(Ppl - Ppl.mean())
+    print(f"[result] learned vs plain FP64: errU {eu:.2e} errP {ep:.2e}")
+    assert eu < 1e-7 and ep < 1e-7
+    ctx.close()
+
+
 def test_poisoned_model_falls_back_to_the_plain_preconditioner():
     inp = make_inputs(scaled_workload(3, 16, 128))
     ctx = _ctx(inp)
