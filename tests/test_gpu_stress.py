@@ -31,7 +31,7 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-@pytest.mark.parametrize("mesh", [(8, 64, 3), (32, 256, 4)])
+@pytest.mark.parametrize("mesh", [(8, 64, 3), (32, 256, 4), (7, 18, 5)])
 @pytest.mark.parametrize("aniso", [False, True])
 def test_stress_matches_oracle(mesh, aniso):
     nr, nt, w = mesh
@@ -69,7 +69,7 @@ def test_stress_on_the_device_velocity_after_a_solve():
     ctx.close()
 
 
-@pytest.mark.parametrize("mesh", [(8, 64, 3), (32, 256, 4)])
+@pytest.mark.parametrize("mesh", [(8, 64, 3), (32, 256, 4), (7, 18, 5)])
 def test_deformation_gradient_matches_oracle(mesh):
     nr, nt, w = mesh
     inp = make_inputs(scaled_workload(w, nr, nt))
@@ -127,7 +127,7 @@ def test_stress_error_paths():
     ctx.close()
 
 
-@pytest.mark.parametrize("mesh", [(8, 64, 3), (32, 256, 4)])
+@pytest.mark.parametrize("mesh", [(8, 64, 3), (32, 256, 4), (7, 18, 5)])
 def test_stats_match_oracle(mesh):
     from oracle import stats_oracle
     nr, nt, w = mesh

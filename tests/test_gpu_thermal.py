@@ -23,7 +23,7 @@ def _ctx(m):
     return _lib.Context.from_mesh(m)
 
 
-@pytest.mark.parametrize("size", [(8, 64), (32, 256)])
+@pytest.mark.parametrize("size", [(8, 64), (32, 256), (7, 18)])
 def test_thermal_matrices_and_step_match_oracle(size):
     m = build_annulus(*size)
     nn = m.theta.size
